@@ -1,0 +1,346 @@
+"""ctypes bindings to the CPU checkers (test infrastructure only).
+
+* ``Oracle``: the C restatement in ``liboracle.so`` (portal_oracle.c).
+* ``Reference``: the reference's own Fortran routines compiled from its sources
+  into ``_ref/libportal_ref.so`` (only present where the reference is mounted and
+  ``make -C oracle ref`` has run).  Fortran passes every argument by reference and
+  expects column-major arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_ORACLE = os.path.join(HERE, "liboracle.so")
+LIB_REF = os.path.join(HERE, "_ref", "libportal_ref.so")
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(ip)
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("node_num", C.c_int), ("bulk_node_num", C.c_int), ("e_num", C.c_int),
+                ("XYZ", dp), ("vel", dp), ("temp", dp), ("b_all", dp), ("h2_dens", dp),
+                ("ab_all", dp), ("dust_p", dp), ("lev_p", dp), ("nbr_ptr", ip), ("nbr_idx", ip),
+                ("tx", C.c_double)]
+
+
+class OrcSpecies(C.Structure):
+    _fields_ = [("e_num", C.c_int), ("r_num", C.c_int), ("c_num", C.c_int), ("c_temp", C.c_int),
+                ("c_par", C.c_int), ("n_trans", C.c_int), ("kmax", C.c_int), ("c_dim", C.c_int),
+                ("j_lev", ip), ("lev_g", ip), ("r_trans", ip), ("c_trans", ip), ("all_trans", ip),
+                ("count_array", ip), ("e_levs_hz", dp), ("t_Aji", dp), ("nu_hz", dp), ("t_Bij", dp),
+                ("kd_v", dp), ("t_Cji", dp), ("col_temps", dp)]
+
+
+class OrcSeParams(C.Structure):
+    _fields_ = [("n_sphere", C.c_int), ("xth", dp), ("nvel", C.c_int), ("dx", C.c_double),
+                ("gvec", dp), ("bvec_all", dp), ("n_depth", C.c_int), ("extra_info", C.c_int)]
+
+
+def _f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _i32(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.int32))
+
+
+class Oracle:
+    def __init__(self, path: str = LIB_ORACLE):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle` (or __graft_entry__.build())")
+        self.lib = C.CDLL(path)
+        L = self.lib
+        L.orc_wig3jj.restype = C.c_double
+        L.orc_wig3jj.argtypes = [C.c_int] * 6
+        L.orc_wig6jj.restype = C.c_double
+        L.orc_wig6jj.argtypes = [C.c_int] * 6
+        L.orc_black_body.restype = C.c_double
+        L.orc_black_body.argtypes = [C.c_double, C.c_double]
+        L.orc_se_pass.restype = C.c_int
+        L.orc_solve_from_J.restype = C.c_int
+        L.orc_image_pass.restype = C.c_int
+        L.orc_inverse_rt.restype = C.c_int
+        L.orc_trace_out.restype = C.c_int
+        L.orc_trace_out_raytrace.restype = C.c_int
+        self._keep = []
+
+    # -- structures -------------------------------------------------------
+    def grid(self, g, lev_p=None):
+        arrs = dict(XYZ=_f64(g.XYZ), vel=_f64(g.vel_all), temp=_f64(g.temp_all), b_all=_f64(g.b_all),
+                    h2=_f64(g.h2_dens), ab=_f64(g.ab_all), dust=_f64(g.dust_p),
+                    lev=_f64(g.lev_p if lev_p is None else lev_p), ptr=_i32(g.nbr_ptr), idx=_i32(g.nbr_idx))
+        s = OrcGrid(g.node_num, g.bulk_node_num, arrs["lev"].shape[0], _d(arrs["XYZ"]), _d(arrs["vel"]),
+                    _d(arrs["temp"]), _d(arrs["b_all"]), _d(arrs["h2"]), _d(arrs["ab"]), _d(arrs["dust"]),
+                    _d(arrs["lev"]), _i(arrs["ptr"]), _i(arrs["idx"]), g.tx)
+        self._keep.append(arrs)
+        return s
+
+    def species(self, sp):
+        arrs = dict(j=_i32(sp.j_lev), g=_i32(sp.lev_g), rt=_i32(sp.r_trans), ct=_i32(sp.c_trans),
+                    at=_i32(sp.all_trans), ca=_i32(sp.count_array), e=_f64(sp.e_levs_hz), A=_f64(sp.t_Aji),
+                    nu=_f64(sp.nu_hz), B=_f64(sp.t_Bij), kd=_f64(sp.kd_v), C=_f64(sp.t_Cji), T=_f64(sp.col_temps))
+        s = OrcSpecies(sp.e_num, sp.r_num, sp.c_num, sp.c_temp, sp.c_par, sp.n_trans, sp.kmax, sp.c_dim,
+                       _i(arrs["j"]), _i(arrs["g"]), _i(arrs["rt"]), _i(arrs["ct"]), _i(arrs["at"]), _i(arrs["ca"]),
+                       _d(arrs["e"]), _d(arrs["A"]), _d(arrs["nu"]), _d(arrs["B"]), _d(arrs["kd"]), _d(arrs["C"]),
+                       _d(arrs["T"]))
+        self._keep.append(arrs)
+        return s
+
+    def params(self, xth, nvel, dx, gvec, bvec_all, n_depth, extra_info=0):
+        arrs = dict(xth=_f64(xth), g=_f64(gvec), b=_f64(bvec_all))
+        s = OrcSeParams(arrs["xth"].shape[1], _d(arrs["xth"]), nvel, dx, _d(arrs["g"]), _d(arrs["b"]),
+                        n_depth, extra_info)
+        self._keep.append(arrs)
+        return s
+
+    # -- wrappers ---------------------------------------------------------
+    def wig3jj(self, *a):
+        return self.lib.orc_wig3jj(*[int(x) for x in a])
+
+    def wig6jj(self, *a):
+        return self.lib.orc_wig6jj(*[int(x) for x in a])
+
+    def se_pass(self, g, sp, xth, nvel, dx, n_depth=4096, node_lo=1, node_hi=None, lev_p=None,
+                extra_info=0, nthreads=0, bvec_all=None, gvec=None):
+        """Nodes node_lo..node_hi of the SE pass.  Returns (pol_pops N x 2L, J0 R x N, J2 R x N[, rates, JJ])."""
+        if node_hi is None:
+            node_hi = g.bulk_node_num
+        G = self.grid(g, lev_p)
+        S = self.species(sp)
+        P = self.params(xth, nvel, dx, g.gvec if gvec is None else gvec,
+                        g.bvec_all if bvec_all is None else bvec_all, n_depth, extra_info)
+        N, L, R = g.node_num, sp.e_num, sp.r_num
+        pp = np.zeros((N, 2 * L), order="F")
+        J0 = np.zeros((R, N), order="F")
+        J2 = np.zeros((R, N), order="F")
+        rates = np.zeros((N, 3, 2 * L), order="F")
+        jj = np.zeros((N, R), order="F")
+        rc = self.lib.orc_se_pass(C.byref(G), C.byref(S), C.byref(P), node_lo, node_hi, _d(pp), _d(rates),
+                                  _d(jj), _d(J0), _d(J2), nthreads)
+        if rc:
+            raise RuntimeError(f"oracle se_pass failed ({rc})")
+        if extra_info:
+            return pp, J0, J2, rates, jj
+        return pp, J0, J2
+
+    def solve_from_J(self, g, sp, J0, J2, node_lo=1, node_hi=None):
+        if node_hi is None:
+            node_hi = g.bulk_node_num
+        G = self.grid(g)
+        S = self.species(sp)
+        pp = np.zeros((g.node_num, 2 * sp.e_num), order="F")
+        J0 = _f64(J0)
+        J2 = _f64(J2)
+        rc = self.lib.orc_solve_from_J(C.byref(G), C.byref(S), node_lo, node_hi, _d(J0), _d(J2), _d(pp))
+        if rc:
+            raise RuntimeError(f"oracle solve_from_J failed ({rc})")
+        return pp
+
+    def setup_stateq(self, sp, AC, Temp):
+        S = self.species(sp)
+        cd = sp.c_dim
+        mats = [np.zeros((cd, cd), order="F") for _ in range(4)]
+        AC = _f64(AC)
+        self.lib.orc_setup_stateq(C.byref(S), _d(AC), C.c_double(Temp), *[_d(m) for m in mats])
+        return mats
+
+    def make_ac(self, sp, nH2, Temp, J0, J2):
+        S = self.species(sp)
+        AC = np.zeros((5, sp.n_trans), order="F")
+        J0 = _f64(J0)
+        J2 = _f64(J2)
+        self.lib.orc_make_ac(C.byref(S), C.c_double(nH2), C.c_double(Temp), _d(J0), _d(J2), _d(AC))
+        return AC
+
+    def get_rhovec(self, sp, AC, Temp, extra_info=0):
+        S = self.species(sp)
+        rho = np.zeros(sp.c_dim)
+        rates = np.zeros((3, sp.c_dim), order="F")
+        AC = _f64(AC)
+        rc = self.lib.orc_get_rhovec(C.byref(S), _d(AC), C.c_double(Temp), _d(rho), extra_info, _d(rates))
+        if rc:
+            raise RuntimeError("singular SE matrix")
+        return (rho, rates) if extra_info else rho
+
+    def coef(self, name, j, k, j2, k2):
+        a = C.c_double()
+        b = C.c_double()
+        fn = getattr(self.lib, f"orc_{name}")
+        if name == "t_spont_const":
+            fn(j, k, j2, k2, C.byref(a))
+            return a.value
+        fn(j, k, j2, k2, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def image_pass(self, g, sp, pol_pops, lines, th_inc, ph_az, nvel, vel_out, im_pix, im_size, dx,
+                   alpha=0.15, n_depth_rt=4096, pix_lo=1, pix_hi=None, nthreads=0, bvec_all=None,
+                   with_dust=False):
+        """One inclination of the image pass.  lines = list of (l1, l2, nu0, Aji, kd)."""
+        n_rt = (2 * im_pix + 1) ** 2
+        if pix_hi is None:
+            pix_hi = n_rt
+        G = self.grid(g)
+        S = self.species(sp)
+        nl = len(lines)
+        l1 = _i32([x[0] for x in lines])
+        l2 = _i32([x[1] for x in lines])
+        nu = _f64([x[2] for x in lines])
+        A = _f64([x[3] for x in lines])
+        kd = _f64([x[4] for x in lines])
+        Cn = 2 * nvel + 1
+        outs = [np.zeros((n_rt, nl, Cn), order="F") for _ in range(4)]
+        dust = np.zeros((4, n_rt, nl, Cn)) if with_dust else None
+        pp = _f64(pol_pops)
+        vo = _f64(vel_out)
+        bv = _f64(g.bvec_all if bvec_all is None else bvec_all)
+        rc = self.lib.orc_image_pass(C.byref(G), C.byref(S), nl, _i(l1), _i(l2), _d(nu), _d(A), _d(kd),
+                                     C.c_double(th_inc), C.c_double(ph_az), _d(bv), _d(pp), nvel, _d(vo), im_pix,
+                                     C.c_double(im_size), C.c_double(dx), C.c_double(alpha), n_depth_rt, pix_lo,
+                                     pix_hi, *[_d(o) for o in outs], _d(dust) if with_dust else None, nthreads)
+        if rc:
+            raise RuntimeError(f"oracle image_pass failed ({rc})")
+        if with_dust:
+            return outs, dust
+        return outs
+
+
+class Reference:
+    """The reference's own routines, compiled from its Fortran sources (never shipped)."""
+
+    def __init__(self, path: str = LIB_REF):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = C.CDLL(path)
+
+    @staticmethod
+    def _r(x, t=C.c_double):
+        return C.byref(t(x))
+
+    def legendre_all(self, nth, nph):
+        xth = np.zeros((3, nth * nph), order="F")
+        n = C.c_int(0)
+        self.lib.legendre_all_(C.byref(C.c_int(nth)), C.byref(C.c_int(nph)), C.byref(n), _d(xth))
+        return np.asfortranarray(xth[:, : n.value])
+
+    def find_dir(self, bvec, gvec, th, ph):
+        b = _f64(bvec)
+        g = _f64(gvec)
+        d = np.zeros(3)
+        self.lib.find_dir_(_d(b), _d(g), self._r(th), self._r(ph), _d(d))
+        return d
+
+    def black_body(self, T, nu):
+        out = C.c_double(0.0)
+        self.lib.black_body_(self._r(T), self._r(nu), C.byref(out))
+        return out.value
+
+    def inverse_rt(self, node, g, sp, del_ne, xth, nvel, dx, bvec, gvec, n_depth, lev_p=None):
+        """inverse_rt.f90:1-137 for one node (1-based) -> (J0, J2)."""
+        N = g.node_num
+        R = sp.r_num
+        ne = _i32(del_ne)
+        nnum = _i32(g.del_num_ne)
+        X = _f64(g.XYZ)
+        lp = _f64(g.lev_p if lev_p is None else lev_p)
+        J0 = np.zeros(R)
+        J2 = np.zeros(R)
+        xt = _f64(xth)
+        b = _f64(bvec)
+        gv = _f64(gvec)
+        rt = _i32(sp.r_trans)
+        ci = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+        cd = lambda v: C.byref(C.c_double(float(v)))  # noqa: E731
+        Xs = np.zeros(3)
+        node_c = C.c_int(int(node))
+        self.lib.inverse_rt_(
+            _i(ne), _i(nnum), ci(N), ci(ne.shape[1]), C.byref(node_c), _d(X), ci(nvel), ci(xt.shape[1]), ci(0),
+            _d(b), _d(gv), cd(dx), cd(g.tx), ci(R), _i(rt), _d(_f64(sp.nu_hz)), ci(sp.e_num), _d(lp),
+            _i(_i32(sp.lev_g)), _d(_f64(g.b_all)), _d(_f64(g.vel_all)), _d(_f64(sp.t_Aji)), _d(_f64(sp.t_Bij)),
+            _d(xt), _d(_f64(g.dust_p)), _d(_f64(sp.kd_v)), _d(_f64(g.temp_all[1])), _d(J0), _d(J2), ci(0), ci(0),
+            cd(0.0), cd(0.0), ci(0), ci(0), _d(Xs), ci(n_depth))
+        return J0, J2
+
+    def trace_out(self, node, g, del_ne, dir_, dx, n_depth):
+        ne = _i32(del_ne)
+        dl = np.zeros((2, n_depth + 1), dtype=np.int32, order="F")
+        nc = C.c_int(0)
+        Xs = np.zeros(3)
+        d = _f64(dir_)
+        self.lib.trace_out_(C.byref(C.c_int(int(node))), C.byref(C.c_int(g.node_num)), _d(_f64(g.XYZ)), _i(ne),
+                            C.byref(C.c_int(ne.shape[1])), _i(_i32(g.del_num_ne)), _d(d), C.byref(C.c_double(dx)),
+                            _i(dl), C.byref(nc), C.byref(C.c_int(n_depth + 1)), C.byref(C.c_int(0)), _d(Xs),
+                            C.byref(C.c_double(0.0)))
+        return dl[:, : nc.value].copy()
+
+    def make_ac(self, sp, nH2, Temp, J0, J2):
+        AC = np.zeros((5, sp.n_trans), order="F")
+        ci = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+        self.lib.make_ac_(_d(_f64(sp.t_Aji)), _d(_f64(sp.t_Cji)), _i(_i32(sp.r_trans)), _i(_i32(sp.c_trans)),
+                          ci(sp.r_num), ci(sp.c_num), ci(sp.c_temp), ci(sp.c_par), ci(sp.e_num),
+                          _d(_f64(sp.e_levs_hz)), C.byref(C.c_double(nH2)), C.byref(C.c_double(Temp)),
+                          _d(_f64(sp.col_temps)), ci(sp.n_trans), _i(_i32(sp.all_trans[:, : sp.n_trans])),
+                          _d(_f64(J0)), _d(_f64(J2)), _d(AC))
+        return AC
+
+    def rotate_all_save(self, g, th, ph, bvec_all):
+        N = g.node_num
+        vn = np.zeros((3, N), order="F")
+        bn = np.zeros((3, N), order="F")
+        xn = np.zeros((3, N), order="F")
+        xs = np.zeros(3)
+        xsn = np.zeros(3)
+        self.lib.rotate_all_save_(C.byref(C.c_int(N)), C.byref(C.c_double(th)), C.byref(C.c_double(ph)),
+                                  _d(_f64(g.vel_all)), _d(_f64(bvec_all)), _d(_f64(g.XYZ)), _d(xs), _d(vn), _d(bn),
+                                  _d(xn), _d(xsn))
+        return vn, bn, xn
+
+    def trace_out_raytrace(self, x, y, g, X_inc, del_ne, dx, n_depth_rt, nu0_all):
+        ne = _i32(del_ne)
+        dl = np.zeros((2, n_depth_rt), dtype=np.int32, order="F")
+        nc = C.c_int(0)
+        nlev = len(nu0_all)
+        I_in = np.zeros(nlev)
+        d = np.array([0.0, 0.0, 1.0])
+        xs = np.zeros(3)
+        self.lib.trace_out_raytrace_(C.byref(C.c_double(x)), C.byref(C.c_double(y)), C.byref(C.c_int(g.node_num)),
+                                     _d(_f64(X_inc)), C.byref(C.c_int(0)), C.byref(C.c_double(0.0)), _d(xs),
+                                     C.byref(C.c_double(0.0)), _i(ne), C.byref(C.c_int(ne.shape[1])),
+                                     _i(_i32(g.del_num_ne)), _d(d), C.byref(C.c_double(dx)),
+                                     C.byref(C.c_int(n_depth_rt)), _i(dl), C.byref(nc), C.byref(C.c_int(nlev)),
+                                     _d(_f64(nu0_all)), _d(I_in))
+        return dl[:, : nc.value].copy(), I_in
+
+    def polarize_prop(self, eI, eQ, pI, pQ, ds, I, Q, U):
+        a = [C.c_double(v) for v in (I, Q, U)]
+        self.lib.polarize_prop_(*[C.byref(C.c_double(v)) for v in (eI, eQ, pI, pQ, ds)], *[C.byref(v) for v in a])
+        return tuple(v.value for v in a)
+
+    def rot_stokes(self, Q, U, b, back=False):
+        qn = C.c_double()
+        un = C.c_double()
+        fn = self.lib.rot_stokes_back_ if back else self.lib.rot_stokes_
+        fn(C.byref(C.c_double(Q)), C.byref(C.c_double(U)), C.byref(qn), C.byref(un), _d(_f64(b)))
+        return qn.value, un.value
+
+    def ray_in_raytrace(self, vel_obs, nc, pol, b_list, bvec_list, dust, alpha, kd, bb_list, vel_list, length_list,
+                        dx, nu0, I_in):
+        outs = [C.c_double() for _ in range(8)]
+        self.lib.ray_in_raytrace_(C.byref(C.c_double(vel_obs)), C.byref(C.c_int(nc)), _d(_f64(pol)),
+                                  _d(_f64(b_list)), _d(_f64(bvec_list)), _d(_f64(dust)), C.byref(C.c_double(alpha)),
+                                  C.byref(C.c_double(kd)), _d(_f64(bb_list)), _d(_f64(vel_list)),
+                                  _i(_i32(length_list)), C.byref(C.c_double(dx)), C.byref(C.c_double(nu0)),
+                                  C.byref(C.c_double(I_in)), *[C.byref(o) for o in outs])
+        return np.array([o.value for o in outs])
