@@ -1,0 +1,157 @@
+/*
+ * portal_hip.h -- C ABI of the MI355X (gfx950) implementation of PORTAL's hot path.
+ *
+ * One call replaces one OpenMP loop of the reference's drivers:
+ *
+ *   portal_se_pass      the node loop of main_para.f90:210-257, i.e. trace_node
+ *                       (main_sub.f90:1-141) for every bulk node: inverse_rt
+ *                       (inverse_rt.f90:1-137) + make_AC (miscellaneous.f90:288-392)
+ *                       + get_rhovec (em_coefs.f90:115-201) + packing.
+ *   portal_inverse_rt   inverse_rt alone for every bulk node (J^0_0, J^2_0).
+ *   portal_se_solve     make_AC + get_rhovec + packing from given J's.
+ *   portal_image_pass   the three loops of one inclination, para_finish.f90:382-449
+ *                       (rotate_all_save, trace_out_raytrace, comp_emcoefs,
+ *                       ray_in_raytrace).
+ *   portal_lambda_iterate  a build extension with no reference counterpart: the
+ *                       SE pass repeated with lev_p <- ab n_H2 sqrt(2J+1) rho^0_0.
+ *
+ * Conventions (those of the reference's Fortran interface):
+ *   - arrays are column-major as declared in the reference; ids are 1-based;
+ *   - all host memory belongs to the caller; device memory belongs to the context;
+ *   - every function returns PORTAL_OK (0) or a negative status code in place of
+ *     the reference's `stop`; portal_last_error() gives the message;
+ *   - one context per device, called from one host thread at a time.
+ *
+ * Fortran binds these through ISO_C_BINDING (portal_amd/fortran/portal_hip_mod.f90).
+ */
+#ifndef PORTAL_HIP_H
+#define PORTAL_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PORTAL_OK 0
+#define PORTAL_ERR_ARG (-1)          /* bad argument or unsupported size */
+#define PORTAL_ERR_DEVICE (-2)       /* HIP runtime error */
+#define PORTAL_ERR_DEPTH (-3)        /* node_count > n_depth, inverse_rt.f90:206-210 */
+#define PORTAL_ERR_DEPTH_RT (-4)     /* node_count > n_depth_rt, raytrace.f90:169-173 */
+#define PORTAL_ERR_LU (-5)           /* singular SE matrix, em_coefs.f90:170-178 */
+#define PORTAL_ERR_TEMP (-6)         /* T_kin above the collision table, miscellaneous.f90:331-360 */
+#define PORTAL_ERR_STATE (-7)        /* grid/species not uploaded */
+#define PORTAL_ERR_NOMEM (-8)        /* device allocation failed */
+#define PORTAL_ERR_UNSUPPORTED (-9)  /* star mode (star_mod=1) */
+
+#define PORTAL_FP64 0   /* every stage in FP64 */
+#define PORTAL_MIXED 1  /* FP32 ray transfer + compensated FP32 J sums; FP64 SE */
+
+typedef struct portal_ctx portal_ctx;
+
+typedef struct {
+    int n_sphere;            /* directions (legendre_all, miscellaneous.f90:1100-1137) */
+    const double *xth;       /* 3 x n_sphere: (mu, phi, weight) */
+    int nvel;                /* channels = 2*nvel+1 (inverse_rt.f90:275) */
+    double dx;               /* length quantum, the reference passes float 1E-6 (main_para.f90:135) */
+    const double *gvec;      /* 3: gauge vector (main_para.f90:172-176) */
+    const double *bvec_all;  /* 3 x node_num: unit field directions */
+    int n_depth;             /* maximum cells per inward ray */
+    int extra_info;          /* 1: also fill pol_rates and JJ_20 */
+    int precision;           /* PORTAL_FP64 or PORTAL_MIXED */
+    int star_mod;            /* must be 0 */
+} portal_se_params;
+
+typedef struct {
+    int max_iter;            /* lambda iterations at most */
+    double tol_pop;          /* stop when max |d(sqrt(g) rho00)| / (sqrt(g) rho00) <= tol_pop */
+    double tol_align;        /* ... and max |d(rho20/rho00)| <= tol_align */
+    int surrogate;           /* 1: the trained MLP supplies the first iterate */
+    double fallback_tol;     /* per-node SE residual above which the MLP iterate is rejected */
+} portal_iter_params;
+
+typedef struct {
+    int nlevels;             /* imaged lines */
+    const int *l1, *l2;      /* lower / upper level of each imaged line (1-based) */
+    const double *nu0_all;   /* line frequencies [Hz] as the driver holds them */
+    const double *Aji_all;
+    const double *kd_all;
+    double th_inc, ph_az;    /* one inclination */
+    int nvel;                /* channels = 2*nvel+1 */
+    const double *vel_out;   /* 2*nvel+1 channel velocities */
+    int im_pix;              /* image is (2*im_pix+1)^2 pixels */
+    double im_size;
+    double dx;               /* march step, the reference passes float 1E-4 (para_finish.f90:201) */
+    double alpha;            /* dust polarisation (para_finish.f90:78) */
+    int n_depth_rt;
+    int precision;           /* PORTAL_FP64 or PORTAL_MIXED */
+} portal_img_params;
+
+typedef struct {
+    double t_walk_ms;        /* inward path construction (done once per geometry) */
+    double t_prep_ms;        /* per-iteration line coefficients */
+    double t_rt_ms;          /* inward ray integration */
+    double t_se_ms;          /* SE assembly + solve */
+    double t_img_ms;         /* image pass */
+    double t_total_ms;       /* device time of the call */
+    double rt_kernel_avg_ms; /* average duration of one inward-integration launch */
+    long long n_cross;       /* ray-cell crossings integrated per pass */
+    long long n_paths;       /* (node, ray) paths */
+    int iterations;
+    int fallbacks;           /* nodes where the surrogate iterate was rejected */
+    int lu_failures;
+    double max_rel_change;   /* last iteration's population change */
+} portal_stats;
+
+/* Context -------------------------------------------------------------------- */
+int portal_ctx_create(portal_ctx **ctx, int device);
+int portal_ctx_destroy(portal_ctx *ctx);
+const char *portal_last_error(const portal_ctx *ctx);
+
+/* Grid: the arrays eat_lime_no_delaunay (eat_lime.f90:71-313) hands to the node
+ * loop, after main_para.f90:119-135.  XYZ/vel_all 3 x N, temp_all 2 x N
+ * (T_kin, T_dust), del_ne N x maxne (1-based, sorted, zero padded). */
+int portal_upload_grid(portal_ctx *ctx, int node_num, int bulk_node_num, const double *XYZ,
+                       const double *vel_all, const double *temp_all, const double *b_all,
+                       const double *h2_dens, const double *ab_all, const double *dust_p,
+                       const int *del_ne, const int *del_num_ne, int maxne, double tx);
+
+/* Species: LAMDA tables after main_para.f90:119-120 and 178-185; the transition
+ * list and state index of main_para.f90:190-198 are built here.  r_trans/c_trans
+ * are 2 x R / 2 x Cn (lower, upper); t_Cji is c_temp x c_num x c_par. */
+int portal_upload_species(portal_ctx *ctx, int e_num, const int *j_lev, const double *e_levs_hz,
+                          int r_num, const int *r_trans, const double *t_Aji, const double *nu_hz,
+                          const double *t_Bij, const double *kd_v, int c_num, const int *c_trans,
+                          int c_temp, int c_par, const double *t_Cji, const double *col_temps,
+                          int kmax);
+int portal_species_info(const portal_ctx *ctx, int *n_trans, int *c_dim);
+
+/* Level densities lev_p (e_num x node_num), the populations the inward rays see
+ * (inverse_rt.f90:121-124). */
+int portal_set_levpop(portal_ctx *ctx, const double *lev_p);
+int portal_get_levpop(portal_ctx *ctx, double *lev_p);
+
+/* SE pass: pol_pops node_num x 2L (main_para.f90:202, written 271-275);
+ * pol_rates node_num x 3 x 2L and JJ_20 node_num x R when extra_info (may be NULL). */
+int portal_se_pass(portal_ctx *ctx, const portal_se_params *p, double *pol_pops, double *pol_rates,
+                   double *JJ_20, portal_stats *stats);
+/* J0, J2: r_num x node_num (zero for nodes that are not solved). */
+int portal_inverse_rt(portal_ctx *ctx, const portal_se_params *p, double *J0, double *J2,
+                      portal_stats *stats);
+int portal_se_solve(portal_ctx *ctx, const double *J0, const double *J2, double *pol_pops,
+                    double *pol_rates, portal_stats *stats);
+
+/* Lambda iteration (no reference counterpart).  lev_p_out (e_num x node_num, may be
+ * NULL) receives the final level densities. */
+int portal_lambda_iterate(portal_ctx *ctx, const portal_se_params *p, const portal_iter_params *it,
+                          double *pol_pops, double *lev_p_out, portal_stats *stats);
+
+/* Image pass for one inclination.  pol_pops is the rescaled table of
+ * para_finish.f90:222-240 (node_num x 2L).  Outputs are n_rt x nlevels x (2nvel+1)
+ * (all_I(:,:,inc,:) of para_finish.f90:356-359); dust (4 such cubes) may be NULL. */
+int portal_image_pass(portal_ctx *ctx, const portal_img_params *p, const double *pol_pops,
+                      const double *bvec_all, double *I, double *Q, double *U, double *TAU,
+                      double *dust, portal_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -136,6 +136,19 @@ class Oracle:
             return pp, J0, J2, rates, jj
         return pp, J0, J2
 
+    def inverse_nodes(self, g, sp, xth, nvel, dx, nodes, n_depth=4096, nthreads=0, lev_p=None):
+        """inverse_rt for the given 1-based nodes -> J0, J2 (R x len(nodes))."""
+        G = self.grid(g, lev_p)
+        S = self.species(sp)
+        P = self.params(xth, nvel, dx, g.gvec, g.bvec_all, n_depth, 0)
+        nd = np.ascontiguousarray(nodes, dtype=np.int32)
+        J0 = np.zeros((sp.r_num, len(nd)), order="F")
+        J2 = np.zeros((sp.r_num, len(nd)), order="F")
+        rc = self.lib.orc_inverse_nodes(C.byref(G), C.byref(S), C.byref(P), _i(nd), len(nd), _d(J0), _d(J2), nthreads)
+        if rc:
+            raise RuntimeError(f"oracle inverse_nodes failed ({rc})")
+        return J0, J2
+
     def solve_from_J(self, g, sp, J0, J2, node_lo=1, node_hi=None):
         if node_hi is None:
             node_hi = g.bulk_node_num

@@ -829,6 +829,23 @@ int orc_se_pass(const orc_grid *g, const orc_species *s, const orc_se_params *p,
     return err;
 }
 
+/* inverse_rt for a list of nodes (1-based); J0/J2 are R x n_nodes. */
+int orc_inverse_nodes(const orc_grid *g, const orc_species *s, const orc_se_params *p, const int *nodes,
+                      int n_nodes, double *J0, double *J2, int nthreads) {
+    const int R = s->r_num;
+    int err = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : err)
+    for (int i = 0; i < n_nodes; i++) {
+        const double *bvec = p->bvec_all + 3 * (size_t)(nodes[i] - 1);
+        int rc = orc_inverse_rt(nodes[i], g, s, p, bvec, J0 + (size_t)R * i, J2 + (size_t)R * i);
+        if (rc) err = rc < err ? rc : err;
+    }
+    return err;
+}
+
 /* SE only: make_AC + get_rhovec + packing from given J0/J2 (R x N). */
 int orc_solve_from_J(const orc_grid *g, const orc_species *s, int node_lo, int node_hi, const double *J0_all,
                      const double *J2_all, double *pol_pops) {

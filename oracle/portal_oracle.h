@@ -94,6 +94,8 @@ int  orc_trace_node(int node, const orc_grid *g, const orc_species *s, const orc
 int  orc_se_pass(const orc_grid *g, const orc_species *s, const orc_se_params *p, int node_lo,
                  int node_hi, double *pol_pops, double *pol_rates, double *JJ_20,
                  double *J0_all, double *J2_all, int nthreads);
+int  orc_inverse_nodes(const orc_grid *g, const orc_species *s, const orc_se_params *p, const int *nodes,
+                       int n_nodes, double *J0, double *J2, int nthreads);
 int  orc_solve_from_J(const orc_grid *g, const orc_species *s, int node_lo, int node_hi,
                       const double *J0_all, const double *J2_all, double *pol_pops);
 

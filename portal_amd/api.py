@@ -1,0 +1,169 @@
+"""Python mirror of the drop-in boundary: one object per device context.
+
+Method names follow the reference's loops they replace (see include/portal_hip.h):
+``se_pass`` is the node loop of main_para.f90:210-257 (trace_node per node),
+``inverse_rt`` the radiation-field part of it, ``se_solve`` the SE part,
+``image_pass`` one inclination of para_finish.f90:382-449.  Arrays are numpy,
+column-major as in the reference; errors raise ``PortalError`` carrying the
+library's status code (the reference would ``stop``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib as L
+
+
+class PortalError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{L.STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _i32(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.int32))
+
+
+def _d(a):
+    return a.ctypes.data_as(L.dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(L.ip)
+
+
+class Portal:
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        self.ctx = C.c_void_p()
+        rc = self.lib.portal_ctx_create(C.byref(self.ctx), device)
+        if rc:
+            raise PortalError(rc, "no usable GPU (portal_ctx_create)")
+        self._keep = []
+        self.grid = None
+        self.species = None
+
+    def close(self):
+        if self.ctx:
+            self.lib.portal_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise PortalError(rc, self.lib.portal_last_error(self.ctx).decode())
+
+    # -- inputs ------------------------------------------------------------------
+    def upload_grid(self, g):
+        del_ne = g.del_ne()
+        self._check(self.lib.portal_upload_grid(
+            self.ctx, g.node_num, g.bulk_node_num, _d(_f64(g.XYZ)), _d(_f64(g.vel_all)), _d(_f64(g.temp_all)),
+            _d(_f64(g.b_all)), _d(_f64(g.h2_dens)), _d(_f64(g.ab_all)), _d(_f64(g.dust_p)), _i(del_ne),
+            _i(_i32(g.del_num_ne)), del_ne.shape[1], C.c_double(g.tx)))
+        self.grid = g
+
+    def upload_species(self, sp):
+        self._check(self.lib.portal_upload_species(
+            self.ctx, sp.e_num, _i(_i32(sp.j_lev)), _d(_f64(sp.e_levs_hz)), sp.r_num, _i(_i32(sp.r_trans)),
+            _d(_f64(sp.t_Aji)), _d(_f64(sp.nu_hz)), _d(_f64(sp.t_Bij)), _d(_f64(sp.kd_v)), sp.c_num,
+            _i(_i32(sp.c_trans)), sp.c_temp, sp.c_par, _d(_f64(sp.t_Cji)), _d(_f64(sp.col_temps)), sp.kmax))
+        self.species = sp
+
+    def set_levpop(self, lev_p):
+        a = _f64(lev_p)
+        self._check(self.lib.portal_set_levpop(self.ctx, _d(a)))
+
+    def get_levpop(self):
+        out = np.zeros((self.species.e_num, self.grid.node_num), order="F")
+        self._check(self.lib.portal_get_levpop(self.ctx, _d(out)))
+        return out
+
+    def species_info(self):
+        nt, cd = C.c_int(), C.c_int()
+        self._check(self.lib.portal_species_info(self.ctx, C.byref(nt), C.byref(cd)))
+        return nt.value, cd.value
+
+    def _se_params(self, xth, nvel, dx, n_depth, extra_info=0, precision=L.PORTAL_FP64, gvec=None, bvec_all=None):
+        g = self.grid
+        arrs = (_f64(xth), _f64(g.gvec if gvec is None else gvec), _f64(g.bvec_all if bvec_all is None else bvec_all))
+        self._keep.append(arrs)
+        return L.SeParams(arrs[0].shape[1], _d(arrs[0]), nvel, dx, _d(arrs[1]), _d(arrs[2]), n_depth, extra_info,
+                          precision, 0)
+
+    # -- passes ------------------------------------------------------------------
+    def se_pass(self, xth, nvel, dx, n_depth=4096, extra_info=0, precision=L.PORTAL_FP64, gvec=None, bvec_all=None):
+        """trace_node for every bulk node.  Returns pol_pops (N x 2L)[, pol_rates, JJ_20], stats."""
+        g, sp = self.grid, self.species
+        p = self._se_params(xth, nvel, dx, n_depth, extra_info, precision, gvec, bvec_all)
+        pp = np.zeros((g.node_num, 2 * sp.e_num), order="F")
+        rates = np.zeros((g.node_num, 3, 2 * sp.e_num), order="F")
+        jj = np.zeros((g.node_num, sp.r_num), order="F")
+        st = L.Stats()
+        self._check(self.lib.portal_se_pass(self.ctx, C.byref(p), _d(pp), _d(rates), _d(jj), C.byref(st)))
+        if extra_info:
+            return pp, rates, jj, st.as_dict()
+        return pp, st.as_dict()
+
+    def inverse_rt(self, xth, nvel, dx, n_depth=4096, precision=L.PORTAL_FP64, gvec=None, bvec_all=None):
+        g, sp = self.grid, self.species
+        p = self._se_params(xth, nvel, dx, n_depth, 0, precision, gvec, bvec_all)
+        J0 = np.zeros((sp.r_num, g.node_num), order="F")
+        J2 = np.zeros((sp.r_num, g.node_num), order="F")
+        st = L.Stats()
+        self._check(self.lib.portal_inverse_rt(self.ctx, C.byref(p), _d(J0), _d(J2), C.byref(st)))
+        return J0, J2, st.as_dict()
+
+    def se_solve(self, J0, J2, rates=False):
+        g, sp = self.grid, self.species
+        pp = np.zeros((g.node_num, 2 * sp.e_num), order="F")
+        rr = np.zeros((g.node_num, 3, 2 * sp.e_num), order="F")
+        st = L.Stats()
+        a0, a2 = _f64(J0), _f64(J2)
+        self._check(self.lib.portal_se_solve(self.ctx, _d(a0), _d(a2), _d(pp), _d(rr) if rates else None,
+                                             C.byref(st)))
+        return (pp, rr) if rates else pp
+
+    def lambda_iterate(self, xth, nvel, dx, max_iter, tol_pop=1e-6, tol_align=1e-8, n_depth=4096,
+                       precision=L.PORTAL_FP64, surrogate=0, fallback_tol=1e-8):
+        g, sp = self.grid, self.species
+        p = self._se_params(xth, nvel, dx, n_depth, 0, precision)
+        it = L.IterParams(max_iter, tol_pop, tol_align, surrogate, fallback_tol)
+        pp = np.zeros((g.node_num, 2 * sp.e_num), order="F")
+        lev = np.zeros((sp.e_num, g.node_num), order="F")
+        st = L.Stats()
+        self._check(self.lib.portal_lambda_iterate(self.ctx, C.byref(p), C.byref(it), _d(pp), _d(lev), C.byref(st)))
+        return pp, lev, st.as_dict()
+
+    def image_pass(self, pol_pops, lines, th_inc, ph_az, nvel, vel_out, im_pix, im_size, dx, alpha=0.15,
+                   n_depth_rt=4096, precision=L.PORTAL_FP64, bvec_all=None, with_dust=False):
+        """One inclination.  lines = [(l1, l2, nu0, Aji, kd)].  Returns I, Q, U, TAU (n_rt x nlines x C)."""
+        g = self.grid
+        nl = len(lines)
+        arrs = (_i32([x[0] for x in lines]), _i32([x[1] for x in lines]), _f64([x[2] for x in lines]),
+                _f64([x[3] for x in lines]), _f64([x[4] for x in lines]), _f64(vel_out))
+        self._keep.append(arrs)
+        p = L.ImgParams(nl, _i(arrs[0]), _i(arrs[1]), _d(arrs[2]), _d(arrs[3]), _d(arrs[4]), th_inc, ph_az, nvel,
+                        _d(arrs[5]), im_pix, im_size, dx, alpha, n_depth_rt, precision)
+        n_rt = (2 * im_pix + 1) ** 2
+        Cn = 2 * nvel + 1
+        outs = [np.zeros((n_rt, nl, Cn), order="F") for _ in range(4)]
+        dust = np.zeros((4, n_rt, nl, Cn)) if with_dust else None
+        bv = _f64(g.bvec_all if bvec_all is None else bvec_all)
+        pp = _f64(pol_pops)
+        st = L.Stats()
+        self._check(self.lib.portal_image_pass(self.ctx, C.byref(p), _d(pp), _d(bv), *[_d(o) for o in outs],
+                                               _d(dust) if with_dust else None, C.byref(st)))
+        if with_dust:
+            return outs, dust, st.as_dict()
+        return outs, st.as_dict()

@@ -1,0 +1,209 @@
+// Internal declarations shared by the host API and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "portal_hip.h"
+
+namespace portal {
+
+// One stored crossing of an inward ray: the cell, its nint(delta/dx) length
+// (inverse_rt.f90:196) and the cell velocity projected on the ray (inverse_rt.f90:103).
+struct __attribute__((aligned(16))) PathEntry {
+    int cell;     // 0-based node id
+    int len;      // length in units of dx
+    double vray;  // v(cell) . dir  [m/s]
+};
+
+// Geometry-invariant inward paths of every (work node, ray).
+struct PathSet {
+    PathEntry *entries = nullptr;   // device
+    long long *offsets = nullptr;   // device, n_work*D + 1
+    int *counts = nullptr;          // device, n_work*D
+    long long total = 0;
+    int n_work = 0, D = 0;
+    size_t cap_entries = 0;
+    // key of the geometry the paths were built for
+    std::vector<double> xth, gvec, bvec;
+    double dx = 0;
+    int n_depth = 0;
+    bool valid = false;
+};
+
+// Statistical-equilibrium coefficient tables of one species (host-built, FP64).
+struct SeTables {
+    int c_dim = 0, n_trans = 0;
+    // per matrix entry (CSR over the c_dim*c_dim entries that have terms)
+    std::vector<int> ent_idx;       // entry = row + c_dim*col
+    std::vector<int> ent_ptr;       // n_ent+1
+    std::vector<int> term_q;        // scalar index 7*t + kind
+    std::vector<double> term_coef;
+    std::vector<int> term_class;    // 0 spontaneous, 1 radiative, 2 collisional
+    int *d_ent_idx = nullptr, *d_ent_ptr = nullptr, *d_term_q = nullptr, *d_term_class = nullptr;
+    double *d_term_coef = nullptr;
+};
+
+// Species after the driver's conversions, plus derived lists.
+struct Species {
+    int e_num = 0, r_num = 0, c_num = 0, c_temp = 0, c_par = 0, kmax = 0;
+    std::vector<int> j_lev, r_trans, c_trans, all_trans, count_array;
+    std::vector<double> e_levs_hz, t_Aji, nu_hz, t_Bij, kd_v, t_Cji, col_temps;
+    int n_trans = 0, c_dim = 0;
+    SeTables se;
+    // per unique transition: radiative line index (-1 if none), collisional index (-1 if none)
+    std::vector<int> trans_rad, trans_col;
+    // device copies
+    int *d_l1 = nullptr, *d_l2 = nullptr;                 // per line, 0-based levels
+    double *d_c1B = nullptr, *d_c2A = nullptr, *d_g1 = nullptr, *d_g2 = nullptr;
+    double *d_cnu_spi = nullptr, *d_Iin = nullptr;
+    double *d_tA = nullptr, *d_tB21 = nullptr, *d_tnu = nullptr, *d_tgul = nullptr;
+    int *d_trad = nullptr, *d_tcol = nullptr;
+    double *d_Cji = nullptr, *d_coltemps = nullptr;
+    int *d_count = nullptr, *d_jlev = nullptr;
+    bool valid = false;
+};
+
+struct Grid {
+    int N = 0, N_b = 0;
+    double tx = 0;
+    std::vector<double> XYZ, vel, temp, b_all, h2, ab, dust;
+    std::vector<int> nbr_ptr, nbr_idx;  // CSR, 0-based ids
+    bool has_dust = false;
+    // device
+    double4 *d_pos = nullptr;    // x, y, z, 0
+    double4 *d_velb = nullptr;   // vx, vy, vz, b
+    double *d_inv_b = nullptr;   // 1/b
+    double *d_tdust = nullptr;   // T_dust
+    double *d_tkin = nullptr;
+    double *d_h2 = nullptr;
+    double *d_dust = nullptr;
+    double *d_ab = nullptr;
+    int *d_nbr_ptr = nullptr, *d_nbr_idx = nullptr;
+    // work list: bulk nodes with ab_all > 0 (main_para.f90:211-213), Morton-sorted
+    std::vector<int> work;
+    int *d_work = nullptr;
+    bool valid = false;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    Grid g;
+    Species sp;
+    PathSet paths;
+    // device lev_p (L x N) and per-iteration buffers
+    double *d_levp = nullptr;
+    double *d_KS = nullptr;     // N x R x {K, S}  (FP64)
+    float *d_KS32 = nullptr;    // N x R x {K, S}  (FP32, mixed path)
+    double *d_dustco = nullptr; // N x R x {Ka, j} when dust is present
+    double *d_J0 = nullptr, *d_J2 = nullptr;  // R x N
+    double *d_pol = nullptr;    // N x 2L
+    double *d_rates = nullptr;  // N x 3 x 2L
+    int *d_flags = nullptr;     // error flags
+    size_t cap_nodes = 0;
+    // scratch of the image pass
+    void *d_img = nullptr;
+    size_t img_bytes = 0;
+    // MLP surrogate state (owned by mlp.hip)
+    void *mlp = nullptr;
+};
+
+// error flags (bits in d_flags[0])
+enum : int { FLAG_DEPTH = 1, FLAG_LU = 2, FLAG_TEMP = 4, FLAG_DEPTH_RT = 8 };
+
+#define HIP_TRY(ctx, expr)                                                          \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);         \
+            return PORTAL_ERR_DEVICE;                                               \
+        }                                                                           \
+    } while (0)
+
+// ---- launchers implemented in the kernel files ------------------------------------
+struct WalkArgs {
+    int n_work, D, N;
+    const int *work;
+    const double4 *pos;
+    const int *nbr_ptr, *nbr_idx;
+    const double *xth;    // 3 x D
+    const double *bvec;   // 3 x N
+    const double *gvec;   // 3
+    const double4 *velb;
+    double dx;
+    int n_depth;
+    int *counts;
+    const long long *offsets;
+    PathEntry *paths;
+    int *flags;
+};
+hipError_t launch_walk(const WalkArgs &a, bool write, hipStream_t s);
+hipError_t launch_scan_counts(const int *counts, long long *offsets, long long n, hipStream_t s,
+                              long long *total_host);
+
+struct PrepArgs {
+    int N, R, L;
+    const double *levp;      // L x N
+    const double4 *velb;
+    const int *l1, *l2;      // 0-based level of each line (device)
+    const double *c1B;       // per line: 2.2102191e-42*nu * B_lu
+    const double *c2A;       // per line: 5.272859e-35*nu * A
+    const double *gratio;    // per line: g_l / g_u   computed as (1.0*g1)/(1.0*g2) split below
+    const double *g1, *g2;
+    const double *cnu_spi;   // per line: c/nu / sqrt(pi)
+    double *KS;
+    float *KS32;
+};
+hipError_t launch_line_prep(const PrepArgs &a, hipStream_t s);
+
+struct RtArgs {
+    int n_work, D, R, nvel, C;
+    const int *work;
+    const double4 *velb;
+    const double *inv_b;
+    const double *KS;       // N x R x 2
+    const float *KS32;
+    const double *dustco;   // N x R x 2 or null
+    const double *xth;      // 3 x D
+    const long long *offsets;
+    const PathEntry *paths;
+    const double *I_in;     // per line
+    double dl_scale;        // tx*dx
+    double *J0, *J2;        // R x N
+    int waves;              // waves per block
+};
+hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s);
+hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s);
+
+struct SeArgs {
+    int n_work, N, L, R, c_dim, n_trans, n_ent;
+    const int *work;
+    const double *J0, *J2;      // R x N
+    const double *tkin, *h2;    // N
+    // per transition
+    const double *t_A, *t_B21, *t_nu, *t_gul;   // A, B21 (a_to_b*A/nu^3), nu (levels), g_u/g_l
+    const int *t_rad, *t_col;                   // radiative line / collisional index or -1
+    const double *t_Cji;                        // c_temp x c_num x c_par
+    const double *col_temps;
+    int c_temp, c_par, c_num;
+    const int *ent_idx, *ent_ptr, *term_q, *term_class;
+    const double *term_coef;
+    const int *count_array, *j_lev;             // L
+    double *pol;                                // N x 2L
+    double *rates;                              // N x 3 x 2L or null
+    int *flags;
+    int extra_info;
+    int waves;
+};
+hipError_t launch_se(const SeArgs &a, hipStream_t s);
+
+hipError_t launch_lambda_update(int n_work, int N, int L, const int *work, const double *ab, const double *h2,
+                                const int *j_lev, const double *pol, const double *pol_prev, double *levp,
+                                unsigned long long *maxes, hipStream_t s);
+
+}  // namespace portal

@@ -1,0 +1,894 @@
+// Host side of the C ABI: context, uploads, pass orchestration.
+//
+// The context keeps the grid, species tables, the inward paths and all per-pass
+// buffers resident on the device; a call moves host inputs in once and the
+// results out once.  Each pass is a short fixed sequence of kernel launches on
+// the context's stream (line coefficients -> inward integration -> SE solve),
+// timed with HIP events on that stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "common.h"
+#include "image.h"
+
+extern "C" double portal_debug_wig6jj(int a, int b, int c, int d, int e, int f);
+
+namespace portal {
+int build_species_tables(Species &s);
+}
+
+using namespace portal;
+
+struct portal_ctx {
+    Ctx c;
+};
+
+namespace {
+
+template <typename T>
+hipError_t upload(T *&d, const std::vector<T> &h) {
+    if (d) hipFree(d);
+    d = nullptr;
+    if (h.empty()) return hipSuccess;
+    hipError_t e = hipMalloc((void **)&d, sizeof(T) * h.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) hipFree((void *)p);
+    p = nullptr;
+}
+
+int fail(Ctx &c, int code, const std::string &msg) {
+    c.err = msg;
+    return code;
+}
+
+uint64_t spread3(uint32_t v) {
+    uint64_t x = v & 0x1fffff;
+    x = (x | x << 32) & 0x1f00000000ffffULL;
+    x = (x | x << 16) & 0x1f0000ff0000ffULL;
+    x = (x | x << 8) & 0x100f00f00f00f00fULL;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+    x = (x | x << 2) & 0x1249249249249249ULL;
+    return x;
+}
+
+double black_body(double T, double nu0) {  // miscellaneous.f90:263-285
+    if (T == 0.0) return 0.0;
+    const double c1 = 1.4744995e-50, c2 = 4.79924e-11 / T;
+    return c1 * std::pow(nu0, 3.0) * std::pow(std::exp(c2 * nu0) - 1.0, -1.0);
+}
+
+struct Timer {
+    hipEvent_t a, b;
+    hipStream_t s;
+    explicit Timer(hipStream_t st) : s(st) {
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, s);
+    }
+    double stop() {
+        hipEventRecord(b, s);
+        hipEventSynchronize(b);
+        float ms = 0;
+        hipEventElapsedTime(&ms, a, b);
+        return ms;
+    }
+    ~Timer() {
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+    }
+};
+
+int ensure_node_buffers(Ctx &c) {
+    const size_t N = c.g.N, R = c.sp.r_num, L = c.sp.e_num;
+    if (!c.g.valid || !c.sp.valid) return fail(c, PORTAL_ERR_STATE, "grid and species must be uploaded first");
+    const size_t need = N * std::max<size_t>(R, 1) * 1000 + L;
+    if (c.cap_nodes == need) return PORTAL_OK;
+    dfree(c.d_KS);
+    dfree(c.d_KS32);
+    dfree(c.d_J0);
+    dfree(c.d_J2);
+    dfree(c.d_pol);
+    dfree(c.d_rates);
+    HIP_TRY(&c, hipMalloc((void **)&c.d_KS, sizeof(double) * 2 * N * R));
+    HIP_TRY(&c, hipMalloc((void **)&c.d_KS32, sizeof(float) * 2 * N * R));
+    HIP_TRY(&c, hipMalloc((void **)&c.d_J0, sizeof(double) * N * R));
+    HIP_TRY(&c, hipMalloc((void **)&c.d_J2, sizeof(double) * N * R));
+    HIP_TRY(&c, hipMalloc((void **)&c.d_pol, sizeof(double) * N * 2 * L));
+    HIP_TRY(&c, hipMalloc((void **)&c.d_rates, sizeof(double) * N * 6 * L));
+    if (!c.d_flags) HIP_TRY(&c, hipMalloc((void **)&c.d_flags, sizeof(int) * 4));
+    if (!c.d_levp) {
+        HIP_TRY(&c, hipMalloc((void **)&c.d_levp, sizeof(double) * N * L));
+        HIP_TRY(&c, hipMemset(c.d_levp, 0, sizeof(double) * N * L));
+    }
+    c.cap_nodes = need;
+    return PORTAL_OK;
+}
+
+bool same(const std::vector<double> &v, const double *p, size_t n) {
+    return v.size() == n && std::memcmp(v.data(), p, n * sizeof(double)) == 0;
+}
+
+// Build (or reuse) the inward paths for this direction set and field geometry.
+int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
+    PathSet &P = c.paths;
+    const int N = c.g.N, D = p->n_sphere;
+    if (P.valid && P.D == D && P.dx == p->dx && P.n_depth == p->n_depth && same(P.xth, p->xth, 3 * (size_t)D) &&
+        same(P.gvec, p->gvec, 3) && same(P.bvec, p->bvec_all, 3 * (size_t)N))
+        return PORTAL_OK;
+    P.valid = false;
+    const int nw = (int)c.g.work.size();
+    double *d_xth = nullptr, *d_bvec = nullptr, *d_gvec = nullptr;
+    P.xth.assign(p->xth, p->xth + 3 * (size_t)D);
+    P.gvec.assign(p->gvec, p->gvec + 3);
+    P.bvec.assign(p->bvec_all, p->bvec_all + 3 * (size_t)N);
+    HIP_TRY(&c, upload(d_xth, P.xth));
+    HIP_TRY(&c, upload(d_gvec, P.gvec));
+    HIP_TRY(&c, upload(d_bvec, P.bvec));
+    dfree(P.counts);
+    dfree(P.offsets);
+    const long long n = (long long)nw * D;
+    HIP_TRY(&c, hipMalloc((void **)&P.counts, sizeof(int) * std::max<long long>(n, 1)));
+    HIP_TRY(&c, hipMalloc((void **)&P.offsets, sizeof(long long) * (n + 1)));
+    HIP_TRY(&c, hipMemsetAsync(c.d_flags, 0, sizeof(int) * 4, c.stream));
+    Timer tm(c.stream);
+    WalkArgs a{};
+    a.n_work = nw;
+    a.D = D;
+    a.N = N;
+    a.work = c.g.d_work;
+    a.pos = c.g.d_pos;
+    a.nbr_ptr = c.g.d_nbr_ptr;
+    a.nbr_idx = c.g.d_nbr_idx;
+    a.xth = d_xth;
+    a.bvec = d_bvec;
+    a.gvec = d_gvec;
+    a.velb = c.g.d_velb;
+    a.dx = p->dx;
+    a.n_depth = p->n_depth;
+    a.counts = P.counts;
+    a.flags = c.d_flags;
+    HIP_TRY(&c, launch_walk(a, false, c.stream));
+    int flags = 0;
+    HIP_TRY(&c, hipMemcpyAsync(&flags, c.d_flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(&c, hipStreamSynchronize(c.stream));
+    if (flags & FLAG_DEPTH) {
+        dfree(d_xth);
+        dfree(d_gvec);
+        dfree(d_bvec);
+        return fail(c, PORTAL_ERR_DEPTH, "an inward ray crosses more than n_depth cells (inverse_rt.f90:206-210)");
+    }
+    long long total = 0;
+    if (n > 0) HIP_TRY(&c, launch_scan_counts(P.counts, P.offsets, n, c.stream, &total));
+    if ((size_t)total > P.cap_entries) {
+        dfree(P.entries);
+        HIP_TRY(&c, hipMalloc((void **)&P.entries, sizeof(PathEntry) * std::max<long long>(total, 1)));
+        P.cap_entries = (size_t)total;
+    }
+    a.offsets = P.offsets;
+    a.paths = P.entries;
+    HIP_TRY(&c, launch_walk(a, true, c.stream));
+    const double ms = tm.stop();
+    dfree(d_xth);
+    dfree(d_gvec);
+    dfree(d_bvec);
+    P.total = total;
+    P.n_work = nw;
+    P.D = D;
+    P.dx = p->dx;
+    P.n_depth = p->n_depth;
+    P.valid = true;
+    if (st) st->t_walk_ms += ms;
+    return PORTAL_OK;
+}
+
+int choose_waves(const Ctx &c) {
+    // enough waves in flight: ~16 per CU over 256 CUs
+    const long long nw = (long long)c.g.work.size();
+    if (nw >= 4096 * 4) return 4;
+    if (nw >= 4096 * 2) return 4;
+    return 4;
+}
+
+int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
+    int rc = ensure_paths(c, p, st);
+    if (rc) return rc;
+    const Species &s = c.sp;
+    Timer tp(c.stream);
+    PrepArgs pa{};
+    pa.N = c.g.N;
+    pa.R = s.r_num;
+    pa.L = s.e_num;
+    pa.levp = c.d_levp;
+    pa.velb = c.g.d_velb;
+    pa.l1 = s.d_l1;
+    pa.l2 = s.d_l2;
+    pa.c1B = s.d_c1B;
+    pa.c2A = s.d_c2A;
+    pa.g1 = s.d_g1;
+    pa.g2 = s.d_g2;
+    pa.cnu_spi = s.d_cnu_spi;
+    const bool mixed = p->precision == PORTAL_MIXED;
+    pa.KS = mixed ? nullptr : c.d_KS;
+    pa.KS32 = mixed ? c.d_KS32 : nullptr;
+    HIP_TRY(&c, launch_line_prep(pa, c.stream));
+    const double t_prep = tp.stop();
+    HIP_TRY(&c, hipMemsetAsync(c.d_J0, 0, sizeof(double) * c.g.N * s.r_num, c.stream));
+    HIP_TRY(&c, hipMemsetAsync(c.d_J2, 0, sizeof(double) * c.g.N * s.r_num, c.stream));
+    RtArgs ra{};
+    ra.n_work = (int)c.g.work.size();
+    ra.D = p->n_sphere;
+    ra.R = s.r_num;
+    ra.nvel = p->nvel;
+    ra.C = 2 * p->nvel + 1;
+    ra.work = c.g.d_work;
+    ra.velb = c.g.d_velb;
+    ra.inv_b = c.g.d_inv_b;
+    ra.KS = c.d_KS;
+    ra.KS32 = c.d_KS32;
+    ra.xth = nullptr;
+    ra.offsets = c.paths.offsets;
+    ra.paths = c.paths.entries;
+    ra.I_in = s.d_Iin;
+    ra.dl_scale = c.g.tx * p->dx;
+    ra.J0 = c.d_J0;
+    ra.J2 = c.d_J2;
+    ra.waves = choose_waves(c);
+    double *d_xth = nullptr;
+    std::vector<double> xv(p->xth, p->xth + 3 * (size_t)p->n_sphere);
+    HIP_TRY(&c, upload(d_xth, xv));
+    ra.xth = d_xth;
+    Timer tr(c.stream);
+    hipError_t e = mixed ? launch_rt_mixed(ra, c.stream) : launch_rt_f64(ra, c.stream);
+    const double t_rt = tr.stop();
+    dfree(d_xth);
+    HIP_TRY(&c, e);
+    if (st) {
+        st->t_prep_ms += t_prep;
+        st->t_rt_ms += t_rt;
+        st->rt_kernel_avg_ms = t_rt;
+        st->n_cross = c.paths.total;
+        st->n_paths = (long long)c.paths.n_work * c.paths.D;
+    }
+    return PORTAL_OK;
+}
+
+int run_se(Ctx &c, int extra_info, portal_stats *st) {
+    const Species &s = c.sp;
+    HIP_TRY(&c, hipMemsetAsync(c.d_flags, 0, sizeof(int) * 4, c.stream));
+    SeArgs a{};
+    a.n_work = (int)c.g.work.size();
+    a.N = c.g.N;
+    a.L = s.e_num;
+    a.R = s.r_num;
+    a.c_dim = s.c_dim;
+    a.n_trans = s.n_trans;
+    a.n_ent = (int)s.se.ent_idx.size();
+    a.work = c.g.d_work;
+    a.J0 = c.d_J0;
+    a.J2 = c.d_J2;
+    a.tkin = c.g.d_tkin;
+    a.h2 = c.g.d_h2;
+    a.t_A = s.d_tA;
+    a.t_B21 = s.d_tB21;
+    a.t_nu = s.d_tnu;
+    a.t_gul = s.d_tgul;
+    a.t_rad = s.d_trad;
+    a.t_col = s.d_tcol;
+    a.t_Cji = s.d_Cji;
+    a.col_temps = s.d_coltemps;
+    a.c_temp = s.c_temp;
+    a.c_par = s.c_par;
+    a.c_num = s.c_num;
+    a.ent_idx = s.se.d_ent_idx;
+    a.ent_ptr = s.se.d_ent_ptr;
+    a.term_q = s.se.d_term_q;
+    a.term_class = s.se.d_term_class;
+    a.term_coef = s.se.d_term_coef;
+    a.count_array = s.d_count;
+    a.j_lev = s.d_jlev;
+    a.pol = c.d_pol;
+    a.rates = extra_info ? c.d_rates : nullptr;
+    a.flags = c.d_flags;
+    a.extra_info = extra_info;
+    HIP_TRY(&c, hipMemsetAsync(c.d_pol, 0, sizeof(double) * c.g.N * 2 * s.e_num, c.stream));
+    if (extra_info) HIP_TRY(&c, hipMemsetAsync(c.d_rates, 0, sizeof(double) * c.g.N * 6 * s.e_num, c.stream));
+    Timer t(c.stream);
+    HIP_TRY(&c, launch_se(a, c.stream));
+    const double ms = t.stop();
+    int flags = 0;
+    HIP_TRY(&c, hipMemcpy(&flags, c.d_flags, sizeof(int), hipMemcpyDeviceToHost));
+    if (st) st->t_se_ms += ms;
+    if (flags & FLAG_TEMP)
+        return fail(c, PORTAL_ERR_TEMP, "kinetic temperature above the collision-rate table (miscellaneous.f90:331-360)");
+    if (flags & FLAG_LU) return fail(c, PORTAL_ERR_LU, "Density-matrix calculation: failed (singular SE matrix)");
+    return PORTAL_OK;
+}
+
+int check_se_params(Ctx &c, const portal_se_params *p) {
+    if (!p || !p->xth || !p->gvec || !p->bvec_all) return fail(c, PORTAL_ERR_ARG, "null parameter");
+    if (p->star_mod != 0) return fail(c, PORTAL_ERR_UNSUPPORTED, "star mode (star_mod=1) is not supported");
+    if (p->n_sphere <= 0 || p->nvel <= 0 || p->n_depth <= 0 || !(p->dx > 0))
+        return fail(c, PORTAL_ERR_ARG, "n_sphere, nvel, n_depth and dx must be positive");
+    if (p->precision != PORTAL_FP64 && p->precision != PORTAL_MIXED) return fail(c, PORTAL_ERR_ARG, "bad precision");
+    if (c.g.has_dust) return fail(c, PORTAL_ERR_UNSUPPORTED, "non-zero dust_p is not supported by the inward pass");
+    return ensure_node_buffers(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int portal_ctx_create(portal_ctx **out, int device) {
+    if (!out) return PORTAL_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return PORTAL_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return PORTAL_ERR_DEVICE;
+    auto *x = new portal_ctx();
+    x->c.device = device;
+    if (hipStreamCreate(&x->c.stream) != hipSuccess) {
+        delete x;
+        return PORTAL_ERR_DEVICE;
+    }
+    *out = x;
+    return PORTAL_OK;
+}
+
+void portal_mlp_free(Ctx &c) { (void)c; }
+
+int portal_ctx_destroy(portal_ctx *x) {
+    if (!x) return PORTAL_OK;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    hipStreamSynchronize(c.stream);
+    portal_mlp_free(c);
+    Grid &g = c.g;
+    dfree(g.d_pos);
+    dfree(g.d_velb);
+    dfree(g.d_inv_b);
+    dfree(g.d_tdust);
+    dfree(g.d_tkin);
+    dfree(g.d_h2);
+    dfree(g.d_dust);
+    dfree(g.d_ab);
+    dfree(g.d_nbr_ptr);
+    dfree(g.d_nbr_idx);
+    dfree(g.d_work);
+    Species &s = c.sp;
+    dfree(s.d_l1);
+    dfree(s.d_l2);
+    dfree(s.d_c1B);
+    dfree(s.d_c2A);
+    dfree(s.d_g1);
+    dfree(s.d_g2);
+    dfree(s.d_cnu_spi);
+    dfree(s.d_Iin);
+    dfree(s.d_tA);
+    dfree(s.d_tB21);
+    dfree(s.d_tnu);
+    dfree(s.d_tgul);
+    dfree(s.d_trad);
+    dfree(s.d_tcol);
+    dfree(s.d_Cji);
+    dfree(s.d_coltemps);
+    dfree(s.d_count);
+    dfree(s.d_jlev);
+    dfree(s.se.d_ent_idx);
+    dfree(s.se.d_ent_ptr);
+    dfree(s.se.d_term_q);
+    dfree(s.se.d_term_class);
+    dfree(s.se.d_term_coef);
+    dfree(c.paths.entries);
+    dfree(c.paths.offsets);
+    dfree(c.paths.counts);
+    dfree(c.d_levp);
+    dfree(c.d_KS);
+    dfree(c.d_KS32);
+    dfree(c.d_dustco);
+    dfree(c.d_J0);
+    dfree(c.d_J2);
+    dfree(c.d_pol);
+    dfree(c.d_rates);
+    dfree(c.d_flags);
+    if (c.d_img) hipFree(c.d_img);
+    hipStreamDestroy(c.stream);
+    delete x;
+    return PORTAL_OK;
+}
+
+const char *portal_last_error(const portal_ctx *x) { return x ? x->c.err.c_str() : "null context"; }
+
+int portal_upload_grid(portal_ctx *x, int N, int N_b, const double *XYZ, const double *vel, const double *temp,
+                       const double *b_all, const double *h2, const double *ab, const double *dust,
+                       const int *del_ne, const int *del_num_ne, int maxne, double tx) {
+    if (!x) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    if (N <= 0 || N_b < 0 || N_b > N || maxne <= 0 || !XYZ || !vel || !temp || !b_all || !h2 || !ab || !dust ||
+        !del_ne || !del_num_ne || !(tx > 0))
+        return fail(c, PORTAL_ERR_ARG, "bad grid arguments");
+    Grid &g = c.g;
+    g.valid = false;
+    c.paths.valid = false;
+    g.N = N;
+    g.N_b = N_b;
+    g.tx = tx;
+    g.XYZ.assign(XYZ, XYZ + 3 * (size_t)N);
+    g.vel.assign(vel, vel + 3 * (size_t)N);
+    g.temp.assign(temp, temp + 2 * (size_t)N);
+    g.b_all.assign(b_all, b_all + N);
+    g.h2.assign(h2, h2 + N);
+    g.ab.assign(ab, ab + N);
+    g.dust.assign(dust, dust + N);
+    g.has_dust = false;
+    for (int i = 0; i < N; i++)
+        if (dust[i] != 0.0) g.has_dust = true;
+    // CSR adjacency from del_ne(N, maxne) (column-major, 1-based, zero padded)
+    g.nbr_ptr.assign(N + 1, 0);
+    g.nbr_idx.clear();
+    for (int i = 0; i < N; i++) {
+        const int k = del_num_ne[i];
+        if (k < 0 || k > maxne) return fail(c, PORTAL_ERR_ARG, "del_num_ne out of range");
+        for (int j = 0; j < k; j++) {
+            const int nb = del_ne[i + (size_t)N * j];
+            if (nb < 1 || nb > N) return fail(c, PORTAL_ERR_ARG, "del_ne entry out of range");
+            g.nbr_idx.push_back(nb - 1);
+        }
+        g.nbr_ptr[i + 1] = (int)g.nbr_idx.size();
+    }
+    std::vector<double4> pos(N), velb(N);
+    std::vector<double> invb(N), tdust(N), tkin(N);
+    for (int i = 0; i < N; i++) {
+        pos[i] = make_double4(XYZ[3 * (size_t)i], XYZ[3 * (size_t)i + 1], XYZ[3 * (size_t)i + 2], 0.0);
+        velb[i] = make_double4(vel[3 * (size_t)i], vel[3 * (size_t)i + 1], vel[3 * (size_t)i + 2], b_all[i]);
+        invb[i] = 1.0 / b_all[i];
+        tkin[i] = temp[2 * (size_t)i];
+        tdust[i] = temp[2 * (size_t)i + 1];
+    }
+    // work list: nodes 1..bulk_node_num with ab_all > 0, ordered along a Morton curve
+    std::vector<std::pair<uint64_t, int>> key;
+    for (int i = 0; i < N_b; i++)
+        if (ab[i] > 0.0) {
+            auto q = [](double v) { return (uint32_t)std::min(2097151.0, std::max(0.0, (v + 1.0) * 0.5 * 2097151.0)); };
+            const uint64_t m = spread3(q(XYZ[3 * (size_t)i])) | spread3(q(XYZ[3 * (size_t)i + 1])) << 1 |
+                               spread3(q(XYZ[3 * (size_t)i + 2])) << 2;
+            key.push_back({m, i});
+        }
+    std::sort(key.begin(), key.end());
+    g.work.resize(key.size());
+    for (size_t i = 0; i < key.size(); i++) g.work[i] = key[i].second;
+    HIP_TRY(&c, upload(g.d_pos, pos));
+    HIP_TRY(&c, upload(g.d_velb, velb));
+    HIP_TRY(&c, upload(g.d_inv_b, invb));
+    HIP_TRY(&c, upload(g.d_tdust, tdust));
+    HIP_TRY(&c, upload(g.d_tkin, tkin));
+    HIP_TRY(&c, upload(g.d_h2, g.h2));
+    HIP_TRY(&c, upload(g.d_dust, g.dust));
+    HIP_TRY(&c, upload(g.d_ab, g.ab));
+    HIP_TRY(&c, upload(g.d_nbr_ptr, g.nbr_ptr));
+    HIP_TRY(&c, upload(g.d_nbr_idx, g.nbr_idx));
+    HIP_TRY(&c, upload(g.d_work, g.work));
+    dfree(c.d_levp);
+    c.cap_nodes = 0;
+    g.valid = true;
+    return PORTAL_OK;
+}
+
+int portal_upload_species(portal_ctx *x, int e_num, const int *j_lev, const double *e_levs_hz, int r_num,
+                          const int *r_trans, const double *t_Aji, const double *nu_hz, const double *t_Bij,
+                          const double *kd_v, int c_num, const int *c_trans, int c_temp, int c_par,
+                          const double *t_Cji, const double *col_temps, int kmax) {
+    if (!x) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    if (e_num <= 0 || r_num <= 0 || c_num < 0 || c_temp <= 0 || (c_par != 1 && c_par != 2) || kmax < 2 ||
+        (kmax & 1) || !j_lev || !e_levs_hz || !r_trans || !t_Aji || !nu_hz || !t_Bij || !kd_v ||
+        (c_num > 0 && (!c_trans || !t_Cji)) || !col_temps)
+        return fail(c, PORTAL_ERR_ARG, "bad species arguments (kmax must be even and >= 2)");
+    if (r_num > 64) return fail(c, PORTAL_ERR_ARG, "at most 64 radiative lines");
+    Species &s = c.sp;
+    s.valid = false;
+    c.paths.valid = false;
+    s.e_num = e_num;
+    s.r_num = r_num;
+    s.c_num = c_num;
+    s.c_temp = c_temp;
+    s.c_par = c_par;
+    s.kmax = kmax;
+    s.j_lev.assign(j_lev, j_lev + e_num);
+    s.e_levs_hz.assign(e_levs_hz, e_levs_hz + e_num);
+    s.r_trans.assign(r_trans, r_trans + 2 * (size_t)r_num);
+    s.c_trans.assign(c_trans ? c_trans : r_trans, c_trans ? c_trans + 2 * (size_t)c_num : r_trans);
+    s.t_Aji.assign(t_Aji, t_Aji + r_num);
+    s.nu_hz.assign(nu_hz, nu_hz + r_num);
+    s.t_Bij.assign(t_Bij, t_Bij + r_num);
+    s.kd_v.assign(kd_v, kd_v + r_num);
+    s.t_Cji.assign(t_Cji ? t_Cji : col_temps, t_Cji ? t_Cji + (size_t)c_temp * c_num * c_par : col_temps);
+    s.col_temps.assign(col_temps, col_temps + c_temp);
+    for (int i = 0; i < 2 * r_num; i++)
+        if (s.r_trans[i] < 1 || s.r_trans[i] > e_num) return fail(c, PORTAL_ERR_ARG, "r_trans out of range");
+    for (int i = 0; i < 2 * c_num; i++)
+        if (s.c_trans[i] < 1 || s.c_trans[i] > e_num) return fail(c, PORTAL_ERR_ARG, "c_trans out of range");
+    build_species_tables(s);
+    const int cd = s.c_dim;
+    if (2 * (size_t)cd * cd * 8 + 7 * (size_t)s.n_trans * 8 > 150 * 1024)
+        return fail(c, PORTAL_ERR_ARG, "SE system too large for one workgroup's LDS");
+    // per radiative line
+    std::vector<int> l1(r_num), l2(r_num);
+    std::vector<double> c1B(r_num), c2A(r_num), g1(r_num), g2(r_num), cnu(r_num), Iin(r_num);
+    const double spi = std::sqrt(std::acos(-1.0));
+    for (int k = 0; k < r_num; k++) {
+        l1[k] = s.r_trans[2 * k] - 1;
+        l2[k] = s.r_trans[2 * k + 1] - 1;
+        const double nu0 = s.nu_hz[k];
+        c1B[k] = 2.2102191e-42 * nu0 * s.t_Bij[k];
+        c2A[k] = 5.272859e-35 * nu0 * s.t_Aji[k];
+        g1[k] = 1.0 * (2 * s.j_lev[l1[k]] + 1);
+        g2[k] = 1.0 * (2 * s.j_lev[l2[k]] + 1);
+        cnu[k] = (299792458.0 / nu0) / spi;
+        Iin[k] = black_body(2.73, nu0);
+    }
+    // per unique transition
+    const int nt = s.n_trans;
+    std::vector<double> tA(nt), tB21(nt), tnu(nt), tgul(nt);
+    for (int t = 0; t < nt; t++) {
+        const int lo = s.all_trans[2 * t] - 1, up = s.all_trans[2 * t + 1] - 1;
+        const double nu0 = s.e_levs_hz[up] - s.e_levs_hz[lo];
+        const double A = s.trans_rad[t] >= 0 ? s.t_Aji[s.trans_rad[t]] : 0.0;
+        tA[t] = A;
+        tnu[t] = nu0;
+        tB21[t] = 6.781962e49 * A / std::pow(nu0, 3.0);
+        tgul[t] = (2.0 * s.j_lev[up] + 1.0) / (2.0 * s.j_lev[lo] + 1.0);
+    }
+    HIP_TRY(&c, upload(s.d_l1, l1));
+    HIP_TRY(&c, upload(s.d_l2, l2));
+    HIP_TRY(&c, upload(s.d_c1B, c1B));
+    HIP_TRY(&c, upload(s.d_c2A, c2A));
+    HIP_TRY(&c, upload(s.d_g1, g1));
+    HIP_TRY(&c, upload(s.d_g2, g2));
+    HIP_TRY(&c, upload(s.d_cnu_spi, cnu));
+    HIP_TRY(&c, upload(s.d_Iin, Iin));
+    HIP_TRY(&c, upload(s.d_tA, tA));
+    HIP_TRY(&c, upload(s.d_tB21, tB21));
+    HIP_TRY(&c, upload(s.d_tnu, tnu));
+    HIP_TRY(&c, upload(s.d_tgul, tgul));
+    HIP_TRY(&c, upload(s.d_trad, s.trans_rad));
+    HIP_TRY(&c, upload(s.d_tcol, s.trans_col));
+    HIP_TRY(&c, upload(s.d_Cji, s.t_Cji));
+    HIP_TRY(&c, upload(s.d_coltemps, s.col_temps));
+    HIP_TRY(&c, upload(s.d_count, s.count_array));
+    HIP_TRY(&c, upload(s.d_jlev, s.j_lev));
+    HIP_TRY(&c, upload(s.se.d_ent_idx, s.se.ent_idx));
+    HIP_TRY(&c, upload(s.se.d_ent_ptr, s.se.ent_ptr));
+    HIP_TRY(&c, upload(s.se.d_term_q, s.se.term_q));
+    HIP_TRY(&c, upload(s.se.d_term_class, s.se.term_class));
+    HIP_TRY(&c, upload(s.se.d_term_coef, s.se.term_coef));
+    dfree(c.d_levp);
+    c.cap_nodes = 0;
+    s.valid = true;
+    return PORTAL_OK;
+}
+
+int portal_species_info(const portal_ctx *x, int *n_trans, int *c_dim) {
+    if (!x || !x->c.sp.valid) return PORTAL_ERR_STATE;
+    if (n_trans) *n_trans = x->c.sp.n_trans;
+    if (c_dim) *c_dim = x->c.sp.c_dim;
+    return PORTAL_OK;
+}
+
+int portal_set_levpop(portal_ctx *x, const double *lev_p) {
+    if (!x || !lev_p) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    int rc = ensure_node_buffers(c);
+    if (rc) return rc;
+    HIP_TRY(&c, hipMemcpy(c.d_levp, lev_p, sizeof(double) * c.g.N * c.sp.e_num, hipMemcpyHostToDevice));
+    return PORTAL_OK;
+}
+
+int portal_get_levpop(portal_ctx *x, double *lev_p) {
+    if (!x || !lev_p) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    int rc = ensure_node_buffers(c);
+    if (rc) return rc;
+    HIP_TRY(&c, hipMemcpy(lev_p, c.d_levp, sizeof(double) * c.g.N * c.sp.e_num, hipMemcpyDeviceToHost));
+    return PORTAL_OK;
+}
+
+int portal_inverse_rt(portal_ctx *x, const portal_se_params *p, double *J0, double *J2, portal_stats *st) {
+    if (!x) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    int rc = check_se_params(c, p);
+    if (rc) return rc;
+    if (st) std::memset(st, 0, sizeof(*st));
+    Timer tt(c.stream);
+    rc = run_inward(c, p, st);
+    if (rc) return rc;
+    const size_t n = (size_t)c.g.N * c.sp.r_num;
+    if (J0) HIP_TRY(&c, hipMemcpyAsync(J0, c.d_J0, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    if (J2) HIP_TRY(&c, hipMemcpyAsync(J2, c.d_J2, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    const double ms = tt.stop();
+    if (st) st->t_total_ms = ms;
+    return PORTAL_OK;
+}
+
+int portal_se_solve(portal_ctx *x, const double *J0, const double *J2, double *pol_pops, double *pol_rates,
+                    portal_stats *st) {
+    if (!x || !J0 || !J2) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    int rc = ensure_node_buffers(c);
+    if (rc) return rc;
+    if (st) std::memset(st, 0, sizeof(*st));
+    const size_t n = (size_t)c.g.N * c.sp.r_num;
+    HIP_TRY(&c, hipMemcpy(c.d_J0, J0, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(&c, hipMemcpy(c.d_J2, J2, sizeof(double) * n, hipMemcpyHostToDevice));
+    rc = run_se(c, pol_rates != nullptr, st);
+    if (rc) return rc;
+    const size_t L = c.sp.e_num, N = c.g.N;
+    if (pol_pops) HIP_TRY(&c, hipMemcpy(pol_pops, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToHost));
+    if (pol_rates) HIP_TRY(&c, hipMemcpy(pol_rates, c.d_rates, sizeof(double) * N * 6 * L, hipMemcpyDeviceToHost));
+    return PORTAL_OK;
+}
+
+int portal_se_pass(portal_ctx *x, const portal_se_params *p, double *pol_pops, double *pol_rates, double *JJ_20,
+                   portal_stats *st) {
+    if (!x) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    int rc = check_se_params(c, p);
+    if (rc) return rc;
+    if (st) std::memset(st, 0, sizeof(*st));
+    Timer tt(c.stream);
+    rc = run_inward(c, p, st);
+    if (rc) return rc;
+    const int extra = p->extra_info == 1;
+    rc = run_se(c, extra, st);
+    if (rc) return rc;
+    if (st) st->t_total_ms = tt.stop();
+    const size_t L = c.sp.e_num, N = c.g.N, R = c.sp.r_num;
+    if (pol_pops) HIP_TRY(&c, hipMemcpy(pol_pops, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToHost));
+    if (extra && pol_rates)
+        HIP_TRY(&c, hipMemcpy(pol_rates, c.d_rates, sizeof(double) * N * 6 * L, hipMemcpyDeviceToHost));
+    if (extra && JJ_20) {
+        // JJ_20 = J2/J0 for solved nodes (main_sub.f90:139), node_num x r_num
+        std::vector<double> J0(N * R), J2(N * R);
+        HIP_TRY(&c, hipMemcpy(J0.data(), c.d_J0, sizeof(double) * N * R, hipMemcpyDeviceToHost));
+        HIP_TRY(&c, hipMemcpy(J2.data(), c.d_J2, sizeof(double) * N * R, hipMemcpyDeviceToHost));
+        std::fill(JJ_20, JJ_20 + N * R, 0.0);
+        for (int node : c.g.work)
+            for (size_t k = 0; k < R; k++) JJ_20[node + N * k] = J2[k + R * node] / J0[k + R * node];
+    }
+    return PORTAL_OK;
+}
+
+int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal_iter_params *it, double *pol_pops,
+                          double *lev_p_out, portal_stats *st) {
+    if (!x || !it) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    int rc = check_se_params(c, p);
+    if (rc) return rc;
+    if (it->max_iter <= 0) return fail(c, PORTAL_ERR_ARG, "max_iter must be positive");
+    if (it->surrogate) return fail(c, PORTAL_ERR_UNSUPPORTED, "surrogate initial iterate not available");
+    if (st) std::memset(st, 0, sizeof(*st));
+    const size_t L = c.sp.e_num, N = c.g.N;
+    double *d_prev = nullptr;
+    unsigned long long *d_max = nullptr;
+    HIP_TRY(&c, hipMalloc((void **)&d_prev, sizeof(double) * N * 2 * L));
+    HIP_TRY(&c, hipMalloc((void **)&d_max, sizeof(unsigned long long) * 2));
+    Timer tt(c.stream);
+    int iters = 0;
+    double dpop = 0, dal = 0;
+    for (int k = 0; k < it->max_iter; k++) {
+        rc = run_inward(c, p, st);
+        if (!rc) rc = run_se(c, 0, st);
+        if (rc) break;
+        HIP_TRY(&c, hipMemsetAsync(d_max, 0, sizeof(unsigned long long) * 2, c.stream));
+        HIP_TRY(&c, launch_lambda_update((int)c.g.work.size(), (int)N, (int)L, c.g.d_work, c.g.d_ab, c.g.d_h2,
+                                         c.sp.d_jlev, c.d_pol, k > 0 ? d_prev : nullptr, c.d_levp, d_max, c.stream));
+        unsigned long long mx[2];
+        HIP_TRY(&c, hipMemcpyAsync(mx, d_max, sizeof(mx), hipMemcpyDeviceToHost, c.stream));
+        HIP_TRY(&c, hipMemcpyAsync(d_prev, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToDevice, c.stream));
+        HIP_TRY(&c, hipStreamSynchronize(c.stream));
+        std::memcpy(&dpop, &mx[0], 8);
+        std::memcpy(&dal, &mx[1], 8);
+        iters = k + 1;
+        if (k > 0 && dpop <= it->tol_pop && dal <= it->tol_align) break;
+    }
+    const double ms = tt.stop();
+    hipFree(d_prev);
+    hipFree(d_max);
+    if (rc) return rc;
+    if (st) {
+        st->t_total_ms = ms;
+        st->iterations = iters;
+        st->max_rel_change = dpop;
+    }
+    if (pol_pops) HIP_TRY(&c, hipMemcpy(pol_pops, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToHost));
+    if (lev_p_out) HIP_TRY(&c, hipMemcpy(lev_p_out, c.d_levp, sizeof(double) * N * L, hipMemcpyDeviceToHost));
+    return PORTAL_OK;
+}
+
+int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *pol_pops, const double *bvec_all,
+                      double *I, double *Q, double *U, double *TAU, double *dust, portal_stats *st) {
+    if (!x) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    if (!c.g.valid || !c.sp.valid) return fail(c, PORTAL_ERR_STATE, "grid and species must be uploaded first");
+    if (!p || !pol_pops || !bvec_all || !I || !Q || !U || !TAU || p->nlevels <= 0 || p->nvel < 0 || p->im_pix < 0 ||
+        !p->l1 || !p->l2 || !p->nu0_all || !p->Aji_all || !p->kd_all || !p->vel_out || p->n_depth_rt <= 0 ||
+        !(p->dx > 0))
+        return fail(c, PORTAL_ERR_ARG, "bad image-pass arguments");
+    if (p->precision != PORTAL_FP64 && p->precision != PORTAL_MIXED) return fail(c, PORTAL_ERR_ARG, "bad precision");
+    const Species &s = c.sp;
+    const int L = s.e_num;
+    for (int j = 0; j < p->nlevels; j++)
+        if (p->l1[j] < 1 || p->l1[j] > L || p->l2[j] < 1 || p->l2[j] > L)
+            return fail(c, PORTAL_ERR_ARG, "imaged line levels out of range");
+    if (st) std::memset(st, 0, sizeof(*st));
+    const int N = c.g.N, nl = p->nlevels, C = 2 * p->nvel + 1, n_side = 2 * p->im_pix + 1;
+    const int n_rt = n_side * n_side;
+    // R_tot = R_z(ph_az) R_x(th_inc) (miscellaneous.f90:870-880)
+    auto rotm = [](const double nv[3], double th, double *R) {
+        const double cth = std::cos(th), octh = 1.0 - cth, sth = std::sin(th);
+        R[0] = cth + octh * nv[0] * nv[0];
+        R[4] = cth + octh * nv[1] * nv[1];
+        R[8] = cth + octh * nv[2] * nv[2];
+        R[3] = nv[0] * nv[1] * octh - nv[2] * sth;
+        R[1] = nv[1] * nv[0] * octh + nv[2] * sth;
+        R[6] = nv[0] * nv[2] * octh + nv[1] * sth;
+        R[2] = nv[2] * nv[0] * octh - nv[1] * sth;
+        R[7] = nv[1] * nv[2] * octh - nv[0] * sth;
+        R[5] = nv[2] * nv[1] * octh + nv[0] * sth;
+    };
+    const double xv[3] = {1, 0, 0}, zv[3] = {0, 0, 1};
+    double Rth[9], Rph[9];
+    std::vector<double> Rt(9);
+    rotm(xv, p->th_inc, Rth);
+    rotm(zv, p->ph_az, Rph);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double sum = 0.0;
+            for (int k = 0; k < 3; k++) sum += Rph[i + 3 * k] * Rth[k + 3 * j];
+            Rt[i + 3 * j] = sum;
+        }
+    int jmax = 0;
+    for (int i = 0; i < L; i++) jmax = std::max(jmax, s.j_lev[i]);
+    std::vector<double> w6((jmax + 1) * (jmax + 1));
+    for (int a = 0; a <= jmax; a++)
+        for (int b = 0; b <= jmax; b++) w6[a * (jmax + 1) + b] = portal_debug_wig6jj(2, 2, 4, 2 * a, 2 * a, 2 * b);
+    std::vector<int> l1(nl), l2(nl);
+    std::vector<double> Iin(nl);
+    for (int j = 0; j < nl; j++) {
+        l1[j] = p->l1[j] - 1;
+        l2[j] = p->l2[j] - 1;
+        Iin[j] = black_body(2.73, p->nu0_all[j]);
+    }
+    // device scratch, one allocation
+    const size_t nrec = (size_t)n_rt * p->n_depth_rt;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+    const size_t o_R = take(9 * 8), o_xyz = take(24 * (size_t)N), o_vel = take(24 * (size_t)N),
+                 o_bv = take(24 * (size_t)N), o_pol = take(16 * (size_t)N * L), o_jl = take(4 * (size_t)L),
+                 o_l1 = take(4 * (size_t)nl), o_l2 = take(4 * (size_t)nl), o_nu = take(8 * (size_t)nl),
+                 o_A = take(8 * (size_t)nl), o_kd = take(8 * (size_t)nl), o_Iin = take(8 * (size_t)nl),
+                 o_vo = take(8 * (size_t)C), o_w6 = take(8 * w6.size()), o_pos = take(32 * (size_t)N),
+                 o_vz = take(8 * (size_t)N), o_geo = take(sizeof(CellGeo) * (size_t)N),
+                 o_pc = take(32 * (size_t)N * nl), o_st = take(32 * (size_t)n_rt), o_sn = take(4 * (size_t)n_rt),
+                 o_rc = take(4 * nrec), o_rs = take(4 * nrec), o_rn = take(4 * (size_t)n_rt),
+                 o_out = take(8 * (size_t)n_rt * nl * C * (dust ? 8 : 4)), o_b = take(8 * (size_t)N),
+                 o_du = take(8 * (size_t)N), o_td = take(8 * (size_t)N);
+    if (off > c.img_bytes) {
+        if (c.d_img) hipFree(c.d_img);
+        c.d_img = nullptr;
+        c.img_bytes = 0;
+        HIP_TRY(&c, hipMalloc(&c.d_img, off));
+        c.img_bytes = off;
+    }
+    char *B = (char *)c.d_img;
+    auto H2D = [&](size_t o, const void *src, size_t bytes) {
+        return hipMemcpyAsync(B + o, src, bytes, hipMemcpyHostToDevice, c.stream);
+    };
+    std::vector<double> tdust(N), dustv(N);
+    for (int i = 0; i < N; i++) tdust[i] = c.g.temp[2 * (size_t)i + 1];
+    HIP_TRY(&c, H2D(o_R, Rt.data(), 72));
+    HIP_TRY(&c, H2D(o_xyz, c.g.XYZ.data(), 24 * (size_t)N));
+    HIP_TRY(&c, H2D(o_vel, c.g.vel.data(), 24 * (size_t)N));
+    HIP_TRY(&c, H2D(o_bv, bvec_all, 24 * (size_t)N));
+    HIP_TRY(&c, H2D(o_pol, pol_pops, 16 * (size_t)N * L));
+    HIP_TRY(&c, H2D(o_jl, s.j_lev.data(), 4 * (size_t)L));
+    HIP_TRY(&c, H2D(o_l1, l1.data(), 4 * (size_t)nl));
+    HIP_TRY(&c, H2D(o_l2, l2.data(), 4 * (size_t)nl));
+    HIP_TRY(&c, H2D(o_nu, p->nu0_all, 8 * (size_t)nl));
+    HIP_TRY(&c, H2D(o_A, p->Aji_all, 8 * (size_t)nl));
+    HIP_TRY(&c, H2D(o_kd, p->kd_all, 8 * (size_t)nl));
+    HIP_TRY(&c, H2D(o_Iin, Iin.data(), 8 * (size_t)nl));
+    HIP_TRY(&c, H2D(o_vo, p->vel_out, 8 * (size_t)C));
+    HIP_TRY(&c, H2D(o_w6, w6.data(), 8 * w6.size()));
+    HIP_TRY(&c, H2D(o_b, c.g.b_all.data(), 8 * (size_t)N));
+    HIP_TRY(&c, H2D(o_du, c.g.dust.data(), 8 * (size_t)N));
+    HIP_TRY(&c, H2D(o_td, tdust.data(), 8 * (size_t)N));
+    if (!c.d_flags) HIP_TRY(&c, hipMalloc((void **)&c.d_flags, sizeof(int) * 4));
+    HIP_TRY(&c, hipMemsetAsync(c.d_flags, 0, sizeof(int) * 4, c.stream));
+    ImgArgs a{};
+    a.N = N;
+    a.N_b = c.g.N_b;
+    a.nlines = nl;
+    a.C = C;
+    a.n_rt = n_rt;
+    a.n_side = n_side;
+    a.im_pix = p->im_pix;
+    a.n_depth_rt = p->n_depth_rt;
+    a.jmax = jmax;
+    a.im_size = p->im_size;
+    a.dx = p->dx;
+    a.dl_scale = c.g.tx * p->dx;
+    a.alpha = p->alpha;
+    a.Rtot = (const double *)(B + o_R);
+    a.XYZ = (const double *)(B + o_xyz);
+    a.vel = (const double *)(B + o_vel);
+    a.bvec = (const double *)(B + o_bv);
+    a.b_all = (const double *)(B + o_b);
+    a.dust = (const double *)(B + o_du);
+    a.tdust = (const double *)(B + o_td);
+    a.pol = (const double *)(B + o_pol);
+    a.jlev = (const int *)(B + o_jl);
+    a.nbr_ptr = c.g.d_nbr_ptr;
+    a.nbr_idx = c.g.d_nbr_idx;
+    a.l1 = (const int *)(B + o_l1);
+    a.l2 = (const int *)(B + o_l2);
+    a.nu0 = (const double *)(B + o_nu);
+    a.Aji = (const double *)(B + o_A);
+    a.kd = (const double *)(B + o_kd);
+    a.Iin = (const double *)(B + o_Iin);
+    a.vel_out = (const double *)(B + o_vo);
+    a.w6tab = (const double *)(B + o_w6);
+    a.pos_r = (double4 *)(B + o_pos);
+    a.vz_r = (double *)(B + o_vz);
+    a.geo = (CellGeo *)(B + o_geo);
+    a.polco = (double4 *)(B + o_pc);
+    a.start = (double4 *)(B + o_st);
+    a.start_node = (int *)(B + o_sn);
+    a.rec_cell = (int *)(B + o_rc);
+    a.rec_step = (int *)(B + o_rs);
+    a.rec_n = (int *)(B + o_rn);
+    a.flags = c.d_flags;
+    const size_t cube = (size_t)n_rt * nl * C;
+    double *outb = (double *)(B + o_out);
+    a.I = outb;
+    a.Q = outb + cube;
+    a.U = outb + 2 * cube;
+    a.TAU = outb + 3 * cube;
+    a.dust_out = dust ? outb + 4 * cube : nullptr;
+    float ms_g = 0, ms_r = 0;
+    HIP_TRY(&c, launch_image(a, c.stream, &ms_g, &ms_r));
+    int flags = 0;
+    HIP_TRY(&c, hipMemcpy(&flags, c.d_flags, sizeof(int), hipMemcpyDeviceToHost));
+    if (flags & FLAG_DEPTH_RT)
+        return fail(c, PORTAL_ERR_DEPTH_RT, "an image ray crosses more than n_depth_rt cells (raytrace.f90:169-173)");
+    HIP_TRY(&c, hipMemcpy(I, a.I, 8 * cube, hipMemcpyDeviceToHost));
+    HIP_TRY(&c, hipMemcpy(Q, a.Q, 8 * cube, hipMemcpyDeviceToHost));
+    HIP_TRY(&c, hipMemcpy(U, a.U, 8 * cube, hipMemcpyDeviceToHost));
+    HIP_TRY(&c, hipMemcpy(TAU, a.TAU, 8 * cube, hipMemcpyDeviceToHost));
+    if (dust) HIP_TRY(&c, hipMemcpy(dust, a.dust_out, 32 * cube, hipMemcpyDeviceToHost));
+    if (st) {
+        st->t_img_ms = ms_g + ms_r;
+        st->t_walk_ms = ms_g;
+        st->t_rt_ms = ms_r;
+        st->rt_kernel_avg_ms = ms_r;
+        st->t_total_ms = ms_g + ms_r;
+    }
+    return PORTAL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,330 @@
+// Image pass of one inclination (FP64): the three OpenMP loops of
+// para_finish.f90:382-449.
+//
+//   rotate_kernel     rotate_all_save (miscellaneous.f90:844-907) on XYZ, v, B
+//   emcoef_kernel     comp_emcoefs (em_coefs.f90:1-113) per bulk node and imaged line,
+//                     plus the per-cell rotation angles of rot_stokes
+//                     (miscellaneous.f90:463-551) and of the dust branch
+//   start_kernel      nearest node to each pixel's entry point, the brute-force O(N)
+//                     search of trace_out_raytrace (raytrace.f90:126-134): node
+//                     positions are staged through LDS and every lane scans them for
+//                     its pixel, first minimal distance wins (strict <)
+//   march_kernel      the fixed-step march of trace_out_raytrace (raytrace.f90:141-167)
+//                     with check_neigh (miscellaneous.f90:1-46), one lane per pixel;
+//                     the photon position is advanced by repeated addition exactly as
+//                     the reference does, so each cell change lands on the same step
+//   stokes_kernel     ray_in_raytrace (raytrace.f90:179-323) with polarize_prop
+//                     (raytrace.f90:1-56), one lane per (pixel, line, channel)
+// Compiled with -ffp-contract=off.
+#include "common.h"
+#include "image.h"
+
+namespace portal {
+
+__global__ void __launch_bounds__(256) rotate_kernel(ImgArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    const double *R = a.Rtot;  // column-major 3x3
+    double v[3], b[3], x[3];
+    for (int j = 0; j < 3; j++) {
+        double sv = 0.0, sb = 0.0, sx = 0.0;
+        for (int k = 0; k < 3; k++) {
+            sv = sv + R[j + 3 * k] * a.vel[3 * (size_t)i + k];
+            sb = sb + R[j + 3 * k] * a.bvec[3 * (size_t)i + k];
+            sx = sx + R[j + 3 * k] * a.XYZ[3 * (size_t)i + k];
+        }
+        v[j] = sv;
+        b[j] = sb;
+        x[j] = sx;
+    }
+    a.pos_r[i] = make_double4(x[0], x[1], x[2], 0.0);
+    a.vz_r[i] = v[2];
+    // rot_stokes angle: chi = (bx, by, 0)/|.|, alpha = acos(chi_y), sign flipped if chi_x > 0
+    const double n = sqrt(b[0] * b[0] + b[1] * b[1] + 0.0 * 0.0);
+    const double chi1 = b[0] / n, chi2 = b[1] / n;
+    double alpha = acos(chi2);
+    if (chi1 > 0.0) alpha = -alpha;
+    // dust angle: phi = acos(by / sqrt(bx^2 + by^2)), sign flipped if bx < 0
+    double phi = acos(b[1] / sqrt(b[0] * b[0] + b[1] * b[1]));
+    if (b[0] < 0.0) phi = -phi;
+    CellGeo cg;
+    cg.c2a = cos(2.0 * alpha);
+    cg.s2a = sin(2.0 * alpha);
+    cg.mu2 = 1.0 - b[2] * b[2];
+    cg.c2p = cos(2.0 * phi);
+    cg.s2p = sin(2.0 * phi);
+    cg.bz = b[2];
+    a.geo[i] = cg;
+}
+
+__device__ static double wig6_1122(int j1, int j2, const double *tab, int jmax) {
+    return tab[j1 * (jmax + 1) + j2];
+}
+
+__global__ void __launch_bounds__(256) emcoef_kernel(ImgArgs a) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)a.N * a.nlines) return;
+    const int node = (int)(id / a.nlines), j = (int)(id % a.nlines);
+    double4 out = make_double4(0.0, 0.0, 0.0, 0.0);
+    const size_t N = a.N;
+    if (node < a.N_b && a.pol[node] != 0.0) {
+        const int lo = a.l1[j], up = a.l2[j];
+        const double r1_0 = a.pol[node + N * (2 * lo)], r2_0 = a.pol[node + N * (2 * up)];
+        const double r1_2 = a.pol[node + N * (2 * lo + 1)], r2_2 = a.pol[node + N * (2 * up + 1)];
+        const int j1 = a.jlev[lo], j2 = a.jlev[up];
+        const double th = acos(a.geo[node].bz);
+        const double nu0 = a.nu0[j], Aji = a.Aji[j];
+        const double fas = pow(-1.0, 1.0 + j1 + j2);
+        const double fac1 = sqrt(3.0 * (2.0 * j1 + 1.0));
+        const double fac2 = sqrt(3.0 * (2.0 * j2 + 1.0));
+        const double w6_1 = wig6_1122(j1, j2, a.w6tab, a.jmax);
+        const double w6_2 = wig6_1122(j2, j1, a.w6tab, a.jmax);
+        const double wjj2_1 = fas * fac1 * w6_1;
+        const double wjj2_2 = fas * fac2 * w6_2;
+        const double g1 = 2.0 * j1 + 1.0, g2 = 2.0 * j2 + 1.0;
+        const double ct = cos(th), sn = sin(th);
+        const double tki = (3.0 * (ct * ct) - 1.0) / (2.0 * sqrt(2.0));
+        const double tkq = -(3.0 * (sn * sn) / (2.0 * sqrt(2.0)));
+        const double na_I = 1.0 + wjj2_1 * (r1_2 / r1_0) * tki;
+        const double na_Q = wjj2_1 * (r1_2 / r1_0) * tkq;
+        const double ns_I = 1.0 + wjj2_2 * (r2_2 / r2_0) * tki;
+        const double ns_Q = wjj2_2 * (r2_2 / r2_0) * tkq;
+        const double a_to_b = 6.781962e49;
+        const double b_to_a = pow(nu0, 3.0) / a_to_b;
+        const double Bji = Aji / b_to_a;
+        double es_I = ns_I, es_Q = ns_Q;
+        double nsI = sqrt(g1) * r1_0 * na_I - sqrt(g2) * r2_0 * ns_I * g1 / g2;
+        double nsQ = sqrt(g1) * r1_0 * na_Q - sqrt(g2) * r2_0 * ns_Q * g1 / g2;
+        const double hv_fac = 5.273e-35 * nu0;
+        es_I = hv_fac * Aji * es_I * sqrt(g2) * r2_0;
+        es_Q = hv_fac * Aji * es_Q * sqrt(g2) * r2_0;
+        nsI = hv_fac * (g2 / g1) * Bji * nsI;
+        nsQ = hv_fac * (g2 / g1) * Bji * nsQ;
+        out = make_double4(nsI, nsQ, es_I, es_Q);
+    }
+    a.polco[id] = out;
+}
+
+// nearest node to every pixel's start point (strict <, first index wins)
+__global__ void __launch_bounds__(256) start_kernel(ImgArgs a) {
+    __shared__ double4 tile[1024];
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = pix < a.n_rt;
+    double sx = 0, sy = 0, sz = 0;
+    if (active) {
+        const int ii = pix / a.n_side - a.im_pix, jj = pix % a.n_side - a.im_pix;
+        sx = ii * a.im_size / a.im_pix;
+        sy = jj * a.im_size / a.im_pix;
+        sz = -1.0 * sqrt(1.0 - sx * sx - sy * sy);
+        a.start[pix] = make_double4(sx, sy, sz, 0.0);
+    }
+    double rsave = 10.0;
+    int node = -1;
+    for (int base = 0; base < a.N; base += 1024) {
+        const int cnt = min(1024, a.N - base);
+        __syncthreads();
+        for (int t = threadIdx.x; t < cnt; t += blockDim.x) tile[t] = a.pos_r[base + t];
+        __syncthreads();
+        if (active)
+            for (int t = 0; t < cnt; t++) {
+                const double4 p = tile[t];
+                const double dx = sx - p.x, dy = sy - p.y, dz = sz - p.z;
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                if (r < rsave) {
+                    rsave = r;
+                    node = base + t;
+                }
+            }
+    }
+    if (active) a.start_node[pix] = node;
+}
+
+__global__ void __launch_bounds__(256) march_kernel(ImgArgs a) {
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= a.n_rt) return;
+    const double4 st = a.start[pix];
+    const double xp0 = st.x, xp1 = st.y;
+    double xp2 = st.z;
+    int node = a.start_node[pix];
+    int *cells = a.rec_cell + (size_t)pix * a.n_depth_rt;
+    int *steps = a.rec_step + (size_t)pix * a.n_depth_rt;
+    int nc = 0, k_cur = 0;
+    bool in_sim = true, over = false;
+    const double dx = a.dx;
+    if (node < 0) {  // no node closer than 10 (empty grid)
+        a.rec_n[pix] = 0;
+        return;
+    }
+    while (in_sim) {
+        // X_phot = X_phot + dx*dir with dir = (0,0,1): x and y are unchanged
+        xp2 = xp2 + dx * 1.0;
+        k_cur++;
+        if (sqrt(xp0 * xp0 + xp1 * xp1 + xp2 * xp2) > 1.0) {
+            nc++;
+            in_sim = false;
+            if (nc <= a.n_depth_rt) {
+                steps[nc - 1] = k_cur;
+                cells[nc - 1] = node;
+            } else {
+                over = true;
+            }
+        }
+        const double4 pn = a.pos_r[node];
+        const double a0 = xp0 - pn.x, a1 = xp1 - pn.y, a2 = xp2 - pn.z;
+        double dis0 = a0 * a0 + a1 * a1 + a2 * a2;
+        int nn = node;
+        const int qe = a.nbr_ptr[node + 1];
+        for (int q = a.nbr_ptr[node]; q < qe; q++) {
+            const int nb = a.nbr_idx[q];
+            const double4 pb = a.pos_r[nb];
+            const double b0 = xp0 - pb.x, b1 = xp1 - pb.y, b2 = xp2 - pb.z;
+            const double dis = b0 * b0 + b1 * b1 + b2 * b2;
+            if (dis < dis0) {
+                dis0 = dis;
+                nn = nb;
+            }
+        }
+        if (nn != node) {
+            nc++;
+            if (nc <= a.n_depth_rt) {
+                steps[nc - 1] = k_cur;
+                cells[nc - 1] = nn;
+            } else {
+                over = true;
+            }
+            node = nn;
+        }
+        if (over) break;
+    }
+    a.rec_n[pix] = over ? -1 : nc;
+    if (over) atomicOr(a.flags, FLAG_DEPTH_RT);
+}
+
+__device__ __forceinline__ void polarize_prop_d(double eta_I, double eta_Q, double eps_I, double eps_Q, double ds,
+                                                double &I, double &Q, double &U) {
+    const double I0 = I, Q0 = Q;
+    const double shQ = sinh(eta_Q * ds);
+    const double chQ = cosh(eta_Q * ds);
+    const double e_tau = exp(-(eta_I * ds));
+    const double fac = 1.0 / (eta_I * eta_I - eta_Q * eta_Q);
+    const double o11 = fac * eta_I * (1.0 - (chQ + (eta_Q / eta_I) * shQ) * e_tau);
+    const double o12 = -(fac * eta_Q * (1.0 - (chQ + (eta_I / eta_Q) * shQ) * e_tau));
+    I = o11 * eps_I + o12 * eps_Q + e_tau * (chQ * I0 - shQ * Q0);
+    Q = o12 * eps_I + o11 * eps_Q + e_tau * (-(shQ * I0) + chQ * Q0);
+    U = U * e_tau;
+}
+
+__device__ __forceinline__ double black_body_d(double T, double nu0) {
+    if (T == 0.0) return 0.0;
+    const double c1 = 1.4744995e-50, c2 = 4.79924e-11 / T;
+    return c1 * pow(nu0, 3.0) * (1.0 / (exp(c2 * nu0) - 1.0));
+}
+
+__global__ void __launch_bounds__(256) stokes_kernel(ImgArgs a) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)a.n_rt * a.nlines * a.C;
+    if (id >= total) return;
+    const int k = (int)(id % a.C);
+    const long long pl = id / a.C;
+    const int j = (int)(pl % a.nlines);
+    const int pix = (int)(pl / a.nlines);
+    const int nc = a.rec_n[pix];
+    const int *cells = a.rec_cell + (size_t)pix * a.n_depth_rt;
+    const int *steps = a.rec_step + (size_t)pix * a.n_depth_rt;
+    const double nu0 = a.nu0[j];
+    const double cnu = (double)2.99792E8f / nu0;
+    const double I_in = a.Iin[j];
+    const double vel_obs = a.vel_out[k];
+    const double spi = sqrt(acos(-1.0));
+    const double kd = a.kd[j], alpha = a.alpha;
+    double I_out = I_in, Q_in = 0.0, U_in = 0.0, Q_out = 0.0, U_out = 0.0, tau = 0.0, tau_dust_tot = 0.0;
+    double I_dust = I_in, sig0 = 0.0, sig2 = 0.0, Qd = 0.0, Ud = 0.0;
+    const double dxl = a.dl_scale;
+    for (int i = 0; i < nc - 1; i++) {
+        const int c = cells[i];
+        const double b = a.b_all[c];
+        const double dv = vel_obs - a.vz_r[c];
+        const double v_corr = (1.0 / (b * spi)) * exp((-(dv * dv)) / (b * b));
+        const CellGeo cg = a.geo[c];
+        Q_out = Q_in * cg.c2a + U_in * cg.s2a;
+        U_out = U_in * cg.c2a - Q_in * cg.s2a;
+        const double4 pc = a.polco[(size_t)c * a.nlines + j];
+        const double eta_I = cnu * v_corr * pc.x;
+        const double eta_Q = cnu * v_corr * pc.y;
+        const double eps_I = cnu * v_corr * pc.z;
+        const double eps_Q = cnu * v_corr * pc.w;
+        const double delta_x = dxl * (double)(steps[i + 1] - steps[i]);
+        if (delta_x * eta_I > 1e-30) {
+            polarize_prop_d(eta_I, eta_Q, eps_I, eps_Q, delta_x, I_out, Q_out, U_out);
+            tau = tau + eta_I * delta_x;
+        }
+        const double tau_dust = kd * a.dust[c] * delta_x;
+        if (tau_dust > 1e-30) {
+            const double I_bb = black_body_d(a.tdust[c], nu0);
+            const double e_dust = exp(-tau_dust);
+            const double sI = 0.5 * alpha * (cg.mu2 - 2.0 / 3.0);
+            const double sQ = alpha * cg.mu2;
+            I_dust = I_dust * e_dust + I_bb * (1.0 - e_dust);
+            sig0 = sig0 * e_dust + I_bb * (1.0 - e_dust);
+            sig2 = sig2 * e_dust + sI * I_bb * (1.0 - e_dust);
+            Qd = Qd * e_dust + sQ * I_bb * (1.0 - e_dust) * cg.c2p;
+            Ud = Ud * e_dust + sQ * I_bb * (1.0 - e_dust) * cg.s2p;
+            tau_dust_tot = tau_dust_tot + tau_dust;
+        }
+        Q_in = Q_out * cg.c2a - U_out * cg.s2a;
+        U_in = U_out * cg.c2a + Q_out * cg.s2a;
+    }
+    const double I_corr = a.Iin[j];
+    I_out = I_out - I_corr;
+    Q_out = Q_in;
+    U_out = U_in;
+    const size_t o = (size_t)pix + (size_t)a.n_rt * ((size_t)j + (size_t)a.nlines * k);
+    a.I[o] = I_out;
+    a.Q[o] = Q_out;
+    a.U[o] = U_out;
+    a.TAU[o] = tau;
+    if (a.dust_out) {
+        I_dust = I_dust - I_corr;
+        double Q_dust, U_dust, tdust;
+        if (I_dust == 0.0) {
+            Q_dust = 0.0;
+            U_dust = 0.0;
+            tdust = 0.0;
+        } else {
+            Q_dust = I_dust * Qd / (sig0 - sig2);
+            U_dust = I_dust * Ud / (sig0 - sig2);
+            tdust = tau_dust_tot;
+        }
+        const size_t cube = (size_t)a.n_rt * a.nlines * a.C;
+        a.dust_out[o] = I_dust;
+        a.dust_out[cube + o] = Q_dust;
+        a.dust_out[2 * cube + o] = U_dust;
+        a.dust_out[3 * cube + o] = tdust;
+    }
+}
+
+hipError_t launch_image(const ImgArgs &a, hipStream_t s, float *ms_geom, float *ms_rt) {
+    hipEvent_t e0, e1, e2;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventCreate(&e2);
+    hipEventRecord(e0, s);
+    hipLaunchKernelGGL(rotate_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, a);
+    const long long ne = (long long)a.N * a.nlines;
+    hipLaunchKernelGGL(emcoef_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(start_kernel, dim3((a.n_rt + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(march_kernel, dim3((a.n_rt + 255) / 256), dim3(256), 0, s, a);
+    hipEventRecord(e1, s);
+    const long long nt = (long long)a.n_rt * a.nlines * a.C;
+    hipLaunchKernelGGL(stokes_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a);
+    hipEventRecord(e2, s);
+    hipError_t err = hipEventSynchronize(e2);
+    if (ms_geom) hipEventElapsedTime(ms_geom, e0, e1);
+    if (ms_rt) hipEventElapsedTime(ms_rt, e1, e2);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(e2);
+    return err == hipSuccess ? hipGetLastError() : err;
+}
+
+}  // namespace portal

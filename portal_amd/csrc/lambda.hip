@@ -1,0 +1,69 @@
+// Lambda-iteration closure (a build extension; the reference makes one SE pass).
+//
+// After an SE pass the level densities the next inward pass sees are
+//   lev_p(i, node) <- ab * n_H2 * sqrt(2 J_i + 1) * rho^0_0(i)
+// which follows from the SE normalisation row sum_i sqrt(2J_i+1) rho^0_0(i) = 1
+// (em_coefs.f90:158-164) and eat_lime.f90:261-263.  The kernel also reduces the
+// largest relative population change and the largest change of rho^2_0/rho^0_0,
+// the convergence measures of the iteration.
+#include "common.h"
+
+namespace portal {
+
+struct LambdaArgs {
+    int n_work, N, L;
+    const int *work;
+    const double *ab, *h2;
+    const int *j_lev;
+    const double *pol;       // N x 2L, this pass
+    const double *pol_prev;  // N x 2L, previous pass (or null)
+    double *levp;            // L x N, updated in place
+    unsigned long long *maxes;  // [0] pop change, [1] alignment change (double bits)
+};
+
+__global__ void __launch_bounds__(256) lambda_update_kernel(LambdaArgs a) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dpop = 0.0, dal = 0.0;
+    if (id < (long long)a.n_work * a.L) {
+        const int w = (int)(id / a.L), i = (int)(id % a.L);
+        const int node = a.work[w];
+        const size_t N = a.N;
+        const double r0 = a.pol[node + N * (2 * i)];
+        const double r2 = a.pol[node + N * (2 * i + 1)];
+        const double nv = a.ab[node] * a.h2[node] * sqrt(2.0 * a.j_lev[i] + 1.0) * r0;
+        const double old = a.levp[(size_t)node * a.L + i];
+        if (old > 0.0) dpop = fabs(nv - old) / old;
+        else if (nv != 0.0) dpop = 1.0;
+        if (a.pol_prev) {
+            const double p0 = a.pol_prev[node + N * (2 * i)], p2 = a.pol_prev[node + N * (2 * i + 1)];
+            const double q = r0 != 0.0 ? r2 / r0 : 0.0, qp = p0 != 0.0 ? p2 / p0 : 0.0;
+            dal = fabs(q - qp);
+        } else {
+            dal = 1.0;
+        }
+        if (!(dpop == dpop)) dpop = 1.0;  // NaN counts as not converged
+        if (!(dal == dal)) dal = 1.0;
+        a.levp[(size_t)node * a.L + i] = nv;
+    }
+    // block max then one atomic per block (non-negative doubles order like their bits)
+    for (int o = 32; o > 0; o >>= 1) {
+        dpop = fmax(dpop, __shfl_xor(dpop, o, 64));
+        dal = fmax(dal, __shfl_xor(dal, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(a.maxes, (unsigned long long)__double_as_longlong(dpop));
+        atomicMax(a.maxes + 1, (unsigned long long)__double_as_longlong(dal));
+    }
+}
+
+hipError_t launch_lambda_update(int n_work, int N, int L, const int *work, const double *ab, const double *h2,
+                                const int *j_lev, const double *pol, const double *pol_prev, double *levp,
+                                unsigned long long *maxes, hipStream_t s) {
+    LambdaArgs a{n_work, N, L, work, ab, h2, j_lev, pol, pol_prev, levp, maxes};
+    const long long n = (long long)n_work * L;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(lambda_update_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace portal

@@ -1,0 +1,227 @@
+// Inward Voronoi walk: the cells every inward ray of every node crosses.
+//
+// Restates trace_out (inverse_rt.f90:139-214) with check_neigh_fast and
+// pos_quadratic (miscellaneous.f90:1017-1098) and the ray direction of find_dir
+// (miscellaneous.f90:48-82, rotmat 146-181, two_mat_vec_prod 183-204).  The walk
+// depends only on the geometry, so it runs once per (grid, direction set) and its
+// result stays resident in HBM for every later pass and lambda iteration.
+//
+// One lane walks one (node, ray): the neighbour test is the reference's sequential
+// loop (ties go to the later neighbour, a NaN resets the running minimum), which a
+// lane reproduces exactly.  Compiled with -ffp-contract=off: every product is
+// rounded as in the reference, so the nint() quantisation sees the same values.
+#include "common.h"
+
+namespace portal {
+
+__device__ static void rotmat_d(const double nv[3], double th, double R[9]) {
+    const double cth = cos(th), octh = 1.0 - cth, sth = sin(th);
+    // column-major R[(i-1) + 3*(j-1)]
+    R[0] = cth + octh * nv[0] * nv[0];
+    R[4] = cth + octh * nv[1] * nv[1];
+    R[8] = cth + octh * nv[2] * nv[2];
+    R[3] = nv[0] * nv[1] * octh - nv[2] * sth;  // (1,2)
+    R[1] = nv[1] * nv[0] * octh + nv[2] * sth;  // (2,1)
+    R[6] = nv[0] * nv[2] * octh + nv[1] * sth;  // (1,3)
+    R[2] = nv[2] * nv[0] * octh - nv[1] * sth;  // (3,1)
+    R[7] = nv[1] * nv[2] * octh - nv[0] * sth;  // (2,3)
+    R[5] = nv[2] * nv[1] * octh + nv[0] * sth;  // (3,2)
+}
+
+__device__ static void find_dir_d(const double b[3], const double g[3], double th, double ph, double dir[3]) {
+    double xv[3];
+    xv[0] = b[1] * g[2] - b[2] * g[1];
+    xv[1] = b[2] * g[0] - b[0] * g[2];
+    xv[2] = b[0] * g[1] - b[1] * g[0];
+    const double xn = sqrt(xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]);
+    xv[0] = xv[0] / xn;
+    xv[1] = xv[1] / xn;
+    xv[2] = xv[2] / xn;
+    double Rth[9], Rph[9];
+    rotmat_d(xv, th, Rth);
+    rotmat_d(b, ph, Rph);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double d = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) d = d + Rph[i + 3 * j] * Rth[j + 3 * k] * b[k];
+        dir[i] = d;
+    }
+}
+
+template <bool WRITE>
+__global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)a.n_work * a.D) return;
+    const int w = (int)(id / a.D), r = (int)(id % a.D);
+    const int node = a.work[w];
+    const double mu = a.xth[3 * r], ph = a.xth[3 * r + 1];
+    const double b[3] = {a.bvec[3 * (size_t)node], a.bvec[3 * (size_t)node + 1], a.bvec[3 * (size_t)node + 2]};
+    const double g[3] = {a.gvec[0], a.gvec[1], a.gvec[2]};
+    double dir[3];
+    find_dir_d(b, g, acos(mu), ph, dir);
+
+    const double4 p0 = a.pos[node];
+    double xp0 = p0.x, xp1 = p0.y, xp2 = p0.z;
+    int cur = node, nc = 0;
+    double delta_x = 0.0;
+    const double thr = (double)1e-10f;  // single-precision literal of miscellaneous.f90:1048
+    const double nd = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    const double qa = nd * nd;
+    long long off = WRITE ? a.offsets[id] : 0;
+    for (;;) {
+        // distance to the unit sphere (pos_quadratic)
+        const double npn = sqrt(xp0 * xp0 + xp1 * xp1 + xp2 * xp2);
+        const double qb = 2.0 * (xp0 * dir[0] + xp1 * dir[1] + xp2 * dir[2]);
+        const double qc = npn * npn - 1.0;
+        const double dis = qb * qb - 4.0 * qa * qc;
+        if (dis > 0) {
+            double x1 = (-qb + sqrt(dis)) / (2.0 * qa);
+            const double x2 = (-qb - sqrt(dis)) / (2.0 * qa);
+            if (x2 > 0) x1 = x2;
+            delta_x = x1;
+        }
+        int next = -1;
+        const double4 pn = a.pos[cur];
+        const double d0 = pn.x - xp0, d1 = pn.y - xp1, d2 = pn.z - xp2;
+        const double t4 = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        const double q4 = t4 * t4;
+        const double q5 = d0 * dir[0] + d1 * dir[1] + d2 * dir[2];
+        const int q_end = a.nbr_ptr[cur + 1];
+        for (int q = a.nbr_ptr[cur]; q < q_end; q++) {
+            const int nb = a.nbr_idx[q];
+            const double4 pb = a.pos[nb];
+            const double e0 = pb.x - xp0, e1 = pb.y - xp1, e2 = pb.z - xp2;
+            const double u = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            const double q1 = u * u;
+            const double q2 = e0 * dir[0] + e1 * dir[1] + e2 * dir[2];
+            const double new_x = 0.5 * (q1 - q4) / (q2 - q5);
+            if (new_x > delta_x || new_x <= thr) continue;
+            delta_x = new_x;
+            next = nb;
+        }
+        xp0 = xp0 + delta_x * dir[0];
+        xp1 = xp1 + delta_x * dir[1];
+        xp2 = xp2 + delta_x * dir[2];
+        nc++;
+        if (nc > a.n_depth) {
+            atomicOr(a.flags, FLAG_DEPTH);
+            break;
+        }
+        if (WRITE) {
+            PathEntry e;
+            e.cell = cur;
+            e.len = (int)round(delta_x / a.dx);
+            const double4 v = a.velb[cur];
+            e.vray = v.x * dir[0] + v.y * dir[1] + v.z * dir[2];
+            a.paths[off + nc - 1] = e;
+        }
+        cur = next;
+        if (next < 0) break;
+    }
+    if (!WRITE) a.counts[id] = nc > a.n_depth ? 0 : nc;
+}
+
+hipError_t launch_walk(const WalkArgs &a, bool write, hipStream_t s) {
+    const long long n = (long long)a.n_work * a.D;
+    const int bs = 256;
+    const unsigned nb = (unsigned)((n + bs - 1) / bs);
+    if (nb == 0) return hipSuccess;
+    if (write)
+        hipLaunchKernelGGL(walk_kernel<true>, dim3(nb), dim3(bs), 0, s, a);
+    else
+        hipLaunchKernelGGL(walk_kernel<false>, dim3(nb), dim3(bs), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---- exclusive prefix sum of the per-path counts (int -> int64 offsets) ----------
+// Three phases: per-block scan of 2048 elements, scan of block sums in one block,
+// add block offsets.  Offsets has n+1 entries; offsets[n] is the total.
+static constexpr int SCAN_BS = 256, SCAN_PER = 8, SCAN_CHUNK = SCAN_BS * SCAN_PER;
+
+__global__ void __launch_bounds__(SCAN_BS) scan_local(const int *in, long long *out, long long *bsum, long long n) {
+    __shared__ long long sh[SCAN_BS];
+    const long long base = (long long)blockIdx.x * SCAN_CHUNK + (long long)threadIdx.x * SCAN_PER;
+    long long v[SCAN_PER];
+    long long acc = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_PER; i++) {
+        const long long k = base + i;
+        v[i] = acc;
+        acc += (k < n) ? in[k] : 0;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 1; o < SCAN_BS; o <<= 1) {
+        const long long t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const long long excl = sh[threadIdx.x] - acc;
+#pragma unroll
+    for (int i = 0; i < SCAN_PER; i++) {
+        const long long k = base + i;
+        if (k < n) out[k] = v[i] + excl;
+    }
+    if (threadIdx.x == SCAN_BS - 1) bsum[blockIdx.x] = sh[SCAN_BS - 1];
+}
+
+__global__ void __launch_bounds__(1024) scan_blocks(long long *bsum, long long nb) {
+    __shared__ long long sh[1024];
+    const long long per = (nb + 1023) / 1024;
+    const long long b0 = threadIdx.x * per;
+    long long acc = 0;
+    for (long long i = 0; i < per; i++) {
+        const long long k = b0 + i;
+        if (k < nb) acc += bsum[k];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const long long t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    long long run = sh[threadIdx.x] - acc;
+    for (long long i = 0; i < per; i++) {
+        const long long k = b0 + i;
+        if (k < nb) {
+            const long long t = bsum[k];
+            bsum[k] = run;
+            run += t;
+        }
+    }
+    if (threadIdx.x == 1023) bsum[nb] = sh[1023];
+}
+
+__global__ void __launch_bounds__(SCAN_BS) scan_add(long long *out, const long long *bsum, long long n) {
+    const long long k = (long long)blockIdx.x * SCAN_CHUNK + threadIdx.x;
+    const long long add = bsum[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_PER; i++) {
+        const long long kk = k + (long long)i * SCAN_BS;
+        if (kk < n) out[kk] += add;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = bsum[gridDim.x];
+}
+
+hipError_t launch_scan_counts(const int *counts, long long *offsets, long long n, hipStream_t s,
+                              long long *total_host) {
+    const long long nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    long long *bsum = nullptr;
+    hipError_t e = hipMalloc((void **)&bsum, sizeof(long long) * (nb + 1));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(scan_local, dim3((unsigned)nb), dim3(SCAN_BS), 0, s, counts, offsets, bsum, n);
+    hipLaunchKernelGGL(scan_blocks, dim3(1), dim3(1024), 0, s, bsum, nb);
+    hipLaunchKernelGGL(scan_add, dim3((unsigned)nb), dim3(SCAN_BS), 0, s, offsets, bsum, n);
+    e = hipMemcpyAsync(total_host, offsets + n, sizeof(long long), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(bsum);
+    return e == hipSuccess ? hipGetLastError() : e;
+}
+
+}  // namespace portal

@@ -1,0 +1,263 @@
+"""Write the golden fixtures of tests/golden/ (run in the build container only).
+
+Two sources:
+
+* the reference's own routines, compiled from its Fortran sources by
+  ``make -C oracle ref`` into ``oracle/_ref/libportal_ref.so``: quadrature,
+  find_dir, black_body, trace_out, inverse_rt, make_AC, rotate_all_save,
+  trace_out_raytrace, polarize_prop, rot_stokes(_back), ray_in_raytrace;
+* exact arithmetic (sympy) for what the reference cannot be built for here
+  (pop_eqs.f90 / em_coefs.f90 need the WIGXJPF library, which this image lacks):
+  Wigner 3j/6j symbols and the L&L04 7.20 coupling constants re-evaluated as
+  exact sums.
+
+The fixtures hold inputs (or the seed that regenerates them) and outputs only.
+Usage:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle.bind import Reference  # noqa: E402
+from portal_amd import cloud  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def cloud_digest(g) -> str:
+    h = hashlib.sha256()
+    for a in (g.XYZ, g.vel_all, g.b_all, g.lev_p, g.nbr_ptr, g.nbr_idx):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def main():
+    R = Reference()
+    rng = np.random.default_rng(20240601)
+
+    # -- quadrature, find_dir, black_body -------------------------------------------
+    quad = {f"xth_{a}_{b}": R.legendre_all(a, b) for a, b in [(6, 11), (9, 32), (4, 8), (64, 64)]}
+    bs, gs, ths, phs, dirs = [], [], [], [], []
+    for _ in range(32):
+        b = rng.normal(size=3)
+        b /= np.linalg.norm(b)
+        g = rng.random(3)
+        g /= np.linalg.norm(g)
+        th, ph = rng.random() * np.pi, rng.random() * 2 * np.pi
+        bs.append(b); gs.append(g); ths.append(th); phs.append(ph)
+        dirs.append(R.find_dir(b, g, th, ph))
+    Ts = np.array([0.0, 2.73, 5.0, 20.0, 100.0, 1e3])
+    nus = np.array([115.2712e9, 230.538e9, 345.796e9, 1.0e12])
+    bb = np.array([[R.black_body(T, nu) for nu in nus] for T in Ts])
+    np.savez_compressed(os.path.join(OUT, "geometry.npz"), **quad, b=np.array(bs), g=np.array(gs), th=np.array(ths),
+                        ph=np.array(phs), dir=np.array(dirs), T=Ts, nu=nus, bb=bb)
+
+    # -- config-1 cloud: walk and inward transfer ---------------------------------
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 4000, seed=1)
+    xth = cloud.legendre_all(6, 11)
+    dne = g.del_ne()
+    nodes = np.unique(np.concatenate([[1, 2, 3, 4000, 4001 - 1], rng.integers(1, g.bulk_node_num, size=59)]))
+    nodes = nodes[g.ab_all[nodes - 1] > 0]
+    J0 = np.zeros((sp.r_num, len(nodes)))
+    J2 = np.zeros((sp.r_num, len(nodes)))
+    for i, n in enumerate(nodes):
+        J0[:, i], J2[:, i] = R.inverse_rt(int(n), g, sp, dne, xth, 32, cloud.DX_INWARD, g.bvec_all[:, n - 1], g.gvec,
+                                          4096)
+    walks, wofs, wnode, wdir = [], [0], [], []
+    for n in nodes[:24]:
+        for k in range(0, xth.shape[1], 7):
+            d = R.find_dir(g.bvec_all[:, n - 1], g.gvec, np.arccos(xth[0, k]), xth[1, k])
+            dl = R.trace_out(int(n), g, dne, d, cloud.DX_INWARD, 4096)
+            walks.append(dl)
+            wofs.append(wofs[-1] + dl.shape[1])
+            wnode.append(n)
+            wdir.append(d)
+    np.savez_compressed(os.path.join(OUT, "inward_cfg1.npz"), digest=cloud_digest(g), nodes=nodes, J0=J0, J2=J2,
+                        walk=np.concatenate(walks, axis=1), walk_ofs=np.array(wofs), walk_node=np.array(wnode),
+                        walk_dir=np.array(wdir))
+
+    # -- make_AC --------------------------------------------------------------------
+    acs = {}
+    for nl in (4, 10):
+        s = cloud.co_like_rotor(nl)
+        J0v = rng.random(s.r_num) * 1e-16
+        J2v = (rng.random(s.r_num) - 0.5) * 1e-18
+        for T in (0.5, 1.0, 7.3, 55.0, 999.0):
+            acs[f"ac_{nl}_{T}"] = R.make_ac(s, 3.2e11, T, J0v, J2v)
+        acs[f"J0_{nl}"] = J0v
+        acs[f"J2_{nl}"] = J2v
+    # two collision partners (ortho/para mix 0.25/0.75), temperature-dependent rates
+    s2 = cloud.co_like_rotor(5)
+    ct = s2.col_temps
+    C = np.stack([np.outer(np.linspace(1.0, 2.0, len(ct)), np.ones(s2.c_num)) * 1e-11,
+                  np.outer(np.linspace(3.0, 1.0, len(ct)), np.ones(s2.c_num)) * 1e-11], axis=2)
+    s2.t_Cji = np.asfortranarray(C)
+    J0v = rng.random(s2.r_num) * 1e-16
+    J2v = (rng.random(s2.r_num) - 0.5) * 1e-18
+    for T in (0.5, 3.0, 42.0, 5e3):
+        acs[f"ac2p_{T}"] = R.make_ac(s2, 1.7e10, T, J0v, J2v)
+    acs["J0_2p"] = J0v
+    acs["J2_2p"] = J2v
+    acs["C_2p"] = C
+    np.savez_compressed(os.path.join(OUT, "make_ac.npz"), **acs)
+
+    # -- image pass pieces ------------------------------------------------------------
+    th_inc, ph_az = np.pi / 4, 0.3
+    vn, bn, xn = R.rotate_all_save(g, th_inc, ph_az, g.bvec_all)
+    sel = np.arange(0, g.node_num, 97)
+    im_pix, im_size = 64, 0.7
+    pix = []
+    for i in range(-im_pix, im_pix + 1, 9):
+        for j in range(-im_pix, im_pix + 1, 13):
+            pix.append((i, j))
+    traces, tofs = [], [0]
+    nu0 = np.array([sp.e_levs_hz[1] - sp.e_levs_hz[0]])
+    for (i, j) in pix:
+        dl, I_in = R.trace_out_raytrace(i * im_size / im_pix, j * im_size / im_pix, g, xn, dne, cloud.DX_IMAGE, 4096,
+                                        nu0)
+        traces.append(dl)
+        tofs.append(tofs[-1] + dl.shape[1])
+    pp = []
+    for _ in range(40):
+        eI = 10 ** rng.uniform(-16, -12)
+        eQ = eI * rng.uniform(-0.3, 0.3)
+        pI = 10 ** rng.uniform(-30, -28)
+        pQ = pI * rng.uniform(-0.3, 0.3)
+        ds = 10 ** rng.uniform(10, 14)
+        I, Q, U = rng.random(3) * 1e-18
+        pp.append([eI, eQ, pI, pQ, ds, I, Q, U, *R.polarize_prop(eI, eQ, pI, pQ, ds, I, Q, U)])
+    rs = []
+    for _ in range(20):
+        b = rng.normal(size=3)
+        b /= np.linalg.norm(b)
+        Q, U = rng.normal(size=2)
+        rs.append([*b, Q, U, *R.rot_stokes(Q, U, b), *R.rot_stokes(Q, U, b, back=True)])
+    # ray_in_raytrace on synthetic paths, with and without dust
+    rir = []
+    rir_in = []
+    for t in range(12):
+        nc = int(rng.integers(1, 30))
+        pol = np.zeros((4, nc))
+        pol[0] = 10 ** rng.uniform(-13, -11, nc)
+        pol[1] = pol[0] * rng.uniform(-0.2, 0.2, nc)
+        pol[2] = 10 ** rng.uniform(-30, -29, nc)
+        pol[3] = pol[2] * rng.uniform(-0.2, 0.2, nc)
+        bl = rng.uniform(150, 600, nc)
+        bv = rng.normal(size=(3, nc))
+        bv /= np.linalg.norm(bv, axis=0)
+        dust = (rng.random(nc) * 1e-16) if t % 3 == 0 else np.zeros(nc)
+        bbl = rng.uniform(5, 40, nc)
+        vl = rng.uniform(-2000, 2000, nc)
+        ll = np.cumsum(rng.integers(0, 800, nc)).astype(np.int32)
+        vo = rng.uniform(-3000, 3000)
+        out = R.ray_in_raytrace(vo, nc, pol, bl, bv, dust, 0.15, 2.0, bbl, vl, ll, 1.5e11 * cloud.DX_IMAGE, 115.27e9,
+                                1e-19)
+        rir_in.append(dict(vo=vo, nc=nc, pol=pol.tolist(), bl=bl.tolist(), bv=bv.tolist(), dust=dust.tolist(),
+                           bbl=bbl.tolist(), vl=vl.tolist(), ll=ll.tolist()))
+        rir.append(out)
+    np.savez_compressed(os.path.join(OUT, "image_pieces.npz"), th_inc=th_inc, ph_az=ph_az, sel=sel, vel_rot=vn[:, sel],
+                        b_rot=bn[:, sel], xyz_rot=xn[:, sel], pix=np.array(pix), trace=np.concatenate(traces, axis=1),
+                        trace_ofs=np.array(tofs), polprop=np.array(pp), rot=np.array(rs), rir_out=np.array(rir))
+    with open(os.path.join(OUT, "ray_in_raytrace_inputs.json"), "w") as f:
+        json.dump(rir_in, f)
+
+    # -- exact Wigner symbols and L&L04 7.20 constants (sympy) -----------------------
+    from sympy import Rational, sqrt
+    from sympy.physics.wigner import wigner_3j, wigner_6j
+
+    w3 = []
+    for _ in range(150):
+        j1, j2 = int(rng.integers(0, 21)), int(rng.integers(0, 21))
+        j3 = int(rng.integers(abs(j1 - j2), j1 + j2 + 1))
+        m1 = int(rng.integers(-j1, j1 + 1))
+        m2 = int(rng.integers(-j2, j2 + 1))
+        m3 = -m1 - m2
+        if abs(m3) > j3:
+            m3 = 0
+            m1 = -m2 if abs(m2) <= j1 else 0
+            m2 = -m1
+        w3.append([j1, j2, j3, m1, m2, m3, float(wigner_3j(j1, j2, j3, m1, m2, m3).evalf(30))])
+    for j in range(0, 21):
+        for k in (0, 2, 4, 6, 8):
+            for m in sorted({0, min(j, 1), j}):
+                w3.append([j, j, k, m, -m, 0, float(wigner_3j(j, j, k, m, -m, 0).evalf(30))])
+    w6 = []
+    for j in range(0, 21):
+        for jp in (j - 1, j, j + 1):
+            if jp < 0:
+                continue
+            for k in (0, 2, 4, 6, 8):
+                w6.append([jp, jp, k, j, j, 1, float(wigner_6j(jp, jp, k, j, j, 1).evalf(30))])
+            w6.append([1, 1, 2, j, j, jp, float(wigner_6j(1, 1, 2, j, j, jp).evalf(30))])
+
+    def coupling_exact(kind, j, k, j2, k2):
+        """The m,q sums of pop_eqs.f90 evaluated exactly."""
+        dj, dk = j2 - j, k2 - k
+        if abs(dj) > 1 or not (dk == 0 or abs(dk) == 2):
+            return 0.0, 0.0
+        f1 = sqrt(3 * (2 * k + 1) * (2 * k2 + 1))
+        s0 = Rational(0)
+        s2 = Rational(0)
+        for m in range(-j, j + 1):
+            for q in (-1, 0, 1):
+                sg = 1
+                if kind == 0:
+                    ml = q + m
+                    a, b, c = wigner_3j(j, j, k, m, -m, 0), wigner_3j(j2, j2, k2, ml, -ml, 0), wigner_3j(j, j2, 1, -m, ml, -q)
+                elif kind == 1:
+                    mu = m - q
+                    a, b, c = wigner_3j(j, j, k, m, -m, 0), wigner_3j(j2, j2, k2, mu, -mu, 0), wigner_3j(j2, j, 1, -mu, m, -q)
+                elif kind == 2:
+                    mu = m - q
+                    a, b, c = wigner_3j(j, j, k, m, -m, 0), wigner_3j(j, j, k2, m, -m, 0), wigner_3j(j2, j, 1, -mu, m, -q)
+                    sg = (-1) ** (1 + q)
+                else:
+                    ml = m + q
+                    a, b, c = wigner_3j(j, j, k, m, -m, 0), wigner_3j(j, j, k2, m, -m, 0), wigner_3j(j, j2, 1, -m, ml, -q)
+                    sg = (-1) ** (1 - q)
+                ww = a * b * c * c
+                if dk == 0:
+                    s0 += sg * ww * wigner_3j(1, 1, 0, q, -q, 0)
+                s2 += sg * ww * wigner_3j(1, 1, 2, q, -q, 0)
+        if kind <= 1:
+            fac = (2 * j2 + 1) * f1 * (-1) ** (1 - dj)
+            return float((s0 * fac).evalf(30)), float((s2 * sqrt(5) * fac).evalf(30))
+        fac = (2 * j + 1) * f1
+        return float((s0 * fac).evalf(30)), float((s2 * fac * sqrt(5)).evalf(30))
+
+    coup = []
+    for kind in range(4):
+        for j in range(0, 5):
+            for j2 in (j - 1, j, j + 1):
+                if j2 < 0:
+                    continue
+                for k in range(0, min(2 * j, 6) + 1, 2):
+                    for k2 in range(0, min(2 * j2, 6) + 1, 2):
+                        if kind >= 2 and k2 > min(2 * j, 6):
+                            continue
+                        c0, c2 = coupling_exact(kind, j, k, j2, k2)
+                        coup.append([kind, j, k, j2, k2, c0, c2])
+    tsp = []
+    for j in range(0, 6):
+        for ju in (j - 1, j, j + 1):
+            if ju < 0:
+                continue
+            for k in range(0, min(2 * j, 2 * ju, 6) + 1, 2):
+                v = (2 * ju + 1) * (-1) ** (1 + j + ju + k) * wigner_6j(ju, ju, k, j, j, 1)
+                tsp.append([j, k, ju, k, float(v.evalf(30))])
+    with open(os.path.join(OUT, "wigner_exact.json"), "w") as f:
+        json.dump({"w3j": w3, "w6j": w6, "coupling": coup, "t_spont": tsp}, f)
+    print("fixtures written to", OUT)
+
+
+if __name__ == "__main__":
+    main()

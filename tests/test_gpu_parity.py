@@ -1,0 +1,232 @@
+"""HIP path against the oracle, through the C ABI (needs an MI355X).
+
+FP64 mode must agree with the oracle to 1e-10 relative: J^0_0 per line and node,
+rho^0_0 per level, and rho^2_0 to 1e-10 of the level's rho^0_0 (rho^2_0 is a small
+difference of large rates, so it is compared on the scale of its level).  Mixed
+mode reports its error and must stay within the stated tolerances.
+"""
+import numpy as np
+import pytest
+
+from portal_amd import cloud
+from portal_amd.api import Portal, PortalError
+from portal_amd.lib import PORTAL_FP64, PORTAL_MIXED
+
+pytestmark = pytest.mark.gpu
+
+NTH = 16  # oracle threads on the GPU host
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 4000, seed=1)
+    xth = cloud.legendre_all(6, 11)
+    return sp, g, xth
+
+
+@pytest.fixture(scope="module")
+def portal1(cfg1):
+    sp, g, xth = cfg1
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    yield pt
+    pt.close()
+
+
+@pytest.fixture(scope="module")
+def oracle_cfg1(oracle, cfg1):
+    sp, g, xth = cfg1
+    pp, J0, J2, rates, jj = oracle.se_pass(g, sp, xth, 32, cloud.DX_INWARD, extra_info=1, nthreads=NTH)
+    return pp, J0, J2, rates, jj
+
+
+def solved(g):
+    return np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+
+
+def assert_pops_close(pp, ref, sp, idx, rtol):
+    for i in range(sp.e_num):
+        r0, o0 = pp[idx, 2 * i], ref[idx, 2 * i]
+        np.testing.assert_allclose(r0, o0, rtol=rtol, atol=0)
+        np.testing.assert_allclose(pp[idx, 2 * i + 1], ref[idx, 2 * i + 1], rtol=0, atol=rtol * np.abs(o0).max())
+
+
+def test_inverse_rt_fp64_parity(portal1, cfg1, oracle_cfg1):
+    sp, g, xth = cfg1
+    _, J0o, J2o, _, _ = oracle_cfg1
+    J0, J2, st = portal1.inverse_rt(xth, 32, cloud.DX_INWARD)
+    idx = solved(g)
+    np.testing.assert_allclose(J0[:, idx], J0o[:, idx], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(J2[:, idx], J2o[:, idx], rtol=0, atol=1e-10 * np.abs(J0o[:, idx]).max(axis=0).min())
+    # nodes that are not solved get zero (main_para.f90:206-208)
+    mask = np.ones(g.node_num, bool)
+    mask[idx] = False
+    assert np.all(J0[:, mask] == 0)
+    assert st["n_cross"] > 0 and st["n_paths"] == len(idx) * xth.shape[1]
+
+
+def test_se_pass_fp64_parity(portal1, cfg1, oracle_cfg1):
+    sp, g, xth = cfg1
+    ppo, _, _, rates_o, jj_o = oracle_cfg1
+    pp, rates, jj, st = portal1.se_pass(xth, 32, cloud.DX_INWARD, extra_info=1)
+    idx = solved(g)
+    assert_pops_close(pp, ppo, sp, idx, 1e-10)
+    # normalisation sum_i sqrt(2J+1) rho00 = 1 on every solved node
+    w = np.sqrt(2 * sp.j_lev + 1.0)
+    np.testing.assert_allclose(pp[idx][:, 0::2] @ w, 1.0, atol=1e-13)
+    np.testing.assert_allclose(jj[idx], jj_o[idx], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(rates[idx], rates_o[idx], rtol=1e-9, atol=1e-9 * np.abs(rates_o[idx]).max())
+
+
+def test_se_solve_from_oracle_J(portal1, cfg1, oracle, oracle_cfg1):
+    sp, g, xth = cfg1
+    _, J0o, J2o, _, _ = oracle_cfg1
+    pp = portal1.se_solve(J0o, J2o)
+    ref = oracle.solve_from_J(g, sp, J0o, J2o)
+    assert_pops_close(pp, ref, sp, solved(g), 1e-11)
+
+
+def test_mixed_precision_error_bounds(portal1, cfg1, oracle_cfg1):
+    sp, g, xth = cfg1
+    ppo, J0o, J2o, _, _ = oracle_cfg1
+    idx = solved(g)
+    J0, J2, _ = portal1.inverse_rt(xth, 32, cloud.DX_INWARD, precision=PORTAL_MIXED)
+    e0 = np.abs(J0[:, idx] / J0o[:, idx] - 1).max()
+    e2 = np.abs(J2[:, idx] - J2o[:, idx]).max() / np.abs(J0o[:, idx]).max()
+    print(f"mixed: max rel J0 err {e0:.3e}, max J2 err / max J0 {e2:.3e}")
+    assert e0 < 2e-5 and e2 < 1e-6
+    pp, _ = portal1.se_pass(xth, 32, cloud.DX_INWARD, precision=PORTAL_MIXED)
+    r0 = np.abs(pp[idx][:, 0::2] / ppo[idx][:, 0::2] - 1).max()
+    print(f"mixed: max rel rho00 err {r0:.3e}")
+    assert r0 < 1e-5
+
+
+def test_lambda_iteration_matches_oracle(cfg1, oracle):
+    # two closure steps lev_p <- ab n sqrt(2J+1) rho00 on a small cloud
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 800, seed=9)
+    xth = cloud.legendre_all(4, 8)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    pp, lev, st = pt.lambda_iterate(xth, 16, cloud.DX_INWARD, max_iter=3, tol_pop=0.0, tol_align=0.0)
+    assert st["iterations"] == 3
+    lev_o = g.lev_p.copy()
+    idx = solved(g)
+    for _ in range(3):
+        ppo, _, _ = oracle.se_pass(g, sp, xth, 16, cloud.DX_INWARD, lev_p=lev_o, nthreads=NTH)
+        for i in range(sp.e_num):
+            lev_o[i, idx] = g.ab_all[idx] * g.h2_dens[idx] * np.sqrt(2 * sp.j_lev[i] + 1.0) * ppo[idx, 2 * i]
+    np.testing.assert_allclose(lev[:, idx], lev_o[:, idx], rtol=1e-10)
+    assert_pops_close(pp, ppo, sp, idx, 1e-10)
+    pt.close()
+
+
+def test_full_size_subset_parity(oracle):
+    """200,000-node cloud with the 10-level rotor, 200 directions, 129 channels:
+    all nodes on the GPU, a seeded subset through the oracle."""
+    sp = cloud.co_like_rotor(10)
+    g = cloud.make_cloud(sp, 200000, seed=2)
+    xth = cloud.legendre_all(9, 32)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    J0, J2, st = pt.inverse_rt(xth, 64, cloud.DX_INWARD)
+    pp = pt.se_solve(J0, J2)
+    idx = solved(g)
+    # size-independent properties on every node: J0 > 0, normalisation
+    assert np.all(J0[:, idx] > 0)
+    w = np.sqrt(2 * sp.j_lev + 1.0)
+    np.testing.assert_allclose(pp[idx][:, 0::2] @ w, 1.0, atol=1e-12)
+    rng = np.random.default_rng(77)
+    sub = np.sort(rng.choice(idx, 32, replace=False)) + 1
+    J0o, J2o = oracle.inverse_nodes(g, sp, xth, 64, cloud.DX_INWARD, sub, nthreads=NTH)
+    np.testing.assert_allclose(J0[:, sub - 1], J0o, rtol=1e-10)
+    np.testing.assert_allclose(J2[:, sub - 1], J2o, rtol=0, atol=1e-10 * J0o.max(axis=0).min())
+    pt.close()
+
+
+def test_error_paths(cfg1):
+    sp, g, xth = cfg1
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    with pytest.raises(PortalError) as e:
+        pt.inverse_rt(xth, 32, cloud.DX_INWARD, n_depth=3)
+    assert e.value.code == -3  # PORTAL_ERR_DEPTH
+    # a kinetic temperature above the collision table (the reference leaves C undefined)
+    g2 = cloud.make_cloud(sp, 300, seed=4)
+    g2.temp_all[0, 5] = 2e4
+    pt.upload_grid(g2)
+    pt.set_levpop(g2.lev_p)
+    with pytest.raises(PortalError) as e:
+        pt.se_pass(cloud.legendre_all(4, 8), 8, cloud.DX_INWARD)
+    assert e.value.code == -6  # PORTAL_ERR_TEMP
+    pt.close()
+
+
+def test_image_pass_fp64_parity(portal1, cfg1, oracle, oracle_cfg1):
+    from portal_amd import driver
+    sp, g, xth = cfg1
+    ppo = oracle_cfg1[0]
+    pp = driver.rescale_pol_pops(ppo, g.lev_p, sp.lev_g)
+    lines = driver.image_lines(sp, [1], [2])
+    vel_out = driver.channel_velocities(32, 93.75)
+    args = (pp, lines, np.pi / 4, 0.0, 32, vel_out, 24, 0.7, cloud.DX_IMAGE)
+    (I, Q, U, T), st = portal1.image_pass(*args, n_depth_rt=2048)
+    Io, Qo, Uo, To = oracle.image_pass(g, sp, pp, lines, np.pi / 4, 0.0, 32, vel_out, 24, 0.7, cloud.DX_IMAGE,
+                                       n_depth_rt=2048, nthreads=NTH)
+    Imax = np.abs(Io).max()
+    assert Imax > 0
+    np.testing.assert_allclose(I, Io, rtol=1e-10, atol=1e-10 * Imax)
+    np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * Imax)
+    np.testing.assert_allclose(U, Uo, rtol=0, atol=1e-10 * Imax)
+    np.testing.assert_allclose(T, To, rtol=1e-10, atol=1e-12)
+    assert np.abs(Qo).max() > 0  # polarisation is present
+
+
+def test_image_pass_dust_branch(cfg1, oracle):
+    # dust present in the image pass (the inward pass rejects dust): both branches agree
+    from portal_amd import driver
+    sp, g0, xth = cfg1
+    g = cloud.make_cloud(sp, 1500, seed=12)
+    g.dust_p[: g.bulk_node_num - 1] = 1e-17 * g.h2_dens[: g.bulk_node_num - 1] / 1e12
+    sp.kd_v = np.full(sp.r_num, 0.3)
+    pp = np.zeros((g.node_num, 2 * sp.e_num), order="F")
+    rng = np.random.default_rng(3)
+    pp[: g.bulk_node_num - 1, 0::2] = 0.2
+    pp[: g.bulk_node_num - 1, 3::2] = rng.uniform(-1e-3, 1e-3, (g.bulk_node_num - 1, sp.e_num - 1))
+    pp = driver.rescale_pol_pops(pp, g.lev_p, sp.lev_g)
+    lines = driver.image_lines(sp, [1, 2], [2, 3])
+    vel_out = driver.channel_velocities(8, 300.0)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    (I, Q, U, T), dust, st = pt.image_pass(pp, lines, 1.1, 0.4, 8, vel_out, 10, 0.68, cloud.DX_IMAGE,
+                                           n_depth_rt=2048, with_dust=True)
+    (Io, Qo, Uo, To), dusto = oracle.image_pass(g, sp, pp, lines, 1.1, 0.4, 8, vel_out, 10, 0.68, cloud.DX_IMAGE,
+                                                n_depth_rt=2048, nthreads=NTH, with_dust=True)
+    sp.kd_v = np.zeros(sp.r_num)
+    Imax = np.abs(Io).max()
+    np.testing.assert_allclose(I, Io, rtol=1e-10, atol=1e-10 * Imax)
+    np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * Imax)
+    dmax = np.abs(dusto[0]).max()
+    assert dmax > 0
+    np.testing.assert_allclose(dust, dusto, rtol=1e-9, atol=1e-10 * dmax)
+    pt.close()
+
+
+def test_image_depth_overflow(portal1, cfg1, oracle_cfg1):
+    from portal_amd import driver
+    sp, g, xth = cfg1
+    pp = driver.rescale_pol_pops(oracle_cfg1[0], g.lev_p, sp.lev_g)
+    with pytest.raises(PortalError) as e:
+        portal1.image_pass(pp, driver.image_lines(sp, [1], [2]), 0.3, 0.0, 2, driver.channel_velocities(2, 100.0),
+                           4, 0.7, cloud.DX_IMAGE, n_depth_rt=3)
+    assert e.value.code == -4  # PORTAL_ERR_DEPTH_RT

@@ -1,0 +1,211 @@
+"""The oracle against the reference's own routines (golden fixtures) and closed forms.
+
+The fixtures were written by tests/golden/make_golden.py from the reference's
+Fortran compiled from its sources (oracle/_ref) and from exact sympy arithmetic.
+"""
+import hashlib
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle.bind import _d, _f64
+from portal_amd import cloud
+import ctypes as C
+
+
+def _digest(g):
+    h = hashlib.sha256()
+    for a in (g.XYZ, g.vel_all, g.b_all, g.lev_p, g.nbr_ptr, g.nbr_idx):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 4000, seed=1)
+    return sp, g
+
+
+def test_quadrature_matches_reference(golden_dir):
+    z = np.load(os.path.join(golden_dir, "geometry.npz"))
+    for key in [k for k in z.files if k.startswith("xth_")]:
+        _, a, b = key.split("_")
+        mine = cloud.legendre_all(int(a), int(b))
+        ref = z[key]
+        assert mine.shape == ref.shape
+        np.testing.assert_allclose(mine, ref, rtol=0, atol=1e-14)
+        assert abs(ref[2].sum() - 1.0) < 1e-13
+    # the direction counts the configs rely on
+    assert cloud.legendre_all(6, 11).shape[1] == 48
+    assert cloud.legendre_all(9, 32).shape[1] == 200
+
+
+def test_oracle_quadrature_nodes(oracle):
+    for n in (4, 6, 9, 64):
+        x = np.zeros(n)
+        w = np.zeros(n)
+        oracle.lib.orc_legendre_nodes(n, _d(x), _d(w))
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        np.testing.assert_allclose(x, xr, atol=1e-15)
+        np.testing.assert_allclose(w, wr, atol=1e-14)
+
+
+def test_find_dir_and_black_body(oracle, golden_dir):
+    z = np.load(os.path.join(golden_dir, "geometry.npz"))
+    for b, g, th, ph, d in zip(z["b"], z["g"], z["th"], z["ph"], z["dir"]):
+        out = np.zeros(3)
+        oracle.lib.orc_find_dir(_d(_f64(b)), _d(_f64(g)), C.c_double(th), C.c_double(ph), _d(out))
+        np.testing.assert_allclose(out, d, rtol=0, atol=1e-15)
+        # closed form: unit vector at angle th from B
+        assert abs(np.linalg.norm(out) - 1) < 1e-14
+        assert abs(math.acos(max(-1.0, min(1.0, out @ b))) - th) < 1e-7
+    for i, T in enumerate(z["T"]):
+        for j, nu in enumerate(z["nu"]):
+            v = oracle.lib.orc_black_body(T, nu)
+            assert v == pytest.approx(z["bb"][i, j], rel=1e-15, abs=0)
+            if T > 0:  # Planck's law with the reference's constants 2h/c^2, h/k
+                assert v == pytest.approx(2 * 6.62607015e-34 * nu**3 / 299792458.0**2 / math.expm1(6.62607015e-34 * nu / (1.380649e-23 * T)), rel=2e-6)
+
+
+def test_walk_matches_reference(oracle, cfg1, golden_dir):
+    sp, g = cfg1
+    z = np.load(os.path.join(golden_dir, "inward_cfg1.npz"))
+    if str(z["digest"]) != _digest(g):
+        pytest.skip("cloud generator output differs from the fixture's")
+    G = oracle.grid(g)
+    walk, ofs = z["walk"], z["walk_ofs"]
+    for k, (n, d) in enumerate(zip(z["walk_node"], z["walk_dir"])):
+        cells = np.zeros(4097, dtype=np.int32)
+        lens = np.zeros(4097, dtype=np.int32)
+        nc = oracle.lib.orc_trace_out(int(n), C.byref(G), _d(_f64(d)), C.c_double(cloud.DX_INWARD), 4096,
+                                      cells.ctypes.data_as(C.POINTER(C.c_int)), lens.ctypes.data_as(C.POINTER(C.c_int)))
+        ref = walk[:, ofs[k]:ofs[k + 1]]
+        assert nc == ref.shape[1]
+        np.testing.assert_array_equal(cells[:nc], ref[0])
+        np.testing.assert_array_equal(lens[:nc], ref[1])
+
+
+def test_inverse_rt_matches_reference(oracle, cfg1, golden_dir):
+    sp, g = cfg1
+    z = np.load(os.path.join(golden_dir, "inward_cfg1.npz"))
+    if str(z["digest"]) != _digest(g):
+        pytest.skip("cloud generator output differs from the fixture's")
+    xth = cloud.legendre_all(6, 11)
+    nodes = z["nodes"]
+    lo, hi = int(nodes.min()), int(nodes.max())
+    for n in nodes[:20]:
+        _, J0, J2 = oracle.se_pass(g, sp, xth, 32, cloud.DX_INWARD, node_lo=int(n), node_hi=int(n))
+        i = list(nodes).index(n)
+        np.testing.assert_allclose(J0[:, n - 1], z["J0"][:, i], rtol=1e-13, atol=0)
+        np.testing.assert_allclose(J2[:, n - 1], z["J2"][:, i], rtol=0, atol=1e-13 * np.abs(z["J0"][:, i]).max())
+    assert lo >= 1 and hi <= g.bulk_node_num
+
+
+def test_make_ac_matches_reference(oracle, golden_dir):
+    z = np.load(os.path.join(golden_dir, "make_ac.npz"))
+    for nl in (4, 10):
+        s = cloud.co_like_rotor(nl)
+        for T in (0.5, 1.0, 7.3, 55.0, 999.0):
+            ac = oracle.make_ac(s, 3.2e11, T, z[f"J0_{nl}"], z[f"J2_{nl}"])
+            np.testing.assert_allclose(ac, z[f"ac_{nl}_{T}"], rtol=1e-15, atol=0)
+    s2 = cloud.co_like_rotor(5)
+    s2.t_Cji = np.asfortranarray(z["C_2p"])
+    for T in (0.5, 3.0, 42.0, 5e3):
+        ac = oracle.make_ac(s2, 1.7e10, T, z["J0_2p"], z["J2_2p"])
+        np.testing.assert_allclose(ac, z[f"ac2p_{T}"], rtol=1e-15, atol=0)
+
+
+def test_wigner_exact(oracle, golden_dir):
+    d = json.load(open(os.path.join(golden_dir, "wigner_exact.json")))
+    for j1, j2, j3, m1, m2, m3, v in d["w3j"]:
+        got = oracle.wig3jj(2 * j1, 2 * j2, 2 * j3, 2 * m1, 2 * m2, 2 * m3)
+        assert got == pytest.approx(v, rel=1e-14, abs=1e-15)
+    for a, b, c, e, f, g, v in d["w6j"]:
+        got = oracle.wig6jj(2 * a, 2 * b, 2 * c, 2 * e, 2 * f, 2 * g)
+        assert got == pytest.approx(v, rel=1e-14, abs=1e-15)
+
+
+def test_coupling_constants_exact(oracle, golden_dir):
+    d = json.load(open(os.path.join(golden_dir, "wigner_exact.json")))
+    names = ["t_abs_const", "t_stim_const", "r_abs_const", "r_stim_const"]
+    for kind, j, k, j2, k2, c0, c2 in d["coupling"]:
+        g0, g2 = oracle.coef(names[kind], j, k, j2, k2)
+        assert g0 == pytest.approx(c0, rel=1e-13, abs=1e-14)
+        assert g2 == pytest.approx(c2, rel=1e-13, abs=1e-14)
+    for j, k, ju, ku, v in d["t_spont"]:
+        assert oracle.coef("t_spont_const", j, k, ju, ku) == pytest.approx(v, rel=1e-13, abs=1e-14)
+
+
+def test_coupling_selection_rules(oracle):
+    # zero unless |dJ| <= 1 and dK in {0, +-2}
+    for j in range(0, 5):
+        for j2 in range(0, 7):
+            for k in range(0, 7, 2):
+                for k2 in range(0, 7, 2):
+                    t0, t2 = oracle.coef("t_abs_const", j, k, j2, k2)
+                    if abs(j2 - j) > 1 or abs(k2 - k) > 2:
+                        assert t0 == 0 and t2 == 0
+                    if k2 != k:
+                        assert t0 == 0
+
+
+def test_image_pieces_match_reference(oracle, cfg1, golden_dir):
+    sp, g = cfg1
+    z = np.load(os.path.join(golden_dir, "image_pieces.npz"))
+    N = g.node_num
+    vn, bn, xn = (np.zeros((3, N), order="F") for _ in range(3))
+    oracle.lib.orc_rotate_all_save(N, C.c_double(float(z["th_inc"])), C.c_double(float(z["ph_az"])), _d(_f64(g.vel_all)),
+                                   _d(_f64(g.bvec_all)), _d(_f64(g.XYZ)), _d(vn), _d(bn), _d(xn))
+    sel = z["sel"]
+    np.testing.assert_allclose(vn[:, sel], z["vel_rot"], rtol=1e-15, atol=1e-12)
+    np.testing.assert_allclose(bn[:, sel], z["b_rot"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(xn[:, sel], z["xyz_rot"], rtol=0, atol=1e-15)
+    ptr, idx = _f64(g.nbr_ptr).astype(np.int32), np.asarray(g.nbr_idx, dtype=np.int32)
+    im_pix, im_size = 64, 0.7
+    for k, (i, j) in enumerate(z["pix"]):
+        cells = np.zeros(4096, dtype=np.int32)
+        steps = np.zeros(4096, dtype=np.int32)
+        nc = C.c_int()
+        rc = oracle.lib.orc_trace_out_raytrace(C.c_double(i * im_size / im_pix), C.c_double(j * im_size / im_pix), N,
+                                               _d(xn), ptr.ctypes.data_as(C.POINTER(C.c_int)),
+                                               idx.ctypes.data_as(C.POINTER(C.c_int)), C.c_double(cloud.DX_IMAGE), 4096,
+                                               cells.ctypes.data_as(C.POINTER(C.c_int)),
+                                               steps.ctypes.data_as(C.POINTER(C.c_int)), C.byref(nc))
+        assert rc == 0
+        ref = z["trace"][:, z["trace_ofs"][k]:z["trace_ofs"][k + 1]]
+        assert nc.value == ref.shape[1]
+        np.testing.assert_array_equal(cells[:nc.value], ref[0])
+        np.testing.assert_array_equal(steps[:nc.value], ref[1])
+    for row in z["polprop"]:
+        eI, eQ, pI, pQ, ds, I, Q, U = row[:8]
+        a = [C.c_double(I), C.c_double(Q), C.c_double(U)]
+        oracle.lib.orc_polarize_prop(*[C.c_double(v) for v in (eI, eQ, pI, pQ, ds)], *[C.byref(v) for v in a])
+        np.testing.assert_allclose([v.value for v in a], row[8:], rtol=1e-13, atol=1e-300)
+    for row in z["rot"]:
+        b = _f64(row[:3])
+        qn, un = C.c_double(), C.c_double()
+        oracle.lib.orc_rot_stokes(C.c_double(row[3]), C.c_double(row[4]), C.byref(qn), C.byref(un), _d(b))
+        np.testing.assert_allclose([qn.value, un.value], row[5:7], rtol=1e-12, atol=1e-13)
+        oracle.lib.orc_rot_stokes_back(C.c_double(row[3]), C.c_double(row[4]), C.byref(qn), C.byref(un), _d(b))
+        np.testing.assert_allclose([qn.value, un.value], row[7:9], rtol=1e-12, atol=1e-13)
+
+
+def test_ray_in_raytrace_matches_reference(oracle, golden_dir):
+    z = np.load(os.path.join(golden_dir, "image_pieces.npz"))
+    inputs = json.load(open(os.path.join(golden_dir, "ray_in_raytrace_inputs.json")))
+    for inp, ref in zip(inputs, z["rir_out"]):
+        out = np.zeros(8)
+        nc = inp["nc"]
+        pol = _f64(np.array(inp["pol"]))  # (4, nc) column-major as pol_rayt_list
+        bv = _f64(np.array(inp["bv"]))    # (3, nc)
+        oracle.lib.orc_ray_in_raytrace(C.c_double(inp["vo"]), nc, _d(pol),
+                                       _d(_f64(inp["bl"])), _d(bv), _d(_f64(inp["dust"])),
+                                       C.c_double(0.15), C.c_double(2.0), _d(_f64(inp["bbl"])), _d(_f64(inp["vl"])),
+                                       np.asarray(inp["ll"], dtype=np.int32).ctypes.data_as(C.POINTER(C.c_int)),
+                                       C.c_double(1.5e11 * cloud.DX_IMAGE), C.c_double(115.27e9), C.c_double(1e-19), _d(out))
+        scale = max(abs(ref[0]), abs(ref[4]), 1e-300)
+        np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-13 * scale)
