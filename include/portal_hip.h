@@ -151,6 +151,29 @@ int portal_image_pass(portal_ctx *ctx, const portal_img_params *p, const double 
                       const double *bvec_all, double *I, double *Q, double *U, double *TAU,
                       double *dust, portal_stats *stats);
 
+/* Surrogate network (no counterpart in the reference).  A dense MLP, depth hidden
+ * layers of `width` with SiLU, BF16 weights and activations, FP32 accumulation and
+ * FP32 master weights (Adam).  Inputs: log10 n_H2[cm^-3], log10 T_kin and, per
+ * radiative line, log10(J^0_0/B_nu(2.73 K)) and 100 J^2_0/J^0_0 of a node
+ * (n_in = 2 + 2 r_num).  Outputs: log10(sqrt(2J_i+1) rho^0_0(i)) per level, then
+ * 1e3 rho^2_0/rho^0_0 per level with J > 0 (n_out = e_num + levels with J > 0).
+ * feats/targets/out are batch x n_in / batch x n_out row-major floats; norm holds the
+ * input means, input standard deviations, output means, output standard deviations.
+ * portal_lambda_iterate with surrogate = 1 uses it for its first iterate. */
+int portal_mlp_create(portal_ctx *ctx, int n_in, int width, int depth, int n_out, unsigned long long seed);
+int portal_mlp_destroy(portal_ctx *ctx);
+long long portal_mlp_num_params(const portal_ctx *ctx);
+int portal_mlp_set_params(portal_ctx *ctx, const float *params, const float *norm);
+int portal_mlp_get_params(portal_ctx *ctx, float *params, float *norm);
+int portal_mlp_forward(portal_ctx *ctx, int batch, const float *feats, float *out);
+int portal_mlp_train_step(portal_ctx *ctx, int batch, const float *feats, const float *targets, float lr,
+                          float *loss, float *grads);
+int portal_mlp_train(portal_ctx *ctx, int n, const float *feats, const float *targets, int epochs, int batch,
+                     float lr, unsigned long long seed, float *loss_hist, double *ms);
+/* Network inputs of every solved node from the last inward pass; nodes (1-based,
+ * may be NULL) gives the node of each row. */
+int portal_surrogate_features(portal_ctx *ctx, int n_in, float *feats, int *nodes, int *n_work);
+
 #ifdef __cplusplus
 }
 #endif

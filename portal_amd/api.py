@@ -38,7 +38,7 @@ def _i(a):
     return a.ctypes.data_as(L.ip)
 
 
-class Portal:
+class _Core:
     def __init__(self, device: int = 0):
         self.lib = L.load()
         self.ctx = C.c_void_p()
@@ -167,3 +167,68 @@ class Portal:
         if with_dust:
             return outs, dust, st.as_dict()
         return outs, st.as_dict()
+
+
+class SurrogateMixin:
+    """Surrogate-network calls of the C ABI (portal_mlp_*, portal_surrogate_features)."""
+
+    def mlp_create(self, n_in, width=256, depth=4, n_out=1, seed=0):
+        self._check(self.lib.portal_mlp_create(self.ctx, n_in, width, depth, n_out, seed))
+        self.mlp_dims = (n_in, width, depth, n_out)
+        return int(self.lib.portal_mlp_num_params(self.ctx))
+
+    def mlp_set_params(self, params=None, norm=None):
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        n = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+        self._check(self.lib.portal_mlp_set_params(self.ctx, None if p is None else p.ctypes.data_as(C.c_void_p),
+                                                   None if n is None else n.ctypes.data_as(C.c_void_p)))
+
+    def mlp_get_params(self):
+        n_in, _, _, n_out = self.mlp_dims
+        p = np.zeros(int(self.lib.portal_mlp_num_params(self.ctx)), dtype=np.float32)
+        nm = np.zeros(2 * n_in + 2 * n_out, dtype=np.float32)
+        self._check(self.lib.portal_mlp_get_params(self.ctx, p.ctypes.data_as(C.c_void_p), nm.ctypes.data_as(C.c_void_p)))
+        return p, nm
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.zeros((x.shape[0], self.mlp_dims[3]), dtype=np.float32)
+        self._check(self.lib.portal_mlp_forward(self.ctx, x.shape[0], x.ctypes.data_as(C.c_void_p),
+                                                out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def mlp_train_step(self, x, t, lr, want_grads=False):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_float()
+        g = np.zeros(int(self.lib.portal_mlp_num_params(self.ctx)), dtype=np.float32) if want_grads else None
+        self._check(self.lib.portal_mlp_train_step(self.ctx, x.shape[0], x.ctypes.data_as(C.c_void_p),
+                                                   t.ctypes.data_as(C.c_void_p), C.c_float(lr), C.byref(loss),
+                                                   None if g is None else g.ctypes.data_as(C.c_void_p)))
+        return (loss.value, g) if want_grads else loss.value
+
+    def mlp_train(self, x, t, epochs, batch, lr, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        hist = np.zeros(epochs, dtype=np.float32)
+        ms = C.c_double()
+        self._check(self.lib.portal_mlp_train(self.ctx, x.shape[0], x.ctypes.data_as(C.c_void_p),
+                                              t.ctypes.data_as(C.c_void_p), epochs, batch, C.c_float(lr), seed,
+                                              hist.ctypes.data_as(C.c_void_p), C.byref(ms)))
+        return hist, ms.value
+
+    def surrogate_features(self):
+        sp = self.species
+        n_in = 2 + 2 * sp.r_num
+        nw_max = self.grid.bulk_node_num
+        f = np.zeros((nw_max, n_in), dtype=np.float32)
+        nodes = np.zeros(nw_max, dtype=np.int32)
+        nw = C.c_int()
+        self._check(self.lib.portal_surrogate_features(self.ctx, n_in, f.ctypes.data_as(C.c_void_p),
+                                                       nodes.ctypes.data_as(C.c_void_p), C.byref(nw)))
+        return f[: nw.value], nodes[: nw.value]
+
+
+
+class Portal(SurrogateMixin, _Core):
+    """One device context: grid/species uploads, SE/image passes, lambda iteration, surrogate network."""

@@ -202,8 +202,29 @@ struct SeArgs {
 };
 hipError_t launch_se(const SeArgs &a, hipStream_t s);
 
+// surrogate features / decoding (lambda.hip) and network inference (mlp.hip)
+struct SurrArgs {
+    int n_work, N, L, R, n_in, n_out;
+    const int *work;
+    const double *h2, *tkin, *ab, *J0, *J2, *Iin;
+    const int *j_lev;
+    float *feats;            // n_work x n_in
+    const float *out;        // n_work x n_out (network outputs, physical units)
+    double *levp, *pol;      // updated for accepted nodes
+    int *n_fallback;
+};
+hipError_t launch_surr_features(const SurrArgs &a, hipStream_t s);
+hipError_t launch_surr_apply(const SurrArgs &a, hipStream_t s);
+hipError_t mlp_infer_device(Ctx &c, int B, const float *feats, float *out);
+int mlp_dims(const Ctx &c, int *n_in, int *n_out);
+
 hipError_t launch_lambda_update(int n_work, int N, int L, const int *work, const double *ab, const double *h2,
                                 const int *j_lev, const double *pol, const double *pol_prev, double *levp,
                                 unsigned long long *maxes, hipStream_t s);
 
 }  // namespace portal
+
+struct portal_ctx {
+    portal::Ctx c;
+};
+void portal_mlp_free(portal::Ctx &c);

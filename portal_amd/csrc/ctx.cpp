@@ -21,10 +21,6 @@ int build_species_tables(Species &s);
 
 using namespace portal;
 
-struct portal_ctx {
-    Ctx c;
-};
-
 namespace {
 
 template <typename T>
@@ -341,7 +337,6 @@ int portal_ctx_create(portal_ctx **out, int device) {
     return PORTAL_OK;
 }
 
-void portal_mlp_free(Ctx &c) { (void)c; }
 
 int portal_ctx_destroy(portal_ctx *x) {
     if (!x) return PORTAL_OK;
@@ -671,6 +666,84 @@ int portal_se_pass(portal_ctx *x, const portal_se_params *p, double *pol_pops, d
     return PORTAL_OK;
 }
 
+static int surrogate_step(Ctx &c, int *n_fallback) {
+    int n_in = 0, n_out = 0;
+    if (!mlp_dims(c, &n_in, &n_out)) return fail(c, PORTAL_ERR_STATE, "no surrogate network (portal_mlp_create)");
+    const Species &s = c.sp;
+    int nj = 0;
+    for (int i = 0; i < s.e_num; i++) nj += s.j_lev[i] != 0;
+    if (n_in != 2 + 2 * s.r_num || n_out != s.e_num + nj)
+        return fail(c, PORTAL_ERR_ARG, "surrogate network shape does not match the species");
+    const int nw = (int)c.g.work.size();
+    float *d_f = nullptr, *d_o = nullptr;
+    int *d_n = nullptr;
+    HIP_TRY(&c, hipMalloc((void **)&d_f, sizeof(float) * (size_t)nw * n_in + 16));
+    HIP_TRY(&c, hipMalloc((void **)&d_o, sizeof(float) * (size_t)nw * n_out + 16));
+    HIP_TRY(&c, hipMalloc((void **)&d_n, sizeof(int)));
+    HIP_TRY(&c, hipMemsetAsync(d_n, 0, sizeof(int), c.stream));
+    SurrArgs a{};
+    a.n_work = nw;
+    a.N = c.g.N;
+    a.L = s.e_num;
+    a.R = s.r_num;
+    a.n_in = n_in;
+    a.n_out = n_out;
+    a.work = c.g.d_work;
+    a.h2 = c.g.d_h2;
+    a.tkin = c.g.d_tkin;
+    a.ab = c.g.d_ab;
+    a.J0 = c.d_J0;
+    a.J2 = c.d_J2;
+    a.Iin = s.d_Iin;
+    a.j_lev = s.d_jlev;
+    a.feats = d_f;
+    a.out = d_o;
+    a.levp = c.d_levp;
+    a.pol = c.d_pol;
+    a.n_fallback = d_n;
+    HIP_TRY(&c, launch_surr_features(a, c.stream));
+    HIP_TRY(&c, mlp_infer_device(c, nw, d_f, d_o));
+    HIP_TRY(&c, launch_surr_apply(a, c.stream));
+    HIP_TRY(&c, hipMemcpyAsync(n_fallback, d_n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(&c, hipStreamSynchronize(c.stream));
+    hipFree(d_f);
+    hipFree(d_o);
+    hipFree(d_n);
+    return PORTAL_OK;
+}
+
+int portal_surrogate_features(portal_ctx *x, int n_in, float *feats, int *nodes, int *n_work) {
+    if (!x || !feats || !n_work) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    if (!c.g.valid || !c.sp.valid || !c.d_J0) return fail(c, PORTAL_ERR_STATE, "no inward pass has run");
+    if (n_in != 2 + 2 * c.sp.r_num) return fail(c, PORTAL_ERR_ARG, "n_in must be 2 + 2*r_num");
+    const int nw = (int)c.g.work.size();
+    float *d_f = nullptr;
+    HIP_TRY(&c, hipMalloc((void **)&d_f, sizeof(float) * (size_t)nw * n_in + 16));
+    SurrArgs a{};
+    a.n_work = nw;
+    a.N = c.g.N;
+    a.L = c.sp.e_num;
+    a.R = c.sp.r_num;
+    a.n_in = n_in;
+    a.work = c.g.d_work;
+    a.h2 = c.g.d_h2;
+    a.tkin = c.g.d_tkin;
+    a.J0 = c.d_J0;
+    a.J2 = c.d_J2;
+    a.Iin = c.sp.d_Iin;
+    a.feats = d_f;
+    HIP_TRY(&c, launch_surr_features(a, c.stream));
+    HIP_TRY(&c, hipMemcpyAsync(feats, d_f, sizeof(float) * (size_t)nw * n_in, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(&c, hipStreamSynchronize(c.stream));
+    hipFree(d_f);
+    if (nodes)
+        for (int i = 0; i < nw; i++) nodes[i] = c.g.work[i] + 1;
+    *n_work = nw;
+    return PORTAL_OK;
+}
+
 int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal_iter_params *it, double *pol_pops,
                           double *lev_p_out, portal_stats *st) {
     if (!x || !it) return PORTAL_ERR_ARG;
@@ -679,7 +752,7 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     int rc = check_se_params(c, p);
     if (rc) return rc;
     if (it->max_iter <= 0) return fail(c, PORTAL_ERR_ARG, "max_iter must be positive");
-    if (it->surrogate) return fail(c, PORTAL_ERR_UNSUPPORTED, "surrogate initial iterate not available");
+    if (it->surrogate && !c.mlp) return fail(c, PORTAL_ERR_STATE, "no surrogate network (portal_mlp_create)");
     if (st) std::memset(st, 0, sizeof(*st));
     const size_t L = c.sp.e_num, N = c.g.N;
     double *d_prev = nullptr;
@@ -687,7 +760,7 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     HIP_TRY(&c, hipMalloc((void **)&d_prev, sizeof(double) * N * 2 * L));
     HIP_TRY(&c, hipMalloc((void **)&d_max, sizeof(unsigned long long) * 2));
     Timer tt(c.stream);
-    int iters = 0;
+    int iters = 0, nfb = 0;
     double dpop = 0, dal = 0;
     for (int k = 0; k < it->max_iter; k++) {
         rc = run_inward(c, p, st);
@@ -696,6 +769,12 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
         HIP_TRY(&c, hipMemsetAsync(d_max, 0, sizeof(unsigned long long) * 2, c.stream));
         HIP_TRY(&c, launch_lambda_update((int)c.g.work.size(), (int)N, (int)L, c.g.d_work, c.g.d_ab, c.g.d_h2,
                                          c.sp.d_jlev, c.d_pol, k > 0 ? d_prev : nullptr, c.d_levp, d_max, c.stream));
+        if (k == 0 && it->surrogate) {
+            // the network's prediction of the converged state replaces the plain first
+            // iterate wherever it passes the validity checks
+            rc = surrogate_step(c, &nfb);
+            if (rc) break;
+        }
         unsigned long long mx[2];
         HIP_TRY(&c, hipMemcpyAsync(mx, d_max, sizeof(mx), hipMemcpyDeviceToHost, c.stream));
         HIP_TRY(&c, hipMemcpyAsync(d_prev, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToDevice, c.stream));
@@ -713,6 +792,7 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
         st->t_total_ms = ms;
         st->iterations = iters;
         st->max_rel_change = dpop;
+        st->fallbacks = nfb;
     }
     if (pol_pops) HIP_TRY(&c, hipMemcpy(pol_pops, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToHost));
     if (lev_p_out) HIP_TRY(&c, hipMemcpy(lev_p_out, c.d_levp, sizeof(double) * N * L, hipMemcpyDeviceToHost));
@@ -730,6 +810,12 @@ int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *p
         !(p->dx > 0))
         return fail(c, PORTAL_ERR_ARG, "bad image-pass arguments");
     if (p->precision != PORTAL_FP64 && p->precision != PORTAL_MIXED) return fail(c, PORTAL_ERR_ARG, "bad precision");
+    if (p->im_pix > 0) {
+        // the corner pixel must start on the model sphere (raytrace.f90:98 takes sqrt(1-x^2-y^2))
+        const double xc = p->im_pix * p->im_size / p->im_pix;
+        if (!(1.0 - xc * xc - xc * xc >= 0.0))
+            return fail(c, PORTAL_ERR_ARG, "image corners lie outside the model sphere (im_size > 1/sqrt(2))");
+    }
     const Species &s = c.sp;
     const int L = s.e_num;
     for (int j = 0; j < p->nlevels; j++)

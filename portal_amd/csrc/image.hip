@@ -151,11 +151,18 @@ __global__ void __launch_bounds__(256) march_kernel(ImgArgs a) {
     int nc = 0, k_cur = 0;
     bool in_sim = true, over = false;
     const double dx = a.dx;
-    if (node < 0) {  // no node closer than 10 (empty grid)
-        a.rec_n[pix] = 0;
+    if (node < 0 || !(xp2 == xp2)) {  // no start node, or a start point off the sphere
+        a.rec_n[pix] = -1;
+        atomicOr(a.flags, FLAG_DEPTH_RT);
         return;
     }
+    // the march crosses the unit sphere in at most 2/dx steps; stop well after that
+    const long long max_steps = (long long)(2.0 / dx) + 16;
     while (in_sim) {
+        if (k_cur > max_steps) {
+            over = true;
+            break;
+        }
         // X_phot = X_phot + dx*dir with dir = (0,0,1): x and y are unchanged
         xp2 = xp2 + dx * 1.0;
         k_cur++;

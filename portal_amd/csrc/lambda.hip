@@ -67,3 +67,74 @@ hipError_t launch_lambda_update(int n_work, int N, int L, const int *work, const
 }
 
 }  // namespace portal
+
+namespace portal {
+
+// Surrogate input features of every work node from the last inward pass:
+//   log10 n_H2[cm^-3], log10 T_kin, and per line log10(J^0_0 / I_cmb) and 100 J^2_0/J^0_0.
+__global__ void __launch_bounds__(256) surr_features_kernel(SurrArgs a) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= a.n_work) return;
+    const int node = a.work[w];
+    float *f = a.feats + (size_t)w * a.n_in;
+    f[0] = (float)log10(a.h2[node] * 1e-6);
+    f[1] = (float)log10(a.tkin[node]);
+    for (int k = 0; k < a.R; k++) {
+        const double j0 = a.J0[(size_t)node * a.R + k], j2 = a.J2[(size_t)node * a.R + k];
+        f[2 + 2 * k] = (float)log10(j0 / a.Iin[k]);
+        f[3 + 2 * k] = (float)(100.0 * j2 / j0);
+    }
+}
+
+// Decode the network output into populations and alignment; reject non-finite or
+// badly normalised predictions (those nodes keep the plain iterate).
+__global__ void __launch_bounds__(256) surr_apply_kernel(SurrArgs a) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= a.n_work) return;
+    const int node = a.work[w];
+    const float *o = a.out + (size_t)w * a.n_out;
+    const size_t N = a.N;
+    double s = 0.0;
+    bool ok = true;
+    for (int i = 0; i < a.L; i++) {
+        const double p = exp10((double)o[i]);
+        ok = ok && isfinite(p);
+        s += p;
+    }
+    ok = ok && s > 0.5 && s < 2.0;
+    int q = a.L;
+    for (int i = 0; i < a.L && ok; i++) {
+        if (a.j_lev[i] == 0) continue;
+        const float r = o[q++];
+        ok = ok && isfinite(r) && fabsf(r) < 500.0f;
+    }
+    if (!ok) {
+        atomicAdd(a.n_fallback, 1);
+        return;
+    }
+    const double nab = a.ab[node] * a.h2[node];
+    q = a.L;
+    for (int i = 0; i < a.L; i++) {
+        const double p = exp10((double)o[i]) / s;
+        const double sg = sqrt(2.0 * a.j_lev[i] + 1.0);
+        const double r0 = p / sg;
+        const double rat = a.j_lev[i] != 0 ? 1e-3 * (double)o[q++] : 0.0;
+        a.levp[(size_t)node * a.L + i] = nab * p;
+        a.pol[node + N * (2 * i)] = r0;
+        a.pol[node + N * (2 * i + 1)] = rat * r0;
+    }
+}
+
+hipError_t launch_surr_features(const SurrArgs &a, hipStream_t s) {
+    if (a.n_work == 0) return hipSuccess;
+    hipLaunchKernelGGL(surr_features_kernel, dim3((a.n_work + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_surr_apply(const SurrArgs &a, hipStream_t s) {
+    if (a.n_work == 0) return hipSuccess;
+    hipLaunchKernelGGL(surr_apply_kernel, dim3((a.n_work + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace portal
