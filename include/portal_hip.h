@@ -65,7 +65,8 @@ typedef struct {
     double tol_pop;          /* stop when max |d(sqrt(g) rho00)| / (sqrt(g) rho00) <= tol_pop */
     double tol_align;        /* ... and max |d(rho20/rho00)| <= tol_align */
     int surrogate;           /* 1: the trained MLP supplies the first iterate */
-    double fallback_tol;     /* per-node SE residual above which the MLP iterate is rejected */
+    double fallback_tol;     /* reserved (the surrogate iterate is rejected per node on the validity tests) */
+    int rebuild_paths;       /* 1: rebuild the inward paths even if the geometry is unchanged */
 } portal_iter_params;
 
 typedef struct {
@@ -138,6 +139,9 @@ int portal_inverse_rt(portal_ctx *ctx, const portal_se_params *p, double *J0, do
                       portal_stats *stats);
 int portal_se_solve(portal_ctx *ctx, const double *J0, const double *J2, double *pol_pops,
                     double *pol_rates, portal_stats *stats);
+
+/* pol_pops (node_num x 2L) of the last SE solve on the device, without recomputing. */
+int portal_get_pol_pops(portal_ctx *ctx, double *pol_pops);
 
 /* Lambda iteration (no reference counterpart).  lev_p_out (e_num x node_num, may be
  * NULL) receives the final level densities. */

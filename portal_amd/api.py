@@ -135,15 +135,25 @@ class _Core:
         return (pp, rr) if rates else pp
 
     def lambda_iterate(self, xth, nvel, dx, max_iter, tol_pop=1e-6, tol_align=1e-8, n_depth=4096,
-                       precision=L.PORTAL_FP64, surrogate=0, fallback_tol=1e-8):
+                       precision=L.PORTAL_FP64, surrogate=0, fallback_tol=1e-8, rebuild_paths=0, outputs=True):
         g, sp = self.grid, self.species
         p = self._se_params(xth, nvel, dx, n_depth, 0, precision)
-        it = L.IterParams(max_iter, tol_pop, tol_align, surrogate, fallback_tol)
+        it = L.IterParams(max_iter, tol_pop, tol_align, surrogate, fallback_tol, rebuild_paths)
+        st = L.Stats()
+        if not outputs:
+            self._check(self.lib.portal_lambda_iterate(self.ctx, C.byref(p), C.byref(it), None, None, C.byref(st)))
+            return st.as_dict()
         pp = np.zeros((g.node_num, 2 * sp.e_num), order="F")
         lev = np.zeros((sp.e_num, g.node_num), order="F")
-        st = L.Stats()
         self._check(self.lib.portal_lambda_iterate(self.ctx, C.byref(p), C.byref(it), _d(pp), _d(lev), C.byref(st)))
         return pp, lev, st.as_dict()
+
+    def get_pol_pops(self):
+        """pol_pops of the last SE solve (N x 2L) without recomputing."""
+        g, sp = self.grid, self.species
+        pp = np.zeros((g.node_num, 2 * sp.e_num), order="F")
+        self._check(self.lib.portal_get_pol_pops(self.ctx, _d(pp)))
+        return pp
 
     def image_pass(self, pol_pops, lines, th_inc, ph_az, nvel, vel_out, im_pix, im_size, dx, alpha=0.15,
                    n_depth_rt=4096, precision=L.PORTAL_FP64, bvec_all=None, with_dust=False):

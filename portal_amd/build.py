@@ -58,7 +58,26 @@ def build(verbose: bool = False) -> str:
     if p.returncode:
         raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
     os.replace(tmp, LIB)
+    build_fortran()
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+DEMO = os.path.join(HERE, "portal_f90_demo")
+
+
+def build_fortran() -> str | None:
+    """The ISO_C_BINDING module and the Fortran demo caller (when a Fortran compiler is present)."""
+    if not os.path.exists(FLANG):
+        return None
+    os.makedirs(OBJ, exist_ok=True)
+    mod_o = os.path.join(OBJ, "portal_hip_mod.o")
+    subprocess.run([FLANG, "-O2", "-fPIC", "-c", os.path.join(FDIR, "portal_hip_mod.f90"), "-J", OBJ, "-o", mod_o],
+                   check=True, capture_output=True)
+    subprocess.run([FLANG, "-O2", "-I", OBJ, os.path.join(FDIR, "portal_f90_demo.f90"), mod_o, "-L", HERE,
+                    "-lportal_hip", "-Wl,-rpath,$ORIGIN", "-o", DEMO], check=True, capture_output=True)
+    return DEMO
 
 
 if __name__ == "__main__":

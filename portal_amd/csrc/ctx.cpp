@@ -712,6 +712,16 @@ static int surrogate_step(Ctx &c, int *n_fallback) {
     return PORTAL_OK;
 }
 
+int portal_get_pol_pops(portal_ctx *x, double *pol_pops) {
+    if (!x || !pol_pops) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    int rc = ensure_node_buffers(c);
+    if (rc) return rc;
+    HIP_TRY(&c, hipMemcpy(pol_pops, c.d_pol, sizeof(double) * c.g.N * 2 * c.sp.e_num, hipMemcpyDeviceToHost));
+    return PORTAL_OK;
+}
+
 int portal_surrogate_features(portal_ctx *x, int n_in, float *feats, int *nodes, int *n_work) {
     if (!x || !feats || !n_work) return PORTAL_ERR_ARG;
     Ctx &c = x->c;
@@ -754,6 +764,7 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     if (it->max_iter <= 0) return fail(c, PORTAL_ERR_ARG, "max_iter must be positive");
     if (it->surrogate && !c.mlp) return fail(c, PORTAL_ERR_STATE, "no surrogate network (portal_mlp_create)");
     if (st) std::memset(st, 0, sizeof(*st));
+    if (it->rebuild_paths) c.paths.valid = false;
     const size_t L = c.sp.e_num, N = c.g.N;
     double *d_prev = nullptr;
     unsigned long long *d_max = nullptr;

@@ -28,7 +28,7 @@ EXPORTS = ["portal_ctx_create", "portal_ctx_destroy", "portal_last_error", "port
            "portal_se_pass", "portal_inverse_rt", "portal_se_solve", "portal_lambda_iterate",
            "portal_image_pass", "portal_mlp_create", "portal_mlp_destroy", "portal_mlp_num_params",
            "portal_mlp_set_params", "portal_mlp_get_params", "portal_mlp_forward", "portal_mlp_train_step",
-           "portal_mlp_train", "portal_surrogate_features"]
+           "portal_mlp_train", "portal_surrogate_features", "portal_get_pol_pops"]
 
 
 class SeParams(C.Structure):
@@ -39,7 +39,7 @@ class SeParams(C.Structure):
 
 class IterParams(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol_pop", C.c_double), ("tol_align", C.c_double),
-                ("surrogate", C.c_int), ("fallback_tol", C.c_double)]
+                ("surrogate", C.c_int), ("fallback_tol", C.c_double), ("rebuild_paths", C.c_int)]
 
 
 class ImgParams(C.Structure):

@@ -1,0 +1,143 @@
+"""Training data, training and loading of the surrogate network.
+
+The network (portal_mlp_*, BF16-MFMA kernels in csrc/mlp.hip) maps a node's local
+state after the first inward pass -- log n_H2, log T_kin and, per line,
+log(J^0_0/B_nu(2.73 K)) and 100 J^2_0/J^0_0 -- to the populations and alignment of
+the converged lambda iteration: log10(sqrt(2J+1) rho^0_0) per level and
+1e3 rho^2_0/rho^0_0 per level with J > 0.  The method produces its own training
+data: seeded clouds are iterated to convergence with the mixed-precision path, and
+the first-pass features are paired with the converged solution.
+
+    python -m portal_amd.surrogate --levels 10 --clouds 8 --nodes 200000
+
+writes portal_amd/surrogate_weights/<species>.npz, which ``load_into`` finds again.
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "surrogate_weights")
+WIDTH, DEPTH = 256, 4
+LOG_FLOOR = -12.0
+
+
+def species_key(sp) -> str:
+    h = hashlib.sha1()
+    for a in (sp.j_lev, sp.r_trans, sp.c_trans):
+        h.update(np.ascontiguousarray(a, dtype=np.int32).tobytes())
+    for a in (sp.e_levs_hz, sp.t_Aji, sp.nu_hz, sp.t_Cji.ravel(order="F"), sp.col_temps):
+        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    h.update(str(sp.kmax).encode())
+    return f"L{sp.e_num}_R{sp.r_num}_{h.hexdigest()[:12]}"
+
+
+def dims(sp):
+    n_in = 2 + 2 * sp.r_num
+    n_out = sp.e_num + int((np.asarray(sp.j_lev) > 0).sum())
+    return n_in, n_out
+
+
+def targets(pol_pops, nodes, sp):
+    """Network targets of the given 1-based nodes from a pol_pops table (N x 2L)."""
+    rows = np.asarray(nodes) - 1
+    L = sp.e_num
+    g = np.sqrt(2.0 * np.asarray(sp.j_lev) + 1.0)
+    r0 = pol_pops[rows][:, 0::2]
+    r2 = pol_pops[rows][:, 1::2]
+    pops = np.clip(r0 * g[None, :], 10.0 ** LOG_FLOOR, None)
+    out = [np.log10(pops)]
+    cols = [i for i in range(L) if sp.j_lev[i] > 0]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rat = np.where(r0[:, cols] != 0, r2[:, cols] / r0[:, cols], 0.0) * 1e3
+    out.append(rat)
+    return np.concatenate(out, axis=1).astype(np.float32)
+
+
+def weights_path(sp) -> str:
+    return os.path.join(WEIGHTS, species_key(sp) + ".npz")
+
+
+def load_into(pt, sp) -> bool:
+    """Create the network on the context and load trained weights; False when none exist."""
+    path = weights_path(sp)
+    if not os.path.exists(path):
+        return False
+    z = np.load(path, allow_pickle=False)
+    n_in, n_out = dims(sp)
+    pt.mlp_create(n_in, int(z["width"]), int(z["depth"]), n_out, seed=0)
+    pt.mlp_set_params(z["params"], z["norm"])
+    return True
+
+
+def make_dataset(pt, sp, g, xth, nvel, dx, max_iter=60, tol=1e-6):
+    """(features, targets, iterations) of one cloud: first-pass features, converged solution."""
+    from .lib import PORTAL_MIXED
+    pt.upload_grid(g)
+    pt.set_levpop(g.lev_p)
+    pt.inverse_rt(xth, nvel, dx, precision=PORTAL_MIXED)
+    feats, nodes = pt.surrogate_features()
+    pt.set_levpop(g.lev_p)
+    st = pt.lambda_iterate(xth, nvel, dx, max_iter, tol_pop=tol, tol_align=1e-9, precision=PORTAL_MIXED,
+                           outputs=False)
+    pp = pt.get_pol_pops()
+    return feats, targets(pp, nodes, sp), st["iterations"]
+
+
+def train(levels=10, clouds=8, nodes=200000, nth=9, nph=32, nvel=64, epochs=40, batch=4096, lr=1e-3,
+          seed0=101, out=None, log=print):
+    from . import cloud
+    from .api import Portal
+    sp = cloud.co_like_rotor(levels)
+    xth = cloud.legendre_all(nth, nph)
+    pt = Portal(0)
+    pt.upload_species(sp)
+    X, Y = [], []
+    for k in range(clouds):
+        t = time.time()
+        rng = np.random.default_rng(seed0 + k)
+        # the production recipe with the density slope and field direction varied per cloud
+        g = cloud.make_cloud(sp, nodes, seed=seed0 + k, n_exp=float(rng.uniform(-1.7, -1.3)))
+        f, y, it = make_dataset(pt, sp, g, xth, nvel, cloud.DX_INWARD)
+        X.append(f)
+        Y.append(y)
+        log(f"cloud {k}: {len(f)} samples, converged in {it} iterations, {time.time() - t:.1f}s")
+    X = np.concatenate(X)
+    Y = np.concatenate(Y)
+    n_in, n_out = dims(sp)
+    norm = np.concatenate([X.mean(0), np.maximum(X.std(0), 1e-3), Y.mean(0), np.maximum(Y.std(0), 1e-3)])
+    pt.mlp_create(n_in, WIDTH, DEPTH, n_out, seed=seed0)
+    pt.mlp_set_params(None, norm.astype(np.float32))
+    perm = np.random.default_rng(seed0).permutation(len(X))
+    X, Y = X[perm], Y[perm]
+    hist1, ms1 = pt.mlp_train(X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed0)
+    hist2, ms2 = pt.mlp_train(X, Y, epochs=max(1, epochs // 4), batch=batch, lr=lr / 5, seed=seed0 + 1)
+    log(f"trained on {len(X)} samples: loss {hist1[0]:.3e} -> {hist2[-1]:.3e} in {(ms1 + ms2) / 1e3:.1f}s")
+    params, nm = pt.mlp_get_params()
+    os.makedirs(WEIGHTS, exist_ok=True)
+    path = out or weights_path(sp)
+    np.savez_compressed(path, params=params, norm=nm, width=WIDTH, depth=DEPTH, samples=len(X),
+                        loss=np.concatenate([hist1, hist2]), key=species_key(sp))
+    pt.close()
+    return path
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--levels", type=int, default=10)
+    ap.add_argument("--clouds", type=int, default=8)
+    ap.add_argument("--nodes", type=int, default=200000)
+    ap.add_argument("--nth", type=int, default=9)
+    ap.add_argument("--nph", type=int, default=32)
+    ap.add_argument("--nvel", type=int, default=64)
+    ap.add_argument("--epochs", type=int, default=40)
+    a = ap.parse_args()
+    sys.path.insert(0, os.path.dirname(HERE))
+    print(train(a.levels, a.clouds, a.nodes, a.nth, a.nph, a.nvel, a.epochs,
+                log=lambda *x: print(*x, flush=True)))
