@@ -100,6 +100,9 @@ typedef struct {
     int fallbacks;           /* nodes where the surrogate iterate was rejected */
     int lu_failures;
     double max_rel_change;   /* last iteration's population change */
+    int rt_variant;          /* inward-integration kernel chosen by the autotuner */
+    int rt_waves;            /* its waves per workgroup */
+    int max_path;            /* longest inward path (cells) */
 } portal_stats;
 
 /* Context -------------------------------------------------------------------- */

@@ -26,6 +26,7 @@ struct PathSet {
     int *counts = nullptr;          // device, n_work*D
     long long total = 0;
     int n_work = 0, D = 0;
+    int max_nc = 0;
     size_t cap_entries = 0;
     // key of the geometry the paths were built for
     std::vector<double> xth, gvec, bvec;
@@ -111,6 +112,10 @@ struct Ctx {
     size_t img_bytes = 0;
     // MLP surrogate state (owned by mlp.hip)
     void *mlp = nullptr;
+    // autotuned inward-integration launch (per precision): variant, waves per block
+    int tuned_key[2] = {0, 0};
+    int tuned_variant[2] = {0, 0}, tuned_waves[2] = {4, 4};
+    double tune_ms[2][4] = {{0}};
 };
 
 // error flags (bits in d_flags[0])
@@ -176,6 +181,8 @@ struct RtArgs {
     double dl_scale;        // tx*dx
     double *J0, *J2;        // R x N
     int waves;              // waves per block
+    int max_nc;             // longest stored path (LDS staging capacity)
+    int variant;            // 0: global-memory kernel, 1: LDS-staged kernel
 };
 hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s);
 hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s);

@@ -151,9 +151,11 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
     a.counts = P.counts;
     a.flags = c.d_flags;
     HIP_TRY(&c, launch_walk(a, false, c.stream));
-    int flags = 0;
-    HIP_TRY(&c, hipMemcpyAsync(&flags, c.d_flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    int fl[2] = {0, 0};
+    HIP_TRY(&c, hipMemcpyAsync(fl, c.d_flags, sizeof(fl), hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(&c, hipStreamSynchronize(c.stream));
+    const int flags = fl[0];
+    P.max_nc = fl[1];
     if (flags & FLAG_DEPTH) {
         dfree(d_xth);
         dfree(d_gvec);
@@ -184,12 +186,39 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
     return PORTAL_OK;
 }
 
-int choose_waves(const Ctx &c) {
-    // enough waves in flight: ~16 per CU over 256 CUs
-    const long long nw = (long long)c.g.work.size();
-    if (nw >= 4096 * 4) return 4;
-    if (nw >= 4096 * 2) return 4;
-    return 4;
+// Autotuner of the inward-integration launch: on the first pass of a geometry the
+// candidate (kernel variant, waves per workgroup) pairs are timed on a slice of the
+// work list and the fastest is kept for every later pass with the same shape.
+int autotune_rt(Ctx &c, RtArgs ra, bool mixed) {
+    const int pi = mixed ? 1 : 0;
+    const int key = (int)((c.paths.total % 1000003) ^ (ra.C * 131 + ra.R * 7 + ra.D));
+    if (c.tuned_key[pi] == key) return PORTAL_OK;
+    struct Cand { int variant, waves; } cands[4] = {{0, 4}, {0, 2}, {1, 4}, {1, 2}};
+    const int sub = std::min(ra.n_work, 2048);
+    ra.n_work = sub;
+    double best = 1e30;
+    int bv = 0, bw = 4;
+    for (int k = 0; k < 4; k++) {
+        ra.variant = cands[k].variant;
+        ra.waves = cands[k].waves;
+        Timer t(c.stream);
+        hipError_t e = mixed ? launch_rt_mixed(ra, c.stream) : launch_rt_f64(ra, c.stream);
+        const double ms = t.stop();
+        c.tune_ms[pi][k] = e == hipSuccess ? ms : -1.0;
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // an LDS budget the variant cannot meet: skip it
+            continue;
+        }
+        if (ms < best) {
+            best = ms;
+            bv = cands[k].variant;
+            bw = cands[k].waves;
+        }
+    }
+    c.tuned_key[pi] = key;
+    c.tuned_variant[pi] = bv;
+    c.tuned_waves[pi] = bw;
+    return PORTAL_OK;
 }
 
 int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
@@ -235,11 +264,14 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
     ra.dl_scale = c.g.tx * p->dx;
     ra.J0 = c.d_J0;
     ra.J2 = c.d_J2;
-    ra.waves = choose_waves(c);
+    ra.max_nc = (c.paths.max_nc + 7) & ~7;
     double *d_xth = nullptr;
     std::vector<double> xv(p->xth, p->xth + 3 * (size_t)p->n_sphere);
     HIP_TRY(&c, upload(d_xth, xv));
     ra.xth = d_xth;
+    autotune_rt(c, ra, mixed);
+    ra.variant = c.tuned_variant[mixed ? 1 : 0];
+    ra.waves = c.tuned_waves[mixed ? 1 : 0];
     Timer tr(c.stream);
     hipError_t e = mixed ? launch_rt_mixed(ra, c.stream) : launch_rt_f64(ra, c.stream);
     const double t_rt = tr.stop();
@@ -251,6 +283,9 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
         st->rt_kernel_avg_ms = t_rt;
         st->n_cross = c.paths.total;
         st->n_paths = (long long)c.paths.n_work * c.paths.D;
+        st->rt_variant = ra.variant;
+        st->rt_waves = ra.waves;
+        st->max_path = c.paths.max_nc;
     }
     return PORTAL_OK;
 }

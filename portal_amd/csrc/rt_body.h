@@ -1,0 +1,248 @@
+// Inward ray integration over stored paths, shared by the FP64 (rt_f64.hip) and
+// mixed-precision (rt_mixed.hip) builds.  Restates ray_in (inverse_rt.f90:217-344)
+// and the J accumulation of inverse_rt (inverse_rt.f90:126-129).
+//
+// One workgroup per node; wave w takes rays w, w+W, ...  Per ray the wave first
+// stages the path into its own LDS slice: per cell the projected velocity, 1/b and
+// the segment length, and per (cell, line) the line coefficients K (tau per unit
+// profile and length) and S (source function).  Lanes then own (line, channel)
+// pairs p = p0 + lane + 64 s and integrate them front to back,
+//   I_out = sum_i S_i (1 - e^-tau_i) prod_{j<i} e^-tau_j + I_in prod_j e^-tau_j,
+// the reference's back-to-front recurrence summed in the other order, with its skip
+// of segments whose tau is not > 1e-30 (NaN included) and no skip for the node's own
+// cell.  GROUP slots are integrated together so that their exponential chains
+// overlap; the cell loop is wave-uniform (cell data are LDS broadcasts) and stops
+// when every lane's transmission is exactly zero.  Per pair the lane accumulates
+// sum_rays w I and sum_rays w c_in I; the channel sums with the profile weight
+// phi0 d_v and the sum over waves are done once per node in a fixed order.
+#pragma once
+#include "common.h"
+
+namespace portal {
+
+__device__ __forceinline__ int xcd_block(int bid, int nblocks) {
+    // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous
+    // range of (Morton-ordered) nodes so that its L2 holds one region of the cloud.
+    const int per = (nblocks + 7) >> 3;
+    const int xcd = bid & 7, slot = bid >> 3;
+    const int full = nblocks - (per - 1) * 8;
+    return xcd < full ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
+}
+
+template <typename Real>
+struct Arith;
+
+template <>
+struct Arith<double> {
+    typedef double2 R2;
+    static constexpr bool kahan = false;
+    __device__ static double ex(double x) { return exp(x); }
+    __device__ static double one_minus(double tau, double et) { return 1.0 - et; }
+    __device__ static double thr() { return 1e-30; }
+    __device__ static R2 ks(const RtArgs &a, size_t i) { return *reinterpret_cast<const double2 *>(a.KS + 2 * i); }
+};
+
+template <>
+struct Arith<float> {
+    typedef float2 R2;
+    static constexpr bool kahan = true;
+    __device__ static float ex(float x) { return __expf(x); }
+    // 1 - exp(-tau) without cancellation for small tau
+    __device__ static float one_minus(float tau, float et) {
+        return tau < 1e-2f ? tau * (1.0f - tau * (0.5f - tau * (0.16666667f - tau * 0.041666668f))) : 1.0f - et;
+    }
+    __device__ static float thr() { return 1e-30f; }
+    __device__ static R2 ks(const RtArgs &a, size_t i) { return *reinterpret_cast<const float2 *>(a.KS32 + 2 * i); }
+};
+
+template <typename Real>
+struct __attribute__((aligned(8))) CellRec {
+    Real vray, inv_b, dl, pad;
+};
+
+template <typename Real>
+__host__ __device__ constexpr size_t stage_bytes_per_cell(int R) {
+    return sizeof(CellRec<Real>) + (size_t)R * sizeof(typename Arith<Real>::R2);
+}
+
+template <typename Real, int MAXS, int GROUP>
+__global__ void __launch_bounds__(256) rt_staged_kernel(RtArgs a, int cap) {
+    typedef Arith<Real> A;
+    typedef typename A::R2 R2;
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int W = blockDim.x >> 6;
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const int node = a.work[bid];
+    const double bd = a.velb[node].w;
+    const Real b = (Real)bd;
+    const Real inv_bn = (Real)(1.0 / bd);
+    const int C = a.C, nvel = a.nvel, R = a.R;
+    const int npairs = R * C;
+    const Real dls = (Real)a.dl_scale;
+    const size_t wave_bytes = (size_t)cap * stage_bytes_per_cell<Real>(R);
+    CellRec<Real> *cr = reinterpret_cast<CellRec<Real> *>(smem + wave * wave_bytes);
+    R2 *ks = reinterpret_cast<R2 *>(cr + cap);
+    // after the ray loop the staging area is reused for the reduction
+    double *red0 = reinterpret_cast<double *>(smem);
+    double *red2 = red0 + 64 * MAXS;
+    double *jacc = red2 + 64 * MAXS;
+
+    double jl0 = 0.0, jl2 = 0.0;  // thread t < R accumulates line t
+    for (int p0 = 0; p0 < npairs; p0 += 64 * MAXS) {
+        Real S0[MAXS], C0[MAXS], S2[MAXS], C2[MAXS];
+#pragma unroll
+        for (int s = 0; s < MAXS; s++) S0[s] = C0[s] = S2[s] = C2[s] = (Real)0;
+        for (int r = wave; r < a.D; r += W) {
+            const Real mu = (Real)a.xth[3 * r], w = (Real)a.xth[3 * r + 2];
+            const Real wc = w * (((Real)3 * (mu * mu) - (Real)1) / (Real)2);
+            const long long off = a.offsets[(long long)bid * a.D + r];
+            const int nc = (int)(a.offsets[(long long)bid * a.D + r + 1] - off);
+            const PathEntry *P = a.paths + off;
+            for (int i = lane; i < nc; i += 64) {
+                const PathEntry e = P[i];
+                CellRec<Real> c;
+                c.vray = (Real)e.vray;
+                c.inv_b = (Real)a.inv_b[e.cell];
+                c.dl = (Real)e.len * dls;
+                c.pad = (Real)0;
+                cr[i] = c;
+                for (int l = 0; l < R; l++) ks[i * R + l] = A::ks(a, (size_t)e.cell * R + l);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const Real vel_node = cr[0].vray;
+            const Real dl0 = cr[0].dl;
+#pragma unroll
+            for (int g0 = 0; g0 < MAXS; g0 += GROUP) {
+                if (p0 + 64 * g0 >= npairs) break;  // wave-uniform
+                Real Tr[GROUP], Em[GROUP], v0s[GROUP];
+                int lg[GROUP];
+                bool on[GROUP];
+#pragma unroll
+                for (int q = 0; q < GROUP; q++) {
+                    const int p = p0 + lane + 64 * (g0 + q);
+                    on[q] = p < npairs;
+                    const int l = on[q] ? p / C : 0;
+                    const int nv = on[q] ? p - l * C - nvel : 0;
+                    lg[q] = l;
+                    const Real v0 = ((Real)2 * b) * (Real)nv / (Real)nvel;
+                    const Real aa = v0 * inv_bn;
+                    const Real e0 = A::ex(-(aa * aa));
+                    const R2 k0 = ks[l];  // cell 0 is the node's own
+                    const Real tau0 = k0.x * e0 * dl0;
+                    Tr[q] = A::ex(-tau0);
+                    Em[q] = k0.y * A::one_minus(tau0, Tr[q]);
+                    v0s[q] = v0 + vel_node;
+                    if (!on[q]) Tr[q] = (Real)0;
+                }
+                for (int i = 1; i < nc; i++) {
+                    bool live = false;
+#pragma unroll
+                    for (int q = 0; q < GROUP; q++) live = live || (Tr[q] != (Real)0);
+                    if (!__any(live)) break;
+                    const CellRec<Real> c = cr[i];
+#pragma unroll
+                    for (int q = 0; q < GROUP; q++) {
+                        const R2 k = ks[i * R + lg[q]];
+                        const Real u = (v0s[q] - c.vray) * c.inv_b;
+                        const Real tau = k.x * A::ex(-(u * u)) * c.dl;
+                        if (tau > A::thr() && Tr[q] != (Real)0) {
+                            const Real et = A::ex(-tau);
+                            Em[q] = Em[q] + Tr[q] * k.y * A::one_minus(tau, et);
+                            Tr[q] = Tr[q] * et;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < GROUP; q++) {
+                    if (!on[q]) continue;
+                    const Real Iout = Em[q] + Tr[q] * (Real)a.I_in[lg[q]];
+                    const int s = g0 + q;
+                    if (A::kahan) {
+                        Real y = w * Iout - C0[s];
+                        Real t = S0[s] + y;
+                        C0[s] = (t - S0[s]) - y;
+                        S0[s] = t;
+                        y = wc * Iout - C2[s];
+                        t = S2[s] + y;
+                        C2[s] = (t - S2[s]) - y;
+                        S2[s] = t;
+                    } else {
+                        S0[s] = S0[s] + w * Iout;
+                        S2[s] = S2[s] + wc * Iout;
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        __syncthreads();  // every wave is done with its staging area
+        for (int ww = 0; ww < W; ww++) {
+            if (wave == ww) {
+#pragma unroll
+                for (int s = 0; s < MAXS; s++) {
+                    const int q = lane + 64 * s;
+                    const double v0 = (double)S0[s] - (double)C0[s];
+                    const double v2 = (double)S2[s] - (double)C2[s];
+                    red0[q] = (ww == 0 ? 0.0 : red0[q]) + v0;
+                    red2[q] = (ww == 0 ? 0.0 : red2[q]) + v2;
+                }
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x < R) {
+            const int l = threadIdx.x;
+            const double spi = 1.7724538509055160273;
+            const double d_v = 2.0 * bd / nvel;
+            for (int c = 0; c < C; c++) {
+                const int p = l * C + c;
+                if (p < p0 || p >= p0 + 64 * MAXS) continue;
+                const double v0 = (2.0 * bd) * (c - nvel) / nvel;
+                const double aa = v0 / bd;
+                const double phi0 = (1.0 / (spi * bd)) * exp(-(aa * aa));
+                jl0 = jl0 + phi0 * d_v * red0[p - p0];
+                jl2 = jl2 + phi0 * d_v * red2[p - p0];
+            }
+        }
+        __syncthreads();  // the staging area is reused by the next pair chunk
+    }
+    (void)jacc;
+    if (threadIdx.x < R) {
+        a.J0[(size_t)node * R + threadIdx.x] = jl0;
+        a.J2[(size_t)node * R + threadIdx.x] = jl2 / 1.4142135623730951;
+    }
+}
+
+// LDS bytes of one workgroup for path capacity `cap`
+template <typename Real, int MAXS>
+size_t staged_lds_bytes(int cap, int R, int W) {
+    const size_t stage = (size_t)W * cap * stage_bytes_per_cell<Real>(R);
+    const size_t red = sizeof(double) * (2 * 64 * MAXS + 128);
+    return stage > red ? stage : red;
+}
+
+template <typename Real>
+hipError_t launch_staged(const RtArgs &a, int cap, hipStream_t s) {
+    const int W = a.waves;
+    const int npairs = a.R * a.C;
+    if (npairs <= 64 * 4) {
+        const size_t lds = staged_lds_bytes<Real, 4>(cap, a.R, W);
+        if (lds > 160 * 1024) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((rt_staged_kernel<Real, 4, 4>), dim3(a.n_work), dim3(64 * W), lds, s, a, cap);
+    } else if (npairs <= 64 * 12) {
+        const size_t lds = staged_lds_bytes<Real, 12>(cap, a.R, W);
+        if (lds > 160 * 1024) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((rt_staged_kernel<Real, 12, 4>), dim3(a.n_work), dim3(64 * W), lds, s, a, cap);
+    } else {
+        const size_t lds = staged_lds_bytes<Real, 20>(cap, a.R, W);
+        if (lds > 160 * 1024) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((rt_staged_kernel<Real, 20, 4>), dim3(a.n_work), dim3(64 * W), lds, s, a, cap);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace portal

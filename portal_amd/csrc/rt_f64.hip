@@ -14,6 +14,7 @@
 // weight phi0*d_v and the sum over waves are done once per node in a fixed order,
 // so J is deterministic.  Compiled with -ffp-contract=off.
 #include "common.h"
+#include "rt_body.h"
 
 namespace portal {
 
@@ -152,6 +153,7 @@ __global__ void __launch_bounds__(256) rt_f64_kernel(RtArgs a) {
 hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
     if (a.R > 64) return hipErrorInvalidValue;
+    if (a.variant == 1) return launch_staged<double>(a, a.max_nc, s);
     const int W = a.waves;
     const int npairs = a.R * a.C;
     if (npairs <= 64 * 4)

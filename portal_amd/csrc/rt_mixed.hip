@@ -9,6 +9,7 @@
 // h nu/4pi) underflow FP32.  Geometry (the stored paths) is the FP64 walk's.
 // The channel sum and the sum over waves are done in FP64.
 #include "common.h"
+#include "rt_body.h"
 
 namespace portal {
 
@@ -130,6 +131,7 @@ __global__ void __launch_bounds__(256) rt_mixed_kernel(RtArgs a) {
 hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
     if (a.R > 64) return hipErrorInvalidValue;
+    if (a.variant == 1) return launch_staged<float>(a, a.max_nc, s);
     const int W = a.waves;
     const int npairs = a.R * a.C;
     if (npairs <= 64 * 4)

@@ -121,7 +121,10 @@ __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
         cur = next;
         if (next < 0) break;
     }
-    if (!WRITE) a.counts[id] = nc > a.n_depth ? 0 : nc;
+    if (!WRITE) {
+        a.counts[id] = nc > a.n_depth ? 0 : nc;
+        atomicMax(a.flags + 1, nc);
+    }
 }
 
 hipError_t launch_walk(const WalkArgs &a, bool write, hipStream_t s) {
