@@ -151,6 +151,17 @@ int portal_get_pol_pops(portal_ctx *ctx, double *pol_pops);
 int portal_lambda_iterate(portal_ctx *ctx, const portal_se_params *p, const portal_iter_params *it,
                           double *pol_pops, double *lev_p_out, portal_stats *stats);
 
+/* Batch of independent models (no reference counterpart): one context per model,
+ * all on one device, all with the same precision.  Every iteration is one replay of
+ * a captured hipGraph holding each model's line preparation, ONE persistent inward
+ * integration over all (model, node) items ordered by cost, ONE persistent SE solve
+ * over all items and each model's closure.  Iterates until every model meets the
+ * tolerances (or max_iter); per_model (n, may be NULL) gets each model's iteration
+ * count at convergence, total the device time of the whole batch.  Results stay in
+ * each context (portal_get_pol_pops / portal_get_levpop). */
+int portal_batch_lambda_iterate(portal_ctx **ctxs, int n_models, const portal_se_params *params,
+                                const portal_iter_params *it, portal_stats *per_model, portal_stats *total);
+
 /* Image pass for one inclination.  pol_pops is the rescaled table of
  * para_finish.f90:222-240 (node_num x 2L).  Outputs are n_rt x nlevels x (2nvel+1)
  * (all_I(:,:,inc,:) of para_finish.f90:356-359); dust (4 such cubes) may be NULL. */

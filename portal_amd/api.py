@@ -242,3 +242,19 @@ class SurrogateMixin:
 
 class Portal(SurrogateMixin, _Core):
     """One device context: grid/species uploads, SE/image passes, lambda iteration, surrogate network."""
+
+
+def batch_lambda_iterate(portals, xths, nvels, dx, max_iter, tol_pop=1e-6, tol_align=1e-8, n_depth=4096,
+                         precision=L.PORTAL_MIXED, rebuild_paths=0):
+    """portal_batch_lambda_iterate over several Portal contexts (one model each)."""
+    n = len(portals)
+    lib = portals[0].lib
+    arr = (C.c_void_p * n)(*[p.ctx.value for p in portals])
+    ps = (L.SeParams * n)(*[p._se_params(x, nv, dx, n_depth, 0, precision) for p, x, nv in zip(portals, xths, nvels)])
+    it = L.IterParams(max_iter, tol_pop, tol_align, 0, 0.0, rebuild_paths)
+    per = (L.Stats * n)()
+    tot = L.Stats()
+    rc = lib.portal_batch_lambda_iterate(arr, n, ps, C.byref(it), per, C.byref(tot))
+    if rc:
+        raise PortalError(rc, lib.portal_last_error(portals[0].ctx).decode())
+    return [s.as_dict() for s in per], tot.as_dict()

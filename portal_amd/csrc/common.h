@@ -227,7 +227,15 @@ int mlp_dims(const Ctx &c, int *n_in, int *n_out);
 
 hipError_t launch_lambda_update(int n_work, int N, int L, const int *work, const double *ab, const double *h2,
                                 const int *j_lev, const double *pol, const double *pol_prev, double *levp,
-                                unsigned long long *maxes, hipStream_t s);
+                                unsigned long long *maxes, hipStream_t s, double *pol_save = nullptr);
+// batched (persistent) launches over (model, node) items
+hipError_t launch_rt_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,
+                                int cap, int R_max, int waves, int n_blocks, hipStream_t s);
+hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd_max,
+                                int nt_max, int n_blocks, hipStream_t s);
+size_t se_lds_bytes(int c_dim, int n_trans);
+size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves);
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 
 }  // namespace portal
 

@@ -747,6 +747,14 @@ static int surrogate_step(Ctx &c, int *n_fallback) {
     return PORTAL_OK;
 }
 
+// the autotuner's timings of the inward-integration candidates (global 4, global 2,
+// staged 4, staged 2 waves) for one precision; -1 for a candidate that could not run
+int portal_debug_tune_ms(const portal_ctx *x, int precision, double *out4) {
+    if (!x || !out4 || precision < 0 || precision > 1) return PORTAL_ERR_ARG;
+    for (int k = 0; k < 4; k++) out4[k] = x->c.tune_ms[precision][k];
+    return PORTAL_OK;
+}
+
 int portal_get_pol_pops(portal_ctx *x, double *pol_pops) {
     if (!x || !pol_pops) return PORTAL_ERR_ARG;
     Ctx &c = x->c;
@@ -1019,6 +1027,260 @@ int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *p
         st->t_rt_ms = ms_r;
         st->rt_kernel_avg_ms = ms_r;
         st->t_total_ms = ms_g + ms_r;
+    }
+    return PORTAL_OK;
+}
+
+
+// ---------------------------------------------------------------------------------
+// Batches of independent models (one context each, same device): every iteration is
+// one hipGraph replay of {reset, line preparation per model, ONE persistent inward
+// integration over all (model, node) items, ONE persistent SE solve over all items,
+// closure per model}.
+int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *params, const portal_iter_params *it,
+                                portal_stats *per_model, portal_stats *total) {
+    if (!xs || n <= 0 || !params || !it || it->max_iter <= 0) return PORTAL_ERR_ARG;
+    Ctx &c0 = xs[0]->c;
+    hipSetDevice(c0.device);
+    if (it->surrogate) return fail(c0, PORTAL_ERR_UNSUPPORTED, "the surrogate iterate is not available for batches");
+    const bool mixed = params[0].precision == PORTAL_MIXED;
+    for (int m = 0; m < n; m++) {
+        if (!xs[m] || xs[m]->c.device != c0.device) return fail(c0, PORTAL_ERR_ARG, "batch contexts must share one device");
+        if ((params[m].precision == PORTAL_MIXED) != mixed) return fail(c0, PORTAL_ERR_ARG, "mixed precisions in one batch");
+        Ctx &c = xs[m]->c;
+        int rc = check_se_params(c, &params[m]);
+        if (rc) return fail(c0, rc, "model " + std::to_string(m) + ": " + c.err);
+    }
+    if (total) std::memset(total, 0, sizeof(*total));
+    if (per_model) std::memset(per_model, 0, sizeof(portal_stats) * n);
+    hipStream_t s = c0.stream;
+    Timer ttot(s);
+    // paths of every model (each on its own stream, synchronised)
+    std::vector<RtArgs> rts(n);
+    std::vector<SeArgs> ses(n);
+    std::vector<std::pair<double, int2>> rt_items, se_items;
+    int cap = 8, R_max = 1, cd_max = 1, nt_max = 1;
+    for (int m = 0; m < n; m++) {
+        Ctx &c = xs[m]->c;
+        if (it->rebuild_paths) c.paths.valid = false;
+        portal_stats st{};
+        int rc = ensure_paths(c, &params[m], &st);
+        if (rc) return fail(c0, rc, "model " + std::to_string(m) + ": " + c.err);
+        if (total) total->t_walk_ms += st.t_walk_ms;
+        const Species &sp = c.sp;
+        const int D = params[m].n_sphere, nw = (int)c.g.work.size();
+        RtArgs &ra = rts[m];
+        ra = RtArgs{};
+        ra.n_work = nw;
+        ra.D = D;
+        ra.R = sp.r_num;
+        ra.nvel = params[m].nvel;
+        ra.C = 2 * params[m].nvel + 1;
+        ra.work = c.g.d_work;
+        ra.velb = c.g.d_velb;
+        ra.inv_b = c.g.d_inv_b;
+        ra.KS = c.d_KS;
+        ra.KS32 = c.d_KS32;
+        ra.offsets = c.paths.offsets;
+        ra.paths = c.paths.entries;
+        ra.I_in = sp.d_Iin;
+        ra.dl_scale = c.g.tx * params[m].dx;
+        ra.J0 = c.d_J0;
+        ra.J2 = c.d_J2;
+        ra.variant = 1;
+        cap = std::max(cap, (c.paths.max_nc + 7) & ~7);
+        R_max = std::max(R_max, sp.r_num);
+        cd_max = std::max(cd_max, sp.c_dim);
+        nt_max = std::max(nt_max, sp.n_trans);
+        SeArgs &a = ses[m];
+        a = SeArgs{};
+        a.n_work = nw;
+        a.N = c.g.N;
+        a.L = sp.e_num;
+        a.R = sp.r_num;
+        a.c_dim = sp.c_dim;
+        a.n_trans = sp.n_trans;
+        a.n_ent = (int)sp.se.ent_idx.size();
+        a.work = c.g.d_work;
+        a.J0 = c.d_J0;
+        a.J2 = c.d_J2;
+        a.tkin = c.g.d_tkin;
+        a.h2 = c.g.d_h2;
+        a.t_A = sp.d_tA;
+        a.t_B21 = sp.d_tB21;
+        a.t_nu = sp.d_tnu;
+        a.t_gul = sp.d_tgul;
+        a.t_rad = sp.d_trad;
+        a.t_col = sp.d_tcol;
+        a.t_Cji = sp.d_Cji;
+        a.col_temps = sp.d_coltemps;
+        a.c_temp = sp.c_temp;
+        a.c_par = sp.c_par;
+        a.c_num = sp.c_num;
+        a.ent_idx = sp.se.d_ent_idx;
+        a.ent_ptr = sp.se.d_ent_ptr;
+        a.term_q = sp.se.d_term_q;
+        a.term_class = sp.se.d_term_class;
+        a.term_coef = sp.se.d_term_coef;
+        a.count_array = sp.d_count;
+        a.j_lev = sp.d_jlev;
+        a.pol = c.d_pol;
+        a.rates = nullptr;
+        a.flags = c0.d_flags;
+        // estimated cost of each node: stored crossings x lines x channels
+        std::vector<long long> off((size_t)nw * D + 1);
+        HIP_TRY(&c0, hipMemcpy(off.data(), c.paths.offsets, sizeof(long long) * off.size(), hipMemcpyDeviceToHost));
+        for (int w = 0; w < nw; w++) {
+            const double cost = (double)(off[(size_t)(w + 1) * D] - off[(size_t)w * D]) * ra.R * ra.C;
+            rt_items.push_back({cost, make_int2(m, w)});
+            se_items.push_back({(double)sp.c_dim, make_int2(m, w)});
+        }
+        HIP_TRY(&c0, hipMemset(c.d_J0, 0, sizeof(double) * c.g.N * sp.r_num));
+        HIP_TRY(&c0, hipMemset(c.d_J2, 0, sizeof(double) * c.g.N * sp.r_num));
+        HIP_TRY(&c0, hipMemset(c.d_pol, 0, sizeof(double) * c.g.N * 2 * sp.e_num));
+    }
+    auto by_cost = [](const std::pair<double, int2> &a, const std::pair<double, int2> &b) { return a.first > b.first; };
+    std::stable_sort(rt_items.begin(), rt_items.end(), by_cost);
+    std::stable_sort(se_items.begin(), se_items.end(), by_cost);
+    std::vector<int2> ri(rt_items.size()), si(se_items.size());
+    for (size_t i = 0; i < ri.size(); i++) ri[i] = rt_items[i].second;
+    for (size_t i = 0; i < si.size(); i++) si[i] = se_items[i].second;
+    const int n_items = (int)ri.size();
+    RtArgs *d_rt = nullptr;
+    SeArgs *d_se = nullptr;
+    int2 *d_ri = nullptr, *d_si = nullptr;
+    int *d_cnt = nullptr;
+    unsigned long long *d_max = nullptr;
+    std::vector<double *> prev(n, nullptr);
+    HIP_TRY(&c0, upload(d_rt, rts));
+    HIP_TRY(&c0, upload(d_se, ses));
+    HIP_TRY(&c0, upload(d_ri, ri));
+    HIP_TRY(&c0, upload(d_si, si));
+    HIP_TRY(&c0, hipMalloc((void **)&d_cnt, sizeof(int) * 4));
+    HIP_TRY(&c0, hipMalloc((void **)&d_max, sizeof(unsigned long long) * 2 * n));
+    for (int m = 0; m < n; m++) {
+        Ctx &c = xs[m]->c;
+        HIP_TRY(&c0, hipMalloc((void **)&prev[m], sizeof(double) * c.g.N * 2 * c.sp.e_num));
+        HIP_TRY(&c0, hipMemset(prev[m], 0, sizeof(double) * c.g.N * 2 * c.sp.e_num));
+    }
+    HIP_TRY(&c0, hipMemset(c0.d_flags, 0, sizeof(int) * 4));
+    int n_cu = 256;
+    hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c0.device);
+    // autotune the persistent integration's waves per workgroup on a slice of the items
+    int best_w = 4;
+    double best_ms = 1e30;
+    for (int w : {2, 4}) {
+        const size_t lds = rt_persistent_lds(mixed, cap, R_max, w);
+        if (lds > 160 * 1024) continue;
+        const int bpc = std::max(1, std::min(8 / (w / 2 > 0 ? w / 2 : 1), (int)((160 * 1024) / lds)));
+        for (int m = 0; m < n; m++) {
+            Ctx &c = xs[m]->c;
+            PrepArgs pa{c.g.N, c.sp.r_num, c.sp.e_num, c.d_levp, c.g.d_velb, c.sp.d_l1, c.sp.d_l2, c.sp.d_c1B,
+                        c.sp.d_c2A, nullptr, c.sp.d_g1, c.sp.d_g2, c.sp.d_cnu_spi, mixed ? nullptr : c.d_KS,
+                        mixed ? c.d_KS32 : nullptr};
+            HIP_TRY(&c0, launch_line_prep(pa, s));
+        }
+        HIP_TRY(&c0, launch_fill_zero(d_cnt, sizeof(int) * 4, s));
+        Timer t(s);
+        hipError_t e = launch_rt_persistent(mixed, d_rt, d_ri, std::min(n_items, 8192), d_cnt, cap, R_max, w,
+                                            n_cu * bpc, s);
+        const double ms = t.stop();
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            continue;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_w = w;
+        }
+    }
+    const size_t rt_lds = rt_persistent_lds(mixed, cap, R_max, best_w);
+    if (rt_lds > 160 * 1024) return fail(c0, PORTAL_ERR_ARG, "batch paths too long for LDS staging");
+    const int rt_blocks = n_cu * std::max(1, std::min(8 / std::max(1, best_w / 2), (int)((160 * 1024) / rt_lds)));
+    const size_t se_lds = se_lds_bytes(cd_max, nt_max);
+    if (se_lds > 160 * 1024) return fail(c0, PORTAL_ERR_ARG, "SE system too large for one workgroup's LDS");
+    const int se_blocks = n_cu * std::max(1, std::min(16, (int)((160 * 1024) / se_lds)));
+    // capture one iteration
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIP_TRY(&c0, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    launch_fill_zero(d_cnt, sizeof(int) * 4, s);
+    launch_fill_zero(d_max, sizeof(unsigned long long) * 2 * n, s);
+    for (int m = 0; m < n; m++) {
+        Ctx &c = xs[m]->c;
+        PrepArgs pa{c.g.N, c.sp.r_num, c.sp.e_num, c.d_levp, c.g.d_velb, c.sp.d_l1, c.sp.d_l2, c.sp.d_c1B, c.sp.d_c2A,
+                    nullptr, c.sp.d_g1, c.sp.d_g2, c.sp.d_cnu_spi, mixed ? nullptr : c.d_KS, mixed ? c.d_KS32 : nullptr};
+        launch_line_prep(pa, s);
+    }
+    launch_rt_persistent(mixed, d_rt, d_ri, n_items, d_cnt, cap, R_max, best_w, rt_blocks, s);
+    launch_se_persistent(d_se, d_si, (int)si.size(), d_cnt + 1, cd_max, nt_max, se_blocks, s);
+    for (int m = 0; m < n; m++) {
+        Ctx &c = xs[m]->c;
+        launch_lambda_update((int)c.g.work.size(), c.g.N, c.sp.e_num, c.g.d_work, c.g.d_ab, c.g.d_h2, c.sp.d_jlev,
+                             c.d_pol, prev[m], c.d_levp, d_max + 2 * m, s, prev[m]);
+    }
+    hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (ce != hipSuccess) {
+        c0.err = std::string("graph capture failed: ") + hipGetErrorString(ce);
+        return PORTAL_ERR_DEVICE;
+    }
+    HIP_TRY(&c0, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    std::vector<unsigned long long> mx(2 * n);
+    std::vector<int> conv_at(n, 0);
+    std::vector<double> last(n, 0.0);
+    int iters = 0;
+    double t_iter = 0;
+    for (int k = 0; k < it->max_iter; k++) {
+        Timer ti(s);
+        HIP_TRY(&c0, hipGraphLaunch(exec, s));
+        HIP_TRY(&c0, hipMemcpyAsync(mx.data(), d_max, sizeof(unsigned long long) * 2 * n, hipMemcpyDeviceToHost, s));
+        t_iter += ti.stop();
+        iters = k + 1;
+        bool all = true;
+        for (int m = 0; m < n; m++) {
+            double dp, da;
+            std::memcpy(&dp, &mx[2 * m], 8);
+            std::memcpy(&da, &mx[2 * m + 1], 8);
+            last[m] = dp;
+            // the first pass has no previous alignment to compare with
+            if (k > 0 && !conv_at[m] && dp <= it->tol_pop && da <= it->tol_align) conv_at[m] = k + 1;
+            all = all && conv_at[m];
+        }
+        if (all) break;
+    }
+    int flags = 0;
+    HIP_TRY(&c0, hipMemcpy(&flags, c0.d_flags, sizeof(int), hipMemcpyDeviceToHost));
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+    hipFree(d_rt);
+    hipFree(d_se);
+    hipFree(d_ri);
+    hipFree(d_si);
+    hipFree(d_cnt);
+    hipFree(d_max);
+    for (auto *p : prev) hipFree(p);
+    const double ms = ttot.stop();
+    if (flags & FLAG_TEMP) return fail(c0, PORTAL_ERR_TEMP, "kinetic temperature above the collision-rate table");
+    if (flags & FLAG_LU) return fail(c0, PORTAL_ERR_LU, "Density-matrix calculation: failed (singular SE matrix)");
+    long long ncross = 0;
+    for (int m = 0; m < n; m++) {
+        Ctx &c = xs[m]->c;
+        ncross += c.paths.total;
+        if (per_model) {
+            per_model[m].iterations = conv_at[m] ? conv_at[m] : iters;
+            per_model[m].max_rel_change = last[m];
+            per_model[m].n_cross = c.paths.total;
+            per_model[m].n_paths = (long long)c.paths.n_work * c.paths.D;
+        }
+    }
+    if (total) {
+        total->t_total_ms = ms;
+        total->t_rt_ms = t_iter;
+        total->iterations = iters;
+        total->n_cross = ncross;
+        total->rt_waves = best_w;
+        total->rt_variant = 2;
+        total->max_path = cap;
     }
     return PORTAL_OK;
 }

@@ -19,6 +19,7 @@ struct LambdaArgs {
     const double *pol_prev;  // N x 2L, previous pass (or null)
     double *levp;            // L x N, updated in place
     unsigned long long *maxes;  // [0] pop change, [1] alignment change (double bits)
+    double *pol_save;        // when set, receives this pass's pol (the next pass's pol_prev)
 };
 
 __global__ void __launch_bounds__(256) lambda_update_kernel(LambdaArgs a) {
@@ -44,6 +45,10 @@ __global__ void __launch_bounds__(256) lambda_update_kernel(LambdaArgs a) {
         if (!(dpop == dpop)) dpop = 1.0;  // NaN counts as not converged
         if (!(dal == dal)) dal = 1.0;
         a.levp[(size_t)node * a.L + i] = nv;
+        if (a.pol_save) {
+            a.pol_save[node + N * (2 * i)] = r0;
+            a.pol_save[node + N * (2 * i + 1)] = r2;
+        }
     }
     // block max then one atomic per block (non-negative doubles order like their bits)
     for (int o = 32; o > 0; o >>= 1) {
@@ -58,8 +63,8 @@ __global__ void __launch_bounds__(256) lambda_update_kernel(LambdaArgs a) {
 
 hipError_t launch_lambda_update(int n_work, int N, int L, const int *work, const double *ab, const double *h2,
                                 const int *j_lev, const double *pol, const double *pol_prev, double *levp,
-                                unsigned long long *maxes, hipStream_t s) {
-    LambdaArgs a{n_work, N, L, work, ab, h2, j_lev, pol, pol_prev, levp, maxes};
+                                unsigned long long *maxes, hipStream_t s, double *pol_save) {
+    LambdaArgs a{n_work, N, L, work, ab, h2, j_lev, pol, pol_prev, levp, maxes, pol_save};
     const long long n = (long long)n_work * L;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(lambda_update_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
@@ -134,6 +139,23 @@ hipError_t launch_surr_features(const SurrArgs &a, hipStream_t s) {
 hipError_t launch_surr_apply(const SurrArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
     hipLaunchKernelGGL(surr_apply_kernel, dim3((a.n_work + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace portal
+
+namespace portal {
+
+// zero fill used in place of hipMemsetAsync inside captured graphs
+__global__ void fill_zero_kernel(unsigned int *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 4;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (unsigned int *)p, n);
     return hipGetLastError();
 }
 

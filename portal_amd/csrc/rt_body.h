@@ -66,14 +66,12 @@ __host__ __device__ constexpr size_t stage_bytes_per_cell(int R) {
 }
 
 template <typename Real, int MAXS, int GROUP>
-__global__ void __launch_bounds__(256) rt_staged_kernel(RtArgs a, int cap) {
+__device__ __forceinline__ void rt_staged_node(const RtArgs &a, const int bid, const int cap, char *smem) {
     typedef Arith<Real> A;
     typedef typename A::R2 R2;
-    extern __shared__ __attribute__((aligned(16))) char smem[];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int W = blockDim.x >> 6;
-    const int bid = xcd_block(blockIdx.x, gridDim.x);
     const int node = a.work[bid];
     const double bd = a.velb[node].w;
     const Real b = (Real)bd;
@@ -217,6 +215,31 @@ __global__ void __launch_bounds__(256) rt_staged_kernel(RtArgs a, int cap) {
     }
 }
 
+template <typename Real, int MAXS, int GROUP>
+__global__ void __launch_bounds__(256) rt_staged_kernel(RtArgs a, int cap) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    rt_staged_node<Real, MAXS, GROUP>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+}
+
+// Persistent form for batches of independent models: every workgroup takes the
+// next (model, node) item from a global counter until the list is exhausted.  The
+// list is ordered by decreasing estimated cost, so the long nodes start first.
+template <typename Real, int MAXS, int GROUP>
+__global__ void __launch_bounds__(256) rt_persistent_kernel(const RtArgs *models, const int2 *items, int n_items,
+                                                            int *counter, int cap) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const int2 mi = items[it];
+        rt_staged_node<Real, MAXS, GROUP>(models[mi.x], mi.y, cap, smem);
+    }
+}
+
 // LDS bytes of one workgroup for path capacity `cap`
 template <typename Real, int MAXS>
 size_t staged_lds_bytes(int cap, int R, int W) {
@@ -242,6 +265,16 @@ hipError_t launch_staged(const RtArgs &a, int cap, hipStream_t s) {
         if (lds > 160 * 1024) return hipErrorInvalidValue;
         hipLaunchKernelGGL((rt_staged_kernel<Real, 20, 4>), dim3(a.n_work), dim3(64 * W), lds, s, a, cap);
     }
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_persistent(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap, int R_max,
+                             int waves, int n_blocks, hipStream_t s) {
+    const size_t lds = staged_lds_bytes<Real, 20>(cap, R_max, waves);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((rt_persistent_kernel<Real, 20, 4>), dim3(n_blocks), dim3(64 * waves), lds, s, models, items,
+                       n_items, counter, cap);
     return hipGetLastError();
 }
 

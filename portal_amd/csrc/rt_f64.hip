@@ -165,4 +165,11 @@ hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_rt_persistent_f64(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
+                                    int R_max, int waves, int n_blocks, hipStream_t s) {
+    return launch_persistent<double>(models, items, n_items, counter, cap, R_max, waves, n_blocks, s);
+}
+
+size_t rt_persistent_lds_f64(int cap, int R_max, int waves) { return staged_lds_bytes<double, 20>(cap, R_max, waves); }
+
 }  // namespace portal

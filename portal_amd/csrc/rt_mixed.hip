@@ -143,4 +143,18 @@ hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_rt_persistent_f64(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
+                                    int R_max, int waves, int n_blocks, hipStream_t s);
+size_t rt_persistent_lds_f64(int cap, int R_max, int waves);
+
+hipError_t launch_rt_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,
+                                int cap, int R_max, int waves, int n_blocks, hipStream_t s) {
+    if (!mixed) return launch_rt_persistent_f64(models, items, n_items, counter, cap, R_max, waves, n_blocks, s);
+    return launch_persistent<float>(models, items, n_items, counter, cap, R_max, waves, n_blocks, s);
+}
+
+size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves) {
+    return mixed ? staged_lds_bytes<float, 20>(cap, R_max, waves) : rt_persistent_lds_f64(cap, R_max, waves);
+}
+
 }  // namespace portal

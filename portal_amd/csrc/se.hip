@@ -20,13 +20,10 @@ __device__ __forceinline__ void two_prod(double a, double b, double &p, double &
     e = fma(a, b, -p);
 }
 
-__global__ void __launch_bounds__(64) se_kernel(SeArgs a) {
+__device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *sm) {
     const int lane = threadIdx.x;
-    const int wi = blockIdx.x;
-    if (wi >= a.n_work) return;
     const int node = a.work[wi];
     const int cd = a.c_dim, nt = a.n_trans, L = a.L, R = a.R;
-    extern __shared__ double sm[];
     double *x = sm;                          // 7*nt scalars
     double *M = x + 7 * nt;                  // LU in place, column-major cd x cd
     double *M0 = M + cd * cd;                // assembled matrix (for the residual)
@@ -217,6 +214,41 @@ __global__ void __launch_bounds__(64) se_kernel(SeArgs a) {
             }
         }
     }
+}
+
+__global__ void __launch_bounds__(64) se_kernel(SeArgs a) {
+    extern __shared__ double sm[];
+    if ((int)blockIdx.x >= a.n_work) return;
+    se_node(a, blockIdx.x, sm);
+}
+
+// Persistent form over the (model, node) items of a batch of models.
+__global__ void __launch_bounds__(64) se_persistent_kernel(const SeArgs *models, const int2 *items, int n_items,
+                                                           int *counter) {
+    extern __shared__ double sm[];
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const int2 mi = items[it];
+        se_node(models[mi.x], mi.y, sm);
+        __syncthreads();
+    }
+}
+
+size_t se_lds_bytes(int c_dim, int n_trans) {
+    return sizeof(double) * (7 * (size_t)n_trans + 2 * (size_t)c_dim * c_dim + 2 * c_dim) + sizeof(int) * c_dim + 16;
+}
+
+hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd_max,
+                                int nt_max, int n_blocks, hipStream_t s) {
+    const size_t lds = se_lds_bytes(cd_max, nt_max);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(se_persistent_kernel, dim3(n_blocks), dim3(64), lds, s, models, items, n_items, counter);
+    return hipGetLastError();
 }
 
 hipError_t launch_se(const SeArgs &a, hipStream_t s) {

@@ -28,7 +28,8 @@ EXPORTS = ["portal_ctx_create", "portal_ctx_destroy", "portal_last_error", "port
            "portal_se_pass", "portal_inverse_rt", "portal_se_solve", "portal_lambda_iterate",
            "portal_image_pass", "portal_mlp_create", "portal_mlp_destroy", "portal_mlp_num_params",
            "portal_mlp_set_params", "portal_mlp_get_params", "portal_mlp_forward", "portal_mlp_train_step",
-           "portal_mlp_train", "portal_surrogate_features", "portal_get_pol_pops"]
+           "portal_mlp_train", "portal_surrogate_features", "portal_get_pol_pops",
+           "portal_batch_lambda_iterate"]
 
 
 class SeParams(C.Structure):
