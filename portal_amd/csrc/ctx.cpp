@@ -186,6 +186,17 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
     return PORTAL_OK;
 }
 
+// Every pointer the inward-integration kernels dereference must be set.
+int check_rt_args(Ctx &c, const RtArgs &a, bool mixed) {
+    const void *need[] = {a.work, a.velb, a.inv_b, a.xth, a.offsets, a.paths, a.I_in, a.J0, a.J2,
+                          mixed ? (const void *)a.KS32 : (const void *)a.KS};
+    for (const void *p : need)
+        if (!p) return fail(c, PORTAL_ERR_STATE, "inward integration launched with an unset buffer");
+    if (a.n_work < 0 || a.D <= 0 || a.R <= 0 || a.R > 64 || a.C <= 0 || a.max_nc < 0)
+        return fail(c, PORTAL_ERR_ARG, "inward integration launched with bad sizes");
+    return PORTAL_OK;
+}
+
 // Autotuner of the inward-integration launch: on the first pass of a geometry the
 // candidate (kernel variant, waves per workgroup) pairs are timed on a slice of the
 // work list and the fastest is kept for every later pass with the same shape.
@@ -269,6 +280,11 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
     std::vector<double> xv(p->xth, p->xth + 3 * (size_t)p->n_sphere);
     HIP_TRY(&c, upload(d_xth, xv));
     ra.xth = d_xth;
+    rc = check_rt_args(c, ra, mixed);
+    if (rc) {
+        dfree(d_xth);
+        return rc;
+    }
     autotune_rt(c, ra, mixed);
     ra.variant = c.tuned_variant[mixed ? 1 : 0];
     ra.waves = c.tuned_waves[mixed ? 1 : 0];
@@ -1059,13 +1075,20 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     std::vector<RtArgs> rts(n);
     std::vector<SeArgs> ses(n);
     std::vector<std::pair<double, int2>> rt_items, se_items;
+    std::vector<double *> d_xths(n, nullptr);
+    auto free_xths = [&]() {
+        for (auto *&p : d_xths) dfree(p);
+    };
     int cap = 8, R_max = 1, cd_max = 1, nt_max = 1;
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
         if (it->rebuild_paths) c.paths.valid = false;
         portal_stats st{};
         int rc = ensure_paths(c, &params[m], &st);
-        if (rc) return fail(c0, rc, "model " + std::to_string(m) + ": " + c.err);
+        if (rc) {
+            free_xths();
+            return fail(c0, rc, "model " + std::to_string(m) + ": " + c.err);
+        }
         if (total) total->t_walk_ms += st.t_walk_ms;
         const Species &sp = c.sp;
         const int D = params[m].n_sphere, nw = (int)c.g.work.size();
@@ -1088,6 +1111,17 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         ra.J0 = c.d_J0;
         ra.J2 = c.d_J2;
         ra.variant = 1;
+        ra.max_nc = (c.paths.max_nc + 7) & ~7;
+        {
+            std::vector<double> xv(params[m].xth, params[m].xth + 3 * (size_t)D);
+            HIP_TRY(&c0, upload(d_xths[m], xv));
+        }
+        ra.xth = d_xths[m];
+        rc = check_rt_args(c, ra, mixed);
+        if (rc) {
+            free_xths();
+            return fail(c0, rc, "model " + std::to_string(m) + ": " + c.err);
+        }
         cap = std::max(cap, (c.paths.max_nc + 7) & ~7);
         R_max = std::max(R_max, sp.r_num);
         cd_max = std::max(cd_max, sp.c_dim);
@@ -1259,6 +1293,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     hipFree(d_cnt);
     hipFree(d_max);
     for (auto *p : prev) hipFree(p);
+    free_xths();
     const double ms = ttot.stop();
     if (flags & FLAG_TEMP) return fail(c0, PORTAL_ERR_TEMP, "kinetic temperature above the collision-rate table");
     if (flags & FLAG_LU) return fail(c0, PORTAL_ERR_LU, "Density-matrix calculation: failed (singular SE matrix)");

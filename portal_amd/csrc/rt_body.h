@@ -32,11 +32,18 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks) {
 template <typename Real>
 struct Arith;
 
+// Per (cell, line) the staging stores L = log(K dl) (natural log in FP64, log2 in
+// FP32) so that tau = K dl exp(-u^2) is one exponential: exp(L - u^2).  A cell with
+// K <= 0 or K = NaN gets L = NaN or -inf, i.e. tau NaN or 0, which the tau > 1e-30
+// test skips exactly as the reference skips its negative, zero or NaN tau.
 template <>
 struct Arith<double> {
     typedef double2 R2;
     static constexpr bool kahan = false;
+    static constexpr double tr_floor = 0.0;  // FP64: a channel stops only at exactly zero transmission
     __device__ static double ex(double x) { return exp(x); }
+    __device__ static double lg(double x) { return log(x); }
+    __device__ static double tau_of(double L, double u) { return exp(L - u * u); }
     __device__ static double one_minus(double tau, double et) { return 1.0 - et; }
     __device__ static double thr() { return 1e-30; }
     __device__ static R2 ks(const RtArgs &a, size_t i) { return *reinterpret_cast<const double2 *>(a.KS + 2 * i); }
@@ -46,7 +53,12 @@ template <>
 struct Arith<float> {
     typedef float2 R2;
     static constexpr bool kahan = true;
+    // mixed: a channel whose transmission fell below 1e-9 stops (the dropped tail is
+    // below 1e-9 of the remaining source function)
+    static constexpr float tr_floor = 1e-9f;
     __device__ static float ex(float x) { return __expf(x); }
+    __device__ static float lg(float x) { return __log2f(x); }
+    __device__ static float tau_of(float L, float u) { return exp2f(fmaf(u * u, -1.44269504f, L)); }
     // 1 - exp(-tau) without cancellation for small tau
     __device__ static float one_minus(float tau, float et) {
         return tau < 1e-2f ? tau * (1.0f - tau * (0.5f - tau * (0.16666667f - tau * 0.041666668f))) : 1.0f - et;
@@ -57,7 +69,7 @@ struct Arith<float> {
 
 template <typename Real>
 struct __attribute__((aligned(8))) CellRec {
-    Real vray, inv_b, dl, pad;
+    Real vb, inv_b, dl, pad;  // v.dir / b, 1/b, segment length
 };
 
 template <typename Real>
@@ -101,17 +113,22 @@ __device__ __forceinline__ void rt_staged_node(const RtArgs &a, const int bid, c
             for (int i = lane; i < nc; i += 64) {
                 const PathEntry e = P[i];
                 CellRec<Real> c;
-                c.vray = (Real)e.vray;
-                c.inv_b = (Real)a.inv_b[e.cell];
+                const double ib = a.inv_b[e.cell];
+                c.vb = (Real)(e.vray * ib);
+                c.inv_b = (Real)ib;
                 c.dl = (Real)e.len * dls;
-                c.pad = (Real)0;
+                c.pad = (Real)e.vray;
                 cr[i] = c;
-                for (int l = 0; l < R; l++) ks[i * R + l] = A::ks(a, (size_t)e.cell * R + l);
+                for (int l = 0; l < R; l++) {
+                    R2 k = A::ks(a, (size_t)e.cell * R + l);
+                    k.x = A::lg(k.x * c.dl);
+                    ks[i * R + l] = k;
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const Real vel_node = cr[0].vray;
+            const Real vel_node = cr[0].pad;
             const Real dl0 = cr[0].dl;
 #pragma unroll
             for (int g0 = 0; g0 < MAXS; g0 += GROUP) {
@@ -129,7 +146,7 @@ __device__ __forceinline__ void rt_staged_node(const RtArgs &a, const int bid, c
                     const Real v0 = ((Real)2 * b) * (Real)nv / (Real)nvel;
                     const Real aa = v0 * inv_bn;
                     const Real e0 = A::ex(-(aa * aa));
-                    const R2 k0 = ks[l];  // cell 0 is the node's own
+                    const R2 k0 = A::ks(a, (size_t)node * R + l);  // the node's own cell, no skip
                     const Real tau0 = k0.x * e0 * dl0;
                     Tr[q] = A::ex(-tau0);
                     Em[q] = k0.y * A::one_minus(tau0, Tr[q]);
@@ -139,14 +156,17 @@ __device__ __forceinline__ void rt_staged_node(const RtArgs &a, const int bid, c
                 for (int i = 1; i < nc; i++) {
                     bool live = false;
 #pragma unroll
-                    for (int q = 0; q < GROUP; q++) live = live || (Tr[q] != (Real)0);
+                    for (int q = 0; q < GROUP; q++) {
+                        if (!(Tr[q] > (Real)A::tr_floor)) Tr[q] = (Real)0;
+                        live = live || (Tr[q] != (Real)0);
+                    }
                     if (!__any(live)) break;
                     const CellRec<Real> c = cr[i];
 #pragma unroll
                     for (int q = 0; q < GROUP; q++) {
                         const R2 k = ks[i * R + lg[q]];
-                        const Real u = (v0s[q] - c.vray) * c.inv_b;
-                        const Real tau = k.x * A::ex(-(u * u)) * c.dl;
+                        const Real u = fma(v0s[q], c.inv_b, -c.vb);
+                        const Real tau = A::tau_of(k.x, u);
                         if (tau > A::thr() && Tr[q] != (Real)0) {
                             const Real et = A::ex(-tau);
                             Em[q] = Em[q] + Tr[q] * k.y * A::one_minus(tau, et);

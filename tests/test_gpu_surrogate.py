@@ -1,0 +1,57 @@
+"""What the trained surrogate gains, and its fallback (needs an MI355X and trained weights)."""
+import numpy as np
+import pytest
+
+from portal_amd import cloud, surrogate
+from portal_amd.api import Portal
+from portal_amd.lib import PORTAL_MIXED
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(levels, n, seed):
+    sp = cloud.co_like_rotor(levels)
+    g = cloud.make_cloud(sp, n, seed=seed)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    return sp, g, pt
+
+
+def test_surrogate_reduces_iterations_to_the_same_answer():
+    sp, g, pt = _setup(4, 4000, seed=777)  # a cloud the network was not trained on
+    if not surrogate.load_into(pt, sp):
+        pt.close()
+        pytest.skip("no trained weights for this species")
+    xth = cloud.legendre_all(6, 11)
+    kw = dict(tol_pop=1e-5, tol_align=1e-7, precision=PORTAL_MIXED)
+    pt.set_levpop(g.lev_p)
+    pp0, lev0, st0 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 200, surrogate=0, **kw)
+    pt.set_levpop(g.lev_p)
+    pp1, lev1, st1 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 200, surrogate=1, **kw)
+    print(f"plain: {st0['iterations']} iterations; surrogate: {st1['iterations']} iterations, "
+          f"{st1['fallbacks']} fallbacks")
+    assert st1["iterations"] < st0["iterations"]
+    idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+    rel = np.abs(lev1[:, idx] - lev0[:, idx]) / np.maximum(np.abs(lev0[:, idx]), 1e-30)
+    big = lev0[:, idx] > 1e-6 * lev0[:, idx].max(axis=0)
+    assert rel[big].max() < 1e-3
+    pt.close()
+
+
+def test_invalid_network_output_falls_back_to_the_plain_iterate():
+    sp, g, pt = _setup(4, 1500, seed=778)
+    n_in, n_out = surrogate.dims(sp)
+    pt.mlp_create(n_in, 64, 2, n_out, seed=1)
+    p, nm = pt.mlp_get_params()
+    nm[2 * n_in: 2 * n_in + n_out] = 1e6  # every decoded population overflows
+    pt.mlp_set_params(p, nm)
+    xth = cloud.legendre_all(4, 8)
+    pt.set_levpop(g.lev_p)
+    pp0, lev0, st0 = pt.lambda_iterate(xth, 16, cloud.DX_INWARD, 3, tol_pop=0, tol_align=0, surrogate=0)
+    pt.set_levpop(g.lev_p)
+    pp1, lev1, st1 = pt.lambda_iterate(xth, 16, cloud.DX_INWARD, 3, tol_pop=0, tol_align=0, surrogate=1)
+    n_solved = int((g.ab_all[: g.bulk_node_num] > 0).sum())
+    assert st1["fallbacks"] == n_solved
+    np.testing.assert_array_equal(lev1, lev0)
+    pt.close()
