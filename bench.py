@@ -51,8 +51,10 @@ def main():
     ap.add_argument("--nth", type=int, default=9)
     ap.add_argument("--nph", type=int, default=32)
     ap.add_argument("--nvel", type=int, default=64)
-    ap.add_argument("--max-iter", type=int, default=50)
-    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--iters", type=int, default=50, help="the FP64 configuration's iteration count")
+    ap.add_argument("--tol", type=float, default=1e-5, help="population-change tolerance of the fast path")
+    ap.add_argument("--tol-align", type=float, default=1e-7)
+    ap.add_argument("--no-plain-mixed", action="store_true")
     ap.add_argument("--im-pix", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample", type=int, default=256)
@@ -88,15 +90,6 @@ def main():
     except Exception:  # torch is plumbing only
         sync = lambda: None  # noqa: E731
 
-    def solve(precision, surr):
-        pt.set_levpop(g.lev_p)
-        sync()
-        t = time.perf_counter()
-        st = pt.lambda_iterate(xth, args.nvel, cloud.DX_INWARD, args.max_iter, tol_pop=args.tol, tol_align=1e-9,
-                               precision=precision, surrogate=surr, rebuild_paths=1, outputs=False)
-        sync()
-        return time.perf_counter() - t, st
-
     lines = driver.image_lines(sp, [1], [2])
     vel_out = driver.channel_velocities(64, 46.875)
 
@@ -109,32 +102,72 @@ def main():
         return (I, Q, U), st
 
     out = {}
+    img64 = None
     if not args.no_fp64:
-        t64, st64 = solve(PORTAL_FP64, 0)
-        log(f"FP64: converged in {st64['iterations']} iterations, {t64:.2f}s "
-            f"(walk {st64['t_walk_ms']:.0f} ms, rt {st64['t_rt_ms']:.0f} ms, se {st64['t_se_ms']:.0f} ms)")
+        # plain FP64 GPU path: the configuration's fixed iteration count, timed in two parts so that the
+        # time at which it met the same tolerance as the fast path is known too
+        pt.set_levpop(g.lev_p)
+        sync()
+        t = time.perf_counter()
+        sta = pt.lambda_iterate(xth, args.nvel, cloud.DX_INWARD, args.iters, tol_pop=args.tol, tol_align=args.tol_align,
+                                precision=PORTAL_FP64, rebuild_paths=1, outputs=False)
+        sync()
+        t_tol = time.perf_counter() - t
+        rest = args.iters - sta["iterations"]
+        t_rest = 0.0
+        if rest > 0:
+            t = time.perf_counter()
+            pt.lambda_iterate(xth, args.nvel, cloud.DX_INWARD, rest, tol_pop=0.0, tol_align=0.0,
+                              precision=PORTAL_FP64, outputs=False)
+            sync()
+            t_rest = time.perf_counter() - t
+        log(f"FP64: tolerance met after {sta['iterations']} iterations in {t_tol:.2f}s; "
+            f"{args.iters} iterations in {t_tol + t_rest:.2f}s (walk {sta['t_walk_ms']:.0f} ms)")
         img64, sti = image()
         log(f"FP64 image: {sti['t_img_ms']:.0f} ms")
-        out["fp64"] = {"seconds": t64, "iterations": st64["iterations"], "nodes_per_s": N_b / t64}
+        out["fp64"] = {"iterations": args.iters, "seconds": t_tol + t_rest, "nodes_per_s": N_b / (t_tol + t_rest),
+                       "iterations_to_tol": sta["iterations"], "seconds_to_tol": t_tol,
+                       "image_ms": sti["t_img_ms"]}
 
+    def fast(surr):
+        pt.set_levpop(g.lev_p)
+        sync()
+        t = time.perf_counter()
+        st = pt.lambda_iterate(xth, args.nvel, cloud.DX_INWARD, args.iters, tol_pop=args.tol,
+                               tol_align=args.tol_align, precision=PORTAL_MIXED, surrogate=surr, rebuild_paths=1,
+                               outputs=False)
+        sync()
+        return time.perf_counter() - t, st
+
+    if not args.no_plain_mixed:
+        t_m, st_m = fast(0)
+        log(f"mixed without surrogate: {st_m['iterations']} iterations, {t_m:.2f}s")
+        img_m0, _ = image()
+        out["mixed_no_surrogate"] = {"iterations": st_m["iterations"], "seconds": t_m,
+                                     "stokes_err_vs_fp64": stokes_error(img64, img_m0) if img64 else None}
     times = []
     stm = None
     for k in range(args.warmup + args.steps):
-        t, stm = solve(PORTAL_MIXED, 1 if have_surr else 0)
+        t, stm = fast(1 if have_surr else 0)
         log(f"mixed{'+surrogate' if have_surr else ''} step {k}: {stm['iterations']} iterations, {t:.2f}s, "
-            f"fallbacks {stm['fallbacks']}")
+            f"fallbacks {stm['fallbacks']}, rt {stm['t_rt_ms']:.0f} ms, walk {stm['t_walk_ms']:.0f} ms")
         if k >= args.warmup:
             times.append(t)
-    T = max(times) if times else float("nan")
+    T = max(times)
     img_m, _ = image()
-    err = stokes_error(img64, img_m) if "fp64" in out else None
+    err = stokes_error(img64, img_m) if img64 else None
 
-    # roofline of the dominant kernel (mixed inward integration): SURVEY per-crossing bytes
+    # roofline of the dominant kernel (the mixed inward integration), per-crossing bytes of
+    # the SURVEY's measurement section: 28 nn + 4 (4 + L)
     nn = float(np.diff(g.nbr_ptr).mean())
     bytes_per_cross = 28.0 * nn + 4.0 * (4 + sp.e_num)
     launches = max(1, stm["iterations"])
     avg_ms = stm["t_rt_ms"] / launches
     achieved = stm["n_cross"] * bytes_per_cross / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "rt_mixed_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
 
     # CPU baseline: the oracle (C restatement, OpenMP) on a bounded node sample, one pass
     cpu = None
@@ -149,10 +182,10 @@ def main():
         O.inverse_nodes(g, sp, xth, args.nvel, cloud.DX_INWARD, sub, nthreads=nthr)
         tc = time.perf_counter() - tc
         per_pass = len(sub) / tc
-        iters = out.get("fp64", {}).get("iterations", stm["iterations"])
-        cpu = {"value": per_pass / iters, "unit": "converged grid nodes/s", "cores": nthr, "kind": "port",
-               "sample": f"inverse_rt of {len(sub)} seeded nodes of the same cloud, one pass, {tc:.1f}s; "
-                         f"converged rate = per-pass rate / {iters} iterations"}
+        cpu = {"value": per_pass / args.iters, "unit": "converged grid nodes/s", "cores": nthr, "kind": "port",
+               "sample": f"inward transfer + SE of {len(sub)} seeded nodes of the same cloud, one FP64 pass "
+                         f"({tc:.1f}s on {nthr} threads); converged rate = per-pass rate / {args.iters} iterations"}
+        log(f"CPU oracle: {per_pass:.1f} nodes/s per pass on {nthr} threads")
     except Exception as e:  # oracle not built
         log(f"cpu baseline skipped: {e}")
 
@@ -170,17 +203,20 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic seeded cloud",
         "config": {"workload": f"{args.nodes}-node cloud, {sp.e_num}-level rotor, {xth.shape[1]} directions, "
-                               f"{2 * args.nvel + 1} channels, lambda iteration to {args.tol:g}",
+                               f"{2 * args.nvel + 1} channels; FP64 path {args.iters} lambda iterations, fast path "
+                               f"to population change <= {args.tol:g}",
                    "nodes": args.nodes, "levels": sp.e_num, "directions": int(xth.shape[1]),
                    "channels": 2 * args.nvel + 1, "surrogate": bool(have_surr),
                    "image": f"{2 * args.im_pix + 1}^2 x 129"},
         "speedup_vs_fp64_gpu": (out["fp64"]["seconds"] / T) if "fp64" in out else None,
         "fp64_gpu": out.get("fp64"),
+        "speedup_vs_fp64_gpu_same_tol": (out["fp64"]["seconds_to_tol"] / T) if "fp64" in out else None,
+        "mixed_no_surrogate": out.get("mixed_no_surrogate"),
         "iterations": stm["iterations"],
         "surrogate_fallbacks": stm["fallbacks"],
         "stokes_err_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "rt_mixed_kernel", "avg_launch_ms": avg_ms, "crossings_per_launch": stm["n_cross"],
                      "bytes_per_crossing": bytes_per_cross},
         "cpu_baseline": cpu,

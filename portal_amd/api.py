@@ -148,6 +148,11 @@ class _Core:
         self._check(self.lib.portal_lambda_iterate(self.ctx, C.byref(p), C.byref(it), _d(pp), _d(lev), C.byref(st)))
         return pp, lev, st.as_dict()
 
+    def force_rt_kernel(self, precision, variant, waves=4):
+        """Test hook: fix the inward-integration kernel (0 global, 1 staged, 2 cooperative)
+        and its waves per workgroup for one precision; variant -1 returns to autotuning."""
+        self._check(self.lib.portal_debug_force_rt(self.ctx, int(precision), int(variant), int(waves)))
+
     def get_pol_pops(self):
         """pol_pops of the last SE solve (N x 2L) without recomputing."""
         g, sp = self.grid, self.species

@@ -115,7 +115,8 @@ struct Ctx {
     // autotuned inward-integration launch (per precision): variant, waves per block
     int tuned_key[2] = {0, 0};
     int tuned_variant[2] = {0, 0}, tuned_waves[2] = {4, 4};
-    double tune_ms[2][4] = {{0}};
+    double tune_ms[2][8] = {{0}};
+    int forced_variant[2] = {-1, -1}, forced_waves[2] = {4, 4};  // test hook: bypass the autotuner
 };
 
 // error flags (bits in d_flags[0])
@@ -182,7 +183,7 @@ struct RtArgs {
     double *J0, *J2;        // R x N
     int waves;              // waves per block
     int max_nc;             // longest stored path (LDS staging capacity)
-    int variant;            // 0: global-memory kernel, 1: LDS-staged kernel
+    int variant;            // 0: global-memory kernel, 1: LDS-staged per-wave rays, 2: cooperative
 };
 hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s);
 hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s);

@@ -203,13 +203,20 @@ int check_rt_args(Ctx &c, const RtArgs &a, bool mixed) {
 int autotune_rt(Ctx &c, RtArgs ra, bool mixed) {
     const int pi = mixed ? 1 : 0;
     const int key = (int)((c.paths.total % 1000003) ^ (ra.C * 131 + ra.R * 7 + ra.D));
+    if (c.forced_variant[pi] >= 0) {
+        c.tuned_variant[pi] = c.forced_variant[pi];
+        c.tuned_waves[pi] = c.forced_waves[pi];
+        c.tuned_key[pi] = 0;
+        return PORTAL_OK;
+    }
     if (c.tuned_key[pi] == key) return PORTAL_OK;
-    struct Cand { int variant, waves; } cands[4] = {{0, 4}, {0, 2}, {1, 4}, {1, 2}};
+    constexpr int NC = 6;
+    struct Cand { int variant, waves; } cands[NC] = {{0, 4}, {1, 4}, {1, 2}, {2, 2}, {2, 4}, {2, 8}};
     const int sub = std::min(ra.n_work, 2048);
     ra.n_work = sub;
     double best = 1e30;
     int bv = 0, bw = 4;
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < NC; k++) {
         ra.variant = cands[k].variant;
         ra.waves = cands[k].waves;
         Timer t(c.stream);
@@ -763,11 +770,23 @@ static int surrogate_step(Ctx &c, int *n_fallback) {
     return PORTAL_OK;
 }
 
-// the autotuner's timings of the inward-integration candidates (global 4, global 2,
-// staged 4, staged 2 waves) for one precision; -1 for a candidate that could not run
-int portal_debug_tune_ms(const portal_ctx *x, int precision, double *out4) {
-    if (!x || !out4 || precision < 0 || precision > 1) return PORTAL_ERR_ARG;
-    for (int k = 0; k < 4; k++) out4[k] = x->c.tune_ms[precision][k];
+// the autotuner's timings of the inward-integration candidates (variant/waves: global 4,
+// staged 4, staged 2, cooperative 2, 4, 8; two unused slots) for one precision; -1 for a
+// candidate that could not run
+int portal_debug_tune_ms(const portal_ctx *x, int precision, double *out8) {
+    if (!x || !out8 || precision < 0 || precision > 1) return PORTAL_ERR_ARG;
+    for (int k = 0; k < 8; k++) out8[k] = x->c.tune_ms[precision][k];
+    return PORTAL_OK;
+}
+
+// test hook: run the inward integration of one precision with a fixed kernel variant
+// (0 global, 1 staged, 2 cooperative) and wave count; variant -1 returns to autotuning
+int portal_debug_force_rt(portal_ctx *x, int precision, int variant, int waves) {
+    if (!x || precision < 0 || precision > 1 || variant < -1 || variant > 2 || waves < 1 || waves > 8)
+        return PORTAL_ERR_ARG;
+    x->c.forced_variant[precision] = variant;
+    x->c.forced_waves[precision] = waves;
+    x->c.tuned_key[precision] = 0;
     return PORTAL_OK;
 }
 

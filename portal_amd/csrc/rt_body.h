@@ -58,10 +58,15 @@ struct Arith<float> {
     static constexpr float tr_floor = 1e-9f;
     __device__ static float ex(float x) { return __expf(x); }
     __device__ static float lg(float x) { return __log2f(x); }
-    __device__ static float tau_of(float L, float u) { return exp2f(fmaf(u * u, -1.44269504f, L)); }
-    // 1 - exp(-tau) without cancellation for small tau
+    // v_exp_f32 directly: a result below the FP32 normal range is < 1e-30 and skipped anyway
+    __device__ static float tau_of(float L, float u) { return __builtin_amdgcn_exp2f(fmaf(u * u, -1.44269504f, L)); }
+    // 1 - exp(-tau) without cancellation for small tau.  Both forms are evaluated and
+    // one is picked by a bit select, so that no branch splits the segment update.
     __device__ static float one_minus(float tau, float et) {
-        return tau < 1e-2f ? tau * (1.0f - tau * (0.5f - tau * (0.16666667f - tau * 0.041666668f))) : 1.0f - et;
+        const float series = tau * (1.0f - tau * (0.5f - tau * (0.16666667f - tau * 0.041666668f)));
+        const float direct = 1.0f - et;
+        const int m = -(int)(tau < 1e-2f);
+        return __int_as_float((__float_as_int(series) & m) | (__float_as_int(direct) & ~m));
     }
     __device__ static float thr() { return 1e-30f; }
     __device__ static R2 ks(const RtArgs &a, size_t i) { return *reinterpret_cast<const float2 *>(a.KS32 + 2 * i); }

@@ -1,0 +1,236 @@
+// Inward ray integration, cooperative form: the waves of a workgroup share one ray
+// at a time and split the (line, channel) pairs between them.  Restates ray_in
+// (inverse_rt.f90:217-344) and the J accumulation of inverse_rt
+// (inverse_rt.f90:126-129) with the same per-segment arithmetic as rt_body.h.
+//
+// Per node, rays are taken in order 0..D-1.  All threads stage ray r+1 (per cell
+// v.dir/b, 1/b, segment length; per (cell, line) log(K dl) and S) into one half of a
+// double-buffered LDS area while the waves integrate ray r from the other half; one
+// workgroup barrier per ray.  Lane `lane` of wave `w` owns pairs
+// p = (q W + w) 64 + lane for q < SLOTS, so the pairs are spread evenly over waves
+// and a lane holds SLOTS accumulators (not all of them, as in the per-wave form),
+// which keeps the register count, and with it occupancy, independent of R C.
+//
+// The segment update is branch-free: tau and e^-tau are evaluated for every lane and
+// the update is selected when tau > 1e-30 (false for NaN), which is the
+// reference's skip; with tau <= 1e-30 or NaN the selected values are the old ones.
+// The check for exhausted transmission (every lane's Tr exactly zero, or below the
+// mixed floor) runs every fourth cell; cells visited after a lane's Tr reached zero
+// add exactly zero.  Per pair the sum over rays is in ray order; the channel sum
+// with phi0 d_v is done per line in channel order.
+#pragma once
+#include "rt_body.h"
+
+namespace portal {
+
+template <typename Real>
+__host__ __device__ constexpr size_t coop_buf_bytes(int cap, int R) {
+    return ((size_t)cap * stage_bytes_per_cell<Real>(R) + 15) & ~(size_t)15;
+}
+
+template <typename Real>
+size_t coop_lds_bytes(int cap, int R, int npairs) {
+    const size_t stage = 2 * coop_buf_bytes<Real>(cap, R);
+    const size_t red = sizeof(double) * 2 * (size_t)npairs;
+    return stage > red ? stage : red;
+}
+
+// All threads of the workgroup stage ray r of work item bid into buf.
+template <typename Real>
+__device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const int r, const int cap, char *buf) {
+    typedef Arith<Real> A;
+    typedef typename A::R2 R2;
+    const int R = a.R;
+    const long long off = a.offsets[(long long)bid * a.D + r];
+    const int nc = (int)(a.offsets[(long long)bid * a.D + r + 1] - off);
+    const PathEntry *P = a.paths + off;
+    CellRec<Real> *cr = reinterpret_cast<CellRec<Real> *>(buf);
+    R2 *ks = reinterpret_cast<R2 *>(cr + cap);
+    const Real dls = (Real)a.dl_scale;
+    const int n = nc * R;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const int i = j / R, l = j - i * R;
+        const PathEntry e = P[i];
+        const double ib = a.inv_b[e.cell];
+        const Real dl = (Real)e.len * dls;
+        R2 k = A::ks(a, (size_t)e.cell * R + l);
+        k.x = A::lg(k.x * dl);
+        ks[j] = k;
+        if (l == 0) {
+            CellRec<Real> c;
+            c.vb = (Real)(e.vray * ib);
+            c.inv_b = (Real)ib;
+            c.dl = dl;
+            c.pad = (Real)e.vray;
+            cr[i] = c;
+        }
+    }
+}
+
+template <typename Real, int SLOTS>
+__device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, const int cap, char *smem) {
+    typedef Arith<Real> A;
+    typedef typename A::R2 R2;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int W = blockDim.x >> 6;
+    const int node = a.work[bid];
+    const double bd = a.velb[node].w;
+    const Real b = (Real)bd;
+    const Real inv_bn = (Real)(1.0 / bd);
+    const int C = a.C, nvel = a.nvel, R = a.R, D = a.D;
+    const int npairs = R * C;
+    const size_t bb = coop_buf_bytes<Real>(cap, R);
+
+    Real v0[SLOTS], e0[SLOTS], k0x[SLOTS], k0y[SLOTS], iin[SLOTS];
+    int lo[SLOTS];
+    bool on[SLOTS];
+    Real S0[SLOTS], C0[SLOTS], S2[SLOTS], C2[SLOTS];
+#pragma unroll
+    for (int q = 0; q < SLOTS; q++) {
+        const int p = (q * W + wave) * 64 + lane;
+        on[q] = p < npairs;
+        const int l = on[q] ? p / C : 0;
+        const int nv = on[q] ? p - l * C - nvel : 0;
+        lo[q] = l;
+        v0[q] = ((Real)2 * b) * (Real)nv / (Real)nvel;
+        const Real aa = v0[q] * inv_bn;
+        e0[q] = A::ex(-(aa * aa));
+        const R2 k = A::ks(a, (size_t)node * R + l);  // the node's own cell, no skip
+        k0x[q] = k.x;
+        k0y[q] = k.y;
+        iin[q] = (Real)a.I_in[l];
+        S0[q] = C0[q] = S2[q] = C2[q] = (Real)0;
+    }
+
+    coop_stage<Real>(a, bid, 0, cap, smem);
+    __syncthreads();
+    for (int r = 0; r < D; r++) {
+        char *cur = smem + (r & 1) * bb;
+        if (r + 1 < D) coop_stage<Real>(a, bid, r + 1, cap, smem + ((r + 1) & 1) * bb);
+        const Real mu = (Real)a.xth[3 * r], w = (Real)a.xth[3 * r + 2];
+        const Real wc = w * (((Real)3 * (mu * mu) - (Real)1) / (Real)2);
+        const long long off = a.offsets[(long long)bid * D + r];
+        const int nc = (int)(a.offsets[(long long)bid * D + r + 1] - off);
+        const CellRec<Real> *cr = reinterpret_cast<const CellRec<Real> *>(cur);
+        const R2 *ks = reinterpret_cast<const R2 *>(cr + cap);
+        const Real vel_node = cr[0].pad;
+        const Real dl0 = cr[0].dl;
+        Real Tr[SLOTS], Em[SLOTS], v0s[SLOTS];
+#pragma unroll
+        for (int q = 0; q < SLOTS; q++) {
+            const Real tau0 = k0x[q] * e0[q] * dl0;
+            Tr[q] = A::ex(-tau0);
+            Em[q] = k0y[q] * A::one_minus(tau0, Tr[q]);
+            v0s[q] = v0[q] + vel_node;
+            if (!on[q]) Tr[q] = (Real)0;
+        }
+        for (int i = 1; i < nc; i++) {
+            if ((i & 3) == 0) {
+                bool live = false;
+#pragma unroll
+                for (int q = 0; q < SLOTS; q++) {
+                    if (!(Tr[q] > (Real)A::tr_floor)) Tr[q] = (Real)0;
+                    live = live || (Tr[q] != (Real)0);
+                }
+                if (!__any(live)) break;
+            }
+            const CellRec<Real> c = cr[i];
+            const R2 *kr = ks + (size_t)i * R;
+#pragma unroll
+            for (int q = 0; q < SLOTS; q++) {
+                const R2 k = kr[lo[q]];
+                const Real u = fma(v0s[q], c.inv_b, -c.vb);
+                const Real tau = A::tau_of(k.x, u);
+                const bool ok = tau > A::thr();
+                const Real et = A::ex(-tau);
+                const Real em = Em[q] + Tr[q] * k.y * A::one_minus(tau, et);
+                const Real tr = Tr[q] * et;
+                Em[q] = ok ? em : Em[q];
+                Tr[q] = ok ? tr : Tr[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < SLOTS; q++) {
+            const Real Iout = Em[q] + Tr[q] * iin[q];
+            if (A::kahan) {
+                Real y = w * Iout - C0[q];
+                Real t = S0[q] + y;
+                C0[q] = (t - S0[q]) - y;
+                S0[q] = t;
+                y = wc * Iout - C2[q];
+                t = S2[q] + y;
+                C2[q] = (t - S2[q]) - y;
+                S2[q] = t;
+            } else {
+                S0[q] = S0[q] + w * Iout;
+                S2[q] = S2[q] + wc * Iout;
+            }
+        }
+        __syncthreads();  // ray r+1 staged; buffer of ray r free for ray r+2
+    }
+
+    double *red0 = reinterpret_cast<double *>(smem);
+    double *red2 = red0 + npairs;
+#pragma unroll
+    for (int q = 0; q < SLOTS; q++) {
+        const int p = (q * W + wave) * 64 + lane;
+        if (on[q]) {
+            red0[p] = (double)S0[q] - (double)C0[q];
+            red2[p] = (double)S2[q] - (double)C2[q];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < R) {
+        const int l = threadIdx.x;
+        const double spi = 1.7724538509055160273;
+        const double d_v = 2.0 * bd / nvel;
+        double j0 = 0.0, j2 = 0.0;
+        for (int c = 0; c < C; c++) {
+            const double vv = (2.0 * bd) * (c - nvel) / nvel;
+            const double aa = vv / bd;
+            const double phi0 = (1.0 / (spi * bd)) * exp(-(aa * aa));
+            j0 = j0 + phi0 * d_v * red0[l * C + c];
+            j2 = j2 + phi0 * d_v * red2[l * C + c];
+        }
+        a.J0[(size_t)node * R + l] = j0;
+        a.J2[(size_t)node * R + l] = j2 / 1.4142135623730951;
+    }
+}
+
+template <typename Real, int SLOTS>
+__global__ void __launch_bounds__(512) rt_coop_kernel(RtArgs a, int cap) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    rt_coop_node<Real, SLOTS>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+}
+
+template <typename Real, int SLOTS>
+hipError_t launch_coop_s(const RtArgs &a, int cap, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((rt_coop_kernel<Real, SLOTS>), dim3(a.n_work), dim3(64 * a.waves), lds, s, a, cap);
+    return hipGetLastError();
+}
+
+// Slots per lane: ceil(R C / (64 W)), rounded up to an instantiated count.
+template <typename Real>
+hipError_t launch_coop(const RtArgs &a, int cap, hipStream_t s) {
+    const int W = a.waves;
+    if (W < 1 || W > 8) return hipErrorInvalidValue;
+    const int npairs = a.R * a.C;
+    const size_t lds = coop_lds_bytes<Real>(cap, a.R, npairs);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const int need = (npairs + 64 * W - 1) / (64 * W);
+    switch (need) {
+        case 1: return launch_coop_s<Real, 1>(a, cap, lds, s);
+        case 2: return launch_coop_s<Real, 2>(a, cap, lds, s);
+        case 3: return launch_coop_s<Real, 3>(a, cap, lds, s);
+        case 4: return launch_coop_s<Real, 4>(a, cap, lds, s);
+        case 5: return launch_coop_s<Real, 5>(a, cap, lds, s);
+        case 6: return launch_coop_s<Real, 6>(a, cap, lds, s);
+        case 7: case 8: return launch_coop_s<Real, 8>(a, cap, lds, s);
+        case 9: case 10: return launch_coop_s<Real, 10>(a, cap, lds, s);
+        case 11: case 12: return launch_coop_s<Real, 12>(a, cap, lds, s);
+        default: return hipErrorInvalidValue;  // more pairs than one workgroup holds
+    }
+}
+
+}  // namespace portal

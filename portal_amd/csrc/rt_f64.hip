@@ -15,6 +15,7 @@
 // so J is deterministic.  Compiled with -ffp-contract=off.
 #include "common.h"
 #include "rt_body.h"
+#include "rt_coop.h"
 
 namespace portal {
 
@@ -154,6 +155,7 @@ hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
     if (a.R > 64) return hipErrorInvalidValue;
     if (a.variant == 1) return launch_staged<double>(a, a.max_nc, s);
+    if (a.variant == 2) return launch_coop<double>(a, a.max_nc, s);
     const int W = a.waves;
     const int npairs = a.R * a.C;
     if (npairs <= 64 * 4)

@@ -10,6 +10,7 @@
 // The channel sum and the sum over waves are done in FP64.
 #include "common.h"
 #include "rt_body.h"
+#include "rt_coop.h"
 
 namespace portal {
 
@@ -132,6 +133,7 @@ hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
     if (a.R > 64) return hipErrorInvalidValue;
     if (a.variant == 1) return launch_staged<float>(a, a.max_nc, s);
+    if (a.variant == 2) return launch_coop<float>(a, a.max_nc, s);
     const int W = a.waves;
     const int npairs = a.R * a.C;
     if (npairs <= 64 * 4)

@@ -151,6 +151,68 @@ def test_full_size_subset_parity(oracle):
     pt.close()
 
 
+KERNELS = [(0, 4), (1, 4), (1, 2), (2, 2), (2, 4), (2, 8)]
+
+
+@pytest.mark.parametrize("variant,waves", KERNELS)
+def test_each_inward_kernel_cfg1(portal1, cfg1, oracle_cfg1, variant, waves):
+    # every kernel the autotuner may pick, FP64 against the oracle and mixed within bounds
+    sp, g, xth = cfg1
+    _, J0o, J2o, _, _ = oracle_cfg1
+    idx = solved(g)
+    try:
+        portal1.force_rt_kernel(PORTAL_FP64, variant, waves)
+        portal1.force_rt_kernel(PORTAL_MIXED, variant, waves)
+        J0, J2, st = portal1.inverse_rt(xth, 32, cloud.DX_INWARD)
+        assert (st["rt_variant"], st["rt_waves"]) == (variant, waves)
+        np.testing.assert_allclose(J0[:, idx], J0o[:, idx], rtol=1e-10, atol=0)
+        np.testing.assert_allclose(J2[:, idx], J2o[:, idx], rtol=0,
+                                   atol=1e-10 * np.abs(J0o[:, idx]).max(axis=0).min())
+        J0m, J2m, _ = portal1.inverse_rt(xth, 32, cloud.DX_INWARD, precision=PORTAL_MIXED)
+        assert np.abs(J0m[:, idx] / J0o[:, idx] - 1).max() < 2e-5
+        assert np.abs(J2m[:, idx] - J2o[:, idx]).max() / np.abs(J0o[:, idx]).max() < 1e-6
+    finally:
+        portal1.force_rt_kernel(PORTAL_FP64, -1)
+        portal1.force_rt_kernel(PORTAL_MIXED, -1)
+
+
+@pytest.fixture(scope="module")
+def cloud10(oracle):
+    # the 10-level rotor with the production angular and velocity grids on a smaller cloud
+    sp = cloud.co_like_rotor(10)
+    g = cloud.make_cloud(sp, 20000, seed=21)
+    xth = cloud.legendre_all(9, 32)
+    rng = np.random.default_rng(5)
+    sub = np.sort(rng.choice(solved(g), 16, replace=False)) + 1
+    J0o, J2o = oracle.inverse_nodes(g, sp, xth, 64, cloud.DX_INWARD, sub, nthreads=NTH)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    yield sp, g, xth, sub, J0o, J2o, pt
+    pt.close()
+
+
+@pytest.mark.parametrize("variant,waves", KERNELS)
+def test_each_inward_kernel_10_levels(cloud10, variant, waves):
+    sp, g, xth, sub, J0o, J2o, pt = cloud10
+    try:
+        pt.force_rt_kernel(PORTAL_FP64, variant, waves)
+        pt.force_rt_kernel(PORTAL_MIXED, variant, waves)
+        J0, J2, st = pt.inverse_rt(xth, 64, cloud.DX_INWARD)
+        np.testing.assert_allclose(J0[:, sub - 1], J0o, rtol=1e-10)
+        np.testing.assert_allclose(J2[:, sub - 1], J2o, rtol=0, atol=1e-10 * J0o.max(axis=0).min())
+        J0m, J2m, _ = pt.inverse_rt(xth, 64, cloud.DX_INWARD, precision=PORTAL_MIXED)
+        idx = solved(g)
+        e0 = np.abs(J0m[:, idx] / J0[:, idx] - 1).max()
+        e2 = np.abs(J2m[:, idx] - J2[:, idx]).max() / np.abs(J0[:, idx]).max()
+        print(f"kernel {variant}/{waves}: mixed max rel J0 err {e0:.3e}, J2 err {e2:.3e}")
+        assert e0 < 2e-5 and e2 < 1e-6
+    finally:
+        pt.force_rt_kernel(PORTAL_FP64, -1)
+        pt.force_rt_kernel(PORTAL_MIXED, -1)
+
+
 def test_error_paths(cfg1):
     sp, g, xth = cfg1
     pt = Portal(0)
