@@ -1,11 +1,15 @@
 """Benchmark: converged grid nodes/s of the mixed-precision + surrogate path against the
 plain FP64 GPU path, on the 200,000-node production cloud (10-level CO-like rotor, 200
-ray directions, 129 channels, lambda iteration to convergence), with the Stokes I/Q/U
-image error of the fast path against the FP64 result.
+ray directions, 129 channels), with the Stokes I/Q/U image error (513 x 513 pixels x 129
+channels) of the fast path against the FP64 result.
 
 A step is one converged solve: from the resident grid and LTE starting populations,
 build the inward paths, then iterate (inward ray integration + SE solve) until the
-population change falls below the tolerance.  value = bulk nodes / step time.
+largest relative level-population change is <= 1e-6 and the largest change of
+rho^2_0/rho^0_0 is <= 1e-8.  value = solved nodes / step time of the fast path
+(mixed precision, surrogate initial iterate, Ng acceleration).  The plain FP64 path
+runs the configuration's 50 iterations (it meets the same tolerance at about the
+50th), timed once; ablations without Ng and without the surrogate are timed once each.
 
     python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--nodes 200000]
 
@@ -52,14 +56,27 @@ def main():
     ap.add_argument("--nph", type=int, default=32)
     ap.add_argument("--nvel", type=int, default=64)
     ap.add_argument("--iters", type=int, default=50, help="the FP64 configuration's iteration count")
-    ap.add_argument("--tol", type=float, default=1e-5, help="population-change tolerance of the fast path")
-    ap.add_argument("--tol-align", type=float, default=1e-7)
-    ap.add_argument("--no-plain-mixed", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-6, help="relative level-population change at convergence")
+    ap.add_argument("--tol-align", type=float, default=1e-8, help="change of rho20/rho00 at convergence")
+    ap.add_argument("--no-ablation", action="store_true", help="skip the runs without Ng / without surrogate")
+    ap.add_argument("--no-ng", action="store_true", help="fast path without Ng acceleration")
     ap.add_argument("--im-pix", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a design "
               "described in DESIGN.md and is not run", file=sys.stderr)
@@ -129,28 +146,36 @@ def main():
                        "iterations_to_tol": sta["iterations"], "seconds_to_tol": t_tol,
                        "image_ms": sti["t_img_ms"]}
 
-    def fast(surr):
+    def fast(surr, ng):
         pt.set_levpop(g.lev_p)
         sync()
         t = time.perf_counter()
-        st = pt.lambda_iterate(xth, args.nvel, cloud.DX_INWARD, args.iters, tol_pop=args.tol,
+        st = pt.lambda_iterate(xth, args.nvel, cloud.DX_INWARD, 4 * args.iters, tol_pop=args.tol,
                                tol_align=args.tol_align, precision=PORTAL_MIXED, surrogate=surr, rebuild_paths=1,
-                               outputs=False)
+                               outputs=False, ng_accel=ng)
         sync()
         return time.perf_counter() - t, st
 
-    if not args.no_plain_mixed:
-        t_m, st_m = fast(0)
-        log(f"mixed without surrogate: {st_m['iterations']} iterations, {t_m:.2f}s")
-        img_m0, _ = image()
-        out["mixed_no_surrogate"] = {"iterations": st_m["iterations"], "seconds": t_m,
-                                     "stokes_err_vs_fp64": stokes_error(img64, img_m0) if img64 else None}
+    surr = 1 if have_surr else 0
+    ng = 0 if args.no_ng else 1
+    if not args.no_ablation:
+        for name, (su, n) in {"mixed": (0, 0), "mixed+surrogate": (surr, 0), "mixed+ng": (0, ng)}.items():
+            if (su, n) == (surr, ng) or (name != "mixed" and (su, n) == (0, 0)):
+                continue
+            t_a, st_a = fast(su, n)
+            img_a, _ = image()
+            log(f"ablation {name}: {st_a['iterations']} iterations, {t_a:.2f}s")
+            out.setdefault("ablation", {})[name] = {
+                "iterations": st_a["iterations"], "seconds": t_a, "nodes_per_s": N_b / t_a,
+                "speedup_vs_fp64_gpu": out["fp64"]["seconds"] / t_a if "fp64" in out else None,
+                "stokes_err_vs_fp64": stokes_error(img64, img_a) if img64 else None}
     times = []
     stm = None
     for k in range(args.warmup + args.steps):
-        t, stm = fast(1 if have_surr else 0)
-        log(f"mixed{'+surrogate' if have_surr else ''} step {k}: {stm['iterations']} iterations, {t:.2f}s, "
-            f"fallbacks {stm['fallbacks']}, rt {stm['t_rt_ms']:.0f} ms, walk {stm['t_walk_ms']:.0f} ms")
+        t, stm = fast(surr, ng)
+        log(f"mixed{'+surrogate' if surr else ''}{'+ng' if ng else ''} step {k}: {stm['iterations']} iterations "
+            f"({stm['ng_steps']} Ng), {t:.2f}s, fallbacks {stm['fallbacks']}, rt {stm['t_rt_ms']:.0f} ms, "
+            f"walk {stm['t_walk_ms']:.0f} ms")
         if k >= args.warmup:
             times.append(t)
     T = max(times)
@@ -164,6 +189,8 @@ def main():
     launches = max(1, stm["iterations"])
     avg_ms = stm["t_rt_ms"] / launches
     achieved = stm["n_cross"] * bytes_per_cross / (avg_ms * 1e-3) / 1e9
+    # the kernel's exponentials: 2 per (crossing, line, channel)
+    exp_rate = 2.0 * sp.r_num * (2 * args.nvel + 1) * stm["n_cross"] / (avg_ms * 1e-3)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "rt_mixed_traffic.json")
     if os.path.exists(tf):
@@ -172,6 +199,8 @@ def main():
     # CPU baseline: the oracle (C restatement, OpenMP) on a bounded node sample, one pass
     cpu = None
     try:
+        if args.cpu_sample <= 0:
+            raise RuntimeError("--cpu-sample 0")
         from oracle.bind import Oracle
         O = Oracle()
         nthr = min(16, os.cpu_count() or 1)
@@ -204,21 +233,24 @@ def main():
         "data": "synthetic seeded cloud",
         "config": {"workload": f"{args.nodes}-node cloud, {sp.e_num}-level rotor, {xth.shape[1]} directions, "
                                f"{2 * args.nvel + 1} channels; FP64 path {args.iters} lambda iterations, fast path "
-                               f"to population change <= {args.tol:g}",
+                               f"to population change <= {args.tol:g} and alignment change <= {args.tol_align:g}",
                    "nodes": args.nodes, "levels": sp.e_num, "directions": int(xth.shape[1]),
                    "channels": 2 * args.nvel + 1, "surrogate": bool(have_surr),
                    "image": f"{2 * args.im_pix + 1}^2 x 129"},
         "speedup_vs_fp64_gpu": (out["fp64"]["seconds"] / T) if "fp64" in out else None,
-        "fp64_gpu": out.get("fp64"),
         "speedup_vs_fp64_gpu_same_tol": (out["fp64"]["seconds_to_tol"] / T) if "fp64" in out else None,
-        "mixed_no_surrogate": out.get("mixed_no_surrogate"),
-        "iterations": stm["iterations"],
-        "surrogate_fallbacks": stm["fallbacks"],
+        "fp64_gpu": out.get("fp64"),
+        "fast_path": {"surrogate": bool(surr), "ng_accel": bool(ng), "iterations": stm["iterations"],
+                      "ng_steps": stm["ng_steps"], "surrogate_fallbacks": stm["fallbacks"],
+                      "walk_ms": stm["t_walk_ms"], "rt_ms": stm["t_rt_ms"], "se_ms": stm["t_se_ms"],
+                      "rt_kernel": f"variant {stm['rt_variant']}, {stm['rt_waves']} waves"},
+        "ablation": out.get("ablation"),
         "stokes_err_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "rt_mixed_kernel", "avg_launch_ms": avg_ms, "crossings_per_launch": stm["n_cross"],
-                     "bytes_per_crossing": bytes_per_cross},
+                     "kernel": "rt_coop_kernel<float>" if stm["rt_variant"] == 2 else f"variant {stm['rt_variant']}",
+                     "avg_launch_ms": avg_ms, "crossings_per_launch": stm["n_cross"],
+                     "bytes_per_crossing": bytes_per_cross, "exp_per_s": exp_rate},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

@@ -67,6 +67,8 @@ typedef struct {
     int surrogate;           /* 1: the trained MLP supplies the first iterate */
     double fallback_tol;     /* reserved (the surrogate iterate is rejected per node on the validity tests) */
     int rebuild_paths;       /* 1: rebuild the inward paths even if the geometry is unchanged */
+    int ng_accel;            /* 1: Ng acceleration of the level densities from the last four iterates,
+                                every third iteration from the fourth on */
 } portal_iter_params;
 
 typedef struct {
@@ -103,6 +105,7 @@ typedef struct {
     int rt_variant;          /* inward-integration kernel chosen by the autotuner */
     int rt_waves;            /* its waves per workgroup */
     int max_path;            /* longest inward path (cells) */
+    int ng_steps;            /* Ng extrapolations applied by the lambda iteration */
 } portal_stats;
 
 /* Context -------------------------------------------------------------------- */

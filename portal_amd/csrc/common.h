@@ -237,6 +237,11 @@ hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_i
 size_t se_lds_bytes(int c_dim, int n_trans);
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+hipError_t launch_ng_dots(int n_work, int L, const int *work, const double *x0, const double *x1, const double *x2,
+                          const double *x3, double *partial, double *out5, hipStream_t s);
+hipError_t launch_ng_apply(int n_work, int L, const int *work, double *x0, const double *x1, const double *x2,
+                           double ca, double cb, hipStream_t s);
+size_t ng_partial_doubles();
 
 }  // namespace portal
 

@@ -848,6 +848,16 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     unsigned long long *d_max = nullptr;
     HIP_TRY(&c, hipMalloc((void **)&d_prev, sizeof(double) * N * 2 * L));
     HIP_TRY(&c, hipMalloc((void **)&d_max, sizeof(unsigned long long) * 2));
+    // Ng acceleration: the three previous level-density iterates and the reduction scratch
+    const bool ng = it->ng_accel != 0;
+    double *d_hist = nullptr, *d_ngp = nullptr;
+    if (ng) {
+        HIP_TRY(&c, hipMalloc((void **)&d_hist, sizeof(double) * N * L * 3));
+        HIP_TRY(&c, hipMalloc((void **)&d_ngp, sizeof(double) * ng_partial_doubles()));
+    }
+    double *hist[3] = {d_hist, d_hist ? d_hist + N * L : nullptr, d_hist ? d_hist + 2 * N * L : nullptr};
+    int nhist = 0, n_ng = 0;
+    const int nw = (int)c.g.work.size();
     Timer tt(c.stream);
     int iters = 0, nfb = 0;
     double dpop = 0, dal = 0;
@@ -872,16 +882,47 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
         std::memcpy(&dal, &mx[1], 8);
         iters = k + 1;
         if (k > 0 && dpop <= it->tol_pop && dal <= it->tol_align) break;
+        if (ng) {
+            // history x1 = hist[0] (newest) .. x3 = hist[2]; x0 is the level densities just formed
+            if (nhist == 3 && k >= 3) {
+                double sums[5];
+                HIP_TRY(&c, launch_ng_dots(nw, (int)L, c.g.d_work, c.d_levp, hist[0], hist[1], hist[2], d_ngp,
+                                           d_ngp + ng_partial_doubles() - 8, c.stream));
+                HIP_TRY(&c, hipMemcpyAsync(sums, d_ngp + ng_partial_doubles() - 8, sizeof(sums),
+                                           hipMemcpyDeviceToHost, c.stream));
+                HIP_TRY(&c, hipStreamSynchronize(c.stream));
+                const double a11 = sums[0], a12 = sums[1], a22 = sums[2], b1 = sums[3], b2 = sums[4];
+                const double det = a11 * a22 - a12 * a12;
+                if (std::isfinite(det) && det > 1e-14 * a11 * a22 && a11 > 0 && a22 > 0) {
+                    const double ca = (b1 * a22 - b2 * a12) / det, cb = (b2 * a11 - b1 * a12) / det;
+                    if (std::isfinite(ca) && std::isfinite(cb)) {
+                        HIP_TRY(&c, launch_ng_apply(nw, (int)L, c.g.d_work, c.d_levp, hist[0], hist[1], ca, cb,
+                                                    c.stream));
+                        n_ng++;
+                    }
+                }
+                nhist = 0;  // the extrapolated iterate starts a new history
+            }
+            double *tail = hist[2];
+            hist[2] = hist[1];
+            hist[1] = hist[0];
+            hist[0] = tail;
+            HIP_TRY(&c, hipMemcpyAsync(hist[0], c.d_levp, sizeof(double) * N * L, hipMemcpyDeviceToDevice, c.stream));
+            nhist = std::min(nhist + 1, 3);
+        }
     }
     const double ms = tt.stop();
     hipFree(d_prev);
     hipFree(d_max);
+    if (d_hist) hipFree(d_hist);
+    if (d_ngp) hipFree(d_ngp);
     if (rc) return rc;
     if (st) {
         st->t_total_ms = ms;
         st->iterations = iters;
         st->max_rel_change = dpop;
         st->fallbacks = nfb;
+        st->ng_steps = n_ng;
     }
     if (pol_pops) HIP_TRY(&c, hipMemcpy(pol_pops, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToHost));
     if (lev_p_out) HIP_TRY(&c, hipMemcpy(lev_p_out, c.d_levp, sizeof(double) * N * L, hipMemcpyDeviceToHost));

@@ -71,6 +71,90 @@ hipError_t launch_lambda_update(int n_work, int N, int L, const int *work, const
     return hipGetLastError();
 }
 
+// Ng acceleration of the iteration (Ng 1974, in the form of Auer 1987, "Acceleration
+// of convergence", Numerical Radiative Transfer, 101-109): from the last four level
+// density iterates x0 (newest) .. x3,
+//   d0 = x0 - x1, d1 = d0 - (x1 - x2), d2 = d0 - (x2 - x3),
+//   [d1.d1 d1.d2; d1.d2 d2.d2] (a, b) = (d0.d1, d0.d2),
+//   x* = (1 - a - b) x0 + a x1 + b x2,
+// with the dot products weighted by 1/x0^2 (relative changes).  Only solved nodes
+// enter; an entry whose extrapolation is not positive keeps x0.
+struct NgArgs {
+    int n_work, L;
+    const int *work;
+    const double *x0, *x1, *x2, *x3;
+    double *partial;  // NG_BLOCKS x 5
+};
+
+constexpr int NG_BLOCKS = 512;
+
+__global__ void __launch_bounds__(256) ng_dots_kernel(NgArgs a) {
+    const long long n = (long long)a.n_work * a.L;
+    double s[5] = {0, 0, 0, 0, 0};
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const int w = (int)(e / a.L), i = (int)(e % a.L);
+        const size_t k = (size_t)a.work[w] * a.L + i;
+        const double x0 = a.x0[k], x1 = a.x1[k], x2 = a.x2[k], x3 = a.x3[k];
+        const double wt = x0 > 0.0 ? 1.0 / (x0 * x0) : 0.0;
+        const double d0 = x0 - x1, d1 = d0 - (x1 - x2), d2 = d0 - (x2 - x3);
+        s[0] += wt * d1 * d1;
+        s[1] += wt * d1 * d2;
+        s[2] += wt * d2 * d2;
+        s[3] += wt * d0 * d1;
+        s[4] += wt * d0 * d2;
+    }
+    __shared__ double red[5][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = 0; j < 5; j++) {
+        double v = s[j];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[j][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        const int j = threadIdx.x;
+        a.partial[blockIdx.x * 5 + j] = ((red[j][0] + red[j][1]) + red[j][2]) + red[j][3];
+    }
+}
+
+// fixed-order sum of the block partials: out[5]
+__global__ void __launch_bounds__(64) ng_sum_kernel(const double *partial, int nb, double *out) {
+    const int j = threadIdx.x;
+    if (j >= 5) return;
+    double v = 0.0;
+    for (int b = 0; b < nb; b++) v += partial[b * 5 + j];
+    out[j] = v;
+}
+
+__global__ void __launch_bounds__(256) ng_apply_kernel(int n_work, int L, const int *work, double *x0, const double *x1,
+                                                       const double *x2, double ca, double cb) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n_work * L) return;
+    const int w = (int)(e / L), i = (int)(e % L);
+    const size_t k = (size_t)work[w] * L + i;
+    const double v = (1.0 - ca - cb) * x0[k] + ca * x1[k] + cb * x2[k];
+    if (v > 0.0 && v < 1e300) x0[k] = v;
+}
+
+hipError_t launch_ng_dots(int n_work, int L, const int *work, const double *x0, const double *x1, const double *x2,
+                          const double *x3, double *partial, double *out5, hipStream_t s) {
+    NgArgs a{n_work, L, work, x0, x1, x2, x3, partial};
+    hipLaunchKernelGGL(ng_dots_kernel, dim3(NG_BLOCKS), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(ng_sum_kernel, dim3(1), dim3(64), 0, s, (const double *)partial, NG_BLOCKS, out5);
+    return hipGetLastError();
+}
+
+hipError_t launch_ng_apply(int n_work, int L, const int *work, double *x0, const double *x1, const double *x2,
+                           double ca, double cb, hipStream_t s) {
+    const long long n = (long long)n_work * L;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(ng_apply_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_work, L, work, x0, x1,
+                       x2, ca, cb);
+    return hipGetLastError();
+}
+
+size_t ng_partial_doubles() { return (size_t)NG_BLOCKS * 5 + 8; }
+
 }  // namespace portal
 
 namespace portal {

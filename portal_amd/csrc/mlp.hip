@@ -132,12 +132,30 @@ __global__ void split_reduce_kernel(const float *P, float *out, int S, long long
     out[i] = s;
 }
 
-// column sums of an M x N fp32 matrix (bias gradients), fixed order per column
-__global__ void colsum_kernel(const float *X, float *out, int M, int N) {
+// column sums of an M x N fp32 matrix (bias gradients) in two fixed-order stages:
+// block (x, y) sums columns 64x..64x+63 over row block y (each wave a quarter of its
+// rows, lane = column), then colsum_final_kernel adds the COLSUM_RB row blocks in order
+constexpr int COLSUM_RB = 16;
+
+__global__ void __launch_bounds__(256) colsum_kernel(const float *X, float *part, int M, int N) {
+    __shared__ float sh[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + lane;
+    const int rows = (M + COLSUM_RB - 1) / COLSUM_RB;
+    const int r0 = blockIdx.y * rows, r1 = min(M, r0 + rows);
+    float s = 0.0f;
+    if (n < N)
+        for (int m = r0 + w; m < r1; m += 4) s += X[(long long)m * N + n];
+    sh[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && n < N) part[(long long)blockIdx.y * N + n] = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
+}
+
+__global__ void colsum_final_kernel(const float *part, float *out, int N) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     float s = 0.0f;
-    for (int m = 0; m < M; m++) s += X[(long long)m * N + n];
+    for (int y = 0; y < COLSUM_RB; y++) s += part[(long long)y * N + n];
     out[n] = s;
 }
 
@@ -175,6 +193,20 @@ __global__ void mse_grad_kernel(const float *Y, const float *T, const float *mu,
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// acc += (sum of the block partials) / ny, in a fixed order (one block of 256)
+__global__ void __launch_bounds__(256) loss_acc_kernel(const float *part, int nbl, long long ny, double *acc) {
+    __shared__ double sh[256];
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nbl; b += 256) v += (double)part[b];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[0] += sh[0] / (double)ny;
 }
 
 __global__ void adam_kernel(float *w, const float *g, float *m, float *v, bf16 *w16, long long n, float lr, float b1,
@@ -216,6 +248,7 @@ struct Mlp {
     long long partial_n = 0;
     float *Xd = nullptr, *Td = nullptr, *Yd = nullptr;
     float *loss_part = nullptr;
+    double *loss_acc = nullptr;  // per-epoch loss sum on device
     long long step = 0;
 };
 
@@ -237,6 +270,7 @@ static void mlp_free_buffers(Mlp &m) {
     f(m.Td);
     f(m.Yd);
     f(m.loss_part);
+    f(m.loss_acc);
     m.cap = 0;
 }
 
@@ -261,6 +295,7 @@ static hipError_t mlp_reserve(Mlp &m, int B) {
     if ((e = hipMalloc((void **)&m.Td, sizeof(float) * (size_t)B * m.n_out))) return e;
     if ((e = hipMalloc((void **)&m.Yd, sizeof(float) * (size_t)B * m.n_out))) return e;
     if ((e = hipMalloc((void **)&m.loss_part, sizeof(float) * ((size_t)B * m.n_out / 256 + 2)))) return e;
+    if (!m.loss_acc && (e = hipMalloc((void **)&m.loss_acc, sizeof(double)))) return e;
     m.cap = B;
     return hipSuccess;
 }
@@ -328,8 +363,11 @@ static hipError_t mlp_backward_dev(Mlp &m, int B, hipStream_t s, float *loss_out
         const long long nw = (long long)din * dout;
         hipLaunchKernelGGL(split_reduce_kernel, dim3(blocks(nw)), dim3(256), 0, s, m.partial, m.g32 + m.woff[l], S,
                            nw);
-        hipLaunchKernelGGL(colsum_kernel, dim3((dout + 255) / 256), dim3(256), 0, s, m.dZ32, m.g32 + m.boff[l], B,
-                           dout);
+        // bias gradient; the split-K buffer is free again once split_reduce has run
+        hipLaunchKernelGGL(colsum_kernel, dim3((dout + 63) / 64, COLSUM_RB), dim3(256), 0, s, (const float *)m.dZ32,
+                           m.partial, B, dout);
+        hipLaunchKernelGGL(colsum_final_kernel, dim3((dout + 255) / 256), dim3(256), 0, s, (const float *)m.partial,
+                           m.g32 + m.boff[l], dout);
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W_l^T) * silu'(Z_{l-1})
             GemmArgs h{};
@@ -576,7 +614,10 @@ int portal_mlp_train(portal_ctx *x, int n, const float *feats, const float *targ
     for (int ep = 0; ep < epochs; ep++) {
         for (int i = 0; i < nb; i++) order[i] = i;
         std::shuffle(order.begin(), order.end(), rng);
-        double tot = 0;
+        // the loss is summed on the device and read once per epoch, so the steps queue
+        // back to back with no host synchronisation
+        HIP_TRY(&c, hipMemsetAsync(m.loss_acc, 0, sizeof(double), c.stream));
+        const long long ny = (long long)batch * m.n_out;
         for (int bi = 0; bi < nb; bi++) {
             const size_t o = (size_t)order[bi] * batch;
             HIP_TRY(&c, hipMemcpyAsync(m.Xd, Xall + o * m.n_in, sizeof(float) * (size_t)batch * m.n_in,
@@ -584,12 +625,18 @@ int portal_mlp_train(portal_ctx *x, int n, const float *feats, const float *targ
             HIP_TRY(&c, hipMemcpyAsync(m.Td, Tall + o * m.n_out, sizeof(float) * (size_t)batch * m.n_out,
                                        hipMemcpyDeviceToDevice, c.stream));
             HIP_TRY(&c, mlp_forward_dev(m, batch, c.stream));
-            float l = 0;
-            HIP_TRY(&c, mlp_backward_dev(m, batch, c.stream, loss_hist ? &l : nullptr));
+            HIP_TRY(&c, mlp_backward_dev(m, batch, c.stream, nullptr));
+            if (loss_hist)
+                hipLaunchKernelGGL(loss_acc_kernel, dim3(1), dim3(256), 0, c.stream, (const float *)m.loss_part,
+                                   (int)blocks(ny), ny, m.loss_acc);
             HIP_TRY(&c, mlp_adam(m, lr, c.stream));
-            tot += l;
         }
-        if (loss_hist) loss_hist[ep] = (float)(tot / std::max(1, nb));
+        if (loss_hist) {
+            double tot = 0;
+            HIP_TRY(&c, hipMemcpyAsync(&tot, m.loss_acc, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            HIP_TRY(&c, hipStreamSynchronize(c.stream));
+            loss_hist[ep] = (float)(tot / std::max(1, nb));
+        }
     }
     hipEventRecord(e1, c.stream);
     hipEventSynchronize(e1);

@@ -31,6 +31,7 @@ module portal_hip
     integer(c_int) :: surrogate
     real(c_double) :: fallback_tol
     integer(c_int) :: rebuild_paths
+    integer(c_int) :: ng_accel
   end type
 
   type, bind(C) :: portal_img_params
@@ -49,7 +50,7 @@ module portal_hip
     integer(c_long_long) :: n_cross, n_paths
     integer(c_int) :: iterations, fallbacks, lu_failures
     real(c_double) :: max_rel_change
-    integer(c_int) :: rt_variant, rt_waves, max_path
+    integer(c_int) :: rt_variant, rt_waves, max_path, ng_steps
   end type
 
   interface

@@ -40,7 +40,8 @@ class SeParams(C.Structure):
 
 class IterParams(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol_pop", C.c_double), ("tol_align", C.c_double),
-                ("surrogate", C.c_int), ("fallback_tol", C.c_double), ("rebuild_paths", C.c_int)]
+                ("surrogate", C.c_int), ("fallback_tol", C.c_double), ("rebuild_paths", C.c_int),
+                ("ng_accel", C.c_int)]
 
 
 class ImgParams(C.Structure):
@@ -56,7 +57,7 @@ class Stats(C.Structure):
                 ("rt_kernel_avg_ms", C.c_double), ("n_cross", C.c_longlong), ("n_paths", C.c_longlong),
                 ("iterations", C.c_int), ("fallbacks", C.c_int), ("lu_failures", C.c_int),
                 ("max_rel_change", C.c_double), ("rt_variant", C.c_int), ("rt_waves", C.c_int),
-                ("max_path", C.c_int)]
+                ("max_path", C.c_int), ("ng_steps", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

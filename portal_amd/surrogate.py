@@ -76,7 +76,7 @@ def load_into(pt, sp) -> bool:
     return True
 
 
-def make_dataset(pt, sp, g, xth, nvel, dx, max_iter=60, tol=1e-6):
+def make_dataset(pt, sp, g, xth, nvel, dx, max_iter=80, tol=1e-6, tol_align=1e-8):
     """(features, targets, iterations) of one cloud: first-pass features, converged solution."""
     from .lib import PORTAL_MIXED
     pt.upload_grid(g)
@@ -84,8 +84,9 @@ def make_dataset(pt, sp, g, xth, nvel, dx, max_iter=60, tol=1e-6):
     pt.inverse_rt(xth, nvel, dx, precision=PORTAL_MIXED)
     feats, nodes = pt.surrogate_features()
     pt.set_levpop(g.lev_p)
-    st = pt.lambda_iterate(xth, nvel, dx, max_iter, tol_pop=tol, tol_align=1e-9, precision=PORTAL_MIXED,
-                           outputs=False)
+    # Ng-accelerated: the same fixed point as the plain iteration, in fewer passes
+    st = pt.lambda_iterate(xth, nvel, dx, max_iter, tol_pop=tol, tol_align=tol_align, precision=PORTAL_MIXED,
+                           outputs=False, ng_accel=1)
     pp = pt.get_pol_pops()
     return feats, targets(pp, nodes, sp), st["iterations"]
 
@@ -99,7 +100,7 @@ def train(levels=10, clouds=8, nodes=200000, nth=9, nph=32, nvel=64, epochs=40, 
     pt = Portal(0)
     pt.upload_species(sp)
     X, Y = [], []
-    for k in range(clouds):
+    for k in range(clouds + 1):  # the last cloud is held out for validation
         t = time.time()
         rng = np.random.default_rng(seed0 + k)
         # the production recipe with the density slope and field direction varied per cloud
@@ -108,6 +109,7 @@ def train(levels=10, clouds=8, nodes=200000, nth=9, nph=32, nvel=64, epochs=40, 
         X.append(f)
         Y.append(y)
         log(f"cloud {k}: {len(f)} samples, converged in {it} iterations, {time.time() - t:.1f}s")
+    Xv, Yv = X.pop(), Y.pop()
     X = np.concatenate(X)
     Y = np.concatenate(Y)
     n_in, n_out = dims(sp)
@@ -119,11 +121,19 @@ def train(levels=10, clouds=8, nodes=200000, nth=9, nph=32, nvel=64, epochs=40, 
     hist1, ms1 = pt.mlp_train(X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed0)
     hist2, ms2 = pt.mlp_train(X, Y, epochs=max(1, epochs // 4), batch=batch, lr=lr / 5, seed=seed0 + 1)
     log(f"trained on {len(X)} samples: loss {hist1[0]:.3e} -> {hist2[-1]:.3e} in {(ms1 + ms2) / 1e3:.1f}s")
+    # held-out error: log10 of sqrt(2J+1) rho00 per level, and 1e3 rho20/rho00
+    P = pt.mlp_forward(Xv)
+    L = sp.e_num
+    e_pop = np.abs(P[:, :L] - Yv[:, :L])
+    e_al = np.abs(P[:, L:] - Yv[:, L:])
+    val = np.array([np.median(e_pop), np.quantile(e_pop, 0.99), e_pop.max(), np.median(e_al), e_al.max()])
+    log(f"held-out cloud: |d log10 pop| median {val[0]:.2e}, 99% {val[1]:.2e}, max {val[2]:.2e}; "
+        f"|d 1e3 rho20/rho00| median {val[3]:.2e}, max {val[4]:.2e}")
     params, nm = pt.mlp_get_params()
-    os.makedirs(WEIGHTS, exist_ok=True)
     path = out or weights_path(sp)
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     np.savez_compressed(path, params=params, norm=nm, width=WIDTH, depth=DEPTH, samples=len(X),
-                        loss=np.concatenate([hist1, hist2]), key=species_key(sp))
+                        loss=np.concatenate([hist1, hist2]), key=species_key(sp), heldout=val)
     pt.close()
     return path
 
@@ -137,7 +147,8 @@ if __name__ == "__main__":
     ap.add_argument("--nph", type=int, default=32)
     ap.add_argument("--nvel", type=int, default=64)
     ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--out", default=None, help="weights file (default: the species' file in surrogate_weights/)")
     a = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
-    print(train(a.levels, a.clouds, a.nodes, a.nth, a.nph, a.nvel, a.epochs,
+    print(train(a.levels, a.clouds, a.nodes, a.nth, a.nph, a.nvel, a.epochs, out=a.out,
                 log=lambda *x: print(*x, flush=True)))

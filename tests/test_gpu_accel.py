@@ -1,0 +1,49 @@
+"""Ng acceleration of the lambda iteration (needs an MI355X).
+
+The accelerated iteration must reach the fixed point of the plain one, in fewer
+iterations, in both precisions.
+"""
+import numpy as np
+import pytest
+
+from portal_amd import cloud
+from portal_amd.api import Portal
+from portal_amd.lib import PORTAL_FP64, PORTAL_MIXED
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small():
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 4000, seed=31)
+    xth = cloud.legendre_all(6, 11)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    yield sp, g, xth, pt
+    pt.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(PORTAL_FP64, 1e-9), (PORTAL_MIXED, 1e-6)])
+def test_ng_reaches_the_plain_fixed_point(small, precision, tol):
+    sp, g, xth, pt = small
+    kw = dict(tol_pop=tol, tol_align=tol * 1e-2, precision=precision)
+    pt.set_levpop(g.lev_p)
+    pp0, lev0, st0 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 400, **kw)
+    pt.set_levpop(g.lev_p)
+    pp1, lev1, st1 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 400, ng_accel=1, **kw)
+    print(f"precision {precision}: plain {st0['iterations']} iterations, Ng {st1['iterations']} "
+          f"({st1['ng_steps']} extrapolations)")
+    assert st0["iterations"] < 400 and st1["iterations"] < 400
+    assert st1["ng_steps"] > 0
+    assert st1["iterations"] < st0["iterations"]
+    idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+    big = lev0[:, idx] > 1e-8 * lev0[:, idx].max(axis=0)
+    rel = np.abs(lev1[:, idx] - lev0[:, idx]) / np.abs(lev0[:, idx])
+    # both stop within their tolerance of the fixed point; the remaining distance of a
+    # geometric iteration is tol / (1 - ratio), ratio ~0.9 on this cloud
+    assert rel[big].max() < 50 * tol
+    al0 = pp0[idx][:, 3::2] / pp0[idx][:, 2::2]
+    al1 = pp1[idx][:, 3::2] / pp1[idx][:, 2::2]
+    assert np.abs(al1 - al0).max() < 50 * tol
