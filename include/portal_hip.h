@@ -106,6 +106,8 @@ typedef struct {
     int rt_waves;            /* its waves per workgroup */
     int max_path;            /* longest inward path (cells) */
     int ng_steps;            /* Ng extrapolations applied by the lambda iteration */
+    int converged;           /* 1: the lambda iteration met both tolerances */
+    double max_align_change; /* last iteration's largest change of rho^2_0/rho^0_0 */
 } portal_stats;
 
 /* Context -------------------------------------------------------------------- */

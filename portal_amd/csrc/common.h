@@ -117,6 +117,10 @@ struct Ctx {
     int tuned_variant[2] = {0, 0}, tuned_waves[2] = {4, 4};
     double tune_ms[2][8] = {{0}};
     int forced_variant[2] = {-1, -1}, forced_waves[2] = {4, 4};  // test hook: bypass the autotuner
+    // d_pol holds the SE solution from which the current d_levp was formed (set by the
+    // lambda iteration; cleared by anything that changes the state otherwise), so that
+    // a lambda iteration continued by a further call measures its first change exactly
+    bool pol_chain = false;
 };
 
 // error flags (bits in d_flags[0])

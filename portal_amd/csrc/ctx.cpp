@@ -463,6 +463,7 @@ int portal_upload_grid(portal_ctx *x, int N, int N_b, const double *XYZ, const d
                        const int *del_ne, const int *del_num_ne, int maxne, double tx) {
     if (!x) return PORTAL_ERR_ARG;
     Ctx &c = x->c;
+    c.pol_chain = false;
     hipSetDevice(c.device);
     if (N <= 0 || N_b < 0 || N_b > N || maxne <= 0 || !XYZ || !vel || !temp || !b_all || !h2 || !ab || !dust ||
         !del_ne || !del_num_ne || !(tx > 0))
@@ -540,6 +541,7 @@ int portal_upload_species(portal_ctx *x, int e_num, const int *j_lev, const doub
                           const double *t_Cji, const double *col_temps, int kmax) {
     if (!x) return PORTAL_ERR_ARG;
     Ctx &c = x->c;
+    c.pol_chain = false;
     hipSetDevice(c.device);
     if (e_num <= 0 || r_num <= 0 || c_num < 0 || c_temp <= 0 || (c_par != 1 && c_par != 2) || kmax < 2 ||
         (kmax & 1) || !j_lev || !e_levs_hz || !r_trans || !t_Aji || !nu_hz || !t_Bij || !kd_v ||
@@ -639,6 +641,7 @@ int portal_species_info(const portal_ctx *x, int *n_trans, int *c_dim) {
 int portal_set_levpop(portal_ctx *x, const double *lev_p) {
     if (!x || !lev_p) return PORTAL_ERR_ARG;
     Ctx &c = x->c;
+    c.pol_chain = false;
     hipSetDevice(c.device);
     int rc = ensure_node_buffers(c);
     if (rc) return rc;
@@ -678,6 +681,7 @@ int portal_se_solve(portal_ctx *x, const double *J0, const double *J2, double *p
                     portal_stats *st) {
     if (!x || !J0 || !J2) return PORTAL_ERR_ARG;
     Ctx &c = x->c;
+    c.pol_chain = false;
     hipSetDevice(c.device);
     int rc = ensure_node_buffers(c);
     if (rc) return rc;
@@ -697,6 +701,7 @@ int portal_se_pass(portal_ctx *x, const portal_se_params *p, double *pol_pops, d
                    portal_stats *st) {
     if (!x) return PORTAL_ERR_ARG;
     Ctx &c = x->c;
+    c.pol_chain = false;
     hipSetDevice(c.device);
     int rc = check_se_params(c, p);
     if (rc) return rc;
@@ -856,18 +861,24 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
         HIP_TRY(&c, hipMalloc((void **)&d_ngp, sizeof(double) * ng_partial_doubles()));
     }
     double *hist[3] = {d_hist, d_hist ? d_hist + N * L : nullptr, d_hist ? d_hist + 2 * N * L : nullptr};
-    int nhist = 0, n_ng = 0;
+    int nhist = 0, n_ng = 0, last_ng = -1, conv = 0;
     const int nw = (int)c.g.work.size();
     Timer tt(c.stream);
     int iters = 0, nfb = 0;
     double dpop = 0, dal = 0;
+    // continuing an earlier call: its last SE solution is this call's first comparison
+    const bool chained = c.pol_chain && !it->surrogate;
+    if (chained)
+        HIP_TRY(&c, hipMemcpyAsync(d_prev, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToDevice, c.stream));
+    c.pol_chain = false;
     for (int k = 0; k < it->max_iter; k++) {
         rc = run_inward(c, p, st);
         if (!rc) rc = run_se(c, 0, st);
         if (rc) break;
         HIP_TRY(&c, hipMemsetAsync(d_max, 0, sizeof(unsigned long long) * 2, c.stream));
         HIP_TRY(&c, launch_lambda_update((int)c.g.work.size(), (int)N, (int)L, c.g.d_work, c.g.d_ab, c.g.d_h2,
-                                         c.sp.d_jlev, c.d_pol, k > 0 ? d_prev : nullptr, c.d_levp, d_max, c.stream));
+                                         c.sp.d_jlev, c.d_pol, (k > 0 || chained) ? d_prev : nullptr, c.d_levp, d_max,
+                                         c.stream));
         if (k == 0 && it->surrogate) {
             // the network's prediction of the converged state replaces the plain first
             // iterate wherever it passes the validity checks
@@ -881,7 +892,10 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
         std::memcpy(&dpop, &mx[0], 8);
         std::memcpy(&dal, &mx[1], 8);
         iters = k + 1;
-        if (k > 0 && dpop <= it->tol_pop && dal <= it->tol_align) break;
+        if ((k > 0 || chained) && dpop <= it->tol_pop && dal <= it->tol_align) {
+            conv = 1;
+            break;
+        }
         if (ng) {
             // history x1 = hist[0] (newest) .. x3 = hist[2]; x0 is the level densities just formed
             if (nhist == 3 && k >= 3) {
@@ -899,6 +913,7 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
                         HIP_TRY(&c, launch_ng_apply(nw, (int)L, c.g.d_work, c.d_levp, hist[0], hist[1], ca, cb,
                                                     c.stream));
                         n_ng++;
+                        last_ng = k;
                     }
                 }
                 nhist = 0;  // the extrapolated iterate starts a new history
@@ -917,12 +932,16 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     if (d_hist) hipFree(d_hist);
     if (d_ngp) hipFree(d_ngp);
     if (rc) return rc;
+    // the Ng history is not carried over, and an extrapolation leaves d_levp ahead of d_pol
+    c.pol_chain = n_ng == 0 || last_ng < iters - 1;
     if (st) {
         st->t_total_ms = ms;
         st->iterations = iters;
         st->max_rel_change = dpop;
         st->fallbacks = nfb;
         st->ng_steps = n_ng;
+        st->converged = conv;
+        st->max_align_change = dal;
     }
     if (pol_pops) HIP_TRY(&c, hipMemcpy(pol_pops, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToHost));
     if (lev_p_out) HIP_TRY(&c, hipMemcpy(lev_p_out, c.d_levp, sizeof(double) * N * L, hipMemcpyDeviceToHost));
@@ -1086,6 +1105,7 @@ int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *p
     a.U = outb + 2 * cube;
     a.TAU = outb + 3 * cube;
     a.dust_out = dust ? outb + 4 * cube : nullptr;
+    a.mixed = p->precision == PORTAL_MIXED ? 1 : 0;
     float ms_g = 0, ms_r = 0;
     HIP_TRY(&c, launch_image(a, c.stream, &ms_g, &ms_r));
     int flags = 0;
@@ -1126,6 +1146,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         Ctx &c = xs[m]->c;
         int rc = check_se_params(c, &params[m]);
         if (rc) return fail(c0, rc, "model " + std::to_string(m) + ": " + c.err);
+        c.pol_chain = false;
     }
     if (total) std::memset(total, 0, sizeof(*total));
     if (per_model) std::memset(per_model, 0, sizeof(portal_stats) * n);

@@ -34,6 +34,7 @@ struct ImgArgs {
     int *flags;
     // outputs: n_rt x nlines x C
     double *I, *Q, *U, *TAU, *dust_out;
+    int mixed;                 // 1: Stokes transfer in FP32 (stokes_f32_kernel)
 };
 
 hipError_t launch_image(const ImgArgs &a, hipStream_t s, float *ms_geom, float *ms_rt);

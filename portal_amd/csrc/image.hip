@@ -310,6 +310,112 @@ __global__ void __launch_bounds__(256) stokes_kernel(ImgArgs a) {
     }
 }
 
+// (1 - e^-x)/x, by its series below |x| = 1e-2 (relative error < 1e-8)
+__device__ __forceinline__ float phi1_f(float x, float ex) {
+    const float series = fmaf(x, fmaf(x, 0.16666667f, -0.5f), 1.0f);
+    return fabsf(x) < 1e-2f ? series : __fdividef(1.0f - ex, x);
+}
+
+// Mixed-precision Stokes transfer: the same cell walk, profile, rotations and skip
+// test as stokes_kernel, evaluated in FP32.  The 2x2 propagator of polarize_prop
+// (raytrace.f90:1-56) is applied in the basis X = I + Q, Y = I - Q, where it is two
+// scalar transfer problems with absorption eta_I +- eta_Q:
+//   X' = X e^-m + (eps_I + eps_Q) ds (1 - e^-m)/m,  m = (eta_I + eta_Q) ds,
+//   Y' = Y e^-p + (eps_I - eps_Q) ds (1 - e^-p)/p,  p = (eta_I - eta_Q) ds,
+//   U' = U e^-(eta_I ds);
+// this is the closed form of the reference in exact arithmetic, without its
+// cancellation 1 - (...) e^-tau at small optical depth.  Intensities are carried in
+// units of the line's background intensity I_in, so that emissivities of weak lines
+// stay in the FP32 normal range; the optional dust branch stays FP64.
+__global__ void __launch_bounds__(256) stokes_f32_kernel(ImgArgs a) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)a.n_rt * a.nlines * a.C;
+    if (id >= total) return;
+    const int k = (int)(id % a.C);
+    const long long pl = id / a.C;
+    const int j = (int)(pl % a.nlines);
+    const int pix = (int)(pl / a.nlines);
+    const int nc = a.rec_n[pix];
+    const int *cells = a.rec_cell + (size_t)pix * a.n_depth_rt;
+    const int *steps = a.rec_step + (size_t)pix * a.n_depth_rt;
+    const double nu0 = a.nu0[j];
+    const double I_in = a.Iin[j];
+    // eta scale c/nu (float literal as in raytrace.f90), eps additionally / I_in
+    const float cnu = (float)((double)2.99792E8f / nu0);
+    const float cnu_e = (float)((double)2.99792E8f / nu0 / I_in);
+    const float vel_obs = (float)a.vel_out[k];
+    const float rspi = 0.56418958f;  // 1/sqrt(pi)
+    const double kd = a.kd[j], alpha = a.alpha;
+    float I = 1.0f, Q_in = 0.0f, U_in = 0.0f, Q_out = 0.0f, U_out = 0.0f, tau = 0.0f;
+    double I_dust = I_in, sig0 = 0.0, sig2 = 0.0, Qd = 0.0, Ud = 0.0, tau_dust_tot = 0.0;
+    const double dxl = a.dl_scale;
+    for (int i = 0; i < nc - 1; i++) {
+        const int c = cells[i];
+        const float ib = (float)(1.0 / a.b_all[c]);
+        const float x = (vel_obs - (float)a.vz_r[c]) * ib;
+        const float v_corr = ib * rspi * __expf(-(x * x));
+        const CellGeo cg = a.geo[c];
+        const float c2a = (float)cg.c2a, s2a = (float)cg.s2a;
+        Q_out = fmaf(Q_in, c2a, U_in * s2a);
+        U_out = fmaf(U_in, c2a, -(Q_in * s2a));
+        const double4 pc = a.polco[(size_t)c * a.nlines + j];
+        const float eta_I = cnu * v_corr * (float)pc.x;
+        const float eta_Q = cnu * v_corr * (float)pc.y;
+        const float eps_I = cnu_e * v_corr * (float)pc.z;
+        const float eps_Q = cnu_e * v_corr * (float)pc.w;
+        const double delta_x = dxl * (double)(steps[i + 1] - steps[i]);
+        const float ds = (float)delta_x;
+        if (ds * eta_I > 1e-30f) {
+            const float m = (eta_I + eta_Q) * ds, p = (eta_I - eta_Q) * ds, t = eta_I * ds;
+            const float em = __expf(-m), ep = __expf(-p), et = __expf(-t);
+            const float X = fmaf(I + Q_out, em, (eps_I + eps_Q) * ds * phi1_f(m, em));
+            const float Y = fmaf(I - Q_out, ep, (eps_I - eps_Q) * ds * phi1_f(p, ep));
+            I = 0.5f * (X + Y);
+            Q_out = 0.5f * (X - Y);
+            U_out = U_out * et;
+            tau = tau + t;
+        }
+        const double tau_dust = kd * a.dust[c] * delta_x;
+        if (tau_dust > 1e-30) {
+            const double I_bb = black_body_d(a.tdust[c], nu0);
+            const double e_dust = exp(-tau_dust);
+            const double sI = 0.5 * alpha * (cg.mu2 - 2.0 / 3.0);
+            const double sQ = alpha * cg.mu2;
+            I_dust = I_dust * e_dust + I_bb * (1.0 - e_dust);
+            sig0 = sig0 * e_dust + I_bb * (1.0 - e_dust);
+            sig2 = sig2 * e_dust + sI * I_bb * (1.0 - e_dust);
+            Qd = Qd * e_dust + sQ * I_bb * (1.0 - e_dust) * cg.c2p;
+            Ud = Ud * e_dust + sQ * I_bb * (1.0 - e_dust) * cg.s2p;
+            tau_dust_tot = tau_dust_tot + tau_dust;
+        }
+        Q_in = fmaf(Q_out, c2a, -(U_out * s2a));
+        U_in = fmaf(U_out, c2a, Q_out * s2a);
+    }
+    const size_t o = (size_t)pix + (size_t)a.n_rt * ((size_t)j + (size_t)a.nlines * k);
+    a.I[o] = (double)(I - 1.0f) * I_in;
+    a.Q[o] = (double)Q_in * I_in;
+    a.U[o] = (double)U_in * I_in;
+    a.TAU[o] = (double)tau;
+    if (a.dust_out) {
+        I_dust = I_dust - I_in;
+        double Q_dust, U_dust, tdust;
+        if (I_dust == 0.0) {
+            Q_dust = 0.0;
+            U_dust = 0.0;
+            tdust = 0.0;
+        } else {
+            Q_dust = I_dust * Qd / (sig0 - sig2);
+            U_dust = I_dust * Ud / (sig0 - sig2);
+            tdust = tau_dust_tot;
+        }
+        const size_t cube = (size_t)a.n_rt * a.nlines * a.C;
+        a.dust_out[o] = I_dust;
+        a.dust_out[cube + o] = Q_dust;
+        a.dust_out[2 * cube + o] = U_dust;
+        a.dust_out[3 * cube + o] = tdust;
+    }
+}
+
 hipError_t launch_image(const ImgArgs &a, hipStream_t s, float *ms_geom, float *ms_rt) {
     hipEvent_t e0, e1, e2;
     hipEventCreate(&e0);
@@ -323,7 +429,10 @@ hipError_t launch_image(const ImgArgs &a, hipStream_t s, float *ms_geom, float *
     hipLaunchKernelGGL(march_kernel, dim3((a.n_rt + 255) / 256), dim3(256), 0, s, a);
     hipEventRecord(e1, s);
     const long long nt = (long long)a.n_rt * a.nlines * a.C;
-    hipLaunchKernelGGL(stokes_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a);
+    if (a.mixed)
+        hipLaunchKernelGGL(stokes_f32_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(stokes_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a);
     hipEventRecord(e2, s);
     hipError_t err = hipEventSynchronize(e2);
     if (ms_geom) hipEventElapsedTime(ms_geom, e0, e1);

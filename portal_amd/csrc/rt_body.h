@@ -32,19 +32,31 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks) {
 template <typename Real>
 struct Arith;
 
-// Per (cell, line) the staging stores L = log(K dl) (natural log in FP64, log2 in
-// FP32) so that tau = K dl exp(-u^2) is one exponential: exp(L - u^2).  A cell with
-// K <= 0 or K = NaN gets L = NaN or -inf, i.e. tau NaN or 0, which the tau > 1e-30
-// test skips exactly as the reference skips its negative, zero or NaN tau.
+// Staged quantities and the segment update.  Per cell the staging stores vb = s v.dir/b
+// and inv_b = s/b, per (cell, line) L and S, so that a segment's profile argument is
+// u = fma(v0 + v_node, inv_b, -vb) and its optical depth one exponential of L - u^2.
+//   FP64:  s = 1, L = log(K dl), tau = exp(L - u^2); a segment is skipped unless
+//          tau > 1e-30 (false for NaN), the reference's test.
+//   mixed: s = sqrt(log2 e), L = log2(K dl log2 e), so exp2(L - u^2) = tau log2 e
+//          ("tau2") and exp2(-tau2) = e^-tau with no further scaling.  A (cell, line)
+//          with K dl not > 0 (zero, negative or NaN: skipped by the reference) is
+//          staged as L = -inf, S = 0, which makes its update an exact no-op, so the
+//          mixed update needs no skip test; a finite tau below 1e-30 changes nothing
+//          at FP32 resolution.
 template <>
 struct Arith<double> {
     typedef double2 R2;
     static constexpr bool kahan = false;
+    static constexpr bool skip_test = true;
+    static constexpr double vscale = 1.0;
     static constexpr double tr_floor = 0.0;  // FP64: a channel stops only at exactly zero transmission
     __device__ static double ex(double x) { return exp(x); }
-    __device__ static double lg(double x) { return log(x); }
-    __device__ static double tau_of(double L, double u) { return exp(L - u * u); }
     __device__ static double one_minus(double tau, double et) { return 1.0 - et; }
+    __device__ static R2 stage(R2 k, double dl, bool) { return double2{log(k.x * dl), k.y}; }
+    __device__ static bool cell_ok(double, double) { return true; }
+    __device__ static double tau_of(double L, double u) { return exp(L - u * u); }
+    __device__ static double att(double tau) { return exp(-tau); }
+    __device__ static double om(double tau, double et) { return 1.0 - et; }
     __device__ static double thr() { return 1e-30; }
     __device__ static R2 ks(const RtArgs &a, size_t i) { return *reinterpret_cast<const double2 *>(a.KS + 2 * i); }
 };
@@ -53,22 +65,41 @@ template <>
 struct Arith<float> {
     typedef float2 R2;
     static constexpr bool kahan = true;
+    static constexpr bool skip_test = false;
+    static constexpr float vscale = 1.2011224087864498f;  // sqrt(log2 e)
     // mixed: a channel whose transmission fell below 1e-9 stops (the dropped tail is
     // below 1e-9 of the remaining source function)
     static constexpr float tr_floor = 1e-9f;
     __device__ static float ex(float x) { return __expf(x); }
-    __device__ static float lg(float x) { return __log2f(x); }
-    // v_exp_f32 directly: a result below the FP32 normal range is < 1e-30 and skipped anyway
-    __device__ static float tau_of(float L, float u) { return __builtin_amdgcn_exp2f(fmaf(u * u, -1.44269504f, L)); }
     // 1 - exp(-tau) without cancellation for small tau.  Both forms are evaluated and
-    // one is picked by a bit select, so that no branch splits the segment update.
+    // one is picked by a bit select, so that no branch splits the update.
     __device__ static float one_minus(float tau, float et) {
         const float series = tau * (1.0f - tau * (0.5f - tau * (0.16666667f - tau * 0.041666668f)));
         const float direct = 1.0f - et;
         const int m = -(int)(tau < 1e-2f);
         return __int_as_float((__float_as_int(series) & m) | (__float_as_int(direct) & ~m));
     }
-    __device__ static float thr() { return 1e-30f; }
+    __device__ static R2 stage(R2 k, float dl, bool cell) {
+        const float kdl = k.x * dl;
+        const bool pos = cell && kdl > 0.0f;
+        return float2{pos ? __log2f(kdl * 1.44269504f) : -INFINITY, pos ? k.y : 0.0f};
+    }
+    // a cell whose profile argument is not finite (b = 0 gives 1/b = inf and, with
+    // v.dir = 0, a NaN product; the reference's tau is then NaN and skipped) is staged
+    // with zero profile terms and L = -inf, an exact no-op
+    __device__ static bool cell_ok(float vb, float inv_b) { return __builtin_isfinite(vb) && __builtin_isfinite(inv_b); }
+    // v_exp_f32 directly: a result below the FP32 normal range is below 1e-30
+    __device__ static float tau_of(float L, float u) { return __builtin_amdgcn_exp2f(fmaf(-u, u, L)); }
+    __device__ static float att(float tau2) { return __builtin_amdgcn_exp2f(-tau2); }
+    // 1 - e^-tau from tau2 = tau log2 e: tau (1 - tau/2 + tau^2/6) below tau = 1e-2
+    // (relative error < 5e-8), 1 - e^-tau above
+    __device__ static float om(float tau2, float et) {
+        const float series = tau2 * (0.693147182f - tau2 * (0.240226507f - tau2 * 0.0555041087f));
+        const float direct = 1.0f - et;
+        const int m = -(int)(tau2 < 1.44269504e-2f);
+        return __int_as_float((__float_as_int(series) & m) | (__float_as_int(direct) & ~m));
+    }
+    __device__ static float thr() { return 1.44269504e-30f; }  // 1e-30 in units of tau2
     __device__ static R2 ks(const RtArgs &a, size_t i) { return *reinterpret_cast<const float2 *>(a.KS32 + 2 * i); }
 };
 
@@ -119,16 +150,14 @@ __device__ __forceinline__ void rt_staged_node(const RtArgs &a, const int bid, c
                 const PathEntry e = P[i];
                 CellRec<Real> c;
                 const double ib = a.inv_b[e.cell];
-                c.vb = (Real)(e.vray * ib);
-                c.inv_b = (Real)ib;
+                c.vb = (Real)(e.vray * ib) * (Real)A::vscale;
+                c.inv_b = (Real)ib * (Real)A::vscale;
+                const bool ok = A::cell_ok(c.vb, c.inv_b);
+                if (!ok) c.vb = c.inv_b = (Real)0;
                 c.dl = (Real)e.len * dls;
                 c.pad = (Real)e.vray;
                 cr[i] = c;
-                for (int l = 0; l < R; l++) {
-                    R2 k = A::ks(a, (size_t)e.cell * R + l);
-                    k.x = A::lg(k.x * c.dl);
-                    ks[i * R + l] = k;
-                }
+                for (int l = 0; l < R; l++) ks[i * R + l] = A::stage(A::ks(a, (size_t)e.cell * R + l), c.dl, ok);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -173,8 +202,8 @@ __device__ __forceinline__ void rt_staged_node(const RtArgs &a, const int bid, c
                         const Real u = fma(v0s[q], c.inv_b, -c.vb);
                         const Real tau = A::tau_of(k.x, u);
                         if (tau > A::thr() && Tr[q] != (Real)0) {
-                            const Real et = A::ex(-tau);
-                            Em[q] = Em[q] + Tr[q] * k.y * A::one_minus(tau, et);
+                            const Real et = A::att(tau);
+                            Em[q] = Em[q] + Tr[q] * k.y * A::om(tau, et);
                             Tr[q] = Tr[q] * et;
                         }
                     }

@@ -11,9 +11,10 @@
 // and a lane holds SLOTS accumulators (not all of them, as in the per-wave form),
 // which keeps the register count, and with it occupancy, independent of R C.
 //
-// The segment update is branch-free: tau and e^-tau are evaluated for every lane and
-// the update is selected when tau > 1e-30 (false for NaN), which is the
-// reference's skip; with tau <= 1e-30 or NaN the selected values are the old ones.
+// The segment update is branch-free: tau and e^-tau are evaluated for every lane.  In
+// FP64 the update is selected when tau > 1e-30 (false for NaN), which is the
+// reference's skip; in mixed precision the staging makes skipped segments exact
+// no-ops (Arith<float>::stage), so no test is needed.
 // The check for exhausted transmission (every lane's Tr exactly zero, or below the
 // mixed floor) runs every fourth cell; cells visited after a lane's Tr reached zero
 // add exactly zero.  Per pair the sum over rays is in ray order; the channel sum
@@ -53,13 +54,13 @@ __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const
         const PathEntry e = P[i];
         const double ib = a.inv_b[e.cell];
         const Real dl = (Real)e.len * dls;
-        R2 k = A::ks(a, (size_t)e.cell * R + l);
-        k.x = A::lg(k.x * dl);
-        ks[j] = k;
+        const Real vb = (Real)(e.vray * ib) * (Real)A::vscale, sib = (Real)ib * (Real)A::vscale;
+        const bool ok = A::cell_ok(vb, sib);
+        ks[j] = A::stage(A::ks(a, (size_t)e.cell * R + l), dl, ok);
         if (l == 0) {
             CellRec<Real> c;
-            c.vb = (Real)(e.vray * ib);
-            c.inv_b = (Real)ib;
+            c.vb = ok ? vb : (Real)0;
+            c.inv_b = ok ? sib : (Real)0;
             c.dl = dl;
             c.pad = (Real)e.vray;
             cr[i] = c;
@@ -142,12 +143,17 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                 const R2 k = kr[lo[q]];
                 const Real u = fma(v0s[q], c.inv_b, -c.vb);
                 const Real tau = A::tau_of(k.x, u);
-                const bool ok = tau > A::thr();
-                const Real et = A::ex(-tau);
-                const Real em = Em[q] + Tr[q] * k.y * A::one_minus(tau, et);
+                const Real et = A::att(tau);
+                const Real em = Em[q] + Tr[q] * k.y * A::om(tau, et);
                 const Real tr = Tr[q] * et;
-                Em[q] = ok ? em : Em[q];
-                Tr[q] = ok ? tr : Tr[q];
+                if constexpr (A::skip_test) {
+                    const bool ok = tau > A::thr();
+                    Em[q] = ok ? em : Em[q];
+                    Tr[q] = ok ? tr : Tr[q];
+                } else {
+                    Em[q] = em;
+                    Tr[q] = tr;
+                }
             }
         }
 #pragma unroll

@@ -50,7 +50,8 @@ module portal_hip
     integer(c_long_long) :: n_cross, n_paths
     integer(c_int) :: iterations, fallbacks, lu_failures
     real(c_double) :: max_rel_change
-    integer(c_int) :: rt_variant, rt_waves, max_path, ng_steps
+    integer(c_int) :: rt_variant, rt_waves, max_path, ng_steps, converged
+    real(c_double) :: max_align_change
   end type
 
   interface

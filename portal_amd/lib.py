@@ -57,7 +57,8 @@ class Stats(C.Structure):
                 ("rt_kernel_avg_ms", C.c_double), ("n_cross", C.c_longlong), ("n_paths", C.c_longlong),
                 ("iterations", C.c_int), ("fallbacks", C.c_int), ("lu_failures", C.c_int),
                 ("max_rel_change", C.c_double), ("rt_variant", C.c_int), ("rt_waves", C.c_int),
-                ("max_path", C.c_int), ("ng_steps", C.c_int)]
+                ("max_path", C.c_int), ("ng_steps", C.c_int), ("converged", C.c_int),
+                ("max_align_change", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -47,3 +47,29 @@ def test_ng_reaches_the_plain_fixed_point(small, precision, tol):
     al0 = pp0[idx][:, 3::2] / pp0[idx][:, 2::2]
     al1 = pp1[idx][:, 3::2] / pp1[idx][:, 2::2]
     assert np.abs(al1 - al0).max() < 50 * tol
+
+
+def test_a_continued_call_is_the_same_iteration(small):
+    # 4 + 3 iterations in two calls equal 7 in one; the tolerance test of the second call
+    # compares with the first call's last SE solution
+    sp, g, xth, pt = small
+    pt.set_levpop(g.lev_p)
+    pp1, lev1, st1 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 7, tol_pop=0.0, tol_align=0.0)
+    pt.set_levpop(g.lev_p)
+    pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 4, tol_pop=0.0, tol_align=0.0)
+    pp2, lev2, st2 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 3, tol_pop=0.0, tol_align=0.0)
+    np.testing.assert_array_equal(lev2, lev1)
+    np.testing.assert_array_equal(pp2, pp1)
+    assert st2["max_rel_change"] == st1["max_rel_change"]
+    # the iteration at which a tolerance is met does not depend on where calls split
+    kw = dict(tol_pop=1e-4, tol_align=1e-6)
+    pt.set_levpop(g.lev_p)
+    n_one = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 200, outputs=False, **kw)["iterations"]
+    pt.set_levpop(g.lev_p)
+    done = 0
+    while True:
+        st = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 5, outputs=False, **kw)
+        done += st["iterations"]
+        if st["converged"] or done > 200:
+            break
+    assert done == n_one

@@ -253,6 +253,25 @@ def test_image_pass_fp64_parity(portal1, cfg1, oracle, oracle_cfg1):
     assert np.abs(Qo).max() > 0  # polarisation is present
 
 
+def test_image_pass_mixed_error(portal1, cfg1, oracle_cfg1):
+    # FP32 Stokes transfer on the same populations: error on the scale of max |I|
+    from portal_amd import driver
+    sp, g, xth = cfg1
+    pp = driver.rescale_pol_pops(oracle_cfg1[0], g.lev_p, sp.lev_g)
+    lines = driver.image_lines(sp, [1, 2], [2, 3])
+    vel_out = driver.channel_velocities(32, 93.75)
+    args = (pp, lines, np.pi / 4, 0.0, 32, vel_out, 24, 0.7, cloud.DX_IMAGE)
+    (I, Q, U, T), _ = portal1.image_pass(*args, n_depth_rt=2048)
+    (Im, Qm, Um, Tm), _ = portal1.image_pass(*args, n_depth_rt=2048, precision=PORTAL_MIXED)
+    s = np.abs(I).max()
+    e = [np.abs(Im - I).max() / s, np.abs(Qm - Q).max() / s, np.abs(Um - U).max() / s]
+    print(f"mixed image: max |dI|, |dQ|, |dU| / max |I| = {e[0]:.2e} {e[1]:.2e} {e[2]:.2e}; "
+          f"max rel dTAU {np.abs(Tm - T).max() / np.abs(T).max():.2e}")
+    assert max(e) < 1e-5
+    np.testing.assert_allclose(Tm, T, rtol=1e-5, atol=1e-6 * np.abs(T).max())
+    assert np.abs(Qm).max() > 0
+
+
 def test_image_pass_dust_branch(cfg1, oracle):
     # dust present in the image pass (the inward pass rejects dust): both branches agree
     from portal_amd import driver
