@@ -67,8 +67,9 @@ typedef struct {
     int surrogate;           /* 1: the trained MLP supplies the first iterate */
     double fallback_tol;     /* reserved (the surrogate iterate is rejected per node on the validity tests) */
     int rebuild_paths;       /* 1: rebuild the inward paths even if the geometry is unchanged */
-    int ng_accel;            /* 1: Ng acceleration of the level densities from the last four iterates,
-                                every third iteration from the fourth on */
+    int ng_accel;            /* 0: plain iteration; 1: Ng acceleration of the level densities from the
+                                last four iterates, every third iteration from the fourth on;
+                                2: Anderson acceleration over a window of 4, every iteration */
 } portal_iter_params;
 
 typedef struct {

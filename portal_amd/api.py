@@ -149,6 +149,12 @@ class _Core:
         self._check(self.lib.portal_lambda_iterate(self.ctx, C.byref(p), C.byref(it), _d(pp), _d(lev), C.byref(st)))
         return pp, lev, st.as_dict()
 
+    def iter_trace(self, max_iter=1000):
+        """(population change, alignment change) of each iteration of the last lambda call."""
+        out = np.zeros(2 * max_iter)
+        n = self.lib.portal_debug_iter_trace(self.ctx, _d(out), max_iter)
+        return out[: 2 * n].reshape(n, 2)
+
     def force_rt_kernel(self, precision, variant, waves=4):
         """Test hook: fix the inward-integration kernel (0 global, 1 staged, 2 cooperative)
         and its waves per workgroup for one precision; variant -1 returns to autotuning."""

@@ -121,6 +121,7 @@ struct Ctx {
     // lambda iteration; cleared by anything that changes the state otherwise), so that
     // a lambda iteration continued by a further call measures its first change exactly
     bool pol_chain = false;
+    std::vector<double> iter_trace;  // per iteration of the last lambda call: population change, alignment change
 };
 
 // error flags (bits in d_flags[0])
@@ -246,6 +247,14 @@ hipError_t launch_ng_dots(int n_work, int L, const int *work, const double *x0, 
 hipError_t launch_ng_apply(int n_work, int L, const int *work, double *x0, const double *x1, const double *x2,
                            double ca, double cb, hipStream_t s);
 size_t ng_partial_doubles();
+constexpr int AA_M_MAX = 4;                               // Anderson window
+constexpr int AA_NSUM = AA_M_MAX * (AA_M_MAX + 1) / 2 + AA_M_MAX;  // dF.dF (upper) and dF.f sums
+hipError_t launch_aa_gather(int n_work, int L, const int *work, const double *levp, double *dst, hipStream_t s);
+hipError_t launch_aa_dots(long long n, int mk, const double *x0, const double *g0, const double *const *xj,
+                          const double *const *gj, double *partial, double *out, hipStream_t s);
+hipError_t launch_aa_apply(int n_work, int L, int mk, const int *work, const double *g0, const double *const *gj,
+                           const double *gamma, double *levp, hipStream_t s);
+size_t aa_partial_doubles();
 
 }  // namespace portal
 

@@ -784,6 +784,15 @@ int portal_debug_tune_ms(const portal_ctx *x, int precision, double *out8) {
     return PORTAL_OK;
 }
 
+// the population and alignment changes of each iteration of the last lambda call
+// (2 doubles per iteration, at most max_iter iterations); returns the iteration count
+int portal_debug_iter_trace(const portal_ctx *x, double *out, int max_iter) {
+    if (!x || !out || max_iter < 0) return PORTAL_ERR_ARG;
+    const int n = (int)std::min<size_t>(x->c.iter_trace.size() / 2, (size_t)max_iter);
+    for (int i = 0; i < 2 * n; i++) out[i] = x->c.iter_trace[i];
+    return n;
+}
+
 // test hook: run the inward integration of one precision with a fixed kernel variant
 // (0 global, 1 staged, 2 cooperative) and wave count; variant -1 returns to autotuning
 int portal_debug_force_rt(portal_ctx *x, int precision, int variant, int waves) {
@@ -853,13 +862,24 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     unsigned long long *d_max = nullptr;
     HIP_TRY(&c, hipMalloc((void **)&d_prev, sizeof(double) * N * 2 * L));
     HIP_TRY(&c, hipMalloc((void **)&d_max, sizeof(unsigned long long) * 2));
-    // Ng acceleration: the three previous level-density iterates and the reduction scratch
-    const bool ng = it->ng_accel != 0;
-    double *d_hist = nullptr, *d_ngp = nullptr;
+    // acceleration: 1 Ng (the three previous level-density iterates), 2 Anderson (the
+    // window of (x, G(x)) pairs, compact over solved nodes)
+    if (it->ng_accel < 0 || it->ng_accel > 2) return fail(c, PORTAL_ERR_ARG, "ng_accel must be 0, 1 or 2");
+    const bool ng = it->ng_accel == 1, aa = it->ng_accel == 2;
+    double *d_hist = nullptr, *d_ngp = nullptr, *d_aa = nullptr;
     if (ng) {
         HIP_TRY(&c, hipMalloc((void **)&d_hist, sizeof(double) * N * L * 3));
         HIP_TRY(&c, hipMalloc((void **)&d_ngp, sizeof(double) * ng_partial_doubles()));
     }
+    const long long n_aa = (long long)c.g.work.size() * (long long)L;
+    constexpr int AW = AA_M_MAX + 1;  // pairs kept
+    if (aa) {
+        HIP_TRY(&c, hipMalloc((void **)&d_aa, sizeof(double) * (size_t)n_aa * 2 * AW));
+        HIP_TRY(&c, hipMalloc((void **)&d_ngp, sizeof(double) * aa_partial_doubles()));
+    }
+    auto aa_x = [&](int slot) { return d_aa + (size_t)n_aa * slot; };
+    auto aa_g = [&](int slot) { return d_aa + (size_t)n_aa * (AW + slot); };
+    int aa_cnt = 0, aa_pos = 0;
     double *hist[3] = {d_hist, d_hist ? d_hist + N * L : nullptr, d_hist ? d_hist + 2 * N * L : nullptr};
     int nhist = 0, n_ng = 0, last_ng = -1, conv = 0;
     const int nw = (int)c.g.work.size();
@@ -871,7 +891,9 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     if (chained)
         HIP_TRY(&c, hipMemcpyAsync(d_prev, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToDevice, c.stream));
     c.pol_chain = false;
+    c.iter_trace.clear();
     for (int k = 0; k < it->max_iter; k++) {
+        if (aa) HIP_TRY(&c, launch_aa_gather(nw, (int)L, c.g.d_work, c.d_levp, aa_x(aa_pos % AW), c.stream));
         rc = run_inward(c, p, st);
         if (!rc) rc = run_se(c, 0, st);
         if (rc) break;
@@ -891,10 +913,78 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
         HIP_TRY(&c, hipStreamSynchronize(c.stream));
         std::memcpy(&dpop, &mx[0], 8);
         std::memcpy(&dal, &mx[1], 8);
+        c.iter_trace.push_back(dpop);
+        c.iter_trace.push_back(dal);
         iters = k + 1;
+        if (aa && !(k == 0 && it->surrogate)) {
+            // the pair (x_k, G(x_k)); the surrogate's replacement of the first iterate is not one
+            HIP_TRY(&c, launch_aa_gather(nw, (int)L, c.g.d_work, c.d_levp, aa_g(aa_pos % AW), c.stream));
+            aa_pos++;
+            aa_cnt = std::min(aa_cnt + 1, AW);
+        }
         if ((k > 0 || chained) && dpop <= it->tol_pop && dal <= it->tol_align) {
             conv = 1;
             break;
+        }
+        if (aa && aa_cnt >= 2) {
+            const int mk = aa_cnt - 1, s0 = (aa_pos - 1) % AW;
+            const double *xj[AA_M_MAX], *gj[AA_M_MAX];
+            for (int j = 0; j < mk; j++) {
+                const int sj = (aa_pos - 1 - (j + 1)) % AW;
+                xj[j] = aa_x(sj);
+                gj[j] = aa_g(sj);
+            }
+            double sums[AA_NSUM];
+            double *d_out = d_ngp + aa_partial_doubles() - 32;
+            HIP_TRY(&c, launch_aa_dots(n_aa, mk, aa_x(s0), aa_g(s0), xj, gj, d_ngp, d_out, c.stream));
+            HIP_TRY(&c, hipMemcpyAsync(sums, d_out, sizeof(sums), hipMemcpyDeviceToHost, c.stream));
+            HIP_TRY(&c, hipStreamSynchronize(c.stream));
+            // normal equations A gamma = b (A: dF.dF, b: dF.f), solved with partial pivoting
+            double A[AA_M_MAX][AA_M_MAX + 1];
+            int t = 0;
+            for (int i = 0; i < AA_M_MAX; i++) {
+                for (int j = i; j < AA_M_MAX; j++) A[i][j] = A[j][i] = sums[t++];
+                A[i][AA_M_MAX] = sums[t++];
+            }
+            double gam[AA_M_MAX] = {0, 0, 0, 0};
+            bool ok = true;
+            double dmax = 0;
+            for (int i = 0; i < mk; i++) dmax = std::max(dmax, A[i][i]);
+            ok = std::isfinite(dmax) && dmax > 0;
+            for (int i = 0; i < mk && ok; i++) A[i][i] += 1e-12 * dmax;
+            double M[AA_M_MAX][AA_M_MAX + 1];
+            for (int i = 0; i < mk; i++) {
+                for (int j = 0; j < mk; j++) M[i][j] = A[i][j];
+                M[i][mk] = A[i][AA_M_MAX];
+            }
+            for (int col = 0; col < mk && ok; col++) {
+                int piv = col;
+                for (int r = col + 1; r < mk; r++)
+                    if (std::fabs(M[r][col]) > std::fabs(M[piv][col])) piv = r;
+                if (!(std::fabs(M[piv][col]) > 1e-300)) {
+                    ok = false;
+                    break;
+                }
+                if (piv != col)
+                    for (int j = 0; j <= mk; j++) std::swap(M[col][j], M[piv][j]);
+                for (int r = col + 1; r < mk; r++) {
+                    const double f = M[r][col] / M[col][col];
+                    for (int j = col; j <= mk; j++) M[r][j] -= f * M[col][j];
+                }
+            }
+            for (int i = mk - 1; i >= 0 && ok; i--) {
+                double v = M[i][mk];
+                for (int j = i + 1; j < mk; j++) v -= M[i][j] * gam[j];
+                gam[i] = v / M[i][i];
+                ok = std::isfinite(gam[i]) && std::fabs(gam[i]) < 1e3;
+            }
+            if (ok) {
+                HIP_TRY(&c, launch_aa_apply(nw, (int)L, mk, c.g.d_work, aa_g(s0), gj, gam, c.d_levp, c.stream));
+                n_ng++;
+                last_ng = k;
+            } else {
+                aa_cnt = 1;  // an ill-conditioned window restarts from the newest pair
+            }
         }
         if (ng) {
             // history x1 = hist[0] (newest) .. x3 = hist[2]; x0 is the level densities just formed
@@ -931,6 +1021,7 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     hipFree(d_max);
     if (d_hist) hipFree(d_hist);
     if (d_ngp) hipFree(d_ngp);
+    if (d_aa) hipFree(d_aa);
     if (rc) return rc;
     // the Ng history is not carried over, and an extrapolation leaves d_levp ahead of d_pol
     c.pol_chain = n_ng == 0 || last_ng < iters - 1;

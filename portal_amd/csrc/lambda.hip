@@ -155,6 +155,139 @@ hipError_t launch_ng_apply(int n_work, int L, const int *work, double *x0, const
 
 size_t ng_partial_doubles() { return (size_t)NG_BLOCKS * 5 + 8; }
 
+// Anderson acceleration (Anderson 1965; the "type II" form of Walker & Ni 2011,
+// SIAM J. Numer. Anal. 49, 1715) of the map G: x (level densities a pass starts
+// from) -> G(x) (level densities its SE solution closes to).  With f = G(x) - x and
+// the window of the AA_M previous pairs, differences to the newest pair
+//   dF_j = f_k - f_{k-j},  dG_j = G_k - G_{k-j},
+// gamma minimises |f_k - sum_j gamma_j dF_j| in the 1/G_k^2-weighted norm, and
+//   x_{k+1} = G_k - sum_j gamma_j dG_j.
+// The pairs are kept compact (solved nodes x levels, in work-list order).
+
+__global__ void __launch_bounds__(256) aa_gather_kernel(int n_work, int L, const int *work, const double *levp,
+                                                        double *dst) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n_work * L) return;
+    const int w = (int)(e / L), i = (int)(e % L);
+    dst[e] = levp[(size_t)work[w] * L + i];
+}
+
+struct AaDotArgs {
+    long long n;
+    int mk;
+    const double *x0, *g0;
+    const double *xj[AA_M_MAX], *gj[AA_M_MAX];
+    double *partial;  // NG_BLOCKS x AA_NSUM
+};
+
+__global__ void __launch_bounds__(256) aa_dots_kernel(AaDotArgs a) {
+    double s[AA_NSUM];
+#pragma unroll
+    for (int t = 0; t < AA_NSUM; t++) s[t] = 0.0;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < a.n; e += (long long)gridDim.x * blockDim.x) {
+        const double g0 = a.g0[e], f0 = g0 - a.x0[e];
+        const double wt = g0 > 0.0 ? 1.0 / (g0 * g0) : 0.0;
+        double df[AA_M_MAX];
+#pragma unroll
+        for (int j = 0; j < AA_M_MAX; j++) df[j] = j < a.mk ? f0 - (a.gj[j][e] - a.xj[j][e]) : 0.0;
+        int t = 0;
+#pragma unroll
+        for (int i = 0; i < AA_M_MAX; i++) {
+#pragma unroll
+            for (int j = i; j < AA_M_MAX; j++) s[t++] += wt * df[i] * df[j];
+            s[t++] += wt * df[i] * f0;
+        }
+    }
+    __shared__ double red[AA_NSUM][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < AA_NSUM; j++) {
+        double v = s[j];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[j][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < AA_NSUM) {
+        const int j = threadIdx.x;
+        a.partial[blockIdx.x * AA_NSUM + j] = ((red[j][0] + red[j][1]) + red[j][2]) + red[j][3];
+    }
+}
+
+__global__ void __launch_bounds__(64) aa_sum_kernel(const double *partial, int nb, double *out) {
+    const int j = threadIdx.x;
+    if (j >= AA_NSUM) return;
+    double v = 0.0;
+    for (int b = 0; b < nb; b++) v += partial[b * AA_NSUM + j];
+    out[j] = v;
+}
+
+struct AaApplyArgs {
+    int n_work, L, mk;
+    const int *work;
+    const double *g0;
+    const double *gj[AA_M_MAX];
+    double gamma[AA_M_MAX];
+    double *levp;
+};
+
+__global__ void __launch_bounds__(256) aa_apply_kernel(AaApplyArgs a) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)a.n_work * a.L) return;
+    const double g0 = a.g0[e];
+    double v = g0;
+#pragma unroll
+    for (int j = 0; j < AA_M_MAX; j++)
+        if (j < a.mk) v -= a.gamma[j] * (g0 - a.gj[j][e]);
+    const int w = (int)(e / a.L), i = (int)(e % a.L);
+    a.levp[(size_t)a.work[w] * a.L + i] = (v > 0.0 && v < 1e300) ? v : g0;
+}
+
+hipError_t launch_aa_gather(int n_work, int L, const int *work, const double *levp, double *dst, hipStream_t s) {
+    const long long n = (long long)n_work * L;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(aa_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_work, L, work, levp,
+                       dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_aa_dots(long long n, int mk, const double *x0, const double *g0, const double *const *xj,
+                          const double *const *gj, double *partial, double *out, hipStream_t s) {
+    AaDotArgs a{};
+    a.n = n;
+    a.mk = mk;
+    a.x0 = x0;
+    a.g0 = g0;
+    for (int j = 0; j < AA_M_MAX; j++) {
+        a.xj[j] = j < mk ? xj[j] : x0;
+        a.gj[j] = j < mk ? gj[j] : g0;
+    }
+    a.partial = partial;
+    hipLaunchKernelGGL(aa_dots_kernel, dim3(NG_BLOCKS), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(aa_sum_kernel, dim3(1), dim3(64), 0, s, (const double *)partial, NG_BLOCKS, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_aa_apply(int n_work, int L, int mk, const int *work, const double *g0, const double *const *gj,
+                           const double *gamma, double *levp, hipStream_t s) {
+    AaApplyArgs a{};
+    a.n_work = n_work;
+    a.L = L;
+    a.mk = mk;
+    a.work = work;
+    a.g0 = g0;
+    for (int j = 0; j < AA_M_MAX; j++) {
+        a.gj[j] = j < mk ? gj[j] : g0;
+        a.gamma[j] = j < mk ? gamma[j] : 0.0;
+    }
+    a.levp = levp;
+    const long long n = (long long)n_work * L;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(aa_apply_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+size_t aa_partial_doubles() { return (size_t)NG_BLOCKS * AA_NSUM + 32; }
+
 }  // namespace portal
 
 namespace portal {

@@ -81,7 +81,7 @@ def small(seed=1):
         pp, lev, st = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 30, tol_pop=0.0, tol_align=0.0, precision=prec,
                                         rebuild_paths=1)
         t = time.perf_counter() - t
-        img, sti = _image(pt, sp, g, 32, 93.75, 64, precision=PORTAL_FP64)
+        img, sti = _image(pt, sp, g, 32, 93.75, 64, precision=prec)
         res[name] = {"pp": pp, "img": img, "seconds": t, "iterations": st["iterations"],
                      "rt_ms": st["t_rt_ms"], "se_ms": st["t_se_ms"], "walk_ms": st["t_walk_ms"],
                      "image_ms": sti["t_img_ms"], "last_change": st["max_rel_change"]}
@@ -156,7 +156,7 @@ def million(n_int=1000000, seed=5):
         st = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 300, tol_pop=TOL_POP, tol_align=TOL_ALIGN, precision=prec,
                                surrogate=surr, ng_accel=ng, rebuild_paths=1, outputs=False)
         t = time.perf_counter() - t
-        img, _ = _image(pt, sp, g, 32, 93.75, 64)
+        img, _ = _image(pt, sp, g, 32, 93.75, 64, precision=prec)
         if ref is None:
             ref = img
         out[name] = {"iterations": st["iterations"], "converged": bool(st["converged"]), "seconds": t,

@@ -1,4 +1,4 @@
-"""Ng acceleration of the lambda iteration (needs an MI355X).
+"""Ng and Anderson acceleration of the lambda iteration (needs an MI355X).
 
 The accelerated iteration must reach the fixed point of the plain one, in fewer
 iterations, in both precisions.
@@ -25,15 +25,16 @@ def small():
     pt.close()
 
 
+@pytest.mark.parametrize("accel", [1, 2], ids=["ng", "anderson"])
 @pytest.mark.parametrize("precision,tol", [(PORTAL_FP64, 1e-9), (PORTAL_MIXED, 1e-6)])
-def test_ng_reaches_the_plain_fixed_point(small, precision, tol):
+def test_acceleration_reaches_the_plain_fixed_point(small, precision, tol, accel):
     sp, g, xth, pt = small
     kw = dict(tol_pop=tol, tol_align=tol * 1e-2, precision=precision)
     pt.set_levpop(g.lev_p)
     pp0, lev0, st0 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 400, **kw)
     pt.set_levpop(g.lev_p)
-    pp1, lev1, st1 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 400, ng_accel=1, **kw)
-    print(f"precision {precision}: plain {st0['iterations']} iterations, Ng {st1['iterations']} "
+    pp1, lev1, st1 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 400, ng_accel=accel, **kw)
+    print(f"precision {precision}: plain {st0['iterations']} iterations, accel {accel}: {st1['iterations']} "
           f"({st1['ng_steps']} extrapolations)")
     assert st0["iterations"] < 400 and st1["iterations"] < 400
     assert st1["ng_steps"] > 0
