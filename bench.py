@@ -192,8 +192,10 @@ def main():
     launches = max(1, stm["iterations"])
     avg_ms = stm["t_rt_ms"] / launches
     achieved = stm["n_cross"] * bytes_per_cross / (avg_ms * 1e-3) / 1e9
-    # the kernel's exponentials: 2 per (crossing, line, channel)
+    # the kernel's exponentials: 2 per (crossing, line, channel), against the measured
+    # v_exp_f32 issue rate of the device
     exp_rate = 2.0 * sp.r_num * (2 * args.nvel + 1) * stm["n_cross"] / (avg_ms * 1e-3)
+    exp_peak = pt.exp_peak()
     traffic = None
     tf = os.path.join(ROOT, "profiles", "rt_mixed_traffic.json")
     if os.path.exists(tf):
@@ -256,7 +258,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "rt_coop_kernel<float>" if stm["rt_variant"] == 2 else f"variant {stm['rt_variant']}",
                      "avg_launch_ms": avg_ms, "crossings_per_launch": stm["n_cross"],
-                     "bytes_per_crossing": bytes_per_cross, "exp_per_s": exp_rate},
+                     "bytes_per_crossing": bytes_per_cross, "exp_per_s": exp_rate,
+                     "exp_peak_per_s": exp_peak, "exp_frac": exp_rate / exp_peak},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

@@ -40,7 +40,7 @@ extern "C" {
 #define PORTAL_ERR_TEMP (-6)         /* T_kin above the collision table, miscellaneous.f90:331-360 */
 #define PORTAL_ERR_STATE (-7)        /* grid/species not uploaded */
 #define PORTAL_ERR_NOMEM (-8)        /* device allocation failed */
-#define PORTAL_ERR_UNSUPPORTED (-9)  /* star mode (star_mod=1) */
+#define PORTAL_ERR_UNSUPPORTED (-9)  /* star mode (star_mod=1); dust in a batch of models */
 
 #define PORTAL_FP64 0   /* every stage in FP64 */
 #define PORTAL_MIXED 1  /* FP32 ray transfer + compensated FP32 J sums; FP64 SE */
@@ -177,10 +177,13 @@ int portal_image_pass(portal_ctx *ctx, const portal_img_params *p, const double 
 
 /* Surrogate network (no counterpart in the reference).  A dense MLP, depth hidden
  * layers of `width` with SiLU, BF16 weights and activations, FP32 accumulation and
- * FP32 master weights (Adam).  Inputs: log10 n_H2[cm^-3], log10 T_kin and, per
- * radiative line, log10(J^0_0/B_nu(2.73 K)) and 100 J^2_0/J^0_0 of a node
- * (n_in = 2 + 2 r_num).  Outputs: log10(sqrt(2J_i+1) rho^0_0(i)) per level, then
- * 1e3 rho^2_0/rho^0_0 per level with J > 0 (n_out = e_num + levels with J > 0).
+ * FP32 master weights (Adam).  Inputs of a node: log10 n_H2[cm^-3], log10 T_kin;
+ * per radiative line log10(J^0_0/B_nu(2.73 K)) and 100 J^2_0/J^0_0; and its SE
+ * solution rho_1 for that J: per level log10(sqrt(2J_i+1) rho^0_0(i)), per level with
+ * J > 0 1e3 rho^2_0/rho^0_0 (n_in = 2 + 2 r_num + e_num + nj, nj = levels with J > 0).
+ * Outputs, the converged solution as corrections to rho_1: per level
+ * log10(rho^0_0(i)/rho_1^0_0(i)), per level with J > 0 1e3 (rho^2_0/rho^0_0 -
+ * rho_1^2_0/rho_1^0_0) (n_out = e_num + nj).
  * feats/targets/out are batch x n_in / batch x n_out row-major floats; norm holds the
  * input means, input standard deviations, output means, output standard deviations.
  * portal_lambda_iterate with surrogate = 1 uses it for its first iterate. */
@@ -194,8 +197,8 @@ int portal_mlp_train_step(portal_ctx *ctx, int batch, const float *feats, const 
                           float *loss, float *grads);
 int portal_mlp_train(portal_ctx *ctx, int n, const float *feats, const float *targets, int epochs, int batch,
                      float lr, unsigned long long seed, float *loss_hist, double *ms);
-/* Network inputs of every solved node from the last inward pass; nodes (1-based,
- * may be NULL) gives the node of each row. */
+/* Network inputs of every solved node from the last SE pass (portal_se_pass or a
+ * lambda iteration); nodes (1-based, may be NULL) gives the node of each row. */
 int portal_surrogate_features(portal_ctx *ctx, int n_in, float *feats, int *nodes, int *n_work);
 
 #ifdef __cplusplus

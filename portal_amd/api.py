@@ -149,6 +149,12 @@ class _Core:
         self._check(self.lib.portal_lambda_iterate(self.ctx, C.byref(p), C.byref(it), _d(pp), _d(lev), C.byref(st)))
         return pp, lev, st.as_dict()
 
+    def exp_peak(self, iters=4096):
+        """Measured v_exp_f32 throughput of the device (exponentials per second)."""
+        out = C.c_double()
+        self._check(self.lib.portal_debug_exp_peak(self.ctx, int(iters), C.byref(out)))
+        return out.value
+
     def iter_trace(self, max_iter=1000):
         """(population change, alignment change) of each iteration of the last lambda call."""
         out = np.zeros(2 * max_iter)
@@ -241,7 +247,7 @@ class SurrogateMixin:
 
     def surrogate_features(self):
         sp = self.species
-        n_in = 2 + 2 * sp.r_num
+        n_in = 2 + 2 * sp.r_num + sp.e_num + int((np.asarray(sp.j_lev) > 0).sum())
         nw_max = self.grid.bulk_node_num
         f = np.zeros((nw_max, n_in), dtype=np.float32)
         nodes = np.zeros(nw_max, dtype=np.int32)

@@ -61,6 +61,7 @@ struct Species {
     int *d_l1 = nullptr, *d_l2 = nullptr;                 // per line, 0-based levels
     double *d_c1B = nullptr, *d_c2A = nullptr, *d_g1 = nullptr, *d_g2 = nullptr;
     double *d_cnu_spi = nullptr, *d_Iin = nullptr;
+    double *d_kd = nullptr, *d_nu = nullptr;              // per line: dust opacity, frequency
     double *d_tA = nullptr, *d_tB21 = nullptr, *d_tnu = nullptr, *d_tgul = nullptr;
     int *d_trad = nullptr, *d_tcol = nullptr;
     double *d_Cji = nullptr, *d_coltemps = nullptr;
@@ -101,6 +102,8 @@ struct Ctx {
     double *d_levp = nullptr;
     double *d_KS = nullptr;     // N x R x {K, S}  (FP64)
     float *d_KS32 = nullptr;    // N x R x {K, S}  (FP32, mixed path)
+    double *d_KSD = nullptr;    // N x R x {alpha_dust, B_nu(T_dust)}, when the grid has dust
+    float *d_KSD32 = nullptr;
     double *d_dustco = nullptr; // N x R x {Ka, j} when dust is present
     double *d_J0 = nullptr, *d_J2 = nullptr;  // R x N
     double *d_pol = nullptr;    // N x 2L
@@ -121,6 +124,7 @@ struct Ctx {
     // lambda iteration; cleared by anything that changes the state otherwise), so that
     // a lambda iteration continued by a further call measures its first change exactly
     bool pol_chain = false;
+    bool pol_valid = false;  // d_pol is the SE solution of the J in d_J0/d_J2
     std::vector<double> iter_trace;  // per iteration of the last lambda call: population change, alignment change
 };
 
@@ -169,6 +173,10 @@ struct PrepArgs {
     const double *cnu_spi;   // per line: c/nu / sqrt(pi)
     double *KS;
     float *KS32;
+    // dust (KSD/KSD32 null: none): per node dust_p and T_dust, per line kd and nu
+    const double *dust, *tdust, *kd, *nu;
+    double *KSD;
+    float *KSD32;
 };
 hipError_t launch_line_prep(const PrepArgs &a, hipStream_t s);
 
@@ -189,6 +197,9 @@ struct RtArgs {
     int waves;              // waves per block
     int max_nc;             // longest stored path (LDS staging capacity)
     int variant;            // 0: global-memory kernel, 1: LDS-staged per-wave rays, 2: cooperative
+    const double *KSD;      // N x R x {alpha_dust, B_nu(T_dust)} when dust is on (cooperative kernel only)
+    const float *KSD32;
+    int dust;
 };
 hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s);
 hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s);

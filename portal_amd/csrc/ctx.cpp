@@ -88,6 +88,8 @@ int ensure_node_buffers(Ctx &c) {
     if (c.cap_nodes == need) return PORTAL_OK;
     dfree(c.d_KS);
     dfree(c.d_KS32);
+    dfree(c.d_KSD);
+    dfree(c.d_KSD32);
     dfree(c.d_J0);
     dfree(c.d_J2);
     dfree(c.d_pol);
@@ -192,6 +194,8 @@ int check_rt_args(Ctx &c, const RtArgs &a, bool mixed) {
                           mixed ? (const void *)a.KS32 : (const void *)a.KS};
     for (const void *p : need)
         if (!p) return fail(c, PORTAL_ERR_STATE, "inward integration launched with an unset buffer");
+    if (a.dust && !(mixed ? (const void *)a.KSD32 : (const void *)a.KSD))
+        return fail(c, PORTAL_ERR_STATE, "inward integration with dust launched without dust coefficients");
     if (a.n_work < 0 || a.D <= 0 || a.R <= 0 || a.R > 64 || a.C <= 0 || a.max_nc < 0)
         return fail(c, PORTAL_ERR_ARG, "inward integration launched with bad sizes");
     return PORTAL_OK;
@@ -202,7 +206,7 @@ int check_rt_args(Ctx &c, const RtArgs &a, bool mixed) {
 // work list and the fastest is kept for every later pass with the same shape.
 int autotune_rt(Ctx &c, RtArgs ra, bool mixed) {
     const int pi = mixed ? 1 : 0;
-    const int key = (int)((c.paths.total % 1000003) ^ (ra.C * 131 + ra.R * 7 + ra.D));
+    const int key = (int)((c.paths.total % 1000003) ^ (ra.C * 131 + ra.R * 7 + ra.D)) * 2 + ra.dust;
     if (c.forced_variant[pi] >= 0) {
         c.tuned_variant[pi] = c.forced_variant[pi];
         c.tuned_waves[pi] = c.forced_waves[pi];
@@ -240,6 +244,7 @@ int autotune_rt(Ctx &c, RtArgs ra, bool mixed) {
 }
 
 int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
+    c.pol_valid = false;
     int rc = ensure_paths(c, p, st);
     if (rc) return rc;
     const Species &s = c.sp;
@@ -260,6 +265,20 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
     const bool mixed = p->precision == PORTAL_MIXED;
     pa.KS = mixed ? nullptr : c.d_KS;
     pa.KS32 = mixed ? c.d_KS32 : nullptr;
+    // dust enters the inward pass when some node has dust and some line a dust opacity
+    bool dust = false;
+    if (c.g.has_dust)
+        for (double kd : s.kd_v) dust = dust || kd != 0.0;
+    if (dust) {
+        if (!c.d_KSD) HIP_TRY(&c, hipMalloc((void **)&c.d_KSD, sizeof(double) * 2 * c.g.N * s.r_num));
+        if (!c.d_KSD32) HIP_TRY(&c, hipMalloc((void **)&c.d_KSD32, sizeof(float) * 2 * c.g.N * s.r_num));
+        pa.dust = c.g.d_dust;
+        pa.tdust = c.g.d_tdust;
+        pa.kd = s.d_kd;
+        pa.nu = s.d_nu;
+        pa.KSD = mixed ? nullptr : c.d_KSD;
+        pa.KSD32 = mixed ? c.d_KSD32 : nullptr;
+    }
     HIP_TRY(&c, launch_line_prep(pa, c.stream));
     const double t_prep = tp.stop();
     HIP_TRY(&c, hipMemsetAsync(c.d_J0, 0, sizeof(double) * c.g.N * s.r_num, c.stream));
@@ -283,6 +302,9 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
     ra.J0 = c.d_J0;
     ra.J2 = c.d_J2;
     ra.max_nc = (c.paths.max_nc + 7) & ~7;
+    ra.dust = dust ? 1 : 0;
+    ra.KSD = c.d_KSD;
+    ra.KSD32 = c.d_KSD32;
     double *d_xth = nullptr;
     std::vector<double> xv(p->xth, p->xth + 3 * (size_t)p->n_sphere);
     HIP_TRY(&c, upload(d_xth, xv));
@@ -362,6 +384,7 @@ int run_se(Ctx &c, int extra_info, portal_stats *st) {
     if (flags & FLAG_TEMP)
         return fail(c, PORTAL_ERR_TEMP, "kinetic temperature above the collision-rate table (miscellaneous.f90:331-360)");
     if (flags & FLAG_LU) return fail(c, PORTAL_ERR_LU, "Density-matrix calculation: failed (singular SE matrix)");
+    c.pol_valid = true;
     return PORTAL_OK;
 }
 
@@ -371,7 +394,6 @@ int check_se_params(Ctx &c, const portal_se_params *p) {
     if (p->n_sphere <= 0 || p->nvel <= 0 || p->n_depth <= 0 || !(p->dx > 0))
         return fail(c, PORTAL_ERR_ARG, "n_sphere, nvel, n_depth and dx must be positive");
     if (p->precision != PORTAL_FP64 && p->precision != PORTAL_MIXED) return fail(c, PORTAL_ERR_ARG, "bad precision");
-    if (c.g.has_dust) return fail(c, PORTAL_ERR_UNSUPPORTED, "non-zero dust_p is not supported by the inward pass");
     return ensure_node_buffers(c);
 }
 
@@ -422,6 +444,8 @@ int portal_ctx_destroy(portal_ctx *x) {
     dfree(s.d_g1);
     dfree(s.d_g2);
     dfree(s.d_cnu_spi);
+    dfree(s.d_kd);
+    dfree(s.d_nu);
     dfree(s.d_Iin);
     dfree(s.d_tA);
     dfree(s.d_tB21);
@@ -444,6 +468,8 @@ int portal_ctx_destroy(portal_ctx *x) {
     dfree(c.d_levp);
     dfree(c.d_KS);
     dfree(c.d_KS32);
+    dfree(c.d_KSD);
+    dfree(c.d_KSD32);
     dfree(c.d_dustco);
     dfree(c.d_J0);
     dfree(c.d_J2);
@@ -464,6 +490,7 @@ int portal_upload_grid(portal_ctx *x, int N, int N_b, const double *XYZ, const d
     if (!x) return PORTAL_ERR_ARG;
     Ctx &c = x->c;
     c.pol_chain = false;
+    c.pol_valid = false;
     hipSetDevice(c.device);
     if (N <= 0 || N_b < 0 || N_b > N || maxne <= 0 || !XYZ || !vel || !temp || !b_all || !h2 || !ab || !dust ||
         !del_ne || !del_num_ne || !(tx > 0))
@@ -542,6 +569,7 @@ int portal_upload_species(portal_ctx *x, int e_num, const int *j_lev, const doub
     if (!x) return PORTAL_ERR_ARG;
     Ctx &c = x->c;
     c.pol_chain = false;
+    c.pol_valid = false;
     hipSetDevice(c.device);
     if (e_num <= 0 || r_num <= 0 || c_num < 0 || c_temp <= 0 || (c_par != 1 && c_par != 2) || kmax < 2 ||
         (kmax & 1) || !j_lev || !e_levs_hz || !r_trans || !t_Aji || !nu_hz || !t_Bij || !kd_v ||
@@ -609,6 +637,8 @@ int portal_upload_species(portal_ctx *x, int e_num, const int *j_lev, const doub
     HIP_TRY(&c, upload(s.d_g1, g1));
     HIP_TRY(&c, upload(s.d_g2, g2));
     HIP_TRY(&c, upload(s.d_cnu_spi, cnu));
+    HIP_TRY(&c, upload(s.d_kd, s.kd_v));
+    HIP_TRY(&c, upload(s.d_nu, s.nu_hz));
     HIP_TRY(&c, upload(s.d_Iin, Iin));
     HIP_TRY(&c, upload(s.d_tA, tA));
     HIP_TRY(&c, upload(s.d_tB21, tB21));
@@ -735,7 +765,7 @@ static int surrogate_step(Ctx &c, int *n_fallback) {
     const Species &s = c.sp;
     int nj = 0;
     for (int i = 0; i < s.e_num; i++) nj += s.j_lev[i] != 0;
-    if (n_in != 2 + 2 * s.r_num || n_out != s.e_num + nj)
+    if (n_in != 2 + 2 * s.r_num + s.e_num + nj || n_out != s.e_num + nj)
         return fail(c, PORTAL_ERR_ARG, "surrogate network shape does not match the species");
     const int nw = (int)c.g.work.size();
     float *d_f = nullptr, *d_o = nullptr;
@@ -818,8 +848,12 @@ int portal_surrogate_features(portal_ctx *x, int n_in, float *feats, int *nodes,
     if (!x || !feats || !n_work) return PORTAL_ERR_ARG;
     Ctx &c = x->c;
     hipSetDevice(c.device);
-    if (!c.g.valid || !c.sp.valid || !c.d_J0) return fail(c, PORTAL_ERR_STATE, "no inward pass has run");
-    if (n_in != 2 + 2 * c.sp.r_num) return fail(c, PORTAL_ERR_ARG, "n_in must be 2 + 2*r_num");
+    if (!c.g.valid || !c.sp.valid || !c.d_J0 || !c.pol_valid)
+        return fail(c, PORTAL_ERR_STATE, "no SE pass has run");
+    int nj = 0;
+    for (int i = 0; i < c.sp.e_num; i++) nj += c.sp.j_lev[i] != 0;
+    if (n_in != 2 + 2 * c.sp.r_num + c.sp.e_num + nj)
+        return fail(c, PORTAL_ERR_ARG, "n_in must be 2 + 2 r_num + e_num + (levels with J > 0)");
     const int nw = (int)c.g.work.size();
     float *d_f = nullptr;
     HIP_TRY(&c, hipMalloc((void **)&d_f, sizeof(float) * (size_t)nw * n_in + 16));
@@ -835,6 +869,8 @@ int portal_surrogate_features(portal_ctx *x, int n_in, float *feats, int *nodes,
     a.J0 = c.d_J0;
     a.J2 = c.d_J2;
     a.Iin = c.sp.d_Iin;
+    a.j_lev = c.sp.d_jlev;
+    a.pol = c.d_pol;
     a.feats = d_f;
     HIP_TRY(&c, launch_surr_features(a, c.stream));
     HIP_TRY(&c, hipMemcpyAsync(feats, d_f, sizeof(float) * (size_t)nw * n_in, hipMemcpyDeviceToHost, c.stream));
@@ -1237,6 +1273,8 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         Ctx &c = xs[m]->c;
         int rc = check_se_params(c, &params[m]);
         if (rc) return fail(c0, rc, "model " + std::to_string(m) + ": " + c.err);
+        if (c.g.has_dust)
+            return fail(c0, PORTAL_ERR_UNSUPPORTED, "model " + std::to_string(m) + ": dust in a batch is not supported");
         c.pol_chain = false;
     }
     if (total) std::memset(total, 0, sizeof(*total));

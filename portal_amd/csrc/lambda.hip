@@ -292,12 +292,19 @@ size_t aa_partial_doubles() { return (size_t)NG_BLOCKS * AA_NSUM + 32; }
 
 namespace portal {
 
-// Surrogate input features of every work node from the last inward pass:
-//   log10 n_H2[cm^-3], log10 T_kin, and per line log10(J^0_0 / I_cmb) and 100 J^2_0/J^0_0.
+// Surrogate input features of every work node, from the last inward pass and the SE
+// solution rho_1 computed from it:
+//   log10 n_H2[cm^-3], log10 T_kin; per line log10(J^0_0 / I_cmb), 100 J^2_0/J^0_0;
+//   per level log10(sqrt(2J+1) rho^0_0); per level with J > 0 1e3 rho^2_0/rho^0_0.
+// The outputs are corrections to rho_1: per level log10(rho^0_0 / rho_1^0_0), per level
+// with J > 0 1e3 (rho^2_0/rho^0_0 - rho_1^2_0/rho_1^0_0).
+__device__ __forceinline__ double surr_logpop(double sg_r0) { return log10(fmax(sg_r0, 1e-12)); }
+
 __global__ void __launch_bounds__(256) surr_features_kernel(SurrArgs a) {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= a.n_work) return;
     const int node = a.work[w];
+    const size_t N = a.N;
     float *f = a.feats + (size_t)w * a.n_in;
     f[0] = (float)log10(a.h2[node] * 1e-6);
     f[1] = (float)log10(a.tkin[node]);
@@ -305,6 +312,16 @@ __global__ void __launch_bounds__(256) surr_features_kernel(SurrArgs a) {
         const double j0 = a.J0[(size_t)node * a.R + k], j2 = a.J2[(size_t)node * a.R + k];
         f[2 + 2 * k] = (float)log10(j0 / a.Iin[k]);
         f[3 + 2 * k] = (float)(100.0 * j2 / j0);
+    }
+    int q = 2 + 2 * a.R;
+    for (int i = 0; i < a.L; i++) {
+        const double r0 = a.pol[node + N * (2 * i)];
+        f[q++] = (float)surr_logpop(sqrt(2.0 * a.j_lev[i] + 1.0) * r0);
+    }
+    for (int i = 0; i < a.L; i++) {
+        if (a.j_lev[i] == 0) continue;
+        const double r0 = a.pol[node + N * (2 * i)], r2 = a.pol[node + N * (2 * i + 1)];
+        f[q++] = (float)(r0 != 0.0 ? 1e3 * r2 / r0 : 0.0);
     }
 }
 
@@ -319,7 +336,8 @@ __global__ void __launch_bounds__(256) surr_apply_kernel(SurrArgs a) {
     double s = 0.0;
     bool ok = true;
     for (int i = 0; i < a.L; i++) {
-        const double p = exp10((double)o[i]);
+        const double sg = sqrt(2.0 * a.j_lev[i] + 1.0);
+        const double p = exp10(surr_logpop(sg * a.pol[node + N * (2 * i)]) + (double)o[i]);
         ok = ok && isfinite(p);
         s += p;
     }
@@ -327,8 +345,9 @@ __global__ void __launch_bounds__(256) surr_apply_kernel(SurrArgs a) {
     int q = a.L;
     for (int i = 0; i < a.L && ok; i++) {
         if (a.j_lev[i] == 0) continue;
-        const float r = o[q++];
-        ok = ok && isfinite(r) && fabsf(r) < 500.0f;
+        const double r0 = a.pol[node + N * (2 * i)], r2 = a.pol[node + N * (2 * i + 1)];
+        const double rat = (r0 != 0.0 ? r2 / r0 : 0.0) + 1e-3 * (double)o[q++];
+        ok = ok && isfinite(rat) && fabs(rat) < 0.5;
     }
     if (!ok) {
         atomicAdd(a.n_fallback, 1);
@@ -337,10 +356,11 @@ __global__ void __launch_bounds__(256) surr_apply_kernel(SurrArgs a) {
     const double nab = a.ab[node] * a.h2[node];
     q = a.L;
     for (int i = 0; i < a.L; i++) {
-        const double p = exp10((double)o[i]) / s;
         const double sg = sqrt(2.0 * a.j_lev[i] + 1.0);
+        const double r0_1 = a.pol[node + N * (2 * i)], r2_1 = a.pol[node + N * (2 * i + 1)];
+        const double p = exp10(surr_logpop(sg * r0_1) + (double)o[i]) / s;
         const double r0 = p / sg;
-        const double rat = a.j_lev[i] != 0 ? 1e-3 * (double)o[q++] : 0.0;
+        const double rat = a.j_lev[i] != 0 ? (r0_1 != 0.0 ? r2_1 / r0_1 : 0.0) + 1e-3 * (double)o[q++] : 0.0;
         a.levp[(size_t)node * a.L + i] = nab * p;
         a.pol[node + N * (2 * i)] = r0;
         a.pol[node + N * (2 * i + 1)] = rat * r0;

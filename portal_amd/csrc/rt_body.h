@@ -59,6 +59,10 @@ struct Arith<double> {
     __device__ static double om(double tau, double et) { return 1.0 - et; }
     __device__ static double thr() { return 1e-30; }
     __device__ static R2 ks(const RtArgs &a, size_t i) { return *reinterpret_cast<const double2 *>(a.KS + 2 * i); }
+    __device__ static R2 ksd(const RtArgs &a, size_t i) { return *reinterpret_cast<const double2 *>(a.KSD + 2 * i); }
+    // dust of a segment: (alpha dl, B_nu(T_dust)) in the units of tau_of
+    __device__ static R2 stage_dust(R2 d, double dl, bool) { return double2{d.x * dl, d.y}; }
+    __device__ static double rcp(double x) { return 1.0 / x; }
 };
 
 template <>
@@ -101,6 +105,12 @@ struct Arith<float> {
     }
     __device__ static float thr() { return 1.44269504e-30f; }  // 1e-30 in units of tau2
     __device__ static R2 ks(const RtArgs &a, size_t i) { return *reinterpret_cast<const float2 *>(a.KS32 + 2 * i); }
+    __device__ static R2 ksd(const RtArgs &a, size_t i) { return *reinterpret_cast<const float2 *>(a.KSD32 + 2 * i); }
+    __device__ static R2 stage_dust(R2 d, float dl, bool cell) {
+        const bool on = cell && d.x * dl > 0.0f;
+        return float2{on ? d.x * dl * 1.44269504f : 0.0f, on ? d.y : 0.0f};
+    }
+    __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 };
 
 template <typename Real>

@@ -4,7 +4,8 @@
 // (inverse_rt.f90:126-129) with the same per-segment arithmetic as rt_body.h.
 //
 // Per node, rays are taken in order 0..D-1.  All threads stage ray r+1 (per cell
-// v.dir/b, 1/b, segment length; per (cell, line) log(K dl) and S) into one half of a
+// v.dir/b, 1/b, segment length; per (cell, line) log(K dl) and S, and with dust
+// alpha dl and B_nu(T_dust)) into one half of a
 // double-buffered LDS area while the waves integrate ray r from the other half; one
 // workgroup barrier per ray.  Lane `lane` of wave `w` owns pairs
 // p = (q W + w) 64 + lane for q < SLOTS, so the pairs are spread evenly over waves
@@ -24,20 +25,23 @@
 
 namespace portal {
 
+// one staging buffer: cap cell records, cap x R (L, S), and with dust cap x R (alpha dl, B_nu(T_dust))
 template <typename Real>
-__host__ __device__ constexpr size_t coop_buf_bytes(int cap, int R) {
-    return ((size_t)cap * stage_bytes_per_cell<Real>(R) + 15) & ~(size_t)15;
+__host__ __device__ constexpr size_t coop_buf_bytes(int cap, int R, bool dust) {
+    return ((size_t)cap * (stage_bytes_per_cell<Real>(R) + (dust ? (size_t)R * sizeof(typename Arith<Real>::R2) : 0)) +
+            15) &
+           ~(size_t)15;
 }
 
 template <typename Real>
-size_t coop_lds_bytes(int cap, int R, int npairs) {
-    const size_t stage = 2 * coop_buf_bytes<Real>(cap, R);
+size_t coop_lds_bytes(int cap, int R, int npairs, bool dust) {
+    const size_t stage = 2 * coop_buf_bytes<Real>(cap, R, dust);
     const size_t red = sizeof(double) * 2 * (size_t)npairs;
     return stage > red ? stage : red;
 }
 
 // All threads of the workgroup stage ray r of work item bid into buf.
-template <typename Real>
+template <typename Real, bool DUST>
 __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const int r, const int cap, char *buf) {
     typedef Arith<Real> A;
     typedef typename A::R2 R2;
@@ -57,6 +61,7 @@ __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const
         const Real vb = (Real)(e.vray * ib) * (Real)A::vscale, sib = (Real)ib * (Real)A::vscale;
         const bool ok = A::cell_ok(vb, sib);
         ks[j] = A::stage(A::ks(a, (size_t)e.cell * R + l), dl, ok);
+        if (DUST) ks[(size_t)cap * R + j] = A::stage_dust(A::ksd(a, (size_t)e.cell * R + l), dl, ok);
         if (l == 0) {
             CellRec<Real> c;
             c.vb = ok ? vb : (Real)0;
@@ -68,7 +73,7 @@ __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const
     }
 }
 
-template <typename Real, int SLOTS>
+template <typename Real, int SLOTS, bool DUST>
 __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, const int cap, char *smem) {
     typedef Arith<Real> A;
     typedef typename A::R2 R2;
@@ -81,9 +86,9 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     const Real inv_bn = (Real)(1.0 / bd);
     const int C = a.C, nvel = a.nvel, R = a.R, D = a.D;
     const int npairs = R * C;
-    const size_t bb = coop_buf_bytes<Real>(cap, R);
+    const size_t bb = coop_buf_bytes<Real>(cap, R, DUST);
 
-    Real v0[SLOTS], e0[SLOTS], k0x[SLOTS], k0y[SLOTS], iin[SLOTS];
+    Real v0[SLOTS], e0[SLOTS], k0x[SLOTS], k0y[SLOTS], iin[SLOTS], d0x[SLOTS], d0y[SLOTS];
     int lo[SLOTS];
     bool on[SLOTS];
     Real S0[SLOTS], C0[SLOTS], S2[SLOTS], C2[SLOTS];
@@ -100,29 +105,44 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
         const R2 k = A::ks(a, (size_t)node * R + l);  // the node's own cell, no skip
         k0x[q] = k.x;
         k0y[q] = k.y;
+        if (DUST) {
+            const R2 d = A::ksd(a, (size_t)node * R + l);
+            d0x[q] = d.x;
+            d0y[q] = d.y;
+        }
         iin[q] = (Real)a.I_in[l];
         S0[q] = C0[q] = S2[q] = C2[q] = (Real)0;
     }
 
-    coop_stage<Real>(a, bid, 0, cap, smem);
+    coop_stage<Real, DUST>(a, bid, 0, cap, smem);
     __syncthreads();
     for (int r = 0; r < D; r++) {
         char *cur = smem + (r & 1) * bb;
-        if (r + 1 < D) coop_stage<Real>(a, bid, r + 1, cap, smem + ((r + 1) & 1) * bb);
+        if (r + 1 < D) coop_stage<Real, DUST>(a, bid, r + 1, cap, smem + ((r + 1) & 1) * bb);
         const Real mu = (Real)a.xth[3 * r], w = (Real)a.xth[3 * r + 2];
         const Real wc = w * (((Real)3 * (mu * mu) - (Real)1) / (Real)2);
         const long long off = a.offsets[(long long)bid * D + r];
         const int nc = (int)(a.offsets[(long long)bid * D + r + 1] - off);
         const CellRec<Real> *cr = reinterpret_cast<const CellRec<Real> *>(cur);
         const R2 *ks = reinterpret_cast<const R2 *>(cr + cap);
+        const R2 *kds = ks + (size_t)cap * R;
         const Real vel_node = cr[0].pad;
         const Real dl0 = cr[0].dl;
         Real Tr[SLOTS], Em[SLOTS], v0s[SLOTS];
 #pragma unroll
         for (int q = 0; q < SLOTS; q++) {
-            const Real tau0 = k0x[q] * e0[q] * dl0;
-            Tr[q] = A::ex(-tau0);
-            Em[q] = k0y[q] * A::one_minus(tau0, Tr[q]);
+            if (DUST) {
+                // S = (e phi + j)/(k phi + alpha), tau = (k phi + alpha) dl (inverse_rt.f90:289-297)
+                const Real kp = k0x[q] * e0[q];
+                const Real tau0 = (kp + d0x[q]) * dl0;
+                const Real S0 = (k0y[q] * kp + d0y[q] * d0x[q]) / (kp + d0x[q]);
+                Tr[q] = A::ex(-tau0);
+                Em[q] = S0 * A::one_minus(tau0, Tr[q]);
+            } else {
+                const Real tau0 = k0x[q] * e0[q] * dl0;
+                Tr[q] = A::ex(-tau0);
+                Em[q] = k0y[q] * A::one_minus(tau0, Tr[q]);
+            }
             v0s[q] = v0[q] + vel_node;
             if (!on[q]) Tr[q] = (Real)0;
         }
@@ -138,13 +158,23 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             }
             const CellRec<Real> c = cr[i];
             const R2 *kr = ks + (size_t)i * R;
+            const R2 *kdr = kds + (size_t)i * R;
 #pragma unroll
             for (int q = 0; q < SLOTS; q++) {
                 const R2 k = kr[lo[q]];
                 const Real u = fma(v0s[q], c.inv_b, -c.vb);
-                const Real tau = A::tau_of(k.x, u);
+                Real tau = A::tau_of(k.x, u), S = k.y;
+                if (DUST) {
+                    // tau = tau_line + alpha dl; S the opacity-weighted mean of the line's
+                    // and the dust's source functions (inverse_rt.f90:273-287)
+                    const R2 d = kdr[lo[q]];
+                    const Real tl = tau;
+                    tau = tl + d.x;
+                    S = (k.y * tl + d.y * d.x) * A::rcp(tau);
+                    if (!A::skip_test && !(tau > (Real)0)) S = (Real)0;
+                }
                 const Real et = A::att(tau);
-                const Real em = Em[q] + Tr[q] * k.y * A::om(tau, et);
+                const Real em = Em[q] + Tr[q] * S * A::om(tau, et);
                 const Real tr = Tr[q] * et;
                 if constexpr (A::skip_test) {
                     const bool ok = tau > A::thr();
@@ -204,15 +234,18 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     }
 }
 
-template <typename Real, int SLOTS>
+template <typename Real, int SLOTS, bool DUST>
 __global__ void __launch_bounds__(512) rt_coop_kernel(RtArgs a, int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    rt_coop_node<Real, SLOTS>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+    rt_coop_node<Real, SLOTS, DUST>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
 }
 
 template <typename Real, int SLOTS>
 hipError_t launch_coop_s(const RtArgs &a, int cap, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((rt_coop_kernel<Real, SLOTS>), dim3(a.n_work), dim3(64 * a.waves), lds, s, a, cap);
+    if (a.dust)
+        hipLaunchKernelGGL((rt_coop_kernel<Real, SLOTS, true>), dim3(a.n_work), dim3(64 * a.waves), lds, s, a, cap);
+    else
+        hipLaunchKernelGGL((rt_coop_kernel<Real, SLOTS, false>), dim3(a.n_work), dim3(64 * a.waves), lds, s, a, cap);
     return hipGetLastError();
 }
 
@@ -222,7 +255,7 @@ hipError_t launch_coop(const RtArgs &a, int cap, hipStream_t s) {
     const int W = a.waves;
     if (W < 1 || W > 8) return hipErrorInvalidValue;
     const int npairs = a.R * a.C;
-    const size_t lds = coop_lds_bytes<Real>(cap, a.R, npairs);
+    const size_t lds = coop_lds_bytes<Real>(cap, a.R, npairs, a.dust != 0);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const int need = (npairs + 64 * W - 1) / (64 * W);
     switch (need) {

@@ -47,6 +47,21 @@ __global__ void __launch_bounds__(256) line_prep_kernel(PrepArgs a) {
         a.KS32[2 * id] = (float)K;
         a.KS32[2 * id + 1] = (float)S;
     }
+    if (a.KSD || a.KSD32) {
+        // dust: alpha = kd dust_p, j = alpha B_nu(T_dust) (inverse_rt.f90:292-297,
+        // black_body miscellaneous.f90:263-285)
+        const double alpha = a.kd[l] * a.dust[n];
+        const double T = a.tdust[n], nu0 = a.nu[l];
+        const double Bb = T == 0.0 ? 0.0 : 1.4744995e-50 * pow(nu0, 3.0) * pow(exp(4.79924e-11 / T * nu0) - 1.0, -1.0);
+        if (a.KSD) {
+            a.KSD[2 * id] = alpha;
+            a.KSD[2 * id + 1] = Bb;
+        }
+        if (a.KSD32) {
+            a.KSD32[2 * id] = (float)alpha;
+            a.KSD32[2 * id + 1] = (float)Bb;
+        }
+    }
 }
 
 hipError_t launch_line_prep(const PrepArgs &a, hipStream_t s) {
@@ -154,6 +169,7 @@ __global__ void __launch_bounds__(256) rt_f64_kernel(RtArgs a) {
 hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
     if (a.R > 64) return hipErrorInvalidValue;
+    if (a.dust && a.variant != 2) return hipErrorInvalidValue;  // dust: cooperative kernel only
     if (a.variant == 1) return launch_staged<double>(a, a.max_nc, s);
     if (a.variant == 2) return launch_coop<double>(a, a.max_nc, s);
     const int W = a.waves;

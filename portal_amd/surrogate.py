@@ -1,14 +1,17 @@
 """Training data, training and loading of the surrogate network.
 
 The network (portal_mlp_*, BF16-MFMA kernels in csrc/mlp.hip) maps a node's local
-state after the first inward pass -- log n_H2, log T_kin and, per line,
-log(J^0_0/B_nu(2.73 K)) and 100 J^2_0/J^0_0 -- to the populations and alignment of
-the converged lambda iteration: log10(sqrt(2J+1) rho^0_0) per level and
-1e3 rho^2_0/rho^0_0 per level with J > 0.  The method produces its own training
-data: seeded clouds are iterated to convergence with the mixed-precision path, and
-the first-pass features are paired with the converged solution.
+state after the first pass -- log n_H2, log T_kin, per line log(J^0_0/B_nu(2.73 K))
+and 100 J^2_0/J^0_0, and the first SE solution rho_1 (per level log10(sqrt(2J+1)
+rho^0_0), per level with J > 0 1e3 rho^2_0/rho^0_0) -- to the converged lambda
+iteration, as corrections to rho_1: per level log10(rho^0_0 / rho_1^0_0) and per level
+with J > 0 1e3 (rho^2_0/rho^0_0 - rho_1^2_0/rho_1^0_0).  Corrections span a far
+smaller range than the populations themselves, which is what a BF16 network with
+about three significant digits per activation resolves best.  The method produces its
+own training data: seeded clouds are iterated to convergence with the mixed-precision
+path, and the first-pass features are paired with the converged solution.
 
-    python -m portal_amd.surrogate --levels 10 --clouds 8 --nodes 200000
+    python -m portal_amd.surrogate --levels 10 --clouds 10 --nodes 200000
 
 writes portal_amd/surrogate_weights/<species>.npz, which ``load_into`` finds again.
 """
@@ -38,26 +41,33 @@ def species_key(sp) -> str:
     return f"L{sp.e_num}_R{sp.r_num}_{h.hexdigest()[:12]}"
 
 
+FORMAT = 2  # corrections to the first SE solution
+
+
 def dims(sp):
-    n_in = 2 + 2 * sp.r_num
-    n_out = sp.e_num + int((np.asarray(sp.j_lev) > 0).sum())
-    return n_in, n_out
+    nj = int((np.asarray(sp.j_lev) > 0).sum())
+    return 2 + 2 * sp.r_num + sp.e_num + nj, sp.e_num + nj
 
 
-def targets(pol_pops, nodes, sp):
-    """Network targets of the given 1-based nodes from a pol_pops table (N x 2L)."""
+def _state(pol_pops, nodes, sp):
+    """log10(sqrt(2J+1) rho00) per level and 1e3 rho20/rho00 per level with J > 0."""
     rows = np.asarray(nodes) - 1
-    L = sp.e_num
     g = np.sqrt(2.0 * np.asarray(sp.j_lev) + 1.0)
     r0 = pol_pops[rows][:, 0::2]
     r2 = pol_pops[rows][:, 1::2]
-    pops = np.clip(r0 * g[None, :], 10.0 ** LOG_FLOOR, None)
-    out = [np.log10(pops)]
-    cols = [i for i in range(L) if sp.j_lev[i] > 0]
+    lp = np.log10(np.clip(r0 * g[None, :], 10.0 ** LOG_FLOOR, None))
+    cols = [i for i in range(sp.e_num) if sp.j_lev[i] > 0]
     with np.errstate(divide="ignore", invalid="ignore"):
         rat = np.where(r0[:, cols] != 0, r2[:, cols] / r0[:, cols], 0.0) * 1e3
-    out.append(rat)
-    return np.concatenate(out, axis=1).astype(np.float32)
+    return lp, rat
+
+
+def targets(pol_conv, pol_first, nodes, sp):
+    """Network targets of the given 1-based nodes: the converged solution (pol_pops
+    table, N x 2L) as corrections to the first SE solution."""
+    lp, rat = _state(pol_conv, nodes, sp)
+    lp1, rat1 = _state(pol_first, nodes, sp)
+    return np.concatenate([lp - lp1, rat - rat1], axis=1).astype(np.float32)
 
 
 def weights_path(sp) -> str:
@@ -70,6 +80,9 @@ def load_into(pt, sp) -> bool:
     if not os.path.exists(path):
         return False
     z = np.load(path, allow_pickle=False)
+    fmt = int(z["format"]) if "format" in z.files else 1
+    if fmt != FORMAT:  # weights of an earlier output definition
+        return False
     n_in, n_out = dims(sp)
     pt.mlp_create(n_in, int(z["width"]), int(z["depth"]), n_out, seed=0)
     pt.mlp_set_params(z["params"], z["norm"])
@@ -81,17 +94,17 @@ def make_dataset(pt, sp, g, xth, nvel, dx, max_iter=80, tol=1e-6, tol_align=1e-8
     from .lib import PORTAL_MIXED
     pt.upload_grid(g)
     pt.set_levpop(g.lev_p)
-    pt.inverse_rt(xth, nvel, dx, precision=PORTAL_MIXED)
+    pp1, _ = pt.se_pass(xth, nvel, dx, precision=PORTAL_MIXED)
     feats, nodes = pt.surrogate_features()
     pt.set_levpop(g.lev_p)
     # Ng-accelerated: the same fixed point as the plain iteration, in fewer passes
     st = pt.lambda_iterate(xth, nvel, dx, max_iter, tol_pop=tol, tol_align=tol_align, precision=PORTAL_MIXED,
                            outputs=False, ng_accel=1)
     pp = pt.get_pol_pops()
-    return feats, targets(pp, nodes, sp), st["iterations"]
+    return feats, targets(pp, pp1, nodes, sp), st["iterations"]
 
 
-def train(levels=10, clouds=8, nodes=200000, nth=9, nph=32, nvel=64, epochs=40, batch=4096, lr=1e-3,
+def train(levels=10, clouds=10, nodes=200000, nth=9, nph=32, nvel=64, epochs=100, batch=4096, lr=1e-3,
           seed0=101, out=None, log=print):
     from . import cloud
     from .api import Portal
@@ -133,7 +146,7 @@ def train(levels=10, clouds=8, nodes=200000, nth=9, nph=32, nvel=64, epochs=40, 
     path = out or weights_path(sp)
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     np.savez_compressed(path, params=params, norm=nm, width=WIDTH, depth=DEPTH, samples=len(X),
-                        loss=np.concatenate([hist1, hist2]), key=species_key(sp), heldout=val)
+                        loss=np.concatenate([hist1, hist2]), key=species_key(sp), heldout=val, format=FORMAT)
     pt.close()
     return path
 
@@ -141,12 +154,12 @@ def train(levels=10, clouds=8, nodes=200000, nth=9, nph=32, nvel=64, epochs=40, 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--levels", type=int, default=10)
-    ap.add_argument("--clouds", type=int, default=8)
+    ap.add_argument("--clouds", type=int, default=10)
     ap.add_argument("--nodes", type=int, default=200000)
     ap.add_argument("--nth", type=int, default=9)
     ap.add_argument("--nph", type=int, default=32)
     ap.add_argument("--nvel", type=int, default=64)
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--out", default=None, help="weights file (default: the species' file in surrogate_weights/)")
     a = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))

@@ -272,8 +272,43 @@ def test_image_pass_mixed_error(portal1, cfg1, oracle_cfg1):
     assert np.abs(Qm).max() > 0
 
 
+@pytest.mark.parametrize("variant,waves", [(2, 2), (2, 4)])
+def test_inward_pass_with_dust(oracle, variant, waves):
+    # dust absorption and emission in the inward integration (inverse_rt.f90:273-297)
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 2500, seed=13)
+    nb = g.bulk_node_num - 1
+    g.dust_p[:nb] = 1e-17 * g.h2_dens[:nb] / 1e12
+    sp.kd_v = np.array([10.0, 20.0, 30.0])  # dust optical depth ~0.1-1 across the cloud
+    xth = cloud.legendre_all(5, 9)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    pt.force_rt_kernel(PORTAL_FP64, variant, waves)
+    pt.force_rt_kernel(PORTAL_MIXED, variant, waves)
+    idx = solved(g)
+    _, J0o, J2o, _, _ = oracle.se_pass(g, sp, xth, 16, cloud.DX_INWARD, extra_info=1, nthreads=NTH)
+    J0, J2, _ = pt.inverse_rt(xth, 16, cloud.DX_INWARD)
+    np.testing.assert_allclose(J0[:, idx], J0o[:, idx], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(J2[:, idx], J2o[:, idx], rtol=0, atol=1e-10 * np.abs(J0o[:, idx]).max(axis=0).min())
+    J0m, J2m, _ = pt.inverse_rt(xth, 16, cloud.DX_INWARD, precision=PORTAL_MIXED)
+    e0 = np.abs(J0m[:, idx] / J0o[:, idx] - 1).max()
+    print(f"dust: mixed max rel J0 err {e0:.3e}")
+    assert e0 < 2e-5
+    # the dust changes J: without it the result differs
+    kd = sp.kd_v.copy()
+    sp.kd_v = np.zeros(3)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    J0n, _, _ = pt.inverse_rt(xth, 16, cloud.DX_INWARD)
+    sp.kd_v = kd
+    assert np.abs(J0n[:, idx] / J0[:, idx] - 1).max() > 1e-3
+    pt.close()
+
+
 def test_image_pass_dust_branch(cfg1, oracle):
-    # dust present in the image pass (the inward pass rejects dust): both branches agree
+    # dust present in the image pass: both branches agree
     from portal_amd import driver
     sp, g0, xth = cfg1
     g = cloud.make_cloud(sp, 1500, seed=12)

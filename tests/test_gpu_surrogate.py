@@ -55,3 +55,26 @@ def test_invalid_network_output_falls_back_to_the_plain_iterate():
     assert st1["fallbacks"] == n_solved
     np.testing.assert_array_equal(lev1, lev0)
     pt.close()
+
+
+def test_features_are_the_first_pass_state():
+    # the device features against the same quantities formed on the host
+    sp, g, pt = _setup(4, 1200, seed=779)
+    xth = cloud.legendre_all(4, 8)
+    pt.set_levpop(g.lev_p)
+    J0, J2, _ = pt.inverse_rt(xth, 16, cloud.DX_INWARD)
+    pt.set_levpop(g.lev_p)
+    pp, _ = pt.se_pass(xth, 16, cloud.DX_INWARD)
+    f, nodes = pt.surrogate_features()
+    n_in, _ = surrogate.dims(sp)
+    assert f.shape == (len(nodes), n_in)
+    r = nodes - 1
+    nu = np.asarray(sp.nu_hz, dtype=np.float64)
+    iin = 1.4744995e-50 * nu ** 3 / (np.exp(4.79924e-11 / 2.73 * nu) - 1.0)  # B_nu(2.73 K), miscellaneous.f90:263-285
+    exp = [np.log10(g.h2_dens[r] * 1e-6), np.log10(g.temp_all[0, r])]
+    for k in range(sp.r_num):
+        exp += [np.log10(J0[k, r] / iin[k]), 100.0 * J2[k, r] / J0[k, r]]
+    lp, rat = surrogate._state(pp, nodes, sp)
+    want = np.column_stack(exp + list(lp.T) + list(rat.T))
+    np.testing.assert_allclose(f, want, rtol=2e-6, atol=2e-6)
+    pt.close()
