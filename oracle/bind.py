@@ -46,7 +46,9 @@ class OrcSpecies(C.Structure):
 
 class OrcSeParams(C.Structure):
     _fields_ = [("n_sphere", C.c_int), ("xth", dp), ("nvel", C.c_int), ("dx", C.c_double),
-                ("gvec", dp), ("bvec_all", dp), ("n_depth", C.c_int), ("extra_info", C.c_int)]
+                ("gvec", dp), ("bvec_all", dp), ("n_depth", C.c_int), ("extra_info", C.c_int),
+                ("star_mod", C.c_int), ("nth", C.c_int), ("nph", C.c_int), ("R_star", C.c_double),
+                ("T_star", C.c_double), ("X_star", dp)]
 
 
 def _f64(a):
@@ -99,12 +101,26 @@ class Oracle:
         self._keep.append(arrs)
         return s
 
-    def params(self, xth, nvel, dx, gvec, bvec_all, n_depth, extra_info=0):
-        arrs = dict(xth=_f64(xth), g=_f64(gvec), b=_f64(bvec_all))
+    def params(self, xth, nvel, dx, gvec, bvec_all, n_depth, extra_info=0, star=None):
+        """star: None, or dict(nth, nph, R_star, T_star, X_star) for star mode."""
+        arrs = dict(xth=_f64(xth), g=_f64(gvec), b=_f64(bvec_all),
+                    xs=_f64(star["X_star"] if star else np.zeros(3)))
         s = OrcSeParams(arrs["xth"].shape[1], _d(arrs["xth"]), nvel, dx, _d(arrs["g"]), _d(arrs["b"]),
-                        n_depth, extra_info)
+                        n_depth, extra_info, 1 if star else 0, star["nth"] if star else 0,
+                        star["nph"] if star else 0, star["R_star"] if star else 0.0,
+                        star["T_star"] if star else 0.0, _d(arrs["xs"]))
         self._keep.append(arrs)
         return s
+
+    def star_all(self, nth, nph, kvec, gvec, bvec, dOm):
+        """(n_star, xth 3 x n_sphere) of orc_star_all."""
+        xth = np.zeros((3, nth * nph), order="F")
+        ns, nv = C.c_int(), C.c_int()
+        rc = self.lib.orc_star_all(nth, nph, _d(_f64(kvec)), _d(_f64(gvec)), _d(_f64(bvec)), C.c_double(dOm),
+                                   C.byref(ns), C.byref(nv), _d(xth))
+        if rc:
+            raise RuntimeError("oracle star_all: rays do not fit")
+        return ns.value, np.asfortranarray(xth[:, : nv.value])
 
     # -- wrappers ---------------------------------------------------------
     def wig3jj(self, *a):
@@ -114,14 +130,14 @@ class Oracle:
         return self.lib.orc_wig6jj(*[int(x) for x in a])
 
     def se_pass(self, g, sp, xth, nvel, dx, n_depth=4096, node_lo=1, node_hi=None, lev_p=None,
-                extra_info=0, nthreads=0, bvec_all=None, gvec=None):
+                extra_info=0, nthreads=0, bvec_all=None, gvec=None, star=None):
         """Nodes node_lo..node_hi of the SE pass.  Returns (pol_pops N x 2L, J0 R x N, J2 R x N[, rates, JJ])."""
         if node_hi is None:
             node_hi = g.bulk_node_num
         G = self.grid(g, lev_p)
         S = self.species(sp)
         P = self.params(xth, nvel, dx, g.gvec if gvec is None else gvec,
-                        g.bvec_all if bvec_all is None else bvec_all, n_depth, extra_info)
+                        g.bvec_all if bvec_all is None else bvec_all, n_depth, extra_info, star)
         N, L, R = g.node_num, sp.e_num, sp.r_num
         pp = np.zeros((N, 2 * L), order="F")
         J0 = np.zeros((R, N), order="F")
@@ -136,11 +152,11 @@ class Oracle:
             return pp, J0, J2, rates, jj
         return pp, J0, J2
 
-    def inverse_nodes(self, g, sp, xth, nvel, dx, nodes, n_depth=4096, nthreads=0, lev_p=None):
+    def inverse_nodes(self, g, sp, xth, nvel, dx, nodes, n_depth=4096, nthreads=0, lev_p=None, star=None):
         """inverse_rt for the given 1-based nodes -> J0, J2 (R x len(nodes))."""
         G = self.grid(g, lev_p)
         S = self.species(sp)
-        P = self.params(xth, nvel, dx, g.gvec, g.bvec_all, n_depth, 0)
+        P = self.params(xth, nvel, dx, g.gvec, g.bvec_all, n_depth, 0, star)
         nd = np.ascontiguousarray(nodes, dtype=np.int32)
         J0 = np.zeros((sp.r_num, len(nd)), order="F")
         J2 = np.zeros((sp.r_num, len(nd)), order="F")
@@ -284,6 +300,47 @@ class Reference:
             _i(_i32(sp.lev_g)), _d(_f64(g.b_all)), _d(_f64(g.vel_all)), _d(_f64(sp.t_Aji)), _d(_f64(sp.t_Bij)),
             _d(xt), _d(_f64(g.dust_p)), _d(_f64(sp.kd_v)), _d(_f64(g.temp_all[1])), _d(J0), _d(J2), ci(0), ci(0),
             cd(0.0), cd(0.0), ci(0), ci(0), _d(Xs), ci(n_depth))
+        return J0, J2
+
+    def star_all(self, nth, nph, kvec, gvec, bvec, dOm):
+        """star_all (miscellaneous.f90:1139-1226) -> (n_star, xth 3 x n_sphere)."""
+        xth = np.zeros((3, nth * nph), order="F")
+        ns, nv = C.c_int(0), C.c_int(0)
+        self.lib.star_all_(C.byref(C.c_int(nth)), C.byref(C.c_int(nph)), _d(_f64(kvec)), _d(_f64(gvec)),
+                           _d(_f64(bvec)), self._r(dOm), C.byref(ns), C.byref(nv), _d(xth))
+        return ns.value, np.asfortranarray(xth[:, : nv.value])
+
+    def circle_points(self, n, th_s, dth_s):
+        x = np.zeros((3, n), order="F")
+        self.lib.circle_points_(C.byref(C.c_int(n)), self._r(th_s), self._r(dth_s), _d(x))
+        return x
+
+    def find_angles(self, bvec, gvec, dir_):
+        th, ph = C.c_double(0.0), C.c_double(0.0)
+        self.lib.find_angles_(_d(_f64(bvec)), _d(_f64(gvec)), _d(_f64(dir_)), C.byref(th), C.byref(ph))
+        return th.value, ph.value
+
+    def inverse_rt_star(self, node, g, sp, del_ne, xth_s, n_star, nvel, dx, bvec, gvec, n_depth, X_star, R_star,
+                        T_star):
+        """inverse_rt.f90:1-137 in star mode for one node, with its star_all directions."""
+        N = g.node_num
+        R = sp.r_num
+        ne = _i32(del_ne)
+        nnum = _i32(g.del_num_ne)
+        X = _f64(g.XYZ)
+        lp = _f64(g.lev_p)
+        J0 = np.zeros(R)
+        J2 = np.zeros(R)
+        xt = _f64(xth_s)
+        ci = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+        cd = lambda v: C.byref(C.c_double(float(v)))  # noqa: E731
+        node_c = C.c_int(int(node))
+        self.lib.inverse_rt_(
+            _i(ne), _i(nnum), ci(N), ci(ne.shape[1]), C.byref(node_c), _d(X), ci(nvel), ci(xt.shape[1]), ci(n_star),
+            _d(_f64(bvec)), _d(_f64(gvec)), cd(dx), cd(g.tx), ci(R), _i(_i32(sp.r_trans)), _d(_f64(sp.nu_hz)),
+            ci(sp.e_num), _d(lp), _i(_i32(sp.lev_g)), _d(_f64(g.b_all)), _d(_f64(g.vel_all)), _d(_f64(sp.t_Aji)),
+            _d(_f64(sp.t_Bij)), _d(xt), _d(_f64(g.dust_p)), _d(_f64(sp.kd_v)), _d(_f64(g.temp_all[1])), _d(J0),
+            _d(J2), ci(0), ci(1), cd(R_star), cd(T_star), ci(0), ci(0), _d(_f64(X_star)), ci(n_depth))
         return J0, J2
 
     def trace_out(self, node, g, del_ne, dir_, dx, n_depth):

@@ -189,9 +189,20 @@ void orc_pos_quadratic(double a, double b, double c, double *x1, int *comp) {
     }
 }
 
-/* check_neigh_fast, miscellaneous.f90:1017-1076 (star branch off) */
+/* check_neigh_fast, miscellaneous.f90:1017-1076, with its star branch (1056-1069):
+ * a star sphere met before the next cell boundary ends the ray there */
+static void check_neigh_fast_s(double *Xp, const double *dir, const double *Xn, int neigh_num, const int *neigh,
+                               const double *X_all, double *delta_x, int *next, int star_mode, const double *Xs,
+                               double R_star);
+
 void orc_check_neigh_fast(double *Xp, const double *dir, const double *Xn, int neigh_num, const int *neigh,
                           const double *X_all, double *delta_x, int *next) {
+    check_neigh_fast_s(Xp, dir, Xn, neigh_num, neigh, X_all, delta_x, next, 0, NULL, 0.0);
+}
+
+static void check_neigh_fast_s(double *Xp, const double *dir, const double *Xn, int neigh_num, const int *neigh,
+                               const double *X_all, double *delta_x, int *next, int star_mode, const double *Xs,
+                               double R_star) {
     int comp;
     double nd = norm2v(dir[0], dir[1], dir[2]);
     double np_ = norm2v(Xp[0], Xp[1], Xp[2]);
@@ -214,6 +225,17 @@ void orc_check_neigh_fast(double *Xp, const double *dir, const double *Xn, int n
         *delta_x = new_x;
         *next = neigh[i];
     }
+    if (star_mode == 1) {
+        double s0 = Xp[0] - Xs[0], s1 = Xp[1] - Xs[1], s2 = Xp[2] - Xs[2];
+        double sn = norm2v(s0, s1, s2);
+        double new_x = 0.0;
+        orc_pos_quadratic(nd * nd, 2.0 * (s0 * dir[0] + s1 * dir[1] + s2 * dir[2]), sn * sn - pow(R_star, 2.0),
+                          &new_x, &comp);
+        if (comp != 1 && new_x < *delta_x && new_x > 0.0) {
+            *delta_x = new_x;
+            *next = 0;
+        }
+    }
     for (int k = 0; k < 3; k++) Xp[k] = Xp[k] + *delta_x * dir[k];
 }
 
@@ -221,6 +243,12 @@ void orc_check_neigh_fast(double *Xp, const double *dir, const double *Xn, int n
  * (cells[0] = node), lens[] the cumulative nint(delta/dx) lengths.  Returns
  * node_count, or -1 when it exceeds n_depth (the reference stops there). */
 int orc_trace_out(int node, const orc_grid *g, const double *dir, double dx, int n_depth, int *cells, int *lens) {
+    return orc_trace_out_s(node, g, dir, dx, n_depth, cells, lens, 0, NULL, 0.0);
+}
+
+/* trace_out with the star test of check_neigh_fast on (star_mode = 1) */
+int orc_trace_out_s(int node, const orc_grid *g, const double *dir, double dx, int n_depth, int *cells, int *lens,
+                    int star_mode, const double *X_star, double R_star) {
     double Xp[3];
     const double *X = g->XYZ;
     for (int k = 0; k < 3; k++) Xp[k] = X[3 * (size_t)(node - 1) + k];
@@ -230,8 +258,8 @@ int orc_trace_out(int node, const orc_grid *g, const double *dir, double dx, int
     for (;;) {
         int nn = g->nbr_ptr[cur] - g->nbr_ptr[cur - 1];
         int nxt;
-        orc_check_neigh_fast(Xp, dir, X + 3 * (size_t)(cur - 1), nn, g->nbr_idx + g->nbr_ptr[cur - 1], X,
-                             &delta_x, &nxt);
+        check_neigh_fast_s(Xp, dir, X + 3 * (size_t)(cur - 1), nn, g->nbr_idx + g->nbr_ptr[cur - 1], X, &delta_x,
+                           &nxt, star_mode, X_star, R_star);
         nc++;
         if (nc > n_depth) return -1;
         lens[nc - 1] = (int)lround(delta_x / dx);
@@ -241,6 +269,105 @@ int orc_trace_out(int node, const orc_grid *g, const double *dir, double dx, int
     }
     for (int i = 1; i < nc; i++) lens[i] = lens[i - 1] + lens[i];
     return nc;
+}
+
+/* find_angles, miscellaneous.f90:86-134: polar angle to B and azimuth about B (from
+ * the x axis B x g) of a direction */
+void orc_find_angles(const double *bvec, const double *gvec, const double *dir, double *th, double *ph) {
+    double xv[3], cbd[3];
+    xv[0] = bvec[1] * gvec[2] - bvec[2] * gvec[1];
+    xv[1] = bvec[2] * gvec[0] - bvec[0] * gvec[2];
+    xv[2] = bvec[0] * gvec[1] - bvec[1] * gvec[0];
+    double xn = sqrt(xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]);
+    for (int i = 0; i < 3; i++) xv[i] = xv[i] / xn;
+    double bd = bvec[0] * dir[0] + bvec[1] * dir[1] + bvec[2] * dir[2];
+    if (bd > 1.0) *th = acos(1.0);
+    else if (bd < -1.0) *th = acos(-1.0);
+    else *th = acos(bd);
+    cbd[0] = bvec[1] * dir[2] - bvec[2] * dir[1];
+    cbd[1] = bvec[2] * dir[0] - bvec[0] * dir[2];
+    cbd[2] = bvec[0] * dir[1] - bvec[1] * dir[0];
+    *ph = atan2(xv[0] * dir[0] + xv[1] * dir[1] + xv[2] * dir[2], xv[0] * cbd[0] + xv[1] * cbd[1] + xv[2] * cbd[2]);
+}
+
+/* circle_points, miscellaneous.f90:1228-1265: n points (cos th, phi, w) on the star's
+ * disc of angular radius th_s; x is 3 x n.  One point sits at (cos 1, cos 1) as the
+ * reference writes it. */
+void orc_circle_points(int n, double th_s, double dth_s, double *x) {
+    const double pi = acos(-1.0);
+    if (n == 1) {
+        x[0] = x[1] = cos(1.0);
+        x[2] = 2.0 * pi * dth_s;
+    } else if (n < 10) {
+        double wph = 2.0 * pi / n, ww = wph * dth_s, xx = cos(th_s / 2.0);
+        for (int i = 1; i <= n; i++) {
+            x[3 * (i - 1)] = xx;
+            x[3 * (i - 1) + 1] = (i - 1) * wph;
+            x[3 * (i - 1) + 2] = ww;
+        }
+    } else {
+        double ga = pi * (3.0 - sqrt(5.0));  /* golden angle (Vogel) */
+        double ww = dth_s * 2.0 * pi / n;
+        for (int i = 1; i <= n; i++) {
+            x[3 * (i - 1)] = cos(th_s * sqrt(1.0 * i / n));
+            x[3 * (i - 1) + 1] = i * ga;
+            x[3 * (i - 1) + 2] = ww;
+        }
+    }
+}
+
+/* star_all, miscellaneous.f90:1139-1226: the direction set of a node that sees a star
+ * of solid angle dOm_star in direction kvec: n_star rays on the star's disc, then
+ * nth rings (with ceil(nph sin th) azimuths each) over the rest of the sphere, all
+ * re-expressed as (cos th, phi) about B.  xth is 3 x nth nph.  Returns 0, or -1 when
+ * the rays do not fit (the reference stops there). */
+int orc_star_all(int nth, int nph, const double *kvec, const double *gvec, const double *bvec, double dOm_star,
+                 int *n_star, int *n_sphere, double *xth) {
+    const double pi = acos(-1.0);
+    double dth_star = dOm_star / (2.0 * pi);
+    double th_star = acos(1 - dth_star);
+    double dOm_max = 4.0 * pi / (nth * nph);
+    int ns = (int)ceil(dOm_star / dOm_max);
+    if (ns > nth * nph) return -1;
+    orc_circle_points(ns, th_star, dth_star, xth);
+    for (int i = 0; i < ns; i++) xth[3 * i + 2] = xth[3 * i + 2] / (4.0 * pi);
+    int n = ns;
+    double dth_rest = (pi - th_star) / nth;
+    double th_i = th_star + dth_rest / 2.0;
+    for (int i = 1; i <= nth; i++) {
+        double th = (i - 1) * dth_rest + th_i, wth;
+        if (i == 1) {
+            double th_u = (i - 0.5) * dth_rest + th_i;
+            wth = cos(th_star) - cos(th_u);
+        } else if (i == nth) {
+            double th_l = (i - 1.5) * dth_rest + th_i;
+            wth = cos(th_l) + 1.0;
+        } else {
+            double th_u = (i - 0.5) * dth_rest + th_i;
+            double th_l = (i - 1.5) * dth_rest + th_i;
+            wth = cos(th_l) - cos(th_u);
+        }
+        int np_ = (int)ceil(nph * sin(th));
+        double wph = 2.0 * pi / np_;
+        double xx = cos(th);
+        for (int j = 1; j <= np_; j++) {
+            n++;
+            if (n > nph * nth) return -1;
+            xth[3 * (n - 1)] = xx;
+            xth[3 * (n - 1) + 1] = (j - 1) * wph;
+            xth[3 * (n - 1) + 2] = wth * wph / (4.0 * pi);
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        double dir[3], th, ph;
+        orc_find_dir(kvec, gvec, acos(xth[3 * i]), xth[3 * i + 1], dir);
+        orc_find_angles(bvec, gvec, dir, &th, &ph);
+        xth[3 * i] = cos(th);
+        xth[3 * i + 1] = ph;
+    }
+    *n_star = ns;
+    *n_sphere = n;
+    return 0;
 }
 
 /* black_body, miscellaneous.f90:263-285 */
@@ -303,8 +430,29 @@ double orc_ray_in(double temp_node, double vel_node, double p1_node, double p2_n
 
 /* inverse_rt, inverse_rt.f90:1-137 (star_mode 0).  J0/J2 of every radiative
  * line at `node` (1-based).  Returns 0, or -1 on path overflow. */
+static int inverse_rt_dirs(int node, const orc_grid *g, const orc_species *s, const orc_se_params *p,
+                           const double *bvec, int n_sphere, const double *xth, int n_star, double *J0, double *J2);
+
 int orc_inverse_rt(int node, const orc_grid *g, const orc_species *s, const orc_se_params *p,
                    const double *bvec, double *J0, double *J2) {
+    if (p->star_mod != 1) return inverse_rt_dirs(node, g, s, p, bvec, p->n_sphere, p->xth, 0, J0, J2);
+    /* trace_node's star branch, main_sub.f90:56-79 */
+    const double pi = acos(-1.0);
+    const double *Xn = g->XYZ + 3 * (size_t)(node - 1);
+    double dv[3] = {p->X_star[0] - Xn[0], p->X_star[1] - Xn[1], p->X_star[2] - Xn[2]};
+    double dn = norm2v(dv[0], dv[1], dv[2]);
+    double dOm = pi * pow(p->R_star / dn, 2.0);
+    for (int k = 0; k < 3; k++) dv[k] = dv[k] / dn;
+    double *xs = (double *)calloc(3 * (size_t)p->nth * p->nph, sizeof(double));
+    int n_star = 0, n_var = 0;
+    int rc = orc_star_all(p->nth, p->nph, dv, p->gvec, bvec, dOm, &n_star, &n_var, xs);
+    if (!rc) rc = inverse_rt_dirs(node, g, s, p, bvec, n_var, xs, n_star, J0, J2);
+    free(xs);
+    return rc ? -1 : 0;
+}
+
+static int inverse_rt_dirs(int node, const orc_grid *g, const orc_species *s, const orc_se_params *p,
+                           const double *bvec, int n_sphere, const double *xth, int n_star, double *J0, double *J2) {
     const int R = s->r_num, L = s->e_num;
     int nd = p->n_depth;
     int *cells = (int *)malloc(sizeof(int) * (nd + 1));
@@ -315,12 +463,13 @@ int orc_inverse_rt(int node, const orc_grid *g, const orc_species *s, const orc_
     int rc = 0;
     for (int k = 0; k < R; k++) J0[k] = J2[k] = 0.0;
     const double *V = g->vel;
-    for (int i = 0; i < p->n_sphere; i++) {
-        double mu = p->xth[3 * i], ph = p->xth[3 * i + 1], w = p->xth[3 * i + 2];
+    for (int i = 0; i < n_sphere; i++) {
+        double mu = xth[3 * i], ph = xth[3 * i + 1], w = xth[3 * i + 2];
         double c_in = (3.0 * (mu * mu) - 1.0) / 2.0;
         double dir[3];
+        const int star_ray = p->star_mod == 1 && i < n_star;
         orc_find_dir(bvec, p->gvec, acos(mu), ph, dir);
-        int nc = orc_trace_out(node, g, dir, p->dx, nd, cells, lens);
+        int nc = orc_trace_out_s(node, g, dir, p->dx, nd, cells, lens, star_ray, p->X_star, p->R_star);
         if (nc < 0) { rc = -1; break; }
         const double *vn = V + 3 * (size_t)(node - 1);
         double vel_node = vn[0] * dir[0] + vn[1] * dir[1] + vn[2] * dir[2];
@@ -335,7 +484,7 @@ int orc_inverse_rt(int node, const orc_grid *g, const orc_species *s, const orc_
         for (int k = 0; k < R; k++) {
             double nu0 = s->nu_hz[k];
             int l1 = s->r_trans[2 * k], l2 = s->r_trans[2 * k + 1];
-            double I_in = orc_black_body(2.73, nu0);
+            double I_in = orc_black_body(star_ray ? p->T_star : 2.73, nu0);
             for (int l = 0; l < nc; l++) {
                 int c = cells[l] - 1;
                 d1[l] = g->lev_p[(size_t)c * L + (l1 - 1)];

@@ -55,6 +55,12 @@ typedef struct {
     const double *bvec_all; /* 3 x N */
     int n_depth;
     int extra_info;
+    /* star mode (star_mod = 1, main_sub.f90:56-79): a star of radius R_star and
+     * temperature T_star at X_star (normalised coordinates); each node's directions are
+     * star_all(nth, nph, ...) instead of xth */
+    int star_mod, nth, nph;
+    double R_star, T_star;
+    const double *X_star;
 } orc_se_params;
 
 /* Wigner symbols, arguments are 2j / 2m like WIGXJPF's fwig3jj/fwig6jj. */
@@ -67,6 +73,12 @@ void orc_find_dir(const double *bvec, const double *gvec, double th, double ph, 
 void orc_pos_quadratic(double a, double b, double c, double *x1, int *comp);
 void orc_check_neigh_fast(double *X_phot, const double *dir, const double *X_node, int neigh_num,
                           const int *neigh, const double *X_all, double *delta_x, int *next);
+void orc_find_angles(const double *bvec, const double *gvec, const double *dir, double *th, double *ph);
+void orc_circle_points(int n, double th_s, double dth_s, double *x);
+int  orc_star_all(int nth, int nph, const double *kvec, const double *gvec, const double *bvec, double dOm_star,
+                  int *n_star, int *n_sphere, double *xth);
+int  orc_trace_out_s(int node, const orc_grid *g, const double *dir, double dx, int n_depth, int *cells,
+                     int *lens, int star_mode, const double *X_star, double R_star);
 int  orc_trace_out(int node, const orc_grid *g, const double *dir, double dx, int n_depth,
                    int *cells, int *lens);
 double orc_black_body(double T, double nu0);

@@ -257,7 +257,66 @@ def main():
     with open(os.path.join(OUT, "wigner_exact.json"), "w") as f:
         json.dump({"w3j": w3, "w6j": w6, "coupling": coup, "t_spont": tsp}, f)
     print("fixtures written to", OUT)
+    make_star(R)
+
+
+def make_star(R):
+    """Star mode: star_all/circle_points/find_angles vectors, and inverse_rt in star mode
+    (inverse_rt.f90:1-137 with star_mode = 1) on nodes of a 2,500-node cloud around a
+    central star of two sizes, so that circle_points' one-point, ring and Vogel branches
+    all occur."""
+    rng = np.random.default_rng(77)
+
+    def unit(v):
+        return v / np.linalg.norm(v)
+
+    sa_in, sa_ns, sa_x, sa_ofs = [], [], [], [0]
+    for t in range(24):
+        k, gv, b = unit(rng.normal(size=3)), unit(rng.normal(size=3)), unit(rng.normal(size=3))
+        dOm = float(10 ** rng.uniform(-4, 0.3))
+        nth, nph = (6, 11) if t % 2 else (9, 32)
+        ns, x = R.star_all(nth, nph, k, gv, b, dOm)
+        sa_in.append(np.concatenate([[nth, nph, dOm], k, gv, b]))
+        sa_ns.append(ns)
+        sa_x.append(x)
+        sa_ofs.append(sa_ofs[-1] + x.shape[1])
+    cp = []
+    for n in (1, 4, 9, 10, 23):
+        th_s, dth_s = float(rng.uniform(0.01, 0.5)), float(rng.uniform(0.001, 0.1))
+        cp.append(np.concatenate([[n, th_s, dth_s], R.circle_points(n, th_s, dth_s).ravel(order="F")]))
+    fa = []
+    for _ in range(16):
+        b, gv, d = unit(rng.normal(size=3)), unit(rng.normal(size=3)), unit(rng.normal(size=3))
+        fa.append(np.concatenate([b, gv, d, R.find_angles(b, gv, d)]))
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 2500, seed=41)
+    del_ne = g.del_ne()
+    r = np.linalg.norm(g.XYZ[:, : g.bulk_node_num - 1], axis=0)
+    Xs, Ts = np.zeros(3), 6000.0
+    rows = []
+    for Rs in (0.03, 0.15):
+        cand = np.where((r > Rs + 0.02) & (g.ab_all[: g.bulk_node_num - 1] > 0))[0]
+        near = cand[np.argsort(r[cand])[:4]]
+        far = rng.choice(cand, 6, replace=False)
+        for n in np.concatenate([near, far]) + 1:
+            X = g.XYZ[:, n - 1]
+            dv = Xs - X
+            dn = np.linalg.norm(dv)
+            dOm = np.pi * (Rs / dn) ** 2
+            b = g.bvec_all[:, n - 1]
+            ns, xs = R.star_all(6, 11, dv / dn, g.gvec, b, dOm)
+            J0, J2 = R.inverse_rt_star(n, g, sp, del_ne, xs, ns, 32, cloud.DX_INWARD, b, g.gvec, 4096, Xs, Rs, Ts)
+            rows.append(np.concatenate([[n, Rs, ns, xs.shape[1]], J0, J2]))
+    np.savez_compressed(os.path.join(OUT, "star.npz"), sa_in=np.array(sa_in), sa_ns=np.array(sa_ns),
+                        sa_x=np.concatenate(sa_x, axis=1), sa_ofs=np.array(sa_ofs),
+                        circle=np.array([np.pad(c, (0, 3 + 3 * 23 - len(c))) for c in cp]),
+                        find_angles=np.array(fa), inverse=np.array(rows), digest=cloud_digest(g), T_star=Ts)
+    print("star fixtures written to", OUT)
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    if sys.argv[1:] == ["star"]:
+        make_star(Reference())
+    else:
+        main()

@@ -279,7 +279,7 @@ def test_inward_pass_with_dust(oracle, variant, waves):
     g = cloud.make_cloud(sp, 2500, seed=13)
     nb = g.bulk_node_num - 1
     g.dust_p[:nb] = 1e-17 * g.h2_dens[:nb] / 1e12
-    sp.kd_v = np.array([10.0, 20.0, 30.0])  # dust optical depth ~0.1-1 across the cloud
+    sp.kd_v = np.array([3000.0, 6000.0, 9000.0])  # enough dust to change J by ~1e-2
     xth = cloud.legendre_all(5, 9)
     pt = Portal(0)
     pt.upload_grid(g)

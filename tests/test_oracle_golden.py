@@ -209,3 +209,41 @@ def test_ray_in_raytrace_matches_reference(oracle, golden_dir):
                                        C.c_double(1.5e11 * cloud.DX_IMAGE), C.c_double(115.27e9), C.c_double(1e-19), _d(out))
         scale = max(abs(ref[0]), abs(ref[4]), 1e-300)
         np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-13 * scale)
+
+
+def test_star_mode_matches_reference(oracle, golden_dir):
+    # star_all, circle_points and find_angles (miscellaneous.f90:86-134, 1139-1265) and
+    # inverse_rt in star mode (inverse_rt.f90:1-137, check_neigh_fast's star test
+    # miscellaneous.f90:1056-1069) against the compiled reference
+    from tests.golden.make_golden import cloud_digest
+    z = np.load(os.path.join(golden_dir, "star.npz"))
+    for k, row in enumerate(z["sa_in"]):
+        nth, nph, dOm = int(row[0]), int(row[1]), row[2]
+        kv, gv, bv = row[3:6], row[6:9], row[9:12]
+        ns, x = oracle.star_all(nth, nph, kv, gv, bv, dOm)
+        ref = z["sa_x"][:, z["sa_ofs"][k]:z["sa_ofs"][k + 1]]
+        assert ns == z["sa_ns"][k] and x.shape == ref.shape
+        np.testing.assert_allclose(x, ref, rtol=0, atol=1e-14)
+    for row in z["circle"]:
+        n, th_s, dth_s = int(row[0]), row[1], row[2]
+        x = np.zeros((3, n), order="F")
+        oracle.lib.orc_circle_points(n, C.c_double(th_s), C.c_double(dth_s), _d(x))
+        np.testing.assert_allclose(x.ravel(order="F"), row[3:3 + 3 * n], rtol=0, atol=1e-15)
+    for row in z["find_angles"]:
+        th, ph = C.c_double(), C.c_double()
+        oracle.lib.orc_find_angles(_d(_f64(row[0:3])), _d(_f64(row[3:6])), _d(_f64(row[6:9])), C.byref(th),
+                                   C.byref(ph))
+        np.testing.assert_allclose([th.value, ph.value], row[9:11], rtol=0, atol=1e-14)
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 2500, seed=41)
+    assert cloud_digest(g) == str(z["digest"])
+    inv = z["inverse"]
+    assert len(set(inv[:, 2])) > 1  # more than one circle_points branch among the nodes
+    xth = cloud.legendre_all(6, 11)
+    for Rs in sorted(set(inv[:, 1])):
+        rows = inv[inv[:, 1] == Rs]
+        nodes = rows[:, 0].astype(np.int32)
+        star = dict(nth=6, nph=11, R_star=float(Rs), T_star=float(z["T_star"]), X_star=np.zeros(3))
+        J0, J2 = oracle.inverse_nodes(g, sp, xth, 32, cloud.DX_INWARD, nodes, star=star)
+        np.testing.assert_allclose(J0.T, rows[:, 4:7], rtol=1e-13, atol=0)
+        np.testing.assert_allclose(J2.T, rows[:, 7:10], rtol=0, atol=1e-13 * np.abs(rows[:, 4:7]).max())
