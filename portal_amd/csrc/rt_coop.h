@@ -7,10 +7,13 @@
 // v.dir/b, 1/b, segment length; per (cell, line) log(K dl) and S, and with dust
 // alpha dl and B_nu(T_dust)) into one half of a
 // double-buffered LDS area while the waves integrate ray r from the other half; one
-// workgroup barrier per ray.  Lane `lane` of wave `w` owns pairs
-// p = (q W + w) 64 + lane for q < SLOTS, so the pairs are spread evenly over waves
-// and a lane holds SLOTS accumulators (not all of them, as in the per-wave form),
-// which keeps the register count, and with it occupancy, independent of R C.
+// workgroup barrier per ray.  Thread t = 64 w + lane owns SLOTS consecutive channels
+// of one line: line t mod R, channels (t div R) SLOTS + q, for t < R G with
+// G = ceil(C / SLOTS).  A lane reads one staged (L, S) per cell for all its slots
+// (the lanes of a wave read R neighbouring entries, conflict-free), every wave holds
+// every line, so the waves' work per ray stays balanced, and a lane holds SLOTS
+// accumulators (not one per pair of the node, as in the per-wave form), which keeps
+// the register count, and with it occupancy, independent of R C.
 //
 // The segment update is branch-free: tau and e^-tau are evaluated for every lane.  In
 // FP64 the update is selected when tau > 1e-30 (false for NaN), which is the
@@ -88,29 +91,25 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     const int npairs = R * C;
     const size_t bb = coop_buf_bytes<Real>(cap, R, DUST);
 
-    Real v0[SLOTS], e0[SLOTS], k0x[SLOTS], k0y[SLOTS], iin[SLOTS], d0x[SLOTS], d0y[SLOTS];
-    int lo[SLOTS];
+    const int G = (C + SLOTS - 1) / SLOTS;  // lanes per line
+    const int t = wave * 64 + lane;
+    const bool lane_on = t < G * R;
+    const int lo = lane_on ? t % R : 0;      // this lane's line
+    const int cb = (t / R) * SLOTS;          // its first channel
+    Real v0[SLOTS], e0[SLOTS];
     bool on[SLOTS];
     Real S0[SLOTS], C0[SLOTS], S2[SLOTS], C2[SLOTS];
+    const R2 k0 = A::ks(a, (size_t)node * R + lo);  // the node's own cell, no skip
+    R2 d0;
+    if (DUST) d0 = A::ksd(a, (size_t)node * R + lo);
+    const Real iin = (Real)a.I_in[lo];
 #pragma unroll
     for (int q = 0; q < SLOTS; q++) {
-        const int p = (q * W + wave) * 64 + lane;
-        on[q] = p < npairs;
-        const int l = on[q] ? p / C : 0;
-        const int nv = on[q] ? p - l * C - nvel : 0;
-        lo[q] = l;
+        on[q] = lane_on && cb + q < C;
+        const int nv = on[q] ? cb + q - nvel : 0;
         v0[q] = ((Real)2 * b) * (Real)nv / (Real)nvel;
         const Real aa = v0[q] * inv_bn;
         e0[q] = A::ex(-(aa * aa));
-        const R2 k = A::ks(a, (size_t)node * R + l);  // the node's own cell, no skip
-        k0x[q] = k.x;
-        k0y[q] = k.y;
-        if (DUST) {
-            const R2 d = A::ksd(a, (size_t)node * R + l);
-            d0x[q] = d.x;
-            d0y[q] = d.y;
-        }
-        iin[q] = (Real)a.I_in[l];
         S0[q] = C0[q] = S2[q] = C2[q] = (Real)0;
     }
 
@@ -133,19 +132,23 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
         for (int q = 0; q < SLOTS; q++) {
             if (DUST) {
                 // S = (e phi + j)/(k phi + alpha), tau = (k phi + alpha) dl (inverse_rt.f90:289-297)
-                const Real kp = k0x[q] * e0[q];
-                const Real tau0 = (kp + d0x[q]) * dl0;
-                const Real S0 = (k0y[q] * kp + d0y[q] * d0x[q]) / (kp + d0x[q]);
+                const Real kp = k0.x * e0[q];
+                const Real tau0 = (kp + d0.x) * dl0;
+                const Real Sc = (k0.y * kp + d0.y * d0.x) / (kp + d0.x);
                 Tr[q] = A::ex(-tau0);
-                Em[q] = S0 * A::one_minus(tau0, Tr[q]);
+                Em[q] = Sc * A::one_minus(tau0, Tr[q]);
             } else {
-                const Real tau0 = k0x[q] * e0[q] * dl0;
+                const Real tau0 = k0.x * e0[q] * dl0;
                 Tr[q] = A::ex(-tau0);
-                Em[q] = k0y[q] * A::one_minus(tau0, Tr[q]);
+                Em[q] = k0.y * A::one_minus(tau0, Tr[q]);
             }
             v0s[q] = v0[q] + vel_node;
             if (!on[q]) Tr[q] = (Real)0;
         }
+        // the next cell's staged data are read one cell ahead
+        CellRec<Real> cn = cr[nc > 1 ? 1 : 0];
+        R2 kn = ks[(size_t)(nc > 1 ? 1 : 0) * R + lo], dn;
+        if (DUST) dn = kds[(size_t)(nc > 1 ? 1 : 0) * R + lo];
         for (int i = 1; i < nc; i++) {
             if ((i & 3) == 0) {
                 bool live = false;
@@ -156,18 +159,21 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                 }
                 if (!__any(live)) break;
             }
-            const CellRec<Real> c = cr[i];
-            const R2 *kr = ks + (size_t)i * R;
-            const R2 *kdr = kds + (size_t)i * R;
+            const CellRec<Real> c = cn;
+            const R2 k = kn;
+            R2 d;
+            if (DUST) d = dn;
+            const int inext = i + 1 < nc ? i + 1 : i;
+            cn = cr[inext];
+            kn = ks[(size_t)inext * R + lo];
+            if (DUST) dn = kds[(size_t)inext * R + lo];
 #pragma unroll
             for (int q = 0; q < SLOTS; q++) {
-                const R2 k = kr[lo[q]];
                 const Real u = fma(v0s[q], c.inv_b, -c.vb);
                 Real tau = A::tau_of(k.x, u), S = k.y;
                 if (DUST) {
                     // tau = tau_line + alpha dl; S the opacity-weighted mean of the line's
                     // and the dust's source functions (inverse_rt.f90:273-287)
-                    const R2 d = kdr[lo[q]];
                     const Real tl = tau;
                     tau = tl + d.x;
                     S = (k.y * tl + d.y * d.x) * A::rcp(tau);
@@ -188,7 +194,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
         }
 #pragma unroll
         for (int q = 0; q < SLOTS; q++) {
-            const Real Iout = Em[q] + Tr[q] * iin[q];
+            const Real Iout = Em[q] + Tr[q] * iin;
             if (A::kahan) {
                 Real y = w * Iout - C0[q];
                 Real t = S0[q] + y;
@@ -210,7 +216,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     double *red2 = red0 + npairs;
 #pragma unroll
     for (int q = 0; q < SLOTS; q++) {
-        const int p = (q * W + wave) * 64 + lane;
+        const int p = lo * C + cb + q;
         if (on[q]) {
             red0[p] = (double)S0[q] - (double)C0[q];
             red2[p] = (double)S2[q] - (double)C2[q];
@@ -249,15 +255,19 @@ hipError_t launch_coop_s(const RtArgs &a, int cap, size_t lds, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Slots per lane: ceil(R C / (64 W)), rounded up to an instantiated count.
+// Slots per lane: the fewest consecutive channels per lane with which every line's
+// channels fit in its share of the 64 W lanes, ceil(C / floor(64 W / R)), rounded up to
+// an instantiated count.
 template <typename Real>
 hipError_t launch_coop(const RtArgs &a, int cap, hipStream_t s) {
     const int W = a.waves;
-    if (W < 1 || W > 8) return hipErrorInvalidValue;
+    if (W < 1 || W > 8 || a.R < 1) return hipErrorInvalidValue;
     const int npairs = a.R * a.C;
     const size_t lds = coop_lds_bytes<Real>(cap, a.R, npairs, a.dust != 0);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    const int need = (npairs + 64 * W - 1) / (64 * W);
+    const int per_line = (64 * W) / a.R;
+    if (per_line < 1) return hipErrorInvalidValue;
+    const int need = (a.C + per_line - 1) / per_line;
     switch (need) {
         case 1: return launch_coop_s<Real, 1>(a, cap, lds, s);
         case 2: return launch_coop_s<Real, 2>(a, cap, lds, s);

@@ -131,9 +131,16 @@ def train(levels=10, clouds=10, nodes=200000, nth=9, nph=32, nvel=64, epochs=100
     pt.mlp_set_params(None, norm.astype(np.float32))
     perm = np.random.default_rng(seed0).permutation(len(X))
     X, Y = X[perm], Y[perm]
-    hist1, ms1 = pt.mlp_train(X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed0)
-    hist2, ms2 = pt.mlp_train(X, Y, epochs=max(1, epochs // 4), batch=batch, lr=lr / 5, seed=seed0 + 1)
-    log(f"trained on {len(X)} samples: loss {hist1[0]:.3e} -> {hist2[-1]:.3e} in {(ms1 + ms2) / 1e3:.1f}s")
+    # step schedule: `epochs` at lr, then a quarter as many at lr/3, lr/10 and lr/30
+    hist, ms = [], 0.0
+    for k, (ep, f) in enumerate(((epochs, 1.0), (epochs // 4, 1 / 3), (epochs // 4, 0.1), (epochs // 4, 1 / 30))):
+        if ep <= 0:
+            continue
+        h, t = pt.mlp_train(X, Y, epochs=ep, batch=batch, lr=lr * f, seed=seed0 + k)
+        hist.append(h)
+        ms += t
+    hist = np.concatenate(hist)
+    log(f"trained on {len(X)} samples: loss {hist[0]:.3e} -> {hist[-1]:.3e} in {ms / 1e3:.1f}s")
     # held-out error: log10 of sqrt(2J+1) rho00 per level, and 1e3 rho20/rho00
     P = pt.mlp_forward(Xv)
     L = sp.e_num
@@ -146,7 +153,7 @@ def train(levels=10, clouds=10, nodes=200000, nth=9, nph=32, nvel=64, epochs=100
     path = out or weights_path(sp)
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     np.savez_compressed(path, params=params, norm=nm, width=WIDTH, depth=DEPTH, samples=len(X),
-                        loss=np.concatenate([hist1, hist2]), key=species_key(sp), heldout=val, format=FORMAT)
+                        loss=hist, key=species_key(sp), heldout=val, format=FORMAT)
     pt.close()
     return path
 
