@@ -196,6 +196,15 @@ def main():
     # v_exp_f32 issue rate of the device
     exp_rate = 2.0 * sp.r_num * (2 * args.nvel + 1) * stm["n_cross"] / (avg_ms * 1e-3)
     exp_peak = pt.exp_peak()
+    # secondary roofline: the SE solve against FP64 peak, with (2/3) c^3 (LU) + 2 c^2
+    # (solve) + 2 c^2 (refinement) + 4 c^2 (assembly, dense bound) flops per node
+    _, c_dim = pt.species_info()
+    se_flops = (2.0 / 3.0) * c_dim ** 3 + 8.0 * c_dim ** 2
+    se_ms = stm["t_se_ms"] / max(1, stm["iterations"])
+    se_tfs = N_b * se_flops / (se_ms * 1e-3) / 1e12
+    secondary = {"se_solve": {"bound": "fp64", "achieved": se_tfs, "peak": 78.6, "unit": "TFLOP/s",
+                              "frac": se_tfs / 78.6, "ms_per_pass": se_ms, "c_dim": c_dim,
+                              "flops_per_node": se_flops}}
     traffic = None
     tf = os.path.join(ROOT, "profiles", "rt_mixed_traffic.json")
     if os.path.exists(tf):
@@ -261,6 +270,7 @@ def main():
                      "bytes_per_crossing": bytes_per_cross, "exp_per_s": exp_rate,
                      "exp_peak_per_s": exp_peak, "exp_frac": exp_rate / exp_peak},
         "cpu_baseline": cpu,
+        "secondary_rooflines": secondary,
     }
     print(json.dumps(res), flush=True)
     pt.close()

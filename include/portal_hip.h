@@ -40,7 +40,7 @@ extern "C" {
 #define PORTAL_ERR_TEMP (-6)         /* T_kin above the collision table, miscellaneous.f90:331-360 */
 #define PORTAL_ERR_STATE (-7)        /* grid/species not uploaded */
 #define PORTAL_ERR_NOMEM (-8)        /* device allocation failed */
-#define PORTAL_ERR_UNSUPPORTED (-9)  /* star mode (star_mod=1); dust in a batch of models */
+#define PORTAL_ERR_UNSUPPORTED (-9)  /* dust or star mode in a batch of models */
 
 #define PORTAL_FP64 0   /* every stage in FP64 */
 #define PORTAL_MIXED 1  /* FP32 ray transfer + compensated FP32 J sums; FP64 SE */
@@ -57,7 +57,12 @@ typedef struct {
     int n_depth;             /* maximum cells per inward ray */
     int extra_info;          /* 1: also fill pol_rates and JJ_20 */
     int precision;           /* PORTAL_FP64 or PORTAL_MIXED */
-    int star_mod;            /* must be 0 */
+    int star_mod;            /* 1: star mode (main_sub.f90:56-79): each node's directions are
+                                star_all(nth, nph) about the star instead of xth; rays towards the
+                                star end on its surface and start with B_nu(T_star) */
+    int nth, nph;            /* star mode: the quadrature of star_all (main_para.f90 nth, nph) */
+    double R_star, T_star;   /* star mode: radius (normalised like XYZ) and temperature [K] */
+    const double *X_star;    /* star mode: position (3, normalised like XYZ) */
 } portal_se_params;
 
 typedef struct {
@@ -87,6 +92,15 @@ typedef struct {
     double alpha;            /* dust polarisation (para_finish.f90:78) */
     int n_depth_rt;
     int precision;           /* PORTAL_FP64 or PORTAL_MIXED */
+    int star_mod;            /* 1: star mode (raytrace.f90:103-114): a pixel within R_star of the
+                                rotated star's centre starts on the star's near side,
+                                z = Z_star + R_star, with B_nu(T_star) as incoming intensity; the
+                                final background subtraction stays B_nu(2.73 K) */
+    double R_star, T_star;   /* star mode: radius (normalised like XYZ) and temperature [K] */
+    const double *X_star;    /* star mode: position (3, normalised like XYZ, before rotation) */
+    int node_pixels;         /* 1: the reference's pix = 0 mode (para_finish.f90:245-330), one pixel
+                                per bulk node at its rotated (x, y), outputs bulk_node_num x nlevels
+                                x (2nvel+1), im_pix and im_size unused; 0: the (2 im_pix+1)^2 grid */
 } portal_img_params;
 
 typedef struct {
@@ -170,7 +184,10 @@ int portal_batch_lambda_iterate(portal_ctx **ctxs, int n_models, const portal_se
 
 /* Image pass for one inclination.  pol_pops is the rescaled table of
  * para_finish.f90:222-240 (node_num x 2L).  Outputs are n_rt x nlevels x (2nvel+1)
- * (all_I(:,:,inc,:) of para_finish.f90:356-359); dust (4 such cubes) may be NULL. */
+ * (all_I(:,:,inc,:) of para_finish.f90:356-359), n_rt = (2 im_pix+1)^2, or
+ * bulk_node_num with node_pixels (para_finish.f90:246-249); dust (4 such cubes) may be
+ * NULL.  A pixel whose start point lies off the model sphere returns PORTAL_ERR_ARG
+ * (the reference's march would not end there, raytrace.f90:98). */
 int portal_image_pass(portal_ctx *ctx, const portal_img_params *p, const double *pol_pops,
                       const double *bvec_all, double *I, double *Q, double *U, double *TAU,
                       double *dust, portal_stats *stats);

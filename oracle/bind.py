@@ -216,9 +216,12 @@ class Oracle:
 
     def image_pass(self, g, sp, pol_pops, lines, th_inc, ph_az, nvel, vel_out, im_pix, im_size, dx,
                    alpha=0.15, n_depth_rt=4096, pix_lo=1, pix_hi=None, nthreads=0, bvec_all=None,
-                   with_dust=False):
-        """One inclination of the image pass.  lines = list of (l1, l2, nu0, Aji, kd)."""
-        n_rt = (2 * im_pix + 1) ** 2
+                   with_dust=False, star=None, node_pixels=False):
+        """One inclination of the image pass.  lines = list of (l1, l2, nu0, Aji, kd).
+        star = dict(R_star=, T_star=, X_star=) (unit-sphere units, X_star before rotation)
+        puts the star behind the pixels it covers (raytrace.f90:103-114).  node_pixels: one
+        pixel per bulk node at its rotated (x, y), the reference's pix = 0 mode."""
+        n_rt = g.bulk_node_num if node_pixels else (2 * im_pix + 1) ** 2
         if pix_hi is None:
             pix_hi = n_rt
         G = self.grid(g)
@@ -238,7 +241,11 @@ class Oracle:
         rc = self.lib.orc_image_pass(C.byref(G), C.byref(S), nl, _i(l1), _i(l2), _d(nu), _d(A), _d(kd),
                                      C.c_double(th_inc), C.c_double(ph_az), _d(bv), _d(pp), nvel, _d(vo), im_pix,
                                      C.c_double(im_size), C.c_double(dx), C.c_double(alpha), n_depth_rt, pix_lo,
-                                     pix_hi, *[_d(o) for o in outs], _d(dust) if with_dust else None, nthreads)
+                                     pix_hi, 1 if node_pixels else 0, 1 if star else 0,
+                                     C.c_double(star["R_star"] if star else 0.0),
+                                     C.c_double(star["T_star"] if star else 0.0),
+                                     _d(_f64(star["X_star"])) if star else None,
+                                     *[_d(o) for o in outs], _d(dust) if with_dust else None, nthreads)
         if rc:
             raise RuntimeError(f"oracle image_pass failed ({rc})")
         if with_dust:
@@ -365,29 +372,33 @@ class Reference:
                           _d(_f64(J0)), _d(_f64(J2)), _d(AC))
         return AC
 
-    def rotate_all_save(self, g, th, ph, bvec_all):
+    def rotate_all_save(self, g, th, ph, bvec_all, X_star=None):
         N = g.node_num
         vn = np.zeros((3, N), order="F")
         bn = np.zeros((3, N), order="F")
         xn = np.zeros((3, N), order="F")
-        xs = np.zeros(3)
+        xs = _f64(np.zeros(3) if X_star is None else X_star)
         xsn = np.zeros(3)
         self.lib.rotate_all_save_(C.byref(C.c_int(N)), C.byref(C.c_double(th)), C.byref(C.c_double(ph)),
                                   _d(_f64(g.vel_all)), _d(_f64(bvec_all)), _d(_f64(g.XYZ)), _d(xs), _d(vn), _d(bn),
                                   _d(xn), _d(xsn))
+        if X_star is not None:
+            return vn, bn, xn, xsn
         return vn, bn, xn
 
-    def trace_out_raytrace(self, x, y, g, X_inc, del_ne, dx, n_depth_rt, nu0_all):
+    def trace_out_raytrace(self, x, y, g, X_inc, del_ne, dx, n_depth_rt, nu0_all, star=None):
+        """star = dict(R_star=, T_star=, Xs=) with Xs the rotated star position."""
         ne = _i32(del_ne)
         dl = np.zeros((2, n_depth_rt), dtype=np.int32, order="F")
         nc = C.c_int(0)
         nlev = len(nu0_all)
         I_in = np.zeros(nlev)
         d = np.array([0.0, 0.0, 1.0])
-        xs = np.zeros(3)
+        xs = _f64(star["Xs"]) if star else np.zeros(3)
+        sm, Rs, Ts = (1, star["R_star"], star["T_star"]) if star else (0, 0.0, 0.0)
         self.lib.trace_out_raytrace_(C.byref(C.c_double(x)), C.byref(C.c_double(y)), C.byref(C.c_int(g.node_num)),
-                                     _d(_f64(X_inc)), C.byref(C.c_int(0)), C.byref(C.c_double(0.0)), _d(xs),
-                                     C.byref(C.c_double(0.0)), _i(ne), C.byref(C.c_int(ne.shape[1])),
+                                     _d(_f64(X_inc)), C.byref(C.c_int(sm)), C.byref(C.c_double(Rs)), _d(xs),
+                                     C.byref(C.c_double(Ts)), _i(ne), C.byref(C.c_int(ne.shape[1])),
                                      _i(_i32(g.del_num_ne)), _d(d), C.byref(C.c_double(dx)),
                                      C.byref(C.c_int(n_depth_rt)), _i(dl), C.byref(nc), C.byref(C.c_int(nlev)),
                                      _d(_f64(nu0_all)), _d(I_in))

@@ -1026,7 +1026,8 @@ int orc_solve_from_J(const orc_grid *g, const orc_species *s, int node_lo, int n
 
 /* rotate_all_save, miscellaneous.f90:844-907 */
 void orc_rotate_all_save(int N, double th_inc, double ph_az, const double *vel, const double *bvec, const double *XYZ,
-                         double *vel_new, double *bvec_new, double *XYZ_new) {
+                         double *vel_new, double *bvec_new, double *XYZ_new, const double *XYZ_star,
+                         double *XYZ_star_new) {
     double xv[3] = {1.0, 0.0, 0.0}, zv[3] = {0.0, 0.0, 1.0}, Rth[9], Rph[9], R[9];
     rotmat(xv, th_inc, Rth);
     rotmat(zv, ph_az, Rph);
@@ -1048,14 +1049,40 @@ void orc_rotate_all_save(int N, double th_inc, double ph_az, const double *vel, 
             bvec_new[3 * (size_t)i + j] = b;
             XYZ_new[3 * (size_t)i + j] = x;
         }
+    /* the star's position, rotated the same way (miscellaneous.f90:900-905) */
+    if (XYZ_star && XYZ_star_new)
+        for (int j = 0; j < 3; j++) {
+            double x = 0.0;
+            for (int k = 0; k < 3; k++) x = x + R[j + 3 * k] * XYZ_star[k];
+            XYZ_star_new[j] = x;
+        }
 }
 
-/* trace_out_raytrace, raytrace.f90:58-177 (star_mod 0) with check_neigh,
- * miscellaneous.f90:1-46.  cells/steps are the records of depth_list (cell id
- * 1-based, cumulative step count).  Returns 0, or -1 on n_depth_rt overflow. */
+/* trace_out_raytrace, raytrace.f90:58-177 with check_neigh, miscellaneous.f90:1-46.
+ * cells/steps are the records of depth_list (cell id 1-based, cumulative step count).
+ * With star_mod 1 a pixel within R_star of the (rotated) star's centre Xs in the sky
+ * plane starts on the star's near side, z = Xs_z + R_star (raytrace.f90:103-114), and
+ * *star_hit is set so that the caller takes B(T_star) as its incoming intensity.
+ * Returns 0, -1 on n_depth_rt overflow, or -2 for a start point off the sphere. */
 int orc_trace_out_raytrace(double x_co, double y_co, int N, const double *X, const int *nbr_ptr, const int *nbr_idx,
-                           double dx, int n_depth_rt, int *cells, int *steps, int *node_count) {
+                           double dx, int n_depth_rt, int star_mod, double R_star, const double *Xs, int *cells,
+                           int *steps, int *node_count, int *star_hit) {
     double z_co = -1.0 * sqrt(1.0 - x_co * x_co - y_co * y_co);
+    int hit = 0;
+    if (star_mod == 1) {
+        const double xc = x_co - Xs[0], yc = y_co - Xs[1];
+        const double rc0 = sqrt(xc * xc + yc * yc);
+        if (rc0 < R_star) {
+            z_co = Xs[2] + R_star;
+            hit = 1;
+        }
+    }
+    if (star_hit) *star_hit = hit;
+    /* a start point off the sphere (NaN z) would march forever in the reference */
+    if (!(z_co == z_co)) {
+        *node_count = 0;
+        return -2;
+    }
     double st[3] = {x_co, y_co, z_co};
     double rsave = 10.0;
     int node = 0;
@@ -1243,13 +1270,18 @@ void orc_ray_in_raytrace(double vel_obs, int node_count, const double *pol, cons
 int orc_image_pass(const orc_grid *g, const orc_species *s, int nlevels, const int *l1, const int *l2,
                    const double *nu0_all, const double *Aji_all, const double *kd_all, double th_inc, double ph_az,
                    const double *bvec_all, const double *pol_pops, int nvel, const double *vel_out, int im_pix,
-                   double im_size, double dx, double alpha, int n_depth_rt, int pix_lo, int pix_hi, double *out_I,
-                   double *out_Q, double *out_U, double *out_TAU, double *dust_out, int nthreads) {
+                   double im_size, double dx, double alpha, int n_depth_rt, int pix_lo, int pix_hi, int node_pixels,
+                   int star_mod, double R_star, double T_star, const double *X_star, double *out_I, double *out_Q,
+                   double *out_U, double *out_TAU, double *dust_out, int nthreads) {
     const int N = g->node_num, L = s->e_num;
-    const int n_side = 2 * im_pix + 1, n_rt = n_side * n_side, C = 2 * nvel + 1;
+    /* node_pixels: the reference's pix = 0 mode (para_finish.f90:245-330), one pixel per
+     * bulk node at its rotated (x, y); else the (2 im_pix + 1)^2 grid (369-376) */
+    const int n_side = 2 * im_pix + 1, n_rt = node_pixels ? g->bulk_node_num : n_side * n_side, C = 2 * nvel + 1;
     double *vi = (double *)malloc(sizeof(double) * 9 * (size_t)N);
     double *bi = vi + 3 * (size_t)N, *xi = bi + 3 * (size_t)N;
-    orc_rotate_all_save(N, th_inc, ph_az, g->vel, bvec_all, g->XYZ, vi, bi, xi);
+    const double xs0[3] = {0.0, 0.0, 0.0};
+    double xs[3] = {0.0, 0.0, 0.0};
+    orc_rotate_all_save(N, th_inc, ph_az, g->vel, bvec_all, g->XYZ, vi, bi, xi, X_star ? X_star : xs0, xs);
     double *pr = (double *)calloc((size_t)nlevels * 4 * N, sizeof(double)); /* all_pol_rayt(nlevels,4,N) */
     for (int node = 1; node <= g->bulk_node_num; node++) {
         if (pol_pops[(node - 1)] != 0.0) {
@@ -1271,12 +1303,21 @@ int orc_image_pass(const orc_grid *g, const orc_species *s, int nlevels, const i
 #endif
 #pragma omp parallel for schedule(dynamic, 8) reduction(min : err)
     for (int pix = pix_lo; pix <= pix_hi; pix++) {
-        int ii = (pix - 1) / n_side - im_pix, jj = (pix - 1) % n_side - im_pix;
-        double x = ii * im_size / im_pix, y = jj * im_size / im_pix;
+        double x, y;
+        if (node_pixels) {
+            x = xi[3 * (size_t)(pix - 1)];
+            y = xi[3 * (size_t)(pix - 1) + 1];
+        } else {
+            int ii = (pix - 1) / n_side - im_pix, jj = (pix - 1) % n_side - im_pix;
+            x = ii * im_size / im_pix;
+            y = jj * im_size / im_pix;
+        }
         int *cells = (int *)malloc(sizeof(int) * 2 * (size_t)n_depth_rt);
         int *steps = cells + n_depth_rt;
         int nc = 0;
-        if (orc_trace_out_raytrace(x, y, N, xi, g->nbr_ptr, g->nbr_idx, dx, n_depth_rt, cells, steps, &nc)) {
+        int hit = 0;
+        if (orc_trace_out_raytrace(x, y, N, xi, g->nbr_ptr, g->nbr_idx, dx, n_depth_rt, star_mod, R_star, xs, cells,
+                                   steps, &nc, &hit)) {
             err = -1;
             free(cells);
             continue;
@@ -1295,7 +1336,7 @@ int orc_image_pass(const orc_grid *g, const orc_species *s, int nlevels, const i
         for (int j = 0; j < nlevels; j++) {
             for (int l = 0; l < nc; l++)
                 for (int k = 0; k < 4; k++) pl[4 * l + k] = pr[(size_t)j + (size_t)nlevels * (k + 4 * (size_t)(cells[l] - 1))];
-            double I_in = orc_black_body(2.73, nu0_all[j]);
+            double I_in = orc_black_body(hit ? T_star : 2.73, nu0_all[j]);
             for (int k = 0; k < C; k++) {
                 double o8[8];
                 orc_ray_in_raytrace(vel_out[k], nc, pl, bl, bv, du, alpha, kd_all[j], bb, vl, steps, g->tx * dx,

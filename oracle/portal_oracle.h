@@ -113,10 +113,11 @@ int  orc_solve_from_J(const orc_grid *g, const orc_species *s, int node_lo, int 
 
 /* image pass */
 void orc_rotate_all_save(int N, double th_inc, double ph_az, const double *vel, const double *bvec,
-                         const double *XYZ, double *vel_new, double *bvec_new, double *XYZ_new);
+                         const double *XYZ, double *vel_new, double *bvec_new, double *XYZ_new,
+                         const double *XYZ_star, double *XYZ_star_new);
 int  orc_trace_out_raytrace(double x_co, double y_co, int N, const double *X_inc, const int *nbr_ptr,
-                            const int *nbr_idx, double dx, int n_depth_rt, int *cells, int *steps,
-                            int *node_count);
+                            const int *nbr_idx, double dx, int n_depth_rt, int star_mod, double R_star,
+                            const double *Xs, int *cells, int *steps, int *node_count, int *star_hit);
 void orc_comp_emcoefs(double r1_0, double r2_0, double r1_2, double r2_2, int j1, int j2, double th,
                       double nu0, double Aji, double *nsI, double *nsQ, double *esI, double *esQ);
 void orc_polarize_prop(double eta_I, double eta_Q, double eps_I, double eps_Q, double ds,
@@ -131,7 +132,8 @@ int  orc_image_pass(const orc_grid *g, const orc_species *s, int nlevels, const 
                     const double *nu0_all, const double *Aji_all, const double *kd_all,
                     double th_inc, double ph_az, const double *bvec_all, const double *pol_pops,
                     int nvel, const double *vel_out, int im_pix, double im_size, double dx,
-                    double alpha, int n_depth_rt, int pix_lo, int pix_hi,
+                    double alpha, int n_depth_rt, int pix_lo, int pix_hi, int node_pixels, int star_mod,
+                    double R_star, double T_star, const double *X_star,
                     double *out_I, double *out_Q, double *out_U, double *out_TAU,
                     double *dust_out, int nthreads);
 

@@ -94,12 +94,19 @@ class _Core:
         self._check(self.lib.portal_species_info(self.ctx, C.byref(nt), C.byref(cd)))
         return nt.value, cd.value
 
+    # star mode (main_sub.f90:56-79): None, or dict(nth, nph, R_star, T_star, X_star) used by
+    # every later pass; the directions then come from star_all instead of xth
+    star = None
+
     def _se_params(self, xth, nvel, dx, n_depth, extra_info=0, precision=L.PORTAL_FP64, gvec=None, bvec_all=None):
         g = self.grid
-        arrs = (_f64(xth), _f64(g.gvec if gvec is None else gvec), _f64(g.bvec_all if bvec_all is None else bvec_all))
+        st = self.star
+        arrs = (_f64(xth), _f64(g.gvec if gvec is None else gvec), _f64(g.bvec_all if bvec_all is None else bvec_all),
+                _f64(st["X_star"] if st else np.zeros(3)))
         self._keep.append(arrs)
         return L.SeParams(arrs[0].shape[1], _d(arrs[0]), nvel, dx, _d(arrs[1]), _d(arrs[2]), n_depth, extra_info,
-                          precision, 0)
+                          precision, 1 if st else 0, st["nth"] if st else 0, st["nph"] if st else 0,
+                          st["R_star"] if st else 0.0, st["T_star"] if st else 0.0, _d(arrs[3]))
 
     # -- passes ------------------------------------------------------------------
     def se_pass(self, xth, nvel, dx, n_depth=4096, extra_info=0, precision=L.PORTAL_FP64, gvec=None, bvec_all=None):
@@ -174,16 +181,25 @@ class _Core:
         return pp
 
     def image_pass(self, pol_pops, lines, th_inc, ph_az, nvel, vel_out, im_pix, im_size, dx, alpha=0.15,
-                   n_depth_rt=4096, precision=L.PORTAL_FP64, bvec_all=None, with_dust=False):
-        """One inclination.  lines = [(l1, l2, nu0, Aji, kd)].  Returns I, Q, U, TAU (n_rt x nlines x C)."""
+                   n_depth_rt=4096, precision=L.PORTAL_FP64, bvec_all=None, with_dust=False, star=None,
+                   node_pixels=False):
+        """One inclination.  lines = [(l1, l2, nu0, Aji, kd)].  Returns I, Q, U, TAU (n_rt x nlines x C).
+        star (default: the context's star mode) = dict(R_star, T_star, X_star) puts the star
+        behind the pixels it covers (raytrace.f90:103-114).  node_pixels: one pixel per bulk
+        node at its rotated (x, y) (the reference's pix = 0 mode); outputs are then
+        bulk_node_num x nlines x C."""
         g = self.grid
         nl = len(lines)
+        st = self.star if star is None else star
         arrs = (_i32([x[0] for x in lines]), _i32([x[1] for x in lines]), _f64([x[2] for x in lines]),
-                _f64([x[3] for x in lines]), _f64([x[4] for x in lines]), _f64(vel_out))
+                _f64([x[3] for x in lines]), _f64([x[4] for x in lines]), _f64(vel_out),
+                _f64(st["X_star"] if st else np.zeros(3)))
         self._keep.append(arrs)
         p = L.ImgParams(nl, _i(arrs[0]), _i(arrs[1]), _d(arrs[2]), _d(arrs[3]), _d(arrs[4]), th_inc, ph_az, nvel,
-                        _d(arrs[5]), im_pix, im_size, dx, alpha, n_depth_rt, precision)
-        n_rt = (2 * im_pix + 1) ** 2
+                        _d(arrs[5]), im_pix, im_size, dx, alpha, n_depth_rt, precision, 1 if st else 0,
+                        st["R_star"] if st else 0.0, st["T_star"] if st else 0.0, _d(arrs[6]),
+                        1 if node_pixels else 0)
+        n_rt = g.bulk_node_num if node_pixels else (2 * im_pix + 1) ** 2
         Cn = 2 * nvel + 1
         outs = [np.zeros((n_rt, nl, Cn), order="F") for _ in range(4)]
         dust = np.zeros((4, n_rt, nl, Cn)) if with_dust else None

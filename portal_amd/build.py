@@ -20,7 +20,8 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libportal_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-NO_CONTRACT = {"walk.hip", "rt_f64.hip", "se.hip", "image.hip", "lambda.hip", "ctx.cpp", "se_tables.cpp"}
+NO_CONTRACT = {"walk.hip", "rt_f64.hip", "se.hip", "image.hip", "lambda.hip", "ctx.cpp", "se_tables.cpp",
+               "star_dirs.cpp"}
 
 
 def _flags(name: str) -> list[str]:

@@ -33,6 +33,12 @@ struct PathSet {
     double dx = 0;
     int n_depth = 0;
     bool valid = false;
+    // star mode: per work item D rays (mu, phi, weight), rays in use, rays towards the star
+    int star = 0, nth = 0, nph = 0;
+    double R_star = 0;
+    std::vector<double> X_star;
+    double *d_xthn = nullptr;
+    int *d_nstar = nullptr;
 };
 
 // Statistical-equilibrium coefficient tables of one species (host-built, FP64).
@@ -129,7 +135,7 @@ struct Ctx {
 };
 
 // error flags (bits in d_flags[0])
-enum : int { FLAG_DEPTH = 1, FLAG_LU = 2, FLAG_TEMP = 4, FLAG_DEPTH_RT = 8 };
+enum : int { FLAG_DEPTH = 1, FLAG_LU = 2, FLAG_TEMP = 4, FLAG_DEPTH_RT = 8, FLAG_START = 16 };
 
 #define HIP_TRY(ctx, expr)                                                          \
     do {                                                                            \
@@ -156,8 +162,16 @@ struct WalkArgs {
     const long long *offsets;
     PathEntry *paths;
     int *flags;
+    // star mode (xthn null: off): per work item D rays (mu, phi, w) and the counts of rays
+    // in use and towards the star; rays towards the star end on its surface
+    const double *xthn;
+    const int *nvar, *nstar;
+    double Xs0, Xs1, Xs2, R_star;
 };
 hipError_t launch_walk(const WalkArgs &a, bool write, hipStream_t s);
+int build_star_dirs(const std::vector<int> &work, const double *XYZ, const double *bvec_all, const double *gvec,
+                    int nth, int nph, const double *X_star, double R_star, std::vector<double> &xthn,
+                    std::vector<int> &nvar, std::vector<int> &nstar);
 hipError_t launch_scan_counts(const int *counts, long long *offsets, long long n, hipStream_t s,
                               long long *total_host);
 
@@ -200,6 +214,11 @@ struct RtArgs {
     const double *KSD;      // N x R x {alpha_dust, B_nu(T_dust)} when dust is on (cooperative kernel only)
     const float *KSD32;
     int dust;
+    // star mode (cooperative kernel only): per work item D rays (mu, phi, w), rays towards
+    // the star (their background is I_star per line instead of I_in)
+    const double *xthn;
+    const int *nstar;
+    const double *I_star;
 };
 hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s);
 hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s);

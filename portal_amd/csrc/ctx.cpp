@@ -116,14 +116,41 @@ bool same(const std::vector<double> &v, const double *p, size_t n) {
 // Build (or reuse) the inward paths for this direction set and field geometry.
 int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
     PathSet &P = c.paths;
-    const int N = c.g.N, D = p->n_sphere;
-    if (P.valid && P.D == D && P.dx == p->dx && P.n_depth == p->n_depth && same(P.xth, p->xth, 3 * (size_t)D) &&
-        same(P.gvec, p->gvec, 3) && same(P.bvec, p->bvec_all, 3 * (size_t)N))
+    const bool star = p->star_mod == 1;
+    const int N = c.g.N, D = star ? p->nth * p->nph : p->n_sphere;
+    const bool same_star = !star ? P.star == 0
+                                 : (P.star == 1 && P.nth == p->nth && P.nph == p->nph && P.R_star == p->R_star &&
+                                    same(P.X_star, p->X_star, 3));
+    if (P.valid && same_star && P.D == D && P.dx == p->dx && P.n_depth == p->n_depth &&
+        (star || same(P.xth, p->xth, 3 * (size_t)D)) && same(P.gvec, p->gvec, 3) &&
+        same(P.bvec, p->bvec_all, 3 * (size_t)N))
         return PORTAL_OK;
     P.valid = false;
     const int nw = (int)c.g.work.size();
     double *d_xth = nullptr, *d_bvec = nullptr, *d_gvec = nullptr;
-    P.xth.assign(p->xth, p->xth + 3 * (size_t)D);
+    int *d_nvar = nullptr;
+    dfree(P.d_xthn);
+    dfree(P.d_nstar);
+    P.star = 0;
+    if (star) {
+        // trace_node's star branch: every node's own direction set (main_sub.f90:56-79)
+        std::vector<double> xthn;
+        std::vector<int> nvar, nstar;
+        const int bad = build_star_dirs(c.g.work, c.g.XYZ.data(), p->bvec_all, p->gvec, p->nth, p->nph, p->X_star,
+                                        p->R_star, xthn, nvar, nstar);
+        if (bad >= 0)
+            return fail(c, PORTAL_ERR_ARG, "star mode: the star_all rays of node " + std::to_string(c.g.work[bad] + 1) +
+                                               " do not fit in nth*nph (is it inside the star?)");
+        HIP_TRY(&c, upload(P.d_xthn, xthn));
+        HIP_TRY(&c, upload(P.d_nstar, nstar));
+        HIP_TRY(&c, upload(d_nvar, nvar));
+        P.star = 1;
+        P.nth = p->nth;
+        P.nph = p->nph;
+        P.R_star = p->R_star;
+        P.X_star.assign(p->X_star, p->X_star + 3);
+    }
+    P.xth.assign(p->xth, p->xth + 3 * (size_t)p->n_sphere);
     P.gvec.assign(p->gvec, p->gvec + 3);
     P.bvec.assign(p->bvec_all, p->bvec_all + 3 * (size_t)N);
     HIP_TRY(&c, upload(d_xth, P.xth));
@@ -152,6 +179,15 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
     a.n_depth = p->n_depth;
     a.counts = P.counts;
     a.flags = c.d_flags;
+    if (star) {
+        a.xthn = P.d_xthn;
+        a.nvar = d_nvar;
+        a.nstar = P.d_nstar;
+        a.Xs0 = p->X_star[0];
+        a.Xs1 = p->X_star[1];
+        a.Xs2 = p->X_star[2];
+        a.R_star = p->R_star;
+    }
     HIP_TRY(&c, launch_walk(a, false, c.stream));
     int fl[2] = {0, 0};
     HIP_TRY(&c, hipMemcpyAsync(fl, c.d_flags, sizeof(fl), hipMemcpyDeviceToHost, c.stream));
@@ -162,6 +198,7 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
         dfree(d_xth);
         dfree(d_gvec);
         dfree(d_bvec);
+        dfree(d_nvar);
         return fail(c, PORTAL_ERR_DEPTH, "an inward ray crosses more than n_depth cells (inverse_rt.f90:206-210)");
     }
     long long total = 0;
@@ -178,6 +215,7 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
     dfree(d_xth);
     dfree(d_gvec);
     dfree(d_bvec);
+    dfree(d_nvar);
     P.total = total;
     P.n_work = nw;
     P.D = D;
@@ -206,7 +244,8 @@ int check_rt_args(Ctx &c, const RtArgs &a, bool mixed) {
 // work list and the fastest is kept for every later pass with the same shape.
 int autotune_rt(Ctx &c, RtArgs ra, bool mixed) {
     const int pi = mixed ? 1 : 0;
-    const int key = (int)((c.paths.total % 1000003) ^ (ra.C * 131 + ra.R * 7 + ra.D)) * 2 + ra.dust;
+    const int key = (int)((c.paths.total % 1000003) ^ (ra.C * 131 + ra.R * 7 + ra.D)) * 4 + ra.dust +
+                    (ra.xthn ? 2 : 0);
     if (c.forced_variant[pi] >= 0) {
         c.tuned_variant[pi] = c.forced_variant[pi];
         c.tuned_waves[pi] = c.forced_waves[pi];
@@ -285,7 +324,7 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
     HIP_TRY(&c, hipMemsetAsync(c.d_J2, 0, sizeof(double) * c.g.N * s.r_num, c.stream));
     RtArgs ra{};
     ra.n_work = (int)c.g.work.size();
-    ra.D = p->n_sphere;
+    ra.D = c.paths.D;
     ra.R = s.r_num;
     ra.nvel = p->nvel;
     ra.C = 2 * p->nvel + 1;
@@ -305,13 +344,23 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
     ra.dust = dust ? 1 : 0;
     ra.KSD = c.d_KSD;
     ra.KSD32 = c.d_KSD32;
-    double *d_xth = nullptr;
+    double *d_xth = nullptr, *d_Istar = nullptr;
     std::vector<double> xv(p->xth, p->xth + 3 * (size_t)p->n_sphere);
     HIP_TRY(&c, upload(d_xth, xv));
     ra.xth = d_xth;
+    if (c.paths.star) {
+        // rays towards the star start with B_nu(T_star) (inverse_rt.f90:113-117)
+        std::vector<double> Is(s.r_num);
+        for (int k = 0; k < s.r_num; k++) Is[k] = black_body(p->T_star, s.nu_hz[k]);
+        HIP_TRY(&c, upload(d_Istar, Is));
+        ra.xthn = c.paths.d_xthn;
+        ra.nstar = c.paths.d_nstar;
+        ra.I_star = d_Istar;
+    }
     rc = check_rt_args(c, ra, mixed);
     if (rc) {
         dfree(d_xth);
+        dfree(d_Istar);
         return rc;
     }
     autotune_rt(c, ra, mixed);
@@ -321,6 +370,7 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
     hipError_t e = mixed ? launch_rt_mixed(ra, c.stream) : launch_rt_f64(ra, c.stream);
     const double t_rt = tr.stop();
     dfree(d_xth);
+    dfree(d_Istar);
     HIP_TRY(&c, e);
     if (st) {
         st->t_prep_ms += t_prep;
@@ -390,7 +440,9 @@ int run_se(Ctx &c, int extra_info, portal_stats *st) {
 
 int check_se_params(Ctx &c, const portal_se_params *p) {
     if (!p || !p->xth || !p->gvec || !p->bvec_all) return fail(c, PORTAL_ERR_ARG, "null parameter");
-    if (p->star_mod != 0) return fail(c, PORTAL_ERR_UNSUPPORTED, "star mode (star_mod=1) is not supported");
+    if (p->star_mod != 0 && p->star_mod != 1) return fail(c, PORTAL_ERR_ARG, "star_mod must be 0 or 1");
+    if (p->star_mod == 1 && (p->nth <= 0 || p->nph <= 0 || !p->X_star || !(p->R_star > 0) || !(p->T_star >= 0)))
+        return fail(c, PORTAL_ERR_ARG, "star mode needs nth, nph > 0, X_star, R_star > 0 and T_star >= 0");
     if (p->n_sphere <= 0 || p->nvel <= 0 || p->n_depth <= 0 || !(p->dx > 0))
         return fail(c, PORTAL_ERR_ARG, "n_sphere, nvel, n_depth and dx must be positive");
     if (p->precision != PORTAL_FP64 && p->precision != PORTAL_MIXED) return fail(c, PORTAL_ERR_ARG, "bad precision");
@@ -465,6 +517,8 @@ int portal_ctx_destroy(portal_ctx *x) {
     dfree(c.paths.entries);
     dfree(c.paths.offsets);
     dfree(c.paths.counts);
+    dfree(c.paths.d_xthn);
+    dfree(c.paths.d_nstar);
     dfree(c.d_levp);
     dfree(c.d_KS);
     dfree(c.d_KS32);
@@ -1086,7 +1140,15 @@ int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *p
         !(p->dx > 0))
         return fail(c, PORTAL_ERR_ARG, "bad image-pass arguments");
     if (p->precision != PORTAL_FP64 && p->precision != PORTAL_MIXED) return fail(c, PORTAL_ERR_ARG, "bad precision");
-    if (p->im_pix > 0) {
+    if (p->star_mod != 0 && p->star_mod != 1) return fail(c, PORTAL_ERR_ARG, "star_mod must be 0 or 1");
+    if (p->star_mod == 1 &&
+        (!p->X_star || !(p->R_star > 0.0) || !std::isfinite(p->R_star) || !(p->T_star >= 0.0) ||
+         !std::isfinite(p->T_star) || !std::isfinite(p->X_star[0]) || !std::isfinite(p->X_star[1]) ||
+         !std::isfinite(p->X_star[2])))
+        return fail(c, PORTAL_ERR_ARG, "star mode needs a finite X_star, R_star > 0 and T_star >= 0");
+    if (p->node_pixels != 0 && p->node_pixels != 1) return fail(c, PORTAL_ERR_ARG, "node_pixels must be 0 or 1");
+    if (p->node_pixels == 1 && c.g.N_b <= 0) return fail(c, PORTAL_ERR_ARG, "node_pixels needs bulk nodes");
+    if (!p->node_pixels && p->im_pix > 0) {
         // the corner pixel must start on the model sphere (raytrace.f90:98 takes sqrt(1-x^2-y^2))
         const double xc = p->im_pix * p->im_size / p->im_pix;
         if (!(1.0 - xc * xc - xc * xc >= 0.0))
@@ -1099,7 +1161,7 @@ int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *p
             return fail(c, PORTAL_ERR_ARG, "imaged line levels out of range");
     if (st) std::memset(st, 0, sizeof(*st));
     const int N = c.g.N, nl = p->nlevels, C = 2 * p->nvel + 1, n_side = 2 * p->im_pix + 1;
-    const int n_rt = n_side * n_side;
+    const int n_rt = p->node_pixels ? c.g.N_b : n_side * n_side;
     // R_tot = R_z(ph_az) R_x(th_inc) (miscellaneous.f90:870-880)
     auto rotm = [](const double nv[3], double th, double *R) {
         const double cth = std::cos(th), octh = 1.0 - cth, sth = std::sin(th);
@@ -1130,12 +1192,22 @@ int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *p
     for (int a = 0; a <= jmax; a++)
         for (int b = 0; b <= jmax; b++) w6[a * (jmax + 1) + b] = portal_debug_wig6jj(2, 2, 4, 2 * a, 2 * a, 2 * b);
     std::vector<int> l1(nl), l2(nl);
-    std::vector<double> Iin(nl);
+    std::vector<double> Iin(2 * (size_t)nl);  // B_nu(2.73 K), then B_nu(T_star)
     for (int j = 0; j < nl; j++) {
         l1[j] = p->l1[j] - 1;
         l2[j] = p->l2[j] - 1;
         Iin[j] = black_body(2.73, p->nu0_all[j]);
+        Iin[nl + j] = p->star_mod == 1 ? black_body(p->T_star, p->nu0_all[j]) : 0.0;
     }
+    // the star's position in the rotated frame, summed as rotate_all_save does
+    // (miscellaneous.f90:900-905)
+    double Xs[3] = {0.0, 0.0, 0.0};
+    if (p->star_mod == 1)
+        for (int j = 0; j < 3; j++) {
+            double sum = 0.0;
+            for (int k = 0; k < 3; k++) sum = sum + Rt[j + 3 * k] * p->X_star[k];
+            Xs[j] = sum;
+        }
     // device scratch, one allocation
     const size_t nrec = (size_t)n_rt * p->n_depth_rt;
     size_t off = 0;
@@ -1143,7 +1215,7 @@ int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *p
     const size_t o_R = take(9 * 8), o_xyz = take(24 * (size_t)N), o_vel = take(24 * (size_t)N),
                  o_bv = take(24 * (size_t)N), o_pol = take(16 * (size_t)N * L), o_jl = take(4 * (size_t)L),
                  o_l1 = take(4 * (size_t)nl), o_l2 = take(4 * (size_t)nl), o_nu = take(8 * (size_t)nl),
-                 o_A = take(8 * (size_t)nl), o_kd = take(8 * (size_t)nl), o_Iin = take(8 * (size_t)nl),
+                 o_A = take(8 * (size_t)nl), o_kd = take(8 * (size_t)nl), o_Iin = take(16 * (size_t)nl),
                  o_vo = take(8 * (size_t)C), o_w6 = take(8 * w6.size()), o_pos = take(32 * (size_t)N),
                  o_vz = take(8 * (size_t)N), o_geo = take(sizeof(CellGeo) * (size_t)N),
                  o_pc = take(32 * (size_t)N * nl), o_st = take(32 * (size_t)n_rt), o_sn = take(4 * (size_t)n_rt),
@@ -1174,7 +1246,7 @@ int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *p
     HIP_TRY(&c, H2D(o_nu, p->nu0_all, 8 * (size_t)nl));
     HIP_TRY(&c, H2D(o_A, p->Aji_all, 8 * (size_t)nl));
     HIP_TRY(&c, H2D(o_kd, p->kd_all, 8 * (size_t)nl));
-    HIP_TRY(&c, H2D(o_Iin, Iin.data(), 8 * (size_t)nl));
+    HIP_TRY(&c, H2D(o_Iin, Iin.data(), 16 * (size_t)nl));
     HIP_TRY(&c, H2D(o_vo, p->vel_out, 8 * (size_t)C));
     HIP_TRY(&c, H2D(o_w6, w6.data(), 8 * w6.size()));
     HIP_TRY(&c, H2D(o_b, c.g.b_all.data(), 8 * (size_t)N));
@@ -1233,10 +1305,19 @@ int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *p
     a.TAU = outb + 3 * cube;
     a.dust_out = dust ? outb + 4 * cube : nullptr;
     a.mixed = p->precision == PORTAL_MIXED ? 1 : 0;
+    a.star = p->star_mod;
+    a.Xs0 = Xs[0];
+    a.Xs1 = Xs[1];
+    a.Xs2 = Xs[2];
+    a.R_star = p->R_star;
+    a.Istar = a.Iin + nl;
+    a.node_pixels = p->node_pixels;
     float ms_g = 0, ms_r = 0;
     HIP_TRY(&c, launch_image(a, c.stream, &ms_g, &ms_r));
     int flags = 0;
     HIP_TRY(&c, hipMemcpy(&flags, c.d_flags, sizeof(int), hipMemcpyDeviceToHost));
+    if (flags & FLAG_START)
+        return fail(c, PORTAL_ERR_ARG, "a pixel's start point lies off the model sphere (raytrace.f90:98)");
     if (flags & FLAG_DEPTH_RT)
         return fail(c, PORTAL_ERR_DEPTH_RT, "an image ray crosses more than n_depth_rt cells (raytrace.f90:169-173)");
     HIP_TRY(&c, hipMemcpy(I, a.I, 8 * cube, hipMemcpyDeviceToHost));
@@ -1273,8 +1354,9 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         Ctx &c = xs[m]->c;
         int rc = check_se_params(c, &params[m]);
         if (rc) return fail(c0, rc, "model " + std::to_string(m) + ": " + c.err);
-        if (c.g.has_dust)
-            return fail(c0, PORTAL_ERR_UNSUPPORTED, "model " + std::to_string(m) + ": dust in a batch is not supported");
+        if (c.g.has_dust || params[m].star_mod)
+            return fail(c0, PORTAL_ERR_UNSUPPORTED,
+                        "model " + std::to_string(m) + ": dust or star mode in a batch is not supported");
         c.pol_chain = false;
     }
     if (total) std::memset(total, 0, sizeof(*total));

@@ -35,6 +35,13 @@ struct ImgArgs {
     // outputs: n_rt x nlines x C
     double *I, *Q, *U, *TAU, *dust_out;
     int mixed;                 // 1: Stokes transfer in FP32 (stokes_f32_kernel)
+    // star mode: pixels within R_star of the rotated star centre (Xs0, Xs1) in the sky
+    // plane start at z = Xs2 + R_star with incoming intensity Istar[line]
+    int star;
+    double Xs0, Xs1, Xs2, R_star;
+    const double *Istar;
+    // 1: one pixel per bulk node at its rotated (x, y) (n_rt = N_b); 0: the pixel grid
+    int node_pixels;
 };
 
 hipError_t launch_image(const ImgArgs &a, hipStream_t s, float *ms_geom, float *ms_rt);

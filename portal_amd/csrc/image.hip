@@ -5,8 +5,11 @@
 //   emcoef_kernel     comp_emcoefs (em_coefs.f90:1-113) per bulk node and imaged line,
 //                     plus the per-cell rotation angles of rot_stokes
 //                     (miscellaneous.f90:463-551) and of the dust branch
-//   start_kernel      nearest node to each pixel's entry point, the brute-force O(N)
-//                     search of trace_out_raytrace (raytrace.f90:126-134): node
+//   start_kernel      each pixel's entry point (raytrace.f90:98-118: the far side of the
+//                     unit sphere, or in star mode the star's near side for pixels on
+//                     its disc, flagged in start.w) and the nearest node to it, the
+//                     brute-force O(N) search of trace_out_raytrace
+//                     (raytrace.f90:126-134): node
 //                     positions are staged through LDS and every lane scans them for
 //                     its pixel, first minimal distance wins (strict <)
 //   march_kernel      the fixed-step march of trace_out_raytrace (raytrace.f90:141-167)
@@ -112,11 +115,25 @@ __global__ void __launch_bounds__(256) start_kernel(ImgArgs a) {
     const bool active = pix < a.n_rt;
     double sx = 0, sy = 0, sz = 0;
     if (active) {
-        const int ii = pix / a.n_side - a.im_pix, jj = pix % a.n_side - a.im_pix;
-        sx = ii * a.im_size / a.im_pix;
-        sy = jj * a.im_size / a.im_pix;
+        if (a.node_pixels) {  // para_finish.f90:296, the node's rotated position
+            const double4 p = a.pos_r[pix];
+            sx = p.x;
+            sy = p.y;
+        } else {  // para_finish.f90:369-376
+            const int ii = pix / a.n_side - a.im_pix, jj = pix % a.n_side - a.im_pix;
+            sx = ii * a.im_size / a.im_pix;
+            sy = jj * a.im_size / a.im_pix;
+        }
         sz = -1.0 * sqrt(1.0 - sx * sx - sy * sy);
-        a.start[pix] = make_double4(sx, sy, sz, 0.0);
+        double on_star = 0.0;
+        if (a.star) {
+            const double xc = sx - a.Xs0, yc = sy - a.Xs1;
+            if (sqrt(xc * xc + yc * yc) < a.R_star) {
+                sz = a.Xs2 + a.R_star;
+                on_star = 1.0;
+            }
+        }
+        a.start[pix] = make_double4(sx, sy, sz, on_star);
     }
     double rsave = 10.0;
     int node = -1;
@@ -153,7 +170,7 @@ __global__ void __launch_bounds__(256) march_kernel(ImgArgs a) {
     const double dx = a.dx;
     if (node < 0 || !(xp2 == xp2)) {  // no start node, or a start point off the sphere
         a.rec_n[pix] = -1;
-        atomicOr(a.flags, FLAG_DEPTH_RT);
+        atomicOr(a.flags, FLAG_START);
         return;
     }
     // the march crosses the unit sphere in at most 2/dx steps; stop well after that
@@ -240,7 +257,8 @@ __global__ void __launch_bounds__(256) stokes_kernel(ImgArgs a) {
     const int *steps = a.rec_step + (size_t)pix * a.n_depth_rt;
     const double nu0 = a.nu0[j];
     const double cnu = (double)2.99792E8f / nu0;
-    const double I_in = a.Iin[j];
+    // B_nu(T_star) behind a pixel on the star's disc, else the CMB (raytrace.f90:98-114)
+    const double I_in = a.start[pix].w != 0.0 ? a.Istar[j] : a.Iin[j];
     const double vel_obs = a.vel_out[k];
     const double spi = sqrt(acos(-1.0));
     const double kd = a.kd[j], alpha = a.alpha;
@@ -339,7 +357,8 @@ __global__ void __launch_bounds__(256) stokes_f32_kernel(ImgArgs a) {
     const int *cells = a.rec_cell + (size_t)pix * a.n_depth_rt;
     const int *steps = a.rec_step + (size_t)pix * a.n_depth_rt;
     const double nu0 = a.nu0[j];
-    const double I_in = a.Iin[j];
+    const double I_cmb = a.Iin[j];
+    const double I_in = a.start[pix].w != 0.0 ? a.Istar[j] : I_cmb;
     // eta scale c/nu (float literal as in raytrace.f90), eps additionally / I_in
     const float cnu = (float)((double)2.99792E8f / nu0);
     const float cnu_e = (float)((double)2.99792E8f / nu0 / I_in);
@@ -392,12 +411,13 @@ __global__ void __launch_bounds__(256) stokes_f32_kernel(ImgArgs a) {
         U_in = fmaf(U_out, c2a, Q_out * s2a);
     }
     const size_t o = (size_t)pix + (size_t)a.n_rt * ((size_t)j + (size_t)a.nlines * k);
-    a.I[o] = (double)(I - 1.0f) * I_in;
+    // I_out - B_nu(2.73 K) with I_out = I I_in; the second term is 0 off the star
+    a.I[o] = (double)(I - 1.0f) * I_in + (I_in - I_cmb);
     a.Q[o] = (double)Q_in * I_in;
     a.U[o] = (double)U_in * I_in;
     a.TAU[o] = (double)tau;
     if (a.dust_out) {
-        I_dust = I_dust - I_in;
+        I_dust = I_dust - I_cmb;
         double Q_dust, U_dust, tdust;
         if (I_dust == 0.0) {
             Q_dust = 0.0;

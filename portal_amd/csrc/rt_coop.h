@@ -102,7 +102,9 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     const R2 k0 = A::ks(a, (size_t)node * R + lo);  // the node's own cell, no skip
     R2 d0;
     if (DUST) d0 = A::ksd(a, (size_t)node * R + lo);
-    const Real iin = (Real)a.I_in[lo];
+    const Real iin_bg = (Real)a.I_in[lo];
+    const Real iin_star = a.xthn ? (Real)a.I_star[lo] : iin_bg;
+    const int n_star = a.xthn ? a.nstar[bid] : 0;
 #pragma unroll
     for (int q = 0; q < SLOTS; q++) {
         on[q] = lane_on && cb + q < C;
@@ -118,7 +120,10 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     for (int r = 0; r < D; r++) {
         char *cur = smem + (r & 1) * bb;
         if (r + 1 < D) coop_stage<Real, DUST>(a, bid, r + 1, cap, smem + ((r + 1) & 1) * bb);
-        const Real mu = (Real)a.xth[3 * r], w = (Real)a.xth[3 * r + 2];
+        // star mode: the node's own directions; rays towards the star start with B_nu(T_star)
+        const double *xr = a.xthn ? a.xthn + 3 * ((size_t)bid * D + r) : a.xth + 3 * r;
+        const Real mu = (Real)xr[0], w = (Real)xr[2];
+        const Real iin = r < n_star ? iin_star : iin_bg;
         const Real wc = w * (((Real)3 * (mu * mu) - (Real)1) / (Real)2);
         const long long off = a.offsets[(long long)bid * D + r];
         const int nc = (int)(a.offsets[(long long)bid * D + r + 1] - off);

@@ -57,7 +57,24 @@ __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
     if (id >= (long long)a.n_work * a.D) return;
     const int w = (int)(id / a.D), r = (int)(id % a.D);
     const int node = a.work[w];
-    const double mu = a.xth[3 * r], ph = a.xth[3 * r + 1];
+    bool star_ray = false;
+    if (a.xthn && r >= a.nvar[w]) {
+        // star mode: a padding ray (weight 0) gets the node's own cell with zero length
+        if (WRITE) {
+            PathEntry e;
+            e.cell = node;
+            e.len = 0;
+            e.vray = 0.0;
+            a.paths[a.offsets[id]] = e;
+        } else {
+            a.counts[id] = 1;
+            atomicMax(a.flags + 1, 1);
+        }
+        return;
+    }
+    if (a.xthn) star_ray = r < a.nstar[w];
+    const double *xr = a.xthn ? a.xthn + 3 * (size_t)id : a.xth + 3 * r;
+    const double mu = xr[0], ph = xr[1];
     const double b[3] = {a.bvec[3 * (size_t)node], a.bvec[3 * (size_t)node + 1], a.bvec[3 * (size_t)node + 2]};
     const double g[3] = {a.gvec[0], a.gvec[1], a.gvec[2]};
     double dir[3];
@@ -101,6 +118,23 @@ __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
             if (new_x > delta_x || new_x <= thr) continue;
             delta_x = new_x;
             next = nb;
+        }
+        if (star_ray) {
+            // check_neigh_fast's star test (miscellaneous.f90:1056-1069)
+            const double s0 = xp0 - a.Xs0, s1 = xp1 - a.Xs1, s2 = xp2 - a.Xs2;
+            const double sn = sqrt(s0 * s0 + s1 * s1 + s2 * s2);
+            const double sb = 2.0 * (s0 * dir[0] + s1 * dir[1] + s2 * dir[2]);
+            const double sc = sn * sn - a.R_star * a.R_star;
+            const double sdis = sb * sb - 4.0 * qa * sc;
+            if (sdis > 0) {
+                double x1 = (-sb + sqrt(sdis)) / (2.0 * qa);
+                const double x2 = (-sb - sqrt(sdis)) / (2.0 * qa);
+                if (x2 > 0) x1 = x2;
+                if (x1 < delta_x && x1 > 0.0) {
+                    delta_x = x1;
+                    next = -1;
+                }
+            }
         }
         xp0 = xp0 + delta_x * dir[0];
         xp1 = xp1 + delta_x * dir[1];

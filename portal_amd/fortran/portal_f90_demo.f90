@@ -1,12 +1,16 @@
 ! Fortran caller of the C ABI: the node loop of an SE pass replaced by one call,
 ! as a PORTAL driver would make it.  Reads a model dumped by the Python tests
 ! (stream file: sizes, then the driver's arrays in the reference's layouts),
-! runs portal_se_pass and writes pol_pops (node_num x 2L) to a stream file.
+! runs portal_se_pass and writes pol_pops (node_num x 2L) to a stream file and,
+! given a third argument, to a text file with the reference's own statements
+! (main_para.f90:271-275: one list-directed record per node), the file that the
+! image-pass driver reads back (para_finish.f90:222-228).
 program portal_f90_demo
   use iso_c_binding
   use portal_hip
   implicit none
-  character(len=512) :: fin, fout
+  character(len=512) :: fin, fout, fdat
+  integer :: node
   integer(c_int) :: node_num, bulk_node_num, e_num, r_num, c_num, c_temp, c_par, kmax, maxne, n_sphere, nvel, rc
   real(c_double) :: tx, dx
   real(c_double), allocatable, target :: XYZ(:,:), vel_all(:,:), temp_all(:,:), b_all(:), h2(:), ab(:), dust(:)
@@ -20,6 +24,7 @@ program portal_f90_demo
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
+  call get_command_argument(3, fdat)
   open(10, file=trim(fin), access='stream', form='unformatted', status='old')
   read(10) node_num, bulk_node_num, e_num, r_num, c_num, c_temp, c_par, kmax, maxne, n_sphere, nvel
   read(10) tx, dx
@@ -50,6 +55,11 @@ program portal_f90_demo
   p%extra_info = 0
   p%precision = PORTAL_FP64
   p%star_mod = 0
+  p%nth = 0
+  p%nph = 0
+  p%R_star = 0.d0
+  p%T_star = 0.d0
+  p%X_star = c_null_ptr
   pol_pops = 0.d0
   if (rc == PORTAL_OK) rc = portal_se_pass(ctx, p, pol_pops, pol_rates, JJ_20, st)
   if (rc /= PORTAL_OK) then
@@ -61,4 +71,11 @@ program portal_f90_demo
   open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(11) pol_pops
   close(11)
+  if (len_trim(fdat) > 0) then
+    open(unit=25, file=trim(fdat), recl=8824)
+    do node = 1, node_num
+      write(25,*) pol_pops(node,:)
+    end do
+    close(25)
+  endif
 end program

@@ -22,6 +22,9 @@ module portal_hip
     integer(c_int) :: extra_info
     integer(c_int) :: precision
     integer(c_int) :: star_mod
+    integer(c_int) :: nth, nph
+    real(c_double) :: R_star, T_star
+    type(c_ptr) :: X_star
   end type
 
   type, bind(C) :: portal_iter_params
@@ -43,6 +46,10 @@ module portal_hip
     integer(c_int) :: im_pix
     real(c_double) :: im_size, dx, alpha
     integer(c_int) :: n_depth_rt, precision
+    integer(c_int) :: star_mod
+    real(c_double) :: R_star, T_star
+    type(c_ptr) :: X_star
+    integer(c_int) :: node_pixels
   end type
 
   type, bind(C) :: portal_stats

@@ -35,7 +35,8 @@ EXPORTS = ["portal_ctx_create", "portal_ctx_destroy", "portal_last_error", "port
 class SeParams(C.Structure):
     _fields_ = [("n_sphere", C.c_int), ("xth", dp), ("nvel", C.c_int), ("dx", C.c_double), ("gvec", dp),
                 ("bvec_all", dp), ("n_depth", C.c_int), ("extra_info", C.c_int), ("precision", C.c_int),
-                ("star_mod", C.c_int)]
+                ("star_mod", C.c_int), ("nth", C.c_int), ("nph", C.c_int), ("R_star", C.c_double),
+                ("T_star", C.c_double), ("X_star", dp)]
 
 
 class IterParams(C.Structure):
@@ -48,7 +49,8 @@ class ImgParams(C.Structure):
     _fields_ = [("nlevels", C.c_int), ("l1", ip), ("l2", ip), ("nu0_all", dp), ("Aji_all", dp), ("kd_all", dp),
                 ("th_inc", C.c_double), ("ph_az", C.c_double), ("nvel", C.c_int), ("vel_out", dp),
                 ("im_pix", C.c_int), ("im_size", C.c_double), ("dx", C.c_double), ("alpha", C.c_double),
-                ("n_depth_rt", C.c_int), ("precision", C.c_int)]
+                ("n_depth_rt", C.c_int), ("precision", C.c_int), ("star_mod", C.c_int), ("R_star", C.c_double),
+                ("T_star", C.c_double), ("X_star", dp), ("node_pixels", C.c_int)]
 
 
 class Stats(C.Structure):

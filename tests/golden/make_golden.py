@@ -258,6 +258,8 @@ def main():
         json.dump({"w3j": w3, "w6j": w6, "coupling": coup, "t_spont": tsp}, f)
     print("fixtures written to", OUT)
     make_star(R)
+    make_star_image(R)
+    make_node_pixels(R)
 
 
 def make_star(R):
@@ -314,9 +316,74 @@ def make_star(R):
     print("star fixtures written to", OUT)
 
 
+def make_star_image(R):
+    """Star mode of the image pass: rotate_all_save's rotated star position
+    (miscellaneous.f90:900-905) and trace_out_raytrace with star_mod = 1
+    (raytrace.f90:103-114: a pixel within R_star of the star's centre starts at
+    z = Xs_z + R_star with I_in = B(T_star)) for pixels inside, on the edge of and
+    outside the star's disc, on the 2,500-node cloud of the star fixtures."""
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 2500, seed=41)
+    dne = g.del_ne()
+    X_star, Rs, Ts = np.array([0.12, -0.07, 0.2]), 0.15, 6000.0
+    th_inc, ph_az = 0.7, 0.4
+    _, _, xn, xs = R.rotate_all_save(g, th_inc, ph_az, g.bvec_all, X_star=X_star)
+    im_pix, im_size = 64, 0.7
+    nu0 = np.array([sp.e_levs_hz[1] - sp.e_levs_hz[0], sp.e_levs_hz[3] - sp.e_levs_hz[2]])
+    pix, traces, tofs, iin = [], [], [0], []
+    for i in range(-im_pix, im_pix + 1, 3):
+        for j in range(-im_pix, im_pix + 1, 3):
+            x, y = i * im_size / im_pix, j * im_size / im_pix
+            r = np.hypot(x - xs[0], y - xs[1])
+            if abs(r - Rs) > 0.05 and (i * 7 + j * 3) % 11:
+                continue  # keep the disc's edge and a sparse sample elsewhere
+            dl, I_in = R.trace_out_raytrace(x, y, g, xn, dne, cloud.DX_IMAGE, 4096, nu0,
+                                            star=dict(R_star=Rs, T_star=Ts, Xs=xs))
+            pix.append((i, j))
+            traces.append(dl)
+            tofs.append(tofs[-1] + dl.shape[1])
+            iin.append(I_in)
+    bb = np.array([[R.black_body(T, nu) for nu in nu0] for T in (2.73, Ts)])
+    np.savez_compressed(os.path.join(OUT, "star_image.npz"), X_star=X_star, R_star=Rs, T_star=Ts, th_inc=th_inc,
+                        ph_az=ph_az, xs_rot=xs, im_pix=im_pix, im_size=im_size, nu0=nu0, pix=np.array(pix),
+                        trace=np.concatenate(traces, axis=1), trace_ofs=np.array(tofs), I_in=np.array(iin), bb=bb,
+                        digest=cloud_digest(g))
+    print("star image fixtures written to", OUT, len(pix), "pixels")
+
+
+def make_node_pixels(R):
+    """The reference's pix = 0 image mode (para_finish.f90:245-330): trace_out_raytrace
+    started at the rotated (x, y) of bulk nodes, on the 2,500-node cloud of the star
+    fixtures, with and without a star."""
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 2500, seed=41)
+    dne = g.del_ne()
+    X_star, Rs, Ts = np.array([0.12, -0.07, 0.2]), 0.15, 6000.0
+    th_inc, ph_az = 1.1, 2.3
+    _, _, xn, xs = R.rotate_all_save(g, th_inc, ph_az, g.bvec_all, X_star=X_star)
+    nu0 = np.array([sp.e_levs_hz[1] - sp.e_levs_hz[0]])
+    rng = np.random.default_rng(5)
+    nodes = np.sort(rng.choice(g.bulk_node_num - 1, 40, replace=False)) + 1
+    traces, tofs, iin = [], [0], []
+    for k, n in enumerate(nodes):
+        star = dict(R_star=Rs, T_star=Ts, Xs=xs) if k % 2 else None
+        dl, I_in = R.trace_out_raytrace(xn[0, n - 1], xn[1, n - 1], g, xn, dne, cloud.DX_IMAGE, 4096, nu0, star=star)
+        traces.append(dl)
+        tofs.append(tofs[-1] + dl.shape[1])
+        iin.append(I_in[0])
+    np.savez_compressed(os.path.join(OUT, "node_pixels.npz"), X_star=X_star, R_star=Rs, T_star=Ts, th_inc=th_inc,
+                        ph_az=ph_az, nodes=nodes, xy=xn[:2, nodes - 1], trace=np.concatenate(traces, axis=1),
+                        trace_ofs=np.array(tofs), I_in=np.array(iin), digest=cloud_digest(g))
+    print("node-pixel fixtures written to", OUT)
+
+
 if __name__ == "__main__":
     import sys
     if sys.argv[1:] == ["star"]:
         make_star(Reference())
+    elif sys.argv[1:] == ["star_image"]:
+        make_star_image(Reference())
+    elif sys.argv[1:] == ["node_pixels"]:
+        make_node_pixels(Reference())
     else:
         main()

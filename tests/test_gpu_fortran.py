@@ -42,9 +42,9 @@ def test_fortran_driver_matches_python_interface(tmp_path):
     sp = cloud.co_like_rotor(4)
     g = cloud.make_cloud(sp, 1200, seed=21)
     xth = cloud.legendre_all(4, 8)
-    fin, fout = str(tmp_path / "model.bin"), str(tmp_path / "pops.bin")
+    fin, fout, fdat = str(tmp_path / "model.bin"), str(tmp_path / "pops.bin"), str(tmp_path / "pol_pops.dat")
     dump(fin, g, sp, xth, 16)
-    r = subprocess.run([DEMO, fin, fout], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DEMO, fin, fout, fdat], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     pf = np.fromfile(fout, dtype=np.float64).reshape((g.node_num, 2 * sp.e_num), order="F")
@@ -55,3 +55,9 @@ def test_fortran_driver_matches_python_interface(tmp_path):
     pp, _ = pt.se_pass(xth, 16, cloud.DX_INWARD)
     pt.close()
     np.testing.assert_array_equal(pf, pp)
+    # pol_pops.dat as main_para.f90:271-275 writes it, read back as para_finish.f90:226-228
+    # does (one list-directed record of 2L doubles per node): the values survive exactly
+    rows = [ln.split() for ln in open(fdat) if ln.strip()]
+    assert len(rows) == g.node_num and all(len(r) == 2 * sp.e_num for r in rows)
+    back = np.array([[float(v.replace("D", "E")) for v in r] for r in rows])
+    np.testing.assert_array_equal(back, pp)

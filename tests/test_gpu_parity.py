@@ -307,6 +307,125 @@ def test_inward_pass_with_dust(oracle, variant, waves):
     pt.close()
 
 
+def test_star_mode(oracle):
+    # star mode (main_sub.f90:56-79): per-node star_all directions, star rays ending on the
+    # star with B_nu(T_star) behind them; nodes inside the star are not solved
+    from portal_amd.api import batch_lambda_iterate
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 2500, seed=42)
+    star = dict(nth=6, nph=11, R_star=0.15, T_star=6000.0, X_star=np.zeros(3))
+    r = np.linalg.norm(g.XYZ, axis=0)
+    inside = np.where(r[: g.bulk_node_num] < 1.3 * star["R_star"])[0]
+    g.ab_all[inside] = 0.0
+    xth = cloud.legendre_all(6, 11)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    pt.star = star
+    idx = solved(g)
+    rng = np.random.default_rng(8)
+    sub = np.sort(np.concatenate([idx[np.argsort(r[idx])[:8]], rng.choice(idx, 24, replace=False)])) + 1
+    sub = np.unique(sub)
+    J0o, J2o = oracle.inverse_nodes(g, sp, xth, 32, cloud.DX_INWARD, sub, star=star, nthreads=NTH)
+    J0, J2, st = pt.inverse_rt(xth, 32, cloud.DX_INWARD)
+    assert st["rt_variant"] == 2
+    np.testing.assert_allclose(J0[:, sub - 1], J0o, rtol=1e-10)
+    np.testing.assert_allclose(J2[:, sub - 1], J2o, rtol=0, atol=1e-10 * np.abs(J0o).max(axis=1).min())
+    J0m, _, _ = pt.inverse_rt(xth, 32, cloud.DX_INWARD, precision=PORTAL_MIXED)
+    e0 = np.abs(J0m[:, idx] / J0[:, idx] - 1).max()
+    print(f"star: mixed max rel J0 err {e0:.3e}")
+    assert e0 < 2e-5
+    # the star is seen: without it J differs
+    pt.star = None
+    J0n, _, _ = pt.inverse_rt(xth, 32, cloud.DX_INWARD)
+    assert np.abs(J0n[:, idx] / J0[:, idx] - 1).max() > 1e-3
+    pt.star = star
+    pp, st = pt.se_pass(xth, 32, cloud.DX_INWARD)
+    w = np.sqrt(2 * sp.j_lev + 1.0)
+    np.testing.assert_allclose(pp[idx][:, 0::2] @ w, 1.0, atol=1e-12)
+    with pytest.raises(PortalError) as e:
+        batch_lambda_iterate([pt], [xth], [32], cloud.DX_INWARD, 2)
+    assert e.value.code == -9  # PORTAL_ERR_UNSUPPORTED
+    pt.close()
+
+
+def test_image_pass_star(oracle):
+    # star mode of the image pass (raytrace.f90:103-114): pixels on the rotated star's disc
+    # start on its near side with B_nu(T_star); both precisions against the oracle
+    from portal_amd import driver
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 2500, seed=41)
+    N, nb = g.node_num, g.bulk_node_num - 1
+    pp = np.zeros((N, 2 * sp.e_num), order="F")
+    rng = np.random.default_rng(8)
+    pp[:nb, 0::2] = g.lev_p[:, :nb].T
+    pp[:nb, 3::2] = rng.uniform(-1e-3, 1e-3, (nb, sp.e_num - 1))
+    pp = driver.rescale_pol_pops(pp, g.lev_p, sp.lev_g)
+    star = dict(R_star=0.15, T_star=6000.0, X_star=np.array([0.12, -0.07, 0.2]))
+    lines = driver.image_lines(sp, [1, 2], [2, 3])
+    vel_out = driver.channel_velocities(8, 200.0)
+    args = (pp, lines, 0.7, 0.4, 8, vel_out, 20, 0.7, cloud.DX_IMAGE)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    (I, Q, U, T), _ = pt.image_pass(*args, n_depth_rt=2048, star=star)
+    (Im, Qm, Um, Tm), _ = pt.image_pass(*args, n_depth_rt=2048, star=star, precision=PORTAL_MIXED)
+    (In, _, _, _), _ = pt.image_pass(*args, n_depth_rt=2048)
+    pt.close()
+    Io, Qo, Uo, To = oracle.image_pass(g, sp, pp, lines, 0.7, 0.4, 8, vel_out, 20, 0.7, cloud.DX_IMAGE,
+                                       n_depth_rt=2048, nthreads=NTH, star=star)
+    on = (I != In).any(axis=(1, 2))
+    assert 20 < on.sum() < len(on) // 4  # pixels on the star's disc
+    Imax = np.abs(Io).max()
+    np.testing.assert_allclose(I, Io, rtol=1e-10, atol=1e-10 * Imax)
+    np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * Imax)
+    np.testing.assert_allclose(U, Uo, rtol=0, atol=1e-10 * Imax)
+    np.testing.assert_allclose(T, To, rtol=1e-10, atol=1e-12)
+    # FP32 transfer: the star's light in units of its own B_nu, the line on the CMB's
+    s_on, s_off = np.abs(I[on]).max(), np.abs(I[~on]).max()
+    e_on = max(np.abs(Im - I)[on].max(), np.abs(Qm - Q)[on].max(), np.abs(Um - U)[on].max()) / s_on
+    e_off = max(np.abs(Im - I)[~on].max(), np.abs(Qm - Q)[~on].max(), np.abs(Um - U)[~on].max()) / s_off
+    print(f"star image: mixed max err {e_on:.2e} on the disc, {e_off:.2e} off it")
+    assert e_on < 1e-5 and e_off < 1e-5
+
+
+def test_image_pass_node_pixels(oracle):
+    # the reference's pix = 0 mode (para_finish.f90:245-330): one pixel per bulk node at its
+    # rotated (x, y), with and without the star, both precisions against the oracle
+    from portal_amd import driver
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 2500, seed=41)
+    N, nb = g.node_num, g.bulk_node_num - 1
+    pp = np.zeros((N, 2 * sp.e_num), order="F")
+    rng = np.random.default_rng(9)
+    pp[:nb, 0::2] = g.lev_p[:, :nb].T
+    pp[:nb, 3::2] = rng.uniform(-1e-3, 1e-3, (nb, sp.e_num - 1))
+    pp = driver.rescale_pol_pops(pp, g.lev_p, sp.lev_g)
+    lines = driver.image_lines(sp, [1, 2], [2, 3])
+    vel_out = driver.channel_velocities(6, 250.0)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    for star in (None, dict(R_star=0.15, T_star=6000.0, X_star=np.array([0.12, -0.07, 0.2]))):
+        args = (pp, lines, 1.1, 2.3, 6, vel_out, 0, 0.0, cloud.DX_IMAGE)
+        (I, Q, U, T), _ = pt.image_pass(*args, n_depth_rt=2048, star=star, node_pixels=True)
+        (Im, Qm, Um, _), _ = pt.image_pass(*args, n_depth_rt=2048, star=star, node_pixels=True,
+                                           precision=PORTAL_MIXED)
+        Io, Qo, Uo, To = oracle.image_pass(g, sp, pp, lines, 1.1, 2.3, 6, vel_out, 0, 0.0, cloud.DX_IMAGE,
+                                           n_depth_rt=2048, nthreads=NTH, star=star, node_pixels=True)
+        assert I.shape == (g.bulk_node_num, 2, 13)
+        Imax = np.abs(Io).max()
+        np.testing.assert_allclose(I, Io, rtol=1e-10, atol=1e-10 * Imax)
+        np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * Imax)
+        np.testing.assert_allclose(U, Uo, rtol=0, atol=1e-10 * Imax)
+        np.testing.assert_allclose(T, To, rtol=1e-10, atol=1e-12)
+        e = max(np.abs(Im - I).max(), np.abs(Qm - Q).max(), np.abs(Um - U).max()) / Imax
+        print(f"node pixels (star {star is not None}): mixed max err {e:.2e} of max |I|")
+        assert e < 1e-5
+    pt.close()
+
+
 def test_image_pass_dust_branch(cfg1, oracle):
     # dust present in the image pass: both branches agree
     from portal_amd import driver

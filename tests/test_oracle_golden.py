@@ -159,7 +159,7 @@ def test_image_pieces_match_reference(oracle, cfg1, golden_dir):
     N = g.node_num
     vn, bn, xn = (np.zeros((3, N), order="F") for _ in range(3))
     oracle.lib.orc_rotate_all_save(N, C.c_double(float(z["th_inc"])), C.c_double(float(z["ph_az"])), _d(_f64(g.vel_all)),
-                                   _d(_f64(g.bvec_all)), _d(_f64(g.XYZ)), _d(vn), _d(bn), _d(xn))
+                                   _d(_f64(g.bvec_all)), _d(_f64(g.XYZ)), _d(vn), _d(bn), _d(xn), None, None)
     sel = z["sel"]
     np.testing.assert_allclose(vn[:, sel], z["vel_rot"], rtol=1e-15, atol=1e-12)
     np.testing.assert_allclose(bn[:, sel], z["b_rot"], rtol=0, atol=1e-15)
@@ -173,8 +173,8 @@ def test_image_pieces_match_reference(oracle, cfg1, golden_dir):
         rc = oracle.lib.orc_trace_out_raytrace(C.c_double(i * im_size / im_pix), C.c_double(j * im_size / im_pix), N,
                                                _d(xn), ptr.ctypes.data_as(C.POINTER(C.c_int)),
                                                idx.ctypes.data_as(C.POINTER(C.c_int)), C.c_double(cloud.DX_IMAGE), 4096,
-                                               cells.ctypes.data_as(C.POINTER(C.c_int)),
-                                               steps.ctypes.data_as(C.POINTER(C.c_int)), C.byref(nc))
+                                               0, C.c_double(0.0), None, cells.ctypes.data_as(C.POINTER(C.c_int)),
+                                               steps.ctypes.data_as(C.POINTER(C.c_int)), C.byref(nc), None)
         assert rc == 0
         ref = z["trace"][:, z["trace_ofs"][k]:z["trace_ofs"][k + 1]]
         assert nc.value == ref.shape[1]
@@ -247,3 +247,124 @@ def test_star_mode_matches_reference(oracle, golden_dir):
         J0, J2 = oracle.inverse_nodes(g, sp, xth, 32, cloud.DX_INWARD, nodes, star=star)
         np.testing.assert_allclose(J0.T, rows[:, 4:7], rtol=1e-13, atol=0)
         np.testing.assert_allclose(J2.T, rows[:, 7:10], rtol=0, atol=1e-13 * np.abs(rows[:, 4:7]).max())
+
+
+def test_star_image_matches_reference(oracle, golden_dir):
+    # the image pass in star mode: rotate_all_save's rotated star (miscellaneous.f90:900-905)
+    # and trace_out_raytrace's star start (raytrace.f90:103-114) against the compiled
+    # reference, for pixels on, near and off the star's disc
+    from portal_amd import driver
+    from tests.golden.make_golden import cloud_digest
+    z = np.load(os.path.join(golden_dir, "star_image.npz"))
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 2500, seed=41)
+    assert cloud_digest(g) == str(z["digest"])
+    N = g.node_num
+    vn, bn, xn = (np.zeros((3, N), order="F") for _ in range(3))
+    xs = np.zeros(3)
+    oracle.lib.orc_rotate_all_save(N, C.c_double(float(z["th_inc"])), C.c_double(float(z["ph_az"])),
+                                   _d(_f64(g.vel_all)), _d(_f64(g.bvec_all)), _d(_f64(g.XYZ)), _d(vn), _d(bn), _d(xn),
+                                   _d(_f64(z["X_star"])), _d(xs))
+    np.testing.assert_allclose(xs, z["xs_rot"], rtol=0, atol=1e-16)
+    ptr, idx = _f64(g.nbr_ptr).astype(np.int32), np.asarray(g.nbr_idx, dtype=np.int32)
+    im_pix, im_size, Rs = int(z["im_pix"]), float(z["im_size"]), float(z["R_star"])
+    hits = 0
+    for k, (i, j) in enumerate(z["pix"]):
+        cells = np.zeros(4096, dtype=np.int32)
+        steps = np.zeros(4096, dtype=np.int32)
+        nc, hit = C.c_int(), C.c_int()
+        rc = oracle.lib.orc_trace_out_raytrace(C.c_double(i * im_size / im_pix), C.c_double(j * im_size / im_pix), N,
+                                               _d(xn), ptr.ctypes.data_as(C.POINTER(C.c_int)),
+                                               idx.ctypes.data_as(C.POINTER(C.c_int)), C.c_double(cloud.DX_IMAGE), 4096,
+                                               1, C.c_double(Rs), _d(xs), cells.ctypes.data_as(C.POINTER(C.c_int)),
+                                               steps.ctypes.data_as(C.POINTER(C.c_int)), C.byref(nc), C.byref(hit))
+        assert rc == 0
+        ref = z["trace"][:, z["trace_ofs"][k]:z["trace_ofs"][k + 1]]
+        assert nc.value == ref.shape[1]
+        np.testing.assert_array_equal(cells[:nc.value], ref[0])
+        np.testing.assert_array_equal(steps[:nc.value], ref[1])
+        np.testing.assert_array_equal(z["I_in"][k], z["bb"][hit.value])  # B(T_star) exactly where the star starts
+        hits += hit.value
+    assert 0 < hits < len(z["pix"])
+    # the whole pass: with no line emission a star pixel reads B(T_star) - B(2.73) and every
+    # other pixel 0; with populations, pixels off the disc are those of the pass without star
+    star = dict(R_star=Rs, T_star=float(z["T_star"]), X_star=z["X_star"])
+    lines = driver.image_lines(sp, [1, 2], [2, 3])
+    vel_out = driver.channel_velocities(3, 300.0)
+    th, ph, ip = float(z["th_inc"]), float(z["ph_az"]), 12
+    zero = np.zeros((N, 2 * sp.e_num), order="F")
+    I0, Q0, U0, T0 = oracle.image_pass(g, sp, zero, lines, th, ph, 3, vel_out, ip, im_size, cloud.DX_IMAGE, star=star)
+    n_side = 2 * ip + 1
+    ii, jj = np.divmod(np.arange(n_side * n_side), n_side)
+    x, y = (ii - ip) * im_size / ip, (jj - ip) * im_size / ip
+    on = np.hypot(x - xs[0], y - xs[1]) < Rs
+    assert 10 < on.sum() < on.size // 4
+    nu = np.array([ln[2] for ln in lines])
+    bb = np.array([[orc_black_body(oracle, T, v) for v in nu] for T in (2.73, star["T_star"])])
+    np.testing.assert_array_equal(I0[~on], 0.0)
+    np.testing.assert_allclose(I0[on], np.broadcast_to((bb[1] - bb[0])[None, :, None], I0[on].shape), rtol=1e-15)
+    assert not Q0.any() and not U0.any() and not T0.any()
+    pp = np.zeros((N, 2 * sp.e_num), order="F")
+    rng = np.random.default_rng(8)
+    nb = g.bulk_node_num - 1
+    pp[:nb, 0::2] = g.lev_p[:, :nb].T
+    pp[:nb, 3::2] = rng.uniform(-1e-3, 1e-3, (nb, sp.e_num - 1))
+    pp = driver.rescale_pol_pops(pp, g.lev_p, sp.lev_g)
+    with_star = oracle.image_pass(g, sp, pp, lines, th, ph, 3, vel_out, ip, im_size, cloud.DX_IMAGE, star=star)
+    without = oracle.image_pass(g, sp, pp, lines, th, ph, 3, vel_out, ip, im_size, cloud.DX_IMAGE)
+    for a, b in zip(with_star, without):
+        np.testing.assert_array_equal(a[~on], b[~on])
+    # the star shines through except at optically thick line centres
+    assert (with_star[0][on] > without[0][on]).mean() > 0.5
+
+
+def test_node_pixels_match_reference(oracle, golden_dir):
+    # the reference's pix = 0 image mode (para_finish.f90:245-330): the march starts at each
+    # bulk node's rotated (x, y); traces with and without the star against the reference
+    from portal_amd import driver
+    from tests.golden.make_golden import cloud_digest
+    z = np.load(os.path.join(golden_dir, "node_pixels.npz"))
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 2500, seed=41)
+    assert cloud_digest(g) == str(z["digest"])
+    N = g.node_num
+    vn, bn, xn = (np.zeros((3, N), order="F") for _ in range(3))
+    xs = np.zeros(3)
+    oracle.lib.orc_rotate_all_save(N, C.c_double(float(z["th_inc"])), C.c_double(float(z["ph_az"])),
+                                   _d(_f64(g.vel_all)), _d(_f64(g.bvec_all)), _d(_f64(g.XYZ)), _d(vn), _d(bn), _d(xn),
+                                   _d(_f64(z["X_star"])), _d(xs))
+    nodes = z["nodes"]
+    np.testing.assert_array_equal(xn[:2, nodes - 1], z["xy"])
+    ptr, idx = _f64(g.nbr_ptr).astype(np.int32), np.asarray(g.nbr_idx, dtype=np.int32)
+    Rs = float(z["R_star"])
+    nu = sp.e_levs_hz[1] - sp.e_levs_hz[0]
+    for k, n in enumerate(nodes):
+        cells = np.zeros(4096, dtype=np.int32)
+        steps = np.zeros(4096, dtype=np.int32)
+        nc, hit = C.c_int(), C.c_int()
+        rc = oracle.lib.orc_trace_out_raytrace(C.c_double(xn[0, n - 1]), C.c_double(xn[1, n - 1]), N, _d(xn),
+                                               ptr.ctypes.data_as(C.POINTER(C.c_int)),
+                                               idx.ctypes.data_as(C.POINTER(C.c_int)), C.c_double(cloud.DX_IMAGE), 4096,
+                                               k % 2, C.c_double(Rs), _d(xs), cells.ctypes.data_as(C.POINTER(C.c_int)),
+                                               steps.ctypes.data_as(C.POINTER(C.c_int)), C.byref(nc), C.byref(hit))
+        assert rc == 0
+        ref = z["trace"][:, z["trace_ofs"][k]:z["trace_ofs"][k + 1]]
+        assert nc.value == ref.shape[1]
+        np.testing.assert_array_equal(cells[:nc.value], ref[0])
+        np.testing.assert_array_equal(steps[:nc.value], ref[1])
+        assert z["I_in"][k] == pytest.approx(orc_black_body(oracle, float(z["T_star"]) if hit.value else 2.73, nu),
+                                             rel=1e-14)
+    # the pass in this mode: one output row per bulk node, the first one's trace started at
+    # node 1's rotated position; with no line emission every row is 0
+    lines = driver.image_lines(sp, [1], [2])
+    vel_out = driver.channel_velocities(2, 300.0)
+    zero = np.zeros((N, 2 * sp.e_num), order="F")
+    out = oracle.image_pass(g, sp, zero, lines, float(z["th_inc"]), float(z["ph_az"]), 2, vel_out, 0, 0.0,
+                            cloud.DX_IMAGE, node_pixels=True)
+    assert all(o.shape == (g.bulk_node_num, 1, 5) for o in out)
+    assert not any(o.any() for o in out)
+
+
+def orc_black_body(oracle, T, nu):
+    oracle.lib.orc_black_body.restype = C.c_double
+    return oracle.lib.orc_black_body(C.c_double(T), C.c_double(nu))
