@@ -75,6 +75,10 @@ typedef struct {
     int ng_accel;            /* 0: plain iteration; 1: Ng acceleration of the level densities from the
                                 last four iterates, every third iteration from the fourth on;
                                 2: Anderson acceleration over a window of 4, every iteration */
+    int ali;                 /* 1: accelerated lambda iteration: each node's own-cell share of J^0_0
+                                (from the inward pass) moves into the K = 0 rates of the SE solve
+                                (Rybicki & Hummer 1991); same fixed point, fewer iterations.
+                                Ignored when dust enters the inward pass */
 } portal_iter_params;
 
 typedef struct {

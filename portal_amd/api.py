@@ -143,10 +143,10 @@ class _Core:
 
     def lambda_iterate(self, xth, nvel, dx, max_iter, tol_pop=1e-6, tol_align=1e-8, n_depth=4096,
                        precision=L.PORTAL_FP64, surrogate=0, fallback_tol=1e-8, rebuild_paths=0, outputs=True,
-                       ng_accel=0):
+                       ng_accel=0, ali=0):
         g, sp = self.grid, self.species
         p = self._se_params(xth, nvel, dx, n_depth, 0, precision)
-        it = L.IterParams(max_iter, tol_pop, tol_align, surrogate, fallback_tol, rebuild_paths, ng_accel)
+        it = L.IterParams(max_iter, tol_pop, tol_align, surrogate, fallback_tol, rebuild_paths, ng_accel, ali)
         st = L.Stats()
         if not outputs:
             self._check(self.lib.portal_lambda_iterate(self.ctx, C.byref(p), C.byref(it), None, None, C.byref(st)))

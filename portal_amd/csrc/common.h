@@ -70,6 +70,7 @@ struct Species {
     double *d_kd = nullptr, *d_nu = nullptr;              // per line: dust opacity, frequency
     double *d_tA = nullptr, *d_tB21 = nullptr, *d_tnu = nullptr, *d_tgul = nullptr;
     int *d_trad = nullptr, *d_tcol = nullptr;
+    int *d_tlo = nullptr, *d_tup = nullptr;               // per transition, 0-based levels
     double *d_Cji = nullptr, *d_coltemps = nullptr;
     int *d_count = nullptr, *d_jlev = nullptr;
     bool valid = false;
@@ -112,6 +113,8 @@ struct Ctx {
     float *d_KSD32 = nullptr;
     double *d_dustco = nullptr; // N x R x {Ka, j} when dust is present
     double *d_J0 = nullptr, *d_J2 = nullptr;  // R x N
+    double *d_LAM = nullptr;    // R x N, own-cell operator of the accelerated lambda iteration
+    bool ali = false;           // the lambda iteration in progress uses it
     double *d_pol = nullptr;    // N x 2L
     double *d_rates = nullptr;  // N x 3 x 2L
     int *d_flags = nullptr;     // error flags
@@ -219,6 +222,10 @@ struct RtArgs {
     const double *xthn;
     const int *nstar;
     const double *I_star;
+    // accelerated lambda iteration (cooperative kernel only): per (node, line) the share of
+    // the node's own cell in its J^0_0 per unit source function, sum over rays and channels
+    // of w phi0 d_v (1 - e^-tau_own) (R x N), or null
+    double *LAM;
 };
 hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s);
 hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s);
@@ -237,6 +244,10 @@ struct SeArgs {
     const int *ent_idx, *ent_ptr, *term_q, *term_class;
     const double *term_coef;
     const int *count_array, *j_lev;             // L
+    // accelerated lambda iteration, or null: lam (R x N) from the inward pass, the level
+    // densities (L x N) that produced J, and each transition's lower/upper level (0-based)
+    const double *lam, *levp;
+    const int *t_lo, *t_up;
     double *pol;                                // N x 2L
     double *rates;                              // N x 3 x 2L or null
     int *flags;

@@ -92,12 +92,14 @@ int ensure_node_buffers(Ctx &c) {
     dfree(c.d_KSD32);
     dfree(c.d_J0);
     dfree(c.d_J2);
+    dfree(c.d_LAM);
     dfree(c.d_pol);
     dfree(c.d_rates);
     HIP_TRY(&c, hipMalloc((void **)&c.d_KS, sizeof(double) * 2 * N * R));
     HIP_TRY(&c, hipMalloc((void **)&c.d_KS32, sizeof(float) * 2 * N * R));
     HIP_TRY(&c, hipMalloc((void **)&c.d_J0, sizeof(double) * N * R));
     HIP_TRY(&c, hipMalloc((void **)&c.d_J2, sizeof(double) * N * R));
+    HIP_TRY(&c, hipMalloc((void **)&c.d_LAM, sizeof(double) * N * R));
     HIP_TRY(&c, hipMalloc((void **)&c.d_pol, sizeof(double) * N * 2 * L));
     HIP_TRY(&c, hipMalloc((void **)&c.d_rates, sizeof(double) * N * 6 * L));
     if (!c.d_flags) HIP_TRY(&c, hipMalloc((void **)&c.d_flags, sizeof(int) * 4));
@@ -366,6 +368,18 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
     autotune_rt(c, ra, mixed);
     ra.variant = c.tuned_variant[mixed ? 1 : 0];
     ra.waves = c.tuned_waves[mixed ? 1 : 0];
+    if (c.ali) {
+        // the own-cell operator comes from the cooperative kernel: its fastest timed shape
+        ra.LAM = c.d_LAM;
+        if (ra.variant != 2) {
+            const int pi = mixed ? 1 : 0, wv[3] = {2, 4, 8};
+            int best = -1;
+            for (int k = 0; k < 3; k++)
+                if (c.tune_ms[pi][3 + k] > 0 && (best < 0 || c.tune_ms[pi][3 + k] < c.tune_ms[pi][3 + best])) best = k;
+            ra.variant = 2;
+            ra.waves = best < 0 ? 4 : wv[best];
+        }
+    }
     Timer tr(c.stream);
     hipError_t e = mixed ? launch_rt_mixed(ra, c.stream) : launch_rt_f64(ra, c.stream);
     const double t_rt = tr.stop();
@@ -419,6 +433,12 @@ int run_se(Ctx &c, int extra_info, portal_stats *st) {
     a.term_coef = s.se.d_term_coef;
     a.count_array = s.d_count;
     a.j_lev = s.d_jlev;
+    a.t_lo = s.d_tlo;
+    a.t_up = s.d_tup;
+    if (c.ali) {
+        a.lam = c.d_LAM;
+        a.levp = c.d_levp;
+    }
     a.pol = c.d_pol;
     a.rates = extra_info ? c.d_rates : nullptr;
     a.flags = c.d_flags;
@@ -505,6 +525,8 @@ int portal_ctx_destroy(portal_ctx *x) {
     dfree(s.d_tgul);
     dfree(s.d_trad);
     dfree(s.d_tcol);
+    dfree(s.d_tlo);
+    dfree(s.d_tup);
     dfree(s.d_Cji);
     dfree(s.d_coltemps);
     dfree(s.d_count);
@@ -527,6 +549,7 @@ int portal_ctx_destroy(portal_ctx *x) {
     dfree(c.d_dustco);
     dfree(c.d_J0);
     dfree(c.d_J2);
+    dfree(c.d_LAM);
     dfree(c.d_pol);
     dfree(c.d_rates);
     dfree(c.d_flags);
@@ -655,7 +678,7 @@ int portal_upload_species(portal_ctx *x, int e_num, const int *j_lev, const doub
         if (s.c_trans[i] < 1 || s.c_trans[i] > e_num) return fail(c, PORTAL_ERR_ARG, "c_trans out of range");
     build_species_tables(s);
     const int cd = s.c_dim;
-    if (2 * (size_t)cd * cd * 8 + 7 * (size_t)s.n_trans * 8 > 150 * 1024)
+    if (2 * (size_t)cd * cd * 8 + 10 * (size_t)s.n_trans * 8 > 150 * 1024)
         return fail(c, PORTAL_ERR_ARG, "SE system too large for one workgroup's LDS");
     // per radiative line
     std::vector<int> l1(r_num), l2(r_num);
@@ -675,8 +698,11 @@ int portal_upload_species(portal_ctx *x, int e_num, const int *j_lev, const doub
     // per unique transition
     const int nt = s.n_trans;
     std::vector<double> tA(nt), tB21(nt), tnu(nt), tgul(nt);
+    std::vector<int> tlo(nt), tup(nt);
     for (int t = 0; t < nt; t++) {
         const int lo = s.all_trans[2 * t] - 1, up = s.all_trans[2 * t + 1] - 1;
+        tlo[t] = lo;
+        tup[t] = up;
         const double nu0 = s.e_levs_hz[up] - s.e_levs_hz[lo];
         const double A = s.trans_rad[t] >= 0 ? s.t_Aji[s.trans_rad[t]] : 0.0;
         tA[t] = A;
@@ -700,6 +726,8 @@ int portal_upload_species(portal_ctx *x, int e_num, const int *j_lev, const doub
     HIP_TRY(&c, upload(s.d_tgul, tgul));
     HIP_TRY(&c, upload(s.d_trad, s.trans_rad));
     HIP_TRY(&c, upload(s.d_tcol, s.trans_col));
+    HIP_TRY(&c, upload(s.d_tlo, tlo));
+    HIP_TRY(&c, upload(s.d_tup, tup));
     HIP_TRY(&c, upload(s.d_Cji, s.t_Cji));
     HIP_TRY(&c, upload(s.d_coltemps, s.col_temps));
     HIP_TRY(&c, upload(s.d_count, s.count_array));
@@ -955,7 +983,18 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     // acceleration: 1 Ng (the three previous level-density iterates), 2 Anderson (the
     // window of (x, G(x)) pairs, compact over solved nodes)
     if (it->ng_accel < 0 || it->ng_accel > 2) return fail(c, PORTAL_ERR_ARG, "ng_accel must be 0, 1 or 2");
+    if (it->ali < 0 || it->ali > 1) return fail(c, PORTAL_ERR_ARG, "ali must be 0 or 1");
     const bool ng = it->ng_accel == 1, aa = it->ng_accel == 2;
+    // the own-cell operator covers the line alone: with dust in the inward pass the
+    // iteration stays plain
+    bool dust_on = false;
+    if (c.g.has_dust)
+        for (double kd : c.sp.kd_v) dust_on = dust_on || kd != 0.0;
+    struct AliScope {
+        Ctx &c;
+        ~AliScope() { c.ali = false; }
+    } ali_scope{c};
+    c.ali = it->ali == 1 && !dust_on;
     double *d_hist = nullptr, *d_ngp = nullptr, *d_aa = nullptr;
     if (ng) {
         HIP_TRY(&c, hipMalloc((void **)&d_hist, sizeof(double) * N * L * 3));

@@ -39,7 +39,7 @@ __host__ __device__ constexpr size_t coop_buf_bytes(int cap, int R, bool dust) {
 template <typename Real>
 size_t coop_lds_bytes(int cap, int R, int npairs, bool dust) {
     const size_t stage = 2 * coop_buf_bytes<Real>(cap, R, dust);
-    const size_t red = sizeof(double) * 2 * (size_t)npairs;
+    const size_t red = sizeof(double) * 3 * (size_t)npairs;
     return stage > red ? stage : red;
 }
 
@@ -98,7 +98,8 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     const int cb = (t / R) * SLOTS;          // its first channel
     Real v0[SLOTS], e0[SLOTS];
     bool on[SLOTS];
-    Real S0[SLOTS], C0[SLOTS], S2[SLOTS], C2[SLOTS];
+    Real S0[SLOTS], C0[SLOTS], S2[SLOTS], C2[SLOTS], Lw[SLOTS];
+    const bool lam_on = a.LAM != nullptr;
     const R2 k0 = A::ks(a, (size_t)node * R + lo);  // the node's own cell, no skip
     R2 d0;
     if (DUST) d0 = A::ksd(a, (size_t)node * R + lo);
@@ -112,7 +113,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
         v0[q] = ((Real)2 * b) * (Real)nv / (Real)nvel;
         const Real aa = v0[q] * inv_bn;
         e0[q] = A::ex(-(aa * aa));
-        S0[q] = C0[q] = S2[q] = C2[q] = (Real)0;
+        S0[q] = C0[q] = S2[q] = C2[q] = Lw[q] = (Real)0;
     }
 
     coop_stage<Real, DUST>(a, bid, 0, cap, smem);
@@ -141,11 +142,16 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                 const Real tau0 = (kp + d0.x) * dl0;
                 const Real Sc = (k0.y * kp + d0.y * d0.x) / (kp + d0.x);
                 Tr[q] = A::ex(-tau0);
-                Em[q] = Sc * A::one_minus(tau0, Tr[q]);
+                const Real om0 = A::one_minus(tau0, Tr[q]);
+                Em[q] = Sc * om0;
+                if (lam_on) Lw[q] = Lw[q] + w * om0;
             } else {
                 const Real tau0 = k0.x * e0[q] * dl0;
                 Tr[q] = A::ex(-tau0);
-                Em[q] = k0.y * A::one_minus(tau0, Tr[q]);
+                const Real om0 = A::one_minus(tau0, Tr[q]);
+                Em[q] = k0.y * om0;
+                // the own cell's share of J per unit of its source function
+                if (lam_on) Lw[q] = Lw[q] + w * om0;
             }
             v0s[q] = v0[q] + vel_node;
             if (!on[q]) Tr[q] = (Real)0;
@@ -219,12 +225,14 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
 
     double *red0 = reinterpret_cast<double *>(smem);
     double *red2 = red0 + npairs;
+    double *red3 = red2 + npairs;
 #pragma unroll
     for (int q = 0; q < SLOTS; q++) {
         const int p = lo * C + cb + q;
         if (on[q]) {
             red0[p] = (double)S0[q] - (double)C0[q];
             red2[p] = (double)S2[q] - (double)C2[q];
+            red3[p] = (double)Lw[q];
         }
     }
     __syncthreads();
@@ -232,16 +240,18 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
         const int l = threadIdx.x;
         const double spi = 1.7724538509055160273;
         const double d_v = 2.0 * bd / nvel;
-        double j0 = 0.0, j2 = 0.0;
+        double j0 = 0.0, j2 = 0.0, lam = 0.0;
         for (int c = 0; c < C; c++) {
             const double vv = (2.0 * bd) * (c - nvel) / nvel;
             const double aa = vv / bd;
             const double phi0 = (1.0 / (spi * bd)) * exp(-(aa * aa));
             j0 = j0 + phi0 * d_v * red0[l * C + c];
             j2 = j2 + phi0 * d_v * red2[l * C + c];
+            if (lam_on) lam = lam + phi0 * d_v * red3[l * C + c];
         }
         a.J0[(size_t)node * R + l] = j0;
         a.J2[(size_t)node * R + l] = j2 / 1.4142135623730951;
+        if (lam_on) a.LAM[(size_t)node * R + l] = lam;
     }
 }
 

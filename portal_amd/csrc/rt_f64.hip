@@ -169,7 +169,8 @@ __global__ void __launch_bounds__(256) rt_f64_kernel(RtArgs a) {
 hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
     if (a.R > 64) return hipErrorInvalidValue;
-    if ((a.dust || a.xthn) && a.variant != 2) return hipErrorInvalidValue;  // dust, star: cooperative only
+    // dust, star and the own-cell operator: cooperative kernel only
+    if ((a.dust || a.xthn || a.LAM) && a.variant != 2) return hipErrorInvalidValue;
     if (a.variant == 1) return launch_staged<double>(a, a.max_nc, s);
     if (a.variant == 2) return launch_coop<double>(a, a.max_nc, s);
     const int W = a.waves;

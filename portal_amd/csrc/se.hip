@@ -5,7 +5,15 @@
 // (main_sub.f90:112-137).  The Wigner-coupled coefficients t_A, t_S, t_E, r_A, r_S
 // (pop_eqs.f90:1-359) depend only on (J,K,J',K'); the host folds them into a term
 // list per matrix entry, so a node's matrix is  A[e] = sum_k coef_k * x[q_k]  with
-// seven node scalars per transition  x = (B21 J0, B21 J2, B12 J0, B12 J2, A, C21, C12).
+// ten node scalars per transition  x = (B21 J0, B21 J2, B12 J0, B12 J2, A, C21, C12,
+// B21 J0*, B12 J0*, A*).  The last three serve the entries that couple the K = 0 slots
+// (the population block); they equal B21 J0, B12 J0 and A unless the accelerated
+// lambda iteration is on.  Then J0* = J0 - lam S and A* = A (1 - lam), with lam the
+// node's own-cell share of J0 per unit source function and S = A n_u / (B12 n_l -
+// B21 n_u) from the level densities that produced J0: the own cell's emission is
+// moved into the rates with the populations being solved for (Rybicki & Hummer 1991,
+// A&A 245, 171, eq. 2-4), and at a fixed point J0* + lam S = J0, so the solution is
+// that of the plain iteration.
 // Row 1 is replaced by sqrt(2J+1) at every level's K=0 slot and A rho = e1 is
 // solved by LU with partial pivoting (first maximal pivot, as DGETRF) in LDS, one
 // row per lane, followed by one step of iterative refinement with a compensated
@@ -24,8 +32,8 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
     const int lane = threadIdx.x;
     const int node = a.work[wi];
     const int cd = a.c_dim, nt = a.n_trans, L = a.L, R = a.R;
-    double *x = sm;                          // 7*nt scalars
-    double *M = x + 7 * nt;                  // LU in place, column-major cd x cd
+    double *x = sm;                          // 10*nt scalars
+    double *M = x + 10 * nt;                 // LU in place, column-major cd x cd
     double *M0 = M + cd * cd;                // assembled matrix (for the residual)
     double *v = M0 + cd * cd;                // solution
     double *y = v + cd;                      // rhs / work
@@ -71,13 +79,28 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
         }
         const double B12 = B21 * gul;
         const double C12 = Cr * gul * exp(-4.79924e-11 * nu / T);
-        x[7 * t + 0] = B21 * J0v;
-        x[7 * t + 1] = B21 * J2v;
-        x[7 * t + 2] = B12 * J0v;
-        x[7 * t + 3] = B12 * J2v;
-        x[7 * t + 4] = A;
-        x[7 * t + 5] = Cr;
-        x[7 * t + 6] = C12;
+        double Jp = J0v, Ap = A;
+        if (a.lam && rad >= 0) {
+            const double lam = a.lam[(size_t)node * R + rad];
+            const double nl = a.levp[(size_t)node * L + a.t_lo[t]], nu_ = a.levp[(size_t)node * L + a.t_up[t]];
+            const double den = B12 * nl - B21 * nu_;
+            const double S = A * nu_ / den;
+            // no operator for an inverted or empty line, or a lam outside [0, 1)
+            if (den > 0.0 && nu_ >= 0.0 && isfinite(S) && lam >= 0.0 && lam < 1.0) {
+                Jp = fmax(J0v - lam * S, 0.0);
+                Ap = A * (1.0 - lam);
+            }
+        }
+        x[10 * t + 0] = B21 * J0v;
+        x[10 * t + 1] = B21 * J2v;
+        x[10 * t + 2] = B12 * J0v;
+        x[10 * t + 3] = B12 * J2v;
+        x[10 * t + 4] = A;
+        x[10 * t + 5] = Cr;
+        x[10 * t + 6] = C12;
+        x[10 * t + 7] = B21 * Jp;
+        x[10 * t + 8] = B12 * Jp;
+        x[10 * t + 9] = Ap;
     }
     for (int i = lane; i < cd * cd; i += 64) M[i] = 0.0;
     __syncthreads();
@@ -240,7 +263,7 @@ __global__ void __launch_bounds__(64) se_persistent_kernel(const SeArgs *models,
 }
 
 size_t se_lds_bytes(int c_dim, int n_trans) {
-    return sizeof(double) * (7 * (size_t)n_trans + 2 * (size_t)c_dim * c_dim + 2 * c_dim) + sizeof(int) * c_dim + 16;
+    return sizeof(double) * (10 * (size_t)n_trans + 2 * (size_t)c_dim * c_dim + 2 * c_dim) + sizeof(int) * c_dim + 16;
 }
 
 hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd_max,
@@ -253,8 +276,7 @@ hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_i
 
 hipError_t launch_se(const SeArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
-    const size_t lds = sizeof(double) * (7 * (size_t)a.n_trans + 2 * (size_t)a.c_dim * a.c_dim + 2 * a.c_dim) +
-                       sizeof(int) * a.c_dim + 16;
+    const size_t lds = se_lds_bytes(a.c_dim, a.n_trans);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL(se_kernel, dim3(a.n_work), dim3(64), lds, s, a);
     return hipGetLastError();

@@ -216,7 +216,13 @@ int build_species_tables(Species &s) {
     auto put = [&](int r, int c, int q, double coef, int cls) {
         if (coef != 0.0) ent[r + cd * c].push_back({q, {coef, cls}});
     };
-    enum { B21J0 = 0, B21J2, B12J0, B12J2, AA, C21, C12 };
+    // scalars per transition; entries between K = 0 slots (the population block) take
+    // J0 and A from their own copies, which the accelerated lambda iteration modifies
+    enum { B21J0 = 0, B21J2, B12J0, B12J2, AA, C21, C12, B21J0P, B12J0P, AAP, NX };
+    auto pop = [](int k1, int k2, int q) {
+        if (k1 != 0 || k2 != 0) return q;
+        return q == B21J0 ? (int)B21J0P : q == B12J0 ? (int)B12J0P : q == AA ? (int)AAP : q;
+    };
     for (int t = 0; t < s.n_trans; t++) {
         const int lo = s.all_trans[2 * t] - 1, up = s.all_trans[2 * t + 1] - 1;
         const int jl = s.j_lev[lo], ju = s.j_lev[up];
@@ -226,39 +232,39 @@ int build_species_tables(Species &s) {
             for (int k2 = 0, c2 = s.count_array[up]; k2 <= kmaxof(up); k2 += 2, c2++) {
                 double ts0, ts2;
                 coupling(1, jl, k1, ju, k2, ts0, ts2);
-                put(c1, c2, 7 * t + B21J0, ts0, 1);
-                put(c1, c2, 7 * t + B21J2, ts2, 1);
-                put(c1, c2, 7 * t + AA, t_spont(jl, k1, ju, k2), 0);
-                if (k1 == 0 && k2 == 0) put(c1, c2, 7 * t + C21, std::sqrt(gu / gl), 2);
+                put(c1, c2, NX * t + pop(k1, k2, B21J0), ts0, 1);
+                put(c1, c2, NX * t + pop(k1, k2, B21J2), ts2, 1);
+                put(c1, c2, NX * t + pop(k1, k2, AA), t_spont(jl, k1, ju, k2), 0);
+                if (k1 == 0 && k2 == 0) put(c1, c2, NX * t + pop(k1, k2, C21), std::sqrt(gu / gl), 2);
             }
         // transfer into the upper level from the lower one (pop_eqs.f90:518-562)
         for (int k1 = 0, c1 = s.count_array[up]; k1 <= kmaxof(up); k1 += 2, c1++)
             for (int k2 = 0, c2 = s.count_array[lo]; k2 <= kmaxof(lo); k2 += 2, c2++) {
                 double ta0, ta2;
                 coupling(0, ju, k1, jl, k2, ta0, ta2);
-                put(c1, c2, 7 * t + B12J0, ta0, 1);
-                put(c1, c2, 7 * t + B12J2, ta2, 1);
-                if (k1 == 0 && k2 == 0) put(c1, c2, 7 * t + C12, std::sqrt(gl / gu), 2);
+                put(c1, c2, NX * t + pop(k1, k2, B12J0), ta0, 1);
+                put(c1, c2, NX * t + pop(k1, k2, B12J2), ta2, 1);
+                if (k1 == 0 && k2 == 0) put(c1, c2, NX * t + pop(k1, k2, C12), std::sqrt(gl / gu), 2);
             }
         // relaxation of the lower level: absorption and collisional excitation
         for (int k1 = 0, c1 = s.count_array[lo]; k1 <= kmaxof(lo); k1 += 2, c1++)
             for (int k2 = 0, c2 = s.count_array[lo]; k2 <= kmaxof(lo); k2 += 2, c2++) {
                 double r0, r2;
                 coupling(2, jl, k1, ju, k2, r0, r2);
-                put(c1, c2, 7 * t + B12J0, -r0, 1);
-                put(c1, c2, 7 * t + B12J2, -r2, 1);
-                if (k1 == k2) put(c1, c2, 7 * t + C12, -1.0, 2);
+                put(c1, c2, NX * t + pop(k1, k2, B12J0), -r0, 1);
+                put(c1, c2, NX * t + pop(k1, k2, B12J2), -r2, 1);
+                if (k1 == k2) put(c1, c2, NX * t + pop(k1, k2, C12), -1.0, 2);
             }
         // relaxation of the upper level: stimulated + spontaneous emission, de-excitation
         for (int k1 = 0, c1 = s.count_array[up]; k1 <= kmaxof(up); k1 += 2, c1++)
             for (int k2 = 0, c2 = s.count_array[up]; k2 <= kmaxof(up); k2 += 2, c2++) {
                 double r0, r2;
                 coupling(3, ju, k1, jl, k2, r0, r2);
-                put(c1, c2, 7 * t + B21J0, -r0, 1);
-                put(c1, c2, 7 * t + B21J2, -r2, 1);
+                put(c1, c2, NX * t + pop(k1, k2, B21J0), -r0, 1);
+                put(c1, c2, NX * t + pop(k1, k2, B21J2), -r2, 1);
                 if (k1 == k2) {
-                    put(c1, c2, 7 * t + AA, -1.0, 0);
-                    put(c1, c2, 7 * t + C21, -1.0, 2);
+                    put(c1, c2, NX * t + pop(k1, k2, AA), -1.0, 0);
+                    put(c1, c2, NX * t + pop(k1, k2, C21), -1.0, 2);
                 }
             }
     }

@@ -35,6 +35,7 @@ module portal_hip
     real(c_double) :: fallback_tol
     integer(c_int) :: rebuild_paths
     integer(c_int) :: ng_accel
+    integer(c_int) :: ali
   end type
 
   type, bind(C) :: portal_img_params

@@ -42,7 +42,7 @@ class SeParams(C.Structure):
 class IterParams(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol_pop", C.c_double), ("tol_align", C.c_double),
                 ("surrogate", C.c_int), ("fallback_tol", C.c_double), ("rebuild_paths", C.c_int),
-                ("ng_accel", C.c_int)]
+                ("ng_accel", C.c_int), ("ali", C.c_int)]
 
 
 class ImgParams(C.Structure):

@@ -116,6 +116,8 @@ def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32):
         pt.set_levpop(g.lev_p)
         pts.append(pt)
         sizes.append((n, L, int((g.ab_all[: g.bulk_node_num] > 0).sum())))
+        if (m + 1) % 8 == 0:
+            _log(f"batch: {m + 1} clouds built, {time.perf_counter() - t0:.1f}s")
     _log(f"batch: {n_models} clouds, {sum(s[0] for s in sizes)} nodes, built in {time.perf_counter() - t0:.1f}s")
     t = time.perf_counter()
     per, tot = batch_lambda_iterate(pts, [xth] * n_models, [nvel] * n_models, cloud.DX_INWARD, 400,
@@ -136,7 +138,7 @@ def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32):
 def million(n_int=1000000, seed=5):
     sp = cloud.co_like_rotor(4)
     t0 = time.perf_counter()
-    g = cloud.make_cloud(sp, n_int, seed=seed)
+    g = cloud.make_cloud(sp, n_int, seed=seed, log=_log)
     xth = cloud.legendre_all(6, 11)
     idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
     _log(f"million: cloud of {g.node_num} nodes built in {time.perf_counter() - t0:.1f}s")
@@ -153,8 +155,20 @@ def million(n_int=1000000, seed=5):
     for name, prec, surr, ng in runs:
         pt.set_levpop(g.lev_p)
         t = time.perf_counter()
-        st = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 300, tol_pop=TOL_POP, tol_align=TOL_ALIGN, precision=prec,
-                               surrogate=surr, ng_accel=ng, rebuild_paths=1, outputs=False)
+        # the plain FP64 iteration in calls of at most 10 iterations, each continuing the
+        # last (the same iteration as one call), with a line per call; the others in one call
+        done, walk = 0, 0.0
+        chunk = 10 if prec == PORTAL_FP64 and not ng else 300
+        while True:
+            st = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, chunk, tol_pop=TOL_POP, tol_align=TOL_ALIGN,
+                                   precision=prec, surrogate=surr if done == 0 else 0, ng_accel=ng,
+                                   rebuild_paths=1 if done == 0 else 0, outputs=False)
+            done += st["iterations"]
+            walk += st["t_walk_ms"]
+            _log(f"million {name}: {done} iterations, {time.perf_counter() - t:.1f}s")
+            if st["converged"] or done >= 300:
+                break
+        st["iterations"], st["t_walk_ms"] = done, walk
         t = time.perf_counter() - t
         img, _ = _image(pt, sp, g, 32, 93.75, 64, precision=prec)
         if ref is None:

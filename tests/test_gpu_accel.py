@@ -1,4 +1,5 @@
-"""Ng and Anderson acceleration of the lambda iteration (needs an MI355X).
+"""Ng and Anderson acceleration and the accelerated lambda iteration (own-cell operator)
+of the lambda iteration (needs an MI355X).
 
 The accelerated iteration must reach the fixed point of the plain one, in fewer
 iterations, in both precisions.
@@ -25,19 +26,19 @@ def small():
     pt.close()
 
 
-@pytest.mark.parametrize("accel", [1, 2], ids=["ng", "anderson"])
+@pytest.mark.parametrize("accel,ali", [(1, 0), (2, 0), (0, 1), (1, 1)], ids=["ng", "anderson", "ali", "ali+ng"])
 @pytest.mark.parametrize("precision,tol", [(PORTAL_FP64, 1e-9), (PORTAL_MIXED, 1e-6)])
-def test_acceleration_reaches_the_plain_fixed_point(small, precision, tol, accel):
+def test_acceleration_reaches_the_plain_fixed_point(small, precision, tol, accel, ali):
     sp, g, xth, pt = small
     kw = dict(tol_pop=tol, tol_align=tol * 1e-2, precision=precision)
     pt.set_levpop(g.lev_p)
     pp0, lev0, st0 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 400, **kw)
     pt.set_levpop(g.lev_p)
-    pp1, lev1, st1 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 400, ng_accel=accel, **kw)
-    print(f"precision {precision}: plain {st0['iterations']} iterations, accel {accel}: {st1['iterations']} "
-          f"({st1['ng_steps']} extrapolations)")
+    pp1, lev1, st1 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 400, ng_accel=accel, ali=ali, **kw)
+    print(f"precision {precision}: plain {st0['iterations']} iterations, accel {accel} ali {ali}: "
+          f"{st1['iterations']} ({st1['ng_steps']} extrapolations)")
     assert st0["iterations"] < 400 and st1["iterations"] < 400
-    assert st1["ng_steps"] > 0
+    assert st1["ng_steps"] > 0 or accel == 0
     assert st1["iterations"] < st0["iterations"]
     idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
     big = lev0[:, idx] > 1e-8 * lev0[:, idx].max(axis=0)
@@ -74,3 +75,24 @@ def test_a_continued_call_is_the_same_iteration(small):
         if st["converged"] or done > 200:
             break
     assert done == n_one
+
+
+def test_ali_fixed_point_is_exact(small):
+    # the own-cell operator changes the path, not the solution: iterated far below the
+    # tolerances above, the plain and accelerated FP64 iterations agree to rounding level
+    sp, g, xth, pt = small
+    kw = dict(tol_pop=1e-13, tol_align=1e-15, precision=PORTAL_FP64)
+    pt.set_levpop(g.lev_p)
+    pp0, lev0, st0 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 600, ng_accel=1, **kw)
+    pt.set_levpop(g.lev_p)
+    pp1, lev1, st1 = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 600, ng_accel=1, ali=1, **kw)
+    print(f"to 1e-13: Ng {st0['iterations']} iterations, Ng + ALI {st1['iterations']}")
+    assert st0["converged"] and st1["converged"]
+    idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+    big = lev0[:, idx] > 1e-8 * lev0[:, idx].max(axis=0)
+    rel = np.abs(lev1[:, idx] - lev0[:, idx]) / np.abs(lev0[:, idx])
+    print(f"max rel difference of the level densities {rel[big].max():.2e}")
+    assert rel[big].max() < 1e-10
+    al0 = pp0[idx][:, 3::2] / pp0[idx][:, 2::2]
+    al1 = pp1[idx][:, 3::2] / pp1[idx][:, 2::2]
+    assert np.abs(al1 - al0).max() < 1e-12

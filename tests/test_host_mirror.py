@@ -71,3 +71,21 @@ def test_image_lines_keep_the_nu0_overwrite():
     assert lines[0][3] == sp.t_Aji[0] and lines[1][3] == sp.t_Aji[1]
     v = driver.channel_velocities(2, 10.0, 1.0)
     assert list(v) == [-19.0, -9.0, 1.0, 11.0, 21.0]
+
+
+def test_box_delaunay_equals_whole_set():
+    # the parallel form used for large clouds (boxes with a halo, one worker process per
+    # box) gives the whole-set triangulation's neighbour lists edge for edge
+    from portal_amd import cloud
+    rng = np.random.default_rng(17)
+    n = 60000
+    r = rng.random(n) ** (1 / 3)
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    u = rng.normal(size=(n // 8, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    pos = np.concatenate([v * r[:, None], u]) * 1.5e15
+    p1, i1 = cloud.delaunay_neighbours(pos, workers=1)
+    p2, i2 = cloud.delaunay_neighbours(pos, workers=4)
+    np.testing.assert_array_equal(p1, p2)
+    np.testing.assert_array_equal(i1, i2)
