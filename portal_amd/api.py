@@ -168,6 +168,11 @@ class _Core:
         n = self.lib.portal_debug_iter_trace(self.ctx, _d(out), max_iter)
         return out[: 2 * n].reshape(n, 2)
 
+    def walk_chunks(self, scale):
+        """Test hook: size the single-pass walk's chunk store at scale x its sampled estimate
+        (a small scale forces the second-walk fallback, scale <= 0 the two-pass walk)."""
+        self._check(self.lib.portal_debug_walk_chunks(self.ctx, C.c_double(scale)))
+
     def force_rt_kernel(self, precision, variant, waves=4):
         """Test hook: fix the inward-integration kernel (0 global, 1 staged, 2 cooperative)
         and its waves per workgroup for one precision; variant -1 returns to autotuning."""

@@ -39,6 +39,11 @@ struct PathSet {
     std::vector<double> X_star;
     double *d_xthn = nullptr;
     int *d_nstar = nullptr;
+    // chunk store of the single-pass walk (kept for the next build)
+    PathEntry *chunks = nullptr;
+    int2 *chunk_meta = nullptr;
+    long long cap_chunks = 0;
+    bool single_pass = false;  // the last build used the single-pass walk
 };
 
 // Statistical-equilibrium coefficient tables of one species (host-built, FP64).
@@ -129,6 +134,7 @@ struct Ctx {
     int tuned_variant[2] = {0, 0}, tuned_waves[2] = {4, 4};
     double tune_ms[2][8] = {{0}};
     int forced_variant[2] = {-1, -1}, forced_waves[2] = {4, 4};  // test hook: bypass the autotuner
+    double walk_chunk_scale = 1.1;  // chunk store of the single-pass walk / sampled estimate
     // d_pol holds the SE solution from which the current d_levp was formed (set by the
     // lambda iteration; cleared by anything that changes the state otherwise), so that
     // a lambda iteration continued by a further call measures its first change exactly
@@ -138,7 +144,7 @@ struct Ctx {
 };
 
 // error flags (bits in d_flags[0])
-enum : int { FLAG_DEPTH = 1, FLAG_LU = 2, FLAG_TEMP = 4, FLAG_DEPTH_RT = 8, FLAG_START = 16 };
+enum : int { FLAG_DEPTH = 1, FLAG_LU = 2, FLAG_TEMP = 4, FLAG_DEPTH_RT = 8, FLAG_START = 16, FLAG_CHUNKS = 32 };
 
 #define HIP_TRY(ctx, expr)                                                          \
     do {                                                                            \
@@ -170,8 +176,21 @@ struct WalkArgs {
     const double *xthn;
     const int *nvar, *nstar;
     double Xs0, Xs1, Xs2, R_star;
+    // single-pass form: entries go to fixed-size chunks taken from a counter, each chunk
+    // tagged with its (path, ordinal); a compaction then writes the contiguous layout
+    PathEntry *chunks;
+    int2 *chunk_meta;
+    unsigned long long *chunk_next;
+    long long chunk_cap;
+    // sampling form (count mode): walk only every sample-th work item
+    int sample;
 };
-hipError_t launch_walk(const WalkArgs &a, bool write, hipStream_t s);
+// walk modes: count (counts and the longest path), write (entries at offsets), chunked
+// (counts and entries in chunks, one pass)
+enum WalkMode : int { WALK_COUNT = 0, WALK_WRITE = 1, WALK_CHUNKED = 2 };
+constexpr int WALK_CHUNK = 16;  // entries per chunk
+hipError_t launch_walk(const WalkArgs &a, int mode, hipStream_t s);
+hipError_t launch_walk_compact(const WalkArgs &a, long long n_chunks, hipStream_t s);
 int build_star_dirs(const std::vector<int> &work, const double *XYZ, const double *bvec_all, const double *gvec,
                     int nth, int nph, const double *X_star, double R_star, std::vector<double> &xthn,
                     std::vector<int> &nvar, std::vector<int> &nstar);
@@ -282,6 +301,11 @@ hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_i
                                 int nt_max, int n_blocks, hipStream_t s);
 size_t se_lds_bytes(int c_dim, int n_trans);
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves);
+// the cooperative form of the persistent integration, one launch per slot count
+hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,
+                                     int cap, int slots, int waves, size_t lds, int n_blocks, hipStream_t s);
+size_t rt_coop_persistent_lds(bool mixed, int cap, int R_max, int C);
+int coop_slots_host(int R, int C, int W);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 hipError_t launch_ng_dots(int n_work, int L, const int *work, const double *x0, const double *x1, const double *x2,
                           const double *x3, double *partial, double *out5, hipStream_t s);

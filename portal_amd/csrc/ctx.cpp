@@ -181,6 +181,7 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
     a.n_depth = p->n_depth;
     a.counts = P.counts;
     a.flags = c.d_flags;
+    a.sample = 1;
     if (star) {
         a.xthn = P.d_xthn;
         a.nvar = d_nvar;
@@ -190,10 +191,60 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
         a.Xs2 = p->X_star[2];
         a.R_star = p->R_star;
     }
-    HIP_TRY(&c, launch_walk(a, false, c.stream));
+    // One walk writes the entries into chunks taken from a counter, and a compaction
+    // lays them out contiguously per path.  The chunk store is sized from a sampled count
+    // over about 4,096 work items (+10% and a margin); if it still overflows, or cannot be
+    // allocated, the entries are written by a second walk at the exact offsets.
+    bool chunked = false;
+    long long n_chunks = 0;
+    unsigned long long *d_next = nullptr;
+    if (n > 0 && n < (1LL << 31) && c.walk_chunk_scale > 0) {
+        a.sample = std::max(1, nw / 4096);
+        const long long ns = (long long)((nw + a.sample - 1) / a.sample) * D;
+        HIP_TRY(&c, launch_walk(a, WALK_COUNT, c.stream));
+        std::vector<int> cs(ns);
+        HIP_TRY(&c, hipMemcpyAsync(cs.data(), P.counts, sizeof(int) * ns, hipMemcpyDeviceToHost, c.stream));
+        HIP_TRY(&c, hipStreamSynchronize(c.stream));
+        a.sample = 1;
+        double per = 0;
+        for (int v : cs) per += (v + WALK_CHUNK - 1) / WALK_CHUNK;
+        per = per / (double)std::max<long long>(ns, 1);
+        const long long need = (long long)(c.walk_chunk_scale * per * (double)n) + 4096;
+        if (need > P.cap_chunks) {
+            dfree(P.chunks);
+            dfree(P.chunk_meta);
+            P.cap_chunks = 0;
+            if (hipMalloc((void **)&P.chunks, sizeof(PathEntry) * WALK_CHUNK * (size_t)need) == hipSuccess &&
+                hipMalloc((void **)&P.chunk_meta, sizeof(int2) * (size_t)need) == hipSuccess) {
+                P.cap_chunks = need;
+            } else {
+                (void)hipGetLastError();
+                dfree(P.chunks);
+                dfree(P.chunk_meta);
+            }
+        }
+        if (P.cap_chunks > 0) {
+            HIP_TRY(&c, hipMalloc((void **)&d_next, sizeof(unsigned long long)));
+            HIP_TRY(&c, hipMemsetAsync(d_next, 0, sizeof(unsigned long long), c.stream));
+            HIP_TRY(&c, hipMemsetAsync(c.d_flags, 0, sizeof(int) * 4, c.stream));
+            a.chunks = P.chunks;
+            a.chunk_meta = P.chunk_meta;
+            a.chunk_next = d_next;
+            a.chunk_cap = P.cap_chunks;
+            HIP_TRY(&c, launch_walk(a, WALK_CHUNKED, c.stream));
+            chunked = true;
+        }
+    }
+    if (!chunked) {
+        HIP_TRY(&c, hipMemsetAsync(c.d_flags, 0, sizeof(int) * 4, c.stream));
+        HIP_TRY(&c, launch_walk(a, WALK_COUNT, c.stream));
+    }
     int fl[2] = {0, 0};
+    unsigned long long used = 0;
     HIP_TRY(&c, hipMemcpyAsync(fl, c.d_flags, sizeof(fl), hipMemcpyDeviceToHost, c.stream));
+    if (d_next) HIP_TRY(&c, hipMemcpyAsync(&used, d_next, sizeof(used), hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(&c, hipStreamSynchronize(c.stream));
+    dfree(d_next);
     const int flags = fl[0];
     P.max_nc = fl[1];
     if (flags & FLAG_DEPTH) {
@@ -203,6 +254,8 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
         dfree(d_nvar);
         return fail(c, PORTAL_ERR_DEPTH, "an inward ray crosses more than n_depth cells (inverse_rt.f90:206-210)");
     }
+    if (chunked && (flags & FLAG_CHUNKS)) chunked = false;  // overflow: write by a second walk
+    n_chunks = (long long)used;
     long long total = 0;
     if (n > 0) HIP_TRY(&c, launch_scan_counts(P.counts, P.offsets, n, c.stream, &total));
     if ((size_t)total > P.cap_entries) {
@@ -212,8 +265,16 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
     }
     a.offsets = P.offsets;
     a.paths = P.entries;
-    HIP_TRY(&c, launch_walk(a, true, c.stream));
+    if (chunked)
+        HIP_TRY(&c, launch_walk_compact(a, n_chunks, c.stream));
+    else
+        HIP_TRY(&c, launch_walk(a, WALK_WRITE, c.stream));
+    P.single_pass = chunked;
     const double ms = tm.stop();
+    // the chunk store (about 1.4x the paths) is not kept between builds
+    dfree(P.chunks);
+    dfree(P.chunk_meta);
+    P.cap_chunks = 0;
     dfree(d_xth);
     dfree(d_gvec);
     dfree(d_bvec);
@@ -244,6 +305,11 @@ int check_rt_args(Ctx &c, const RtArgs &a, bool mixed) {
 // Autotuner of the inward-integration launch: on the first pass of a geometry the
 // candidate (kernel variant, waves per workgroup) pairs are timed on a slice of the
 // work list and the fastest is kept for every later pass with the same shape.
+// inward-integration launch shapes the autotuner times: (variant, waves per workgroup)
+constexpr int N_RT_CANDS = 7;
+constexpr struct RtCand { int variant, waves; } RT_CANDS[N_RT_CANDS] = {{0, 4}, {1, 4}, {1, 2}, {2, 2}, {2, 4},
+                                                                      {2, 5}, {2, 8}};
+
 int autotune_rt(Ctx &c, RtArgs ra, bool mixed) {
     const int pi = mixed ? 1 : 0;
     const int key = (int)((c.paths.total % 1000003) ^ (ra.C * 131 + ra.R * 7 + ra.D)) * 4 + ra.dust +
@@ -255,8 +321,8 @@ int autotune_rt(Ctx &c, RtArgs ra, bool mixed) {
         return PORTAL_OK;
     }
     if (c.tuned_key[pi] == key) return PORTAL_OK;
-    constexpr int NC = 6;
-    struct Cand { int variant, waves; } cands[NC] = {{0, 4}, {1, 4}, {1, 2}, {2, 2}, {2, 4}, {2, 8}};
+    constexpr int NC = N_RT_CANDS;
+    const RtCand *cands = RT_CANDS;
     const int sub = std::min(ra.n_work, 2048);
     ra.n_work = sub;
     double best = 1e30;
@@ -372,12 +438,14 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
         // the own-cell operator comes from the cooperative kernel: its fastest timed shape
         ra.LAM = c.d_LAM;
         if (ra.variant != 2) {
-            const int pi = mixed ? 1 : 0, wv[3] = {2, 4, 8};
+            const int pi = mixed ? 1 : 0;
             int best = -1;
-            for (int k = 0; k < 3; k++)
-                if (c.tune_ms[pi][3 + k] > 0 && (best < 0 || c.tune_ms[pi][3 + k] < c.tune_ms[pi][3 + best])) best = k;
+            for (int k = 0; k < N_RT_CANDS; k++)
+                if (RT_CANDS[k].variant == 2 && c.tune_ms[pi][k] > 0 &&
+                    (best < 0 || c.tune_ms[pi][k] < c.tune_ms[pi][best]))
+                    best = k;
             ra.variant = 2;
-            ra.waves = best < 0 ? 4 : wv[best];
+            ra.waves = best < 0 ? 4 : RT_CANDS[best].waves;
         }
     }
     Timer tr(c.stream);
@@ -541,6 +609,8 @@ int portal_ctx_destroy(portal_ctx *x) {
     dfree(c.paths.counts);
     dfree(c.paths.d_xthn);
     dfree(c.paths.d_nstar);
+    dfree(c.paths.chunks);
+    dfree(c.paths.chunk_meta);
     dfree(c.d_levp);
     dfree(c.d_KS);
     dfree(c.d_KS32);
@@ -907,6 +977,16 @@ int portal_debug_iter_trace(const portal_ctx *x, double *out, int max_iter) {
 
 // test hook: run the inward integration of one precision with a fixed kernel variant
 // (0 global, 1 staged, 2 cooperative) and wave count; variant -1 returns to autotuning
+// Test hook: the chunk store of the single-pass walk is sized at scale x the sampled
+// estimate (default 1.1); a small scale forces the overflow fallback, scale <= 0 the
+// two-pass walk.  The next path build uses it.
+int portal_debug_walk_chunks(portal_ctx *x, double scale) {
+    if (!x) return PORTAL_ERR_ARG;
+    x->c.walk_chunk_scale = scale;
+    x->c.paths.valid = false;
+    return PORTAL_OK;
+}
+
 int portal_debug_force_rt(portal_ctx *x, int precision, int variant, int waves) {
     if (!x || precision < 0 || precision > 1 || variant < -1 || variant > 2 || waves < 1 || waves > 8)
         return PORTAL_ERR_ARG;
@@ -1387,6 +1467,9 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     hipSetDevice(c0.device);
     if (it->surrogate) return fail(c0, PORTAL_ERR_UNSUPPORTED, "the surrogate iterate is not available for batches");
     const bool mixed = params[0].precision == PORTAL_MIXED;
+    // a model whose last SE solution formed its current level densities continues that
+    // iteration: its first pass here is compared with that solution
+    std::vector<char> chained(n, 0);
     for (int m = 0; m < n; m++) {
         if (!xs[m] || xs[m]->c.device != c0.device) return fail(c0, PORTAL_ERR_ARG, "batch contexts must share one device");
         if ((params[m].precision == PORTAL_MIXED) != mixed) return fail(c0, PORTAL_ERR_ARG, "mixed precisions in one batch");
@@ -1396,6 +1479,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         if (c.g.has_dust || params[m].star_mod)
             return fail(c0, PORTAL_ERR_UNSUPPORTED,
                         "model " + std::to_string(m) + ": dust or star mode in a batch is not supported");
+        chained[m] = c.pol_chain ? 1 : 0;
         c.pol_chain = false;
     }
     if (total) std::memset(total, 0, sizeof(*total));
@@ -1502,7 +1586,8 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         }
         HIP_TRY(&c0, hipMemset(c.d_J0, 0, sizeof(double) * c.g.N * sp.r_num));
         HIP_TRY(&c0, hipMemset(c.d_J2, 0, sizeof(double) * c.g.N * sp.r_num));
-        HIP_TRY(&c0, hipMemset(c.d_pol, 0, sizeof(double) * c.g.N * 2 * sp.e_num));
+        // a continued model keeps its last SE solution: it is the first comparison
+        if (!chained[m]) HIP_TRY(&c0, hipMemset(c.d_pol, 0, sizeof(double) * c.g.N * 2 * sp.e_num));
     }
     auto by_cost = [](const std::pair<double, int2> &a, const std::pair<double, int2> &b) { return a.first > b.first; };
     std::stable_sort(rt_items.begin(), rt_items.end(), by_cost);
@@ -1511,30 +1596,81 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     for (size_t i = 0; i < ri.size(); i++) ri[i] = rt_items[i].second;
     for (size_t i = 0; i < si.size(); i++) si[i] = se_items[i].second;
     const int n_items = (int)ri.size();
+    // the cooperative form: the items split by the slot count of their model's lines and
+    // channels, each class one persistent launch (in cost order within the class); the
+    // staged per-wave form serves batches that some class does not fit
+    constexpr int COOP_W = 4;
+    struct CoopClass {
+        int slots = 0, cap = 8, R_max = 1, C_max = 1;
+        std::vector<int2> items;
+        size_t lds = 0;
+        int2 *d_items = nullptr;
+    };
+    std::vector<CoopClass> classes;
+    bool coop = true;
+    {
+        std::vector<int> cls_of(n, -1);
+        for (int m = 0; m < n && coop; m++) {
+            const int sl = coop_slots_host(rts[m].R, rts[m].C, COOP_W);
+            if (sl == 0) {
+                coop = false;
+                break;
+            }
+            for (size_t k = 0; k < classes.size(); k++)
+                if (classes[k].slots == sl) cls_of[m] = (int)k;
+            if (cls_of[m] < 0) {
+                cls_of[m] = (int)classes.size();
+                classes.emplace_back();
+                classes.back().slots = sl;
+            }
+            CoopClass &k = classes[cls_of[m]];
+            k.cap = std::max(k.cap, rts[m].max_nc);
+            k.R_max = std::max(k.R_max, rts[m].R);
+            k.C_max = std::max(k.C_max, rts[m].C);
+        }
+        if (coop) {
+            for (const int2 &itm : ri) classes[cls_of[itm.x]].items.push_back(itm);
+            for (auto &k : classes) {
+                k.lds = rt_coop_persistent_lds(mixed, k.cap, k.R_max, k.C_max);
+                coop = coop && k.lds <= 160 * 1024;
+            }
+        }
+    }
     RtArgs *d_rt = nullptr;
     SeArgs *d_se = nullptr;
     int2 *d_ri = nullptr, *d_si = nullptr;
     int *d_cnt = nullptr;
     unsigned long long *d_max = nullptr;
     std::vector<double *> prev(n, nullptr);
+    auto free_classes = [&]() {
+        for (auto &k : classes) dfree(k.d_items);
+    };
     HIP_TRY(&c0, upload(d_rt, rts));
     HIP_TRY(&c0, upload(d_se, ses));
     HIP_TRY(&c0, upload(d_ri, ri));
     HIP_TRY(&c0, upload(d_si, si));
-    HIP_TRY(&c0, hipMalloc((void **)&d_cnt, sizeof(int) * 4));
+    if (coop)
+        for (auto &k : classes) HIP_TRY(&c0, upload(k.d_items, k.items));
+    const int n_cnt = 2 + (int)classes.size();
+    HIP_TRY(&c0, hipMalloc((void **)&d_cnt, sizeof(int) * n_cnt));
     HIP_TRY(&c0, hipMalloc((void **)&d_max, sizeof(unsigned long long) * 2 * n));
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
         HIP_TRY(&c0, hipMalloc((void **)&prev[m], sizeof(double) * c.g.N * 2 * c.sp.e_num));
-        HIP_TRY(&c0, hipMemset(prev[m], 0, sizeof(double) * c.g.N * 2 * c.sp.e_num));
+        if (chained[m])
+            HIP_TRY(&c0, hipMemcpy(prev[m], c.d_pol, sizeof(double) * c.g.N * 2 * c.sp.e_num, hipMemcpyDeviceToDevice));
+        else
+            HIP_TRY(&c0, hipMemset(prev[m], 0, sizeof(double) * c.g.N * 2 * c.sp.e_num));
     }
     HIP_TRY(&c0, hipMemset(c0.d_flags, 0, sizeof(int) * 4));
     int n_cu = 256;
     hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c0.device);
-    // autotune the persistent integration's waves per workgroup on a slice of the items
-    int best_w = 4;
+    // autotune the staged persistent integration's waves per workgroup on a slice of the
+    // items (when the cooperative form serves the batch, it runs with COOP_W waves)
+    int best_w = coop ? COOP_W : 4;
     double best_ms = 1e30;
     for (int w : {2, 4}) {
+        if (coop) break;
         const size_t lds = rt_persistent_lds(mixed, cap, R_max, w);
         if (lds > 160 * 1024) continue;
         const int bpc = std::max(1, std::min(8 / (w / 2 > 0 ? w / 2 : 1), (int)((160 * 1024) / lds)));
@@ -1560,7 +1696,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         }
     }
     const size_t rt_lds = rt_persistent_lds(mixed, cap, R_max, best_w);
-    if (rt_lds > 160 * 1024) return fail(c0, PORTAL_ERR_ARG, "batch paths too long for LDS staging");
+    if (!coop && rt_lds > 160 * 1024) return fail(c0, PORTAL_ERR_ARG, "batch paths too long for LDS staging");
     const int rt_blocks = n_cu * std::max(1, std::min(8 / std::max(1, best_w / 2), (int)((160 * 1024) / rt_lds)));
     const size_t se_lds = se_lds_bytes(cd_max, nt_max);
     if (se_lds > 160 * 1024) return fail(c0, PORTAL_ERR_ARG, "SE system too large for one workgroup's LDS");
@@ -1569,7 +1705,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIP_TRY(&c0, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    launch_fill_zero(d_cnt, sizeof(int) * 4, s);
+    launch_fill_zero(d_cnt, sizeof(int) * n_cnt, s);
     launch_fill_zero(d_max, sizeof(unsigned long long) * 2 * n, s);
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
@@ -1577,7 +1713,16 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
                     nullptr, c.sp.d_g1, c.sp.d_g2, c.sp.d_cnu_spi, mixed ? nullptr : c.d_KS, mixed ? c.d_KS32 : nullptr};
         launch_line_prep(pa, s);
     }
-    launch_rt_persistent(mixed, d_rt, d_ri, n_items, d_cnt, cap, R_max, best_w, rt_blocks, s);
+    if (coop) {
+        for (size_t k = 0; k < classes.size(); k++) {
+            const CoopClass &cl = classes[k];
+            const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / std::max<size_t>(cl.lds, 1))));
+            launch_rt_coop_persistent(mixed, d_rt, cl.d_items, (int)cl.items.size(), d_cnt + 2 + k, cl.cap, cl.slots,
+                                      COOP_W, cl.lds, n_cu * bpc, s);
+        }
+    } else {
+        launch_rt_persistent(mixed, d_rt, d_ri, n_items, d_cnt, cap, R_max, best_w, rt_blocks, s);
+    }
     launch_se_persistent(d_se, d_si, (int)si.size(), d_cnt + 1, cd_max, nt_max, se_blocks, s);
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
@@ -1592,7 +1737,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     HIP_TRY(&c0, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     std::vector<unsigned long long> mx(2 * n);
     std::vector<int> conv_at(n, 0);
-    std::vector<double> last(n, 0.0);
+    std::vector<double> last(n, 0.0), last_al(n, 0.0);
     int iters = 0;
     double t_iter = 0;
     for (int k = 0; k < it->max_iter; k++) {
@@ -1607,8 +1752,9 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
             std::memcpy(&dp, &mx[2 * m], 8);
             std::memcpy(&da, &mx[2 * m + 1], 8);
             last[m] = dp;
-            // the first pass has no previous alignment to compare with
-            if (k > 0 && !conv_at[m] && dp <= it->tol_pop && da <= it->tol_align) conv_at[m] = k + 1;
+            last_al[m] = da;
+            // the first pass has no previous alignment to compare with, unless it continues
+            if ((k > 0 || chained[m]) && !conv_at[m] && dp <= it->tol_pop && da <= it->tol_align) conv_at[m] = k + 1;
             all = all && conv_at[m];
         }
         if (all) break;
@@ -1625,16 +1771,22 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     hipFree(d_max);
     for (auto *p : prev) hipFree(p);
     free_xths();
+    free_classes();
     const double ms = ttot.stop();
     if (flags & FLAG_TEMP) return fail(c0, PORTAL_ERR_TEMP, "kinetic temperature above the collision-rate table");
     if (flags & FLAG_LU) return fail(c0, PORTAL_ERR_LU, "Density-matrix calculation: failed (singular SE matrix)");
     long long ncross = 0;
+    bool all_conv = true;
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
         ncross += c.paths.total;
+        c.pol_chain = true;  // d_pol formed d_levp in the last pass
+        all_conv = all_conv && conv_at[m];
         if (per_model) {
+            per_model[m].converged = conv_at[m] ? 1 : 0;
             per_model[m].iterations = conv_at[m] ? conv_at[m] : iters;
             per_model[m].max_rel_change = last[m];
+            per_model[m].max_align_change = last_al[m];
             per_model[m].n_cross = c.paths.total;
             per_model[m].n_paths = (long long)c.paths.n_work * c.paths.D;
         }
@@ -1645,8 +1797,9 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         total->iterations = iters;
         total->n_cross = ncross;
         total->rt_waves = best_w;
-        total->rt_variant = 2;
+        total->rt_variant = coop ? 2 : 1;
         total->max_path = cap;
+        total->converged = all_conv ? 1 : 0;
     }
     return PORTAL_OK;
 }

@@ -5,7 +5,10 @@
 // which follows from the SE normalisation row sum_i sqrt(2J_i+1) rho^0_0(i) = 1
 // (em_coefs.f90:158-164) and eat_lime.f90:261-263.  The kernel also reduces the
 // largest relative population change and the largest change of rho^2_0/rho^0_0,
-// the convergence measures of the iteration.
+// the convergence measures of the iteration.  Levels that hold less than 1e-10 of a
+// node's population (sqrt(2J+1) rho^0_0 < 1e-10, the SE normalisation summing these to
+// 1) are left out of both measures: their relative change is rounding noise in the
+// lines' J and their lines carry nothing at the images' tolerance.
 #include "common.h"
 
 namespace portal {
@@ -44,6 +47,7 @@ __global__ void __launch_bounds__(256) lambda_update_kernel(LambdaArgs a) {
         }
         if (!(dpop == dpop)) dpop = 1.0;  // NaN counts as not converged
         if (!(dal == dal)) dal = 1.0;
+        if (sqrt(2.0 * a.j_lev[i] + 1.0) * r0 < 1e-10) dpop = dal = 0.0;  // false for NaN
         a.levp[(size_t)node * a.L + i] = nv;
         if (a.pol_save) {
             a.pol_save[node + N * (2 * i)] = r0;

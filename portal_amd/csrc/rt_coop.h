@@ -270,9 +270,60 @@ hipError_t launch_coop_s(const RtArgs &a, int cap, size_t lds, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Persistent form over the (model, work item) items of a batch of models whose lines
+// and channels need the same slot count: workgroups take items from a counter until the
+// list is exhausted (every workgroup reaches the exit test after each item).
+template <typename Real, int SLOTS>
+__global__ void __launch_bounds__(512) rt_coop_persistent_kernel(const RtArgs *models, const int2 *items,
+                                                                 int n_items, int *counter, int cap) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const int2 mi = items[it];
+        rt_coop_node<Real, SLOTS, false>(models[mi.x], mi.y, cap, smem);
+        __syncthreads();
+    }
+}
+
 // Slots per lane: the fewest consecutive channels per lane with which every line's
 // channels fit in its share of the 64 W lanes, ceil(C / floor(64 W / R)), rounded up to
-// an instantiated count.
+// an instantiated count (0: more pairs than one workgroup holds).
+inline int coop_slots(int R, int C, int W) {
+    const int per_line = (64 * W) / R;
+    if (per_line < 1) return 0;
+    const int need = (C + per_line - 1) / per_line;
+    if (need <= 6) return need;
+    if (need <= 8) return 8;
+    if (need <= 10) return 10;
+    if (need <= 12) return 12;
+    return 0;
+}
+
+template <typename Real>
+hipError_t launch_coop_persistent(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
+                                  int slots, int waves, size_t lds, int n_blocks, hipStream_t s) {
+    if (lds > 160 * 1024 || waves < 1 || waves > 8) return hipErrorInvalidValue;
+    const dim3 g(n_blocks), b(64 * waves);
+    switch (slots) {
+        case 1: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 1>), g, b, lds, s, models, items, n_items, counter, cap); break;
+        case 2: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 2>), g, b, lds, s, models, items, n_items, counter, cap); break;
+        case 3: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 3>), g, b, lds, s, models, items, n_items, counter, cap); break;
+        case 4: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 4>), g, b, lds, s, models, items, n_items, counter, cap); break;
+        case 5: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 5>), g, b, lds, s, models, items, n_items, counter, cap); break;
+        case 6: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 6>), g, b, lds, s, models, items, n_items, counter, cap); break;
+        case 8: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 8>), g, b, lds, s, models, items, n_items, counter, cap); break;
+        case 10: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 10>), g, b, lds, s, models, items, n_items, counter, cap); break;
+        case 12: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 12>), g, b, lds, s, models, items, n_items, counter, cap); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <typename Real>
 hipError_t launch_coop(const RtArgs &a, int cap, hipStream_t s) {
     const int W = a.waves;

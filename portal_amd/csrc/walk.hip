@@ -51,23 +51,52 @@ __device__ static void find_dir_d(const double b[3], const double g[3], double t
     }
 }
 
-template <bool WRITE>
+// Chunked output of one path: entries go to chunks of WALK_CHUNK taken from a global
+// counter; a chunk records its (path, ordinal).  Past the capacity nothing is written and
+// FLAG_CHUNKS is raised (the caller then writes the entries by a second walk).
+struct ChunkWriter {
+    long long chunk = -1;
+    int slot = WALK_CHUNK, ord = -1;
+    __device__ void put(const WalkArgs &a, long long id, const PathEntry &e) {
+        if (slot == WALK_CHUNK) {
+            ord++;
+            slot = 0;
+            const unsigned long long c = atomicAdd(a.chunk_next, 1ull);
+            if ((long long)c < a.chunk_cap) {
+                chunk = (long long)c;
+                a.chunk_meta[c] = make_int2((int)id, ord);
+            } else {
+                chunk = -2;
+                atomicOr(a.flags, FLAG_CHUNKS);
+            }
+        }
+        if (chunk >= 0) a.chunks[chunk * WALK_CHUNK + slot] = e;
+        slot++;
+    }
+};
+
+template <int MODE>
 __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long long)a.n_work * a.D) return;
+    constexpr bool WRITE = MODE == WALK_WRITE, CHUNKED = MODE == WALK_CHUNKED;
+    const long long n_paths = (long long)((a.n_work + a.sample - 1) / a.sample) * a.D;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= n_paths) return;
+    // with sampling (count mode only) path tid walks ray tid % D of work item sample*(tid / D)
+    const long long id = a.sample > 1 ? (tid / a.D) * a.sample * a.D + tid % a.D : tid;
     const int w = (int)(id / a.D), r = (int)(id % a.D);
     const int node = a.work[w];
     bool star_ray = false;
+    ChunkWriter cw;
     if (a.xthn && r >= a.nvar[w]) {
         // star mode: a padding ray (weight 0) gets the node's own cell with zero length
-        if (WRITE) {
-            PathEntry e;
-            e.cell = node;
-            e.len = 0;
-            e.vray = 0.0;
-            a.paths[a.offsets[id]] = e;
-        } else {
-            a.counts[id] = 1;
+        PathEntry e;
+        e.cell = node;
+        e.len = 0;
+        e.vray = 0.0;
+        if (WRITE) a.paths[a.offsets[id]] = e;
+        if (CHUNKED) cw.put(a, id, e);
+        if (!WRITE) {
+            a.counts[tid] = 1;
             atomicMax(a.flags + 1, 1);
         }
         return;
@@ -106,10 +135,19 @@ __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
         const double t4 = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
         const double q4 = t4 * t4;
         const double q5 = d0 * dir[0] + d1 * dir[1] + d2 * dir[2];
-        const int q_end = a.nbr_ptr[cur + 1];
-        for (int q = a.nbr_ptr[cur]; q < q_end; q++) {
-            const int nb = a.nbr_idx[q];
-            const double4 pb = a.pos[nb];
+        const int q0 = a.nbr_ptr[cur], q_end = a.nbr_ptr[cur + 1];
+        // a neighbour's position is loaded one neighbour, its index two neighbours, ahead of
+        // its test, so that the gathers overlap the arithmetic (past the end of the list the
+        // loads read the current node's own entry, which is never used)
+        int nb_n = q0 < q_end ? a.nbr_idx[q0] : cur;
+        double4 pb_n = a.pos[nb_n];
+        int nb_nn = q0 + 1 < q_end ? a.nbr_idx[q0 + 1] : cur;
+        for (int q = q0; q < q_end; q++) {
+            const int nb = nb_n;
+            const double4 pb = pb_n;
+            nb_n = nb_nn;
+            pb_n = a.pos[nb_n];
+            nb_nn = q + 2 < q_end ? a.nbr_idx[q + 2] : cur;
             const double e0 = pb.x - xp0, e1 = pb.y - xp1, e2 = pb.z - xp2;
             const double u = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
             const double q1 = u * u;
@@ -144,32 +182,54 @@ __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
             atomicOr(a.flags, FLAG_DEPTH);
             break;
         }
-        if (WRITE) {
+        if (WRITE || CHUNKED) {
             PathEntry e;
             e.cell = cur;
             e.len = (int)round(delta_x / a.dx);
             const double4 v = a.velb[cur];
             e.vray = v.x * dir[0] + v.y * dir[1] + v.z * dir[2];
-            a.paths[off + nc - 1] = e;
+            if (WRITE) a.paths[off + nc - 1] = e;
+            if (CHUNKED) cw.put(a, id, e);
         }
         cur = next;
         if (next < 0) break;
     }
     if (!WRITE) {
-        a.counts[id] = nc > a.n_depth ? 0 : nc;
+        a.counts[tid] = nc > a.n_depth ? 0 : nc;
         atomicMax(a.flags + 1, nc);
     }
 }
 
-hipError_t launch_walk(const WalkArgs &a, bool write, hipStream_t s) {
-    const long long n = (long long)a.n_work * a.D;
+// one lane per (chunk, slot): entry ord*WALK_CHUNK + slot of its path, if the path has it
+__global__ void __launch_bounds__(256) walk_compact_kernel(WalkArgs a, long long n_chunks) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long c = t / WALK_CHUNK;
+    if (c >= n_chunks) return;
+    const int slot = (int)(t % WALK_CHUNK);
+    const int2 m = a.chunk_meta[c];
+    const int k = m.y * WALK_CHUNK + slot;
+    if (k < a.counts[m.x]) a.paths[a.offsets[m.x] + k] = a.chunks[t];
+}
+
+hipError_t launch_walk(const WalkArgs &a, int mode, hipStream_t s) {
+    if (a.sample < 1 || (a.sample > 1 && mode != WALK_COUNT)) return hipErrorInvalidValue;
+    const long long n = (long long)((a.n_work + a.sample - 1) / a.sample) * a.D;
     const int bs = 256;
     const unsigned nb = (unsigned)((n + bs - 1) / bs);
     if (nb == 0) return hipSuccess;
-    if (write)
-        hipLaunchKernelGGL(walk_kernel<true>, dim3(nb), dim3(bs), 0, s, a);
+    if (mode == WALK_WRITE)
+        hipLaunchKernelGGL(walk_kernel<WALK_WRITE>, dim3(nb), dim3(bs), 0, s, a);
+    else if (mode == WALK_CHUNKED)
+        hipLaunchKernelGGL(walk_kernel<WALK_CHUNKED>, dim3(nb), dim3(bs), 0, s, a);
     else
-        hipLaunchKernelGGL(walk_kernel<false>, dim3(nb), dim3(bs), 0, s, a);
+        hipLaunchKernelGGL(walk_kernel<WALK_COUNT>, dim3(nb), dim3(bs), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk_compact(const WalkArgs &a, long long n_chunks, hipStream_t s) {
+    const long long n = n_chunks * WALK_CHUNK;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(walk_compact_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, n_chunks);
     return hipGetLastError();
 }
 

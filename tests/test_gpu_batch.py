@@ -69,3 +69,46 @@ def test_batch_converges_each_model():
     assert all(p["max_rel_change"] < 1e-3 for p in per)
     for pt in pts:
         pt.close()
+
+
+def test_batch_continued_calls_are_one_iteration():
+    # 4 + 3 batch iterations in two calls equal 7 in one (the second call continues each
+    # model's iteration), and the iteration at which each model meets the tolerance does
+    # not depend on where the calls split
+    models = _models()
+    xth = cloud.legendre_all(4, 8)
+
+    def fresh():
+        pts = []
+        for sp, g in models:
+            pt = Portal(0)
+            pt.upload_grid(g)
+            pt.upload_species(sp)
+            pt.set_levpop(g.lev_p)
+            pts.append(pt)
+        return pts
+
+    a = fresh()
+    batch_lambda_iterate(a, [xth] * 3, [8] * 3, cloud.DX_INWARD, 7, tol_pop=0.0, tol_align=0.0)
+    b = fresh()
+    batch_lambda_iterate(b, [xth] * 3, [8] * 3, cloud.DX_INWARD, 4, tol_pop=0.0, tol_align=0.0)
+    batch_lambda_iterate(b, [xth] * 3, [8] * 3, cloud.DX_INWARD, 3, tol_pop=0.0, tol_align=0.0)
+    for pa, pb in zip(a, b):
+        np.testing.assert_array_equal(pa.get_levpop(), pb.get_levpop())
+    kw = dict(tol_pop=1e-5, tol_align=1e-7)
+    c = fresh()
+    per, tot = batch_lambda_iterate(c, [xth] * 3, [8] * 3, cloud.DX_INWARD, 200, **kw)
+    assert tot["converged"]
+    d = fresh()
+    at, done = [0, 0, 0], 0
+    while True:
+        per2, tot2 = batch_lambda_iterate(d, [xth] * 3, [8] * 3, cloud.DX_INWARD, 5, **kw)
+        for m, p in enumerate(per2):
+            if not at[m] and p["converged"]:
+                at[m] = done + p["iterations"]
+        done += tot2["iterations"]
+        if tot2["converged"] or done > 200:
+            break
+    assert at == [p["iterations"] for p in per]
+    for pt in a + b + c + d:
+        pt.close()

@@ -68,6 +68,29 @@ def test_inverse_rt_fp64_parity(portal1, cfg1, oracle_cfg1):
     assert st["n_cross"] > 0 and st["n_paths"] == len(idx) * xth.shape[1]
 
 
+def test_walk_forms_agree():
+    # the single-pass walk (chunked entries + compaction, store sized from a sampled count),
+    # its overflow fallback (a second walk at exact offsets) and the two-pass walk build
+    # the same paths: the inward pass gives identical J
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 20000, seed=7)
+    xth = cloud.legendre_all(4, 8)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    out = []
+    for scale in (1.1, 0.05, 0.0):
+        pt.walk_chunks(scale)
+        J0, J2, st = pt.inverse_rt(xth, 16, cloud.DX_INWARD)
+        out.append((J0, J2, st["n_cross"], st["max_path"]))
+    pt.close()
+    for J0, J2, nc, mp in out[1:]:
+        np.testing.assert_array_equal(J0, out[0][0])
+        np.testing.assert_array_equal(J2, out[0][1])
+        assert nc == out[0][2] and mp == out[0][3]
+
+
 def test_se_pass_fp64_parity(portal1, cfg1, oracle_cfg1):
     sp, g, xth = cfg1
     ppo, _, _, rates_o, jj_o = oracle_cfg1
@@ -151,7 +174,7 @@ def test_full_size_subset_parity(oracle):
     pt.close()
 
 
-KERNELS = [(0, 4), (1, 4), (1, 2), (2, 2), (2, 4), (2, 8)]
+KERNELS = [(0, 4), (1, 4), (1, 2), (2, 2), (2, 4), (2, 5), (2, 8)]
 
 
 @pytest.mark.parametrize("variant,waves", KERNELS)
