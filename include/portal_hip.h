@@ -177,12 +177,16 @@ int portal_lambda_iterate(portal_ctx *ctx, const portal_se_params *p, const port
 
 /* Batch of independent models (no reference counterpart): one context per model,
  * all on one device, all with the same precision.  Every iteration is one replay of
- * a captured hipGraph holding each model's line preparation, ONE persistent inward
- * integration over all (model, node) items ordered by cost, ONE persistent SE solve
- * over all items and each model's closure.  Iterates until every model meets the
+ * a captured hipGraph holding each model's line preparation, the persistent inward
+ * integration over all (model, node) items ordered by cost (one launch per class of
+ * models with the same slot count), ONE persistent SE solve over all items and each
+ * model's closure.  it->ng_accel = 1 applies Ng acceleration per model between
+ * replays; Anderson acceleration, the own-cell operator and the surrogate are for
+ * single models (PORTAL_ERR_UNSUPPORTED).  Iterates until every model meets the
  * tolerances (or max_iter); per_model (n, may be NULL) gets each model's iteration
- * count at convergence, total the device time of the whole batch.  Results stay in
- * each context (portal_get_pol_pops / portal_get_levpop). */
+ * count at convergence and whether it converged, total the device time of the whole
+ * batch.  A further call continues each model's iteration.  Results stay in each
+ * context (portal_get_pol_pops / portal_get_levpop). */
 int portal_batch_lambda_iterate(portal_ctx **ctxs, int n_models, const portal_se_params *params,
                                 const portal_iter_params *it, portal_stats *per_model, portal_stats *total);
 

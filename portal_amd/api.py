@@ -168,6 +168,10 @@ class _Core:
         n = self.lib.portal_debug_iter_trace(self.ctx, _d(out), max_iter)
         return out[: 2 * n].reshape(n, 2)
 
+    def ng_schedule(self, first=3, gap=3):
+        """Test hook: first iteration and least gap of the Ng extrapolations."""
+        self._check(self.lib.portal_debug_ng_schedule(self.ctx, int(first), int(gap)))
+
     def walk_chunks(self, scale):
         """Test hook: size the single-pass walk's chunk store at scale x its sampled estimate
         (a small scale forces the second-walk fallback, scale <= 0 the two-pass walk)."""
@@ -284,13 +288,13 @@ class Portal(SurrogateMixin, _Core):
 
 
 def batch_lambda_iterate(portals, xths, nvels, dx, max_iter, tol_pop=1e-6, tol_align=1e-8, n_depth=4096,
-                         precision=L.PORTAL_MIXED, rebuild_paths=0):
+                         precision=L.PORTAL_MIXED, rebuild_paths=0, ng_accel=0):
     """portal_batch_lambda_iterate over several Portal contexts (one model each)."""
     n = len(portals)
     lib = portals[0].lib
     arr = (C.c_void_p * n)(*[p.ctx.value for p in portals])
     ps = (L.SeParams * n)(*[p._se_params(x, nv, dx, n_depth, 0, precision) for p, x, nv in zip(portals, xths, nvels)])
-    it = L.IterParams(max_iter, tol_pop, tol_align, 0, 0.0, rebuild_paths)
+    it = L.IterParams(max_iter, tol_pop, tol_align, 0, 0.0, rebuild_paths, ng_accel)
     per = (L.Stats * n)()
     tot = L.Stats()
     rc = lib.portal_batch_lambda_iterate(arr, n, ps, C.byref(it), per, C.byref(tot))

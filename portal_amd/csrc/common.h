@@ -135,6 +135,7 @@ struct Ctx {
     double tune_ms[2][8] = {{0}};
     int forced_variant[2] = {-1, -1}, forced_waves[2] = {4, 4};  // test hook: bypass the autotuner
     double walk_chunk_scale = 1.1;  // chunk store of the single-pass walk / sampled estimate
+    int ng_first = 3, ng_gap = 3;   // Ng schedule: first iteration, least gap between extrapolations
     // d_pol holds the SE solution from which the current d_levp was formed (set by the
     // lambda iteration; cleared by anything that changes the state otherwise), so that
     // a lambda iteration continued by a further call measures its first change exactly

@@ -193,13 +193,14 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
     }
     // One walk writes the entries into chunks taken from a counter, and a compaction
     // lays them out contiguously per path.  The chunk store is sized from a sampled count
-    // over about 4,096 work items (+10% and a margin); if it still overflows, or cannot be
-    // allocated, the entries are written by a second walk at the exact offsets.
+    // over 4,096 to 8,192 work items (+10% and a margin); if it still overflows, or cannot
+    // be allocated, the entries are written by a second walk at the exact offsets.
     bool chunked = false;
     long long n_chunks = 0;
     unsigned long long *d_next = nullptr;
-    if (n > 0 && n < (1LL << 31) && c.walk_chunk_scale > 0) {
-        a.sample = std::max(1, nw / 4096);
+    // (at most 8,192 work items: the two-pass walk, which costs no more there)
+    if (n > 0 && n < (1LL << 31) && c.walk_chunk_scale > 0 && nw > 8192) {
+        a.sample = nw / 4096;
         const long long ns = (long long)((nw + a.sample - 1) / a.sample) * D;
         HIP_TRY(&c, launch_walk(a, WALK_COUNT, c.stream));
         std::vector<int> cs(ns);
@@ -980,6 +981,15 @@ int portal_debug_iter_trace(const portal_ctx *x, double *out, int max_iter) {
 // Test hook: the chunk store of the single-pass walk is sized at scale x the sampled
 // estimate (default 1.1); a small scale forces the overflow fallback, scale <= 0 the
 // two-pass walk.  The next path build uses it.
+// Test hook: the Ng schedule of the single-model iteration, the first iteration
+// (0-based, at least 3) and the least gap between extrapolations (at least 3).
+int portal_debug_ng_schedule(portal_ctx *x, int first, int gap) {
+    if (!x || first < 3 || gap < 3) return PORTAL_ERR_ARG;
+    x->c.ng_first = first;
+    x->c.ng_gap = gap;
+    return PORTAL_OK;
+}
+
 int portal_debug_walk_chunks(portal_ctx *x, double scale) {
     if (!x) return PORTAL_ERR_ARG;
     x->c.walk_chunk_scale = scale;
@@ -1197,7 +1207,7 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
         }
         if (ng) {
             // history x1 = hist[0] (newest) .. x3 = hist[2]; x0 is the level densities just formed
-            if (nhist == 3 && k >= 3) {
+            if (nhist == 3 && k >= c.ng_first && (last_ng < 0 || k - last_ng >= c.ng_gap)) {
                 double sums[5];
                 HIP_TRY(&c, launch_ng_dots(nw, (int)L, c.g.d_work, c.d_levp, hist[0], hist[1], hist[2], d_ngp,
                                            d_ngp + ng_partial_doubles() - 8, c.stream));
@@ -1466,6 +1476,10 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     Ctx &c0 = xs[0]->c;
     hipSetDevice(c0.device);
     if (it->surrogate) return fail(c0, PORTAL_ERR_UNSUPPORTED, "the surrogate iterate is not available for batches");
+    if (it->ng_accel < 0 || it->ng_accel > 1)
+        return fail(c0, it->ng_accel == 2 ? PORTAL_ERR_UNSUPPORTED : PORTAL_ERR_ARG,
+                    "a batch takes ng_accel 0 or 1 (Anderson acceleration is for single models)");
+    if (it->ali) return fail(c0, PORTAL_ERR_UNSUPPORTED, "the own-cell operator is not available for batches");
     const bool mixed = params[0].precision == PORTAL_MIXED;
     // a model whose last SE solution formed its current level densities continues that
     // iteration: its first pass here is compared with that solution
@@ -1740,6 +1754,26 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     std::vector<double> last(n, 0.0), last_al(n, 0.0);
     int iters = 0;
     double t_iter = 0;
+    // Ng acceleration per model between replays, on the same schedule as the
+    // single-model iteration: from the fourth iterate on, every third
+    const bool ng = it->ng_accel == 1;
+    std::vector<double *> hist(3 * (size_t)n, nullptr);
+    double *d_ngp = nullptr, *d_ngs = nullptr;
+    auto free_ng = [&]() {
+        for (auto *&p : hist) dfree(p);
+        dfree(d_ngp);
+        dfree(d_ngs);
+    };
+    if (ng) {
+        for (int m = 0; m < n; m++)
+            for (int h = 0; h < 3; h++)
+                HIP_TRY(&c0, hipMalloc((void **)&hist[3 * m + h],
+                                       sizeof(double) * xs[m]->c.g.N * xs[m]->c.sp.e_num));
+        HIP_TRY(&c0, hipMalloc((void **)&d_ngp, sizeof(double) * ng_partial_doubles()));
+        HIP_TRY(&c0, hipMalloc((void **)&d_ngs, sizeof(double) * 8 * n));
+    }
+    int nhist = 0, n_ng = 0, last_ng = -1;
+    std::vector<double> sums(8 * (size_t)n);
     for (int k = 0; k < it->max_iter; k++) {
         Timer ti(s);
         HIP_TRY(&c0, hipGraphLaunch(exec, s));
@@ -1758,7 +1792,46 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
             all = all && conv_at[m];
         }
         if (all) break;
+        if (ng) {
+            // history x1 = hist[3m] (newest) .. x3 = hist[3m+2]; x0 the level densities just formed
+            bool applied = false;
+            if (nhist == 3 && k >= 3) {
+                for (int m = 0; m < n; m++) {
+                    Ctx &c = xs[m]->c;
+                    HIP_TRY(&c0, launch_ng_dots((int)c.g.work.size(), c.sp.e_num, c.g.d_work, c.d_levp, hist[3 * m],
+                                                hist[3 * m + 1], hist[3 * m + 2], d_ngp, d_ngs + 8 * m, s));
+                }
+                HIP_TRY(&c0, hipMemcpyAsync(sums.data(), d_ngs, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, s));
+                HIP_TRY(&c0, hipStreamSynchronize(s));
+                for (int m = 0; m < n; m++) {
+                    Ctx &c = xs[m]->c;
+                    const double *q = &sums[8 * (size_t)m];
+                    const double a11 = q[0], a12 = q[1], a22 = q[2], b1 = q[3], b2 = q[4];
+                    const double det = a11 * a22 - a12 * a12;
+                    if (!(std::isfinite(det) && det > 1e-14 * a11 * a22 && a11 > 0 && a22 > 0)) continue;
+                    const double ca = (b1 * a22 - b2 * a12) / det, cb = (b2 * a11 - b1 * a12) / det;
+                    if (!(std::isfinite(ca) && std::isfinite(cb))) continue;
+                    HIP_TRY(&c0, launch_ng_apply((int)c.g.work.size(), c.sp.e_num, c.g.d_work, c.d_levp, hist[3 * m],
+                                                 hist[3 * m + 1], ca, cb, s));
+                    applied = true;
+                }
+                if (applied) {
+                    n_ng++;
+                    last_ng = k;
+                }
+                nhist = 0;  // the extrapolated iterate starts a new history
+            }
+            for (int m = 0; m < n; m++) {
+                Ctx &c = xs[m]->c;
+                std::swap(hist[3 * m + 2], hist[3 * m + 1]);
+                std::swap(hist[3 * m + 1], hist[3 * m]);
+                HIP_TRY(&c0, hipMemcpyAsync(hist[3 * m], c.d_levp, sizeof(double) * c.g.N * c.sp.e_num,
+                                            hipMemcpyDeviceToDevice, s));
+            }
+            nhist = std::min(nhist + 1, 3);
+        }
     }
+    free_ng();
     int flags = 0;
     HIP_TRY(&c0, hipMemcpy(&flags, c0.d_flags, sizeof(int), hipMemcpyDeviceToHost));
     hipGraphExecDestroy(exec);
@@ -1780,7 +1853,8 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
         ncross += c.paths.total;
-        c.pol_chain = true;  // d_pol formed d_levp in the last pass
+        // d_pol formed d_levp in the last pass, unless an extrapolation followed it
+        c.pol_chain = last_ng < iters - 1;
         all_conv = all_conv && conv_at[m];
         if (per_model) {
             per_model[m].converged = conv_at[m] ? 1 : 0;
@@ -1798,6 +1872,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         total->n_cross = ncross;
         total->rt_waves = best_w;
         total->rt_variant = coop ? 2 : 1;
+        total->ng_steps = n_ng;
         total->max_path = cap;
         total->converged = all_conv ? 1 : 0;
     }

@@ -1,7 +1,7 @@
 """The other reference configurations, run on one GPU and reported as JSON lines.
 
     python -m portal_amd.workloads small        # 4,000 nodes, 4 levels: FP64, then mixed with its errors
-    python -m portal_amd.workloads batch        # 64 independent clouds in one batched iteration
+    python -m portal_amd.workloads batch        # 64 independent clouds in one batched iteration (Ng)
     python -m portal_amd.workloads million      # the 4,000-node recipe at 1e6 nodes with the surrogate
 
 `small` runs the 4,000-node cloud (48 directions, 65 channels, 30 iterations, a 129 x 129
@@ -99,7 +99,7 @@ def small(seed=1):
     return out
 
 
-def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32):
+def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True):
     rng = np.random.default_rng(seed)
     xth = cloud.legendre_all(nth, nph)
     pts, sizes = [], []
@@ -126,7 +126,7 @@ def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32):
     while True:
         per, tot = batch_lambda_iterate(pts, [xth] * n_models, [nvel] * n_models, cloud.DX_INWARD, 10,
                                         tol_pop=TOL_POP, tol_align=TOL_ALIGN, precision=PORTAL_MIXED,
-                                        rebuild_paths=1 if done == 0 else 0)
+                                        rebuild_paths=1 if done == 0 else 0, ng_accel=1 if ng else 0)
         for m, p in enumerate(per):
             if not iters_at[m] and p["converged"]:
                 iters_at[m] = done + p["iterations"]
@@ -141,7 +141,8 @@ def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32):
     t = time.perf_counter() - t
     n_solved = sum(s[2] for s in sizes)
     out = {"workload": f"{n_models} clouds, nodes log-uniform in [5e3, 1e5], 4..20 levels, "
-                       f"{xth.shape[1]} directions, {2 * nvel + 1} channels, mixed, to {TOL_POP:g}/{TOL_ALIGN:g}",
+                       f"{xth.shape[1]} directions, {2 * nvel + 1} channels, mixed{' + Ng' if ng else ''}, "
+                       f"to {TOL_POP:g}/{TOL_ALIGN:g}",
            "nodes": n_solved, "seconds": t, "converged_nodes_per_s": n_solved / t,
            "iterations_max": done, "all_converged": bool(all(iters_at)),
            "iterations_per_model": [v if v else done for v in iters_at],
@@ -204,8 +205,9 @@ def million(n_int=1000000, seed=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("which", choices=["small", "batch", "million"])
+    ap.add_argument("--no-ng", action="store_true", help="batch: plain iteration")
     a = ap.parse_args()
-    res = {"small": small, "batch": batch, "million": million}[a.which]()
+    res = batch(ng=not a.no_ng) if a.which == "batch" else {"small": small, "million": million}[a.which]()
     print(json.dumps(res), flush=True)
 
 

@@ -112,3 +112,31 @@ def test_batch_continued_calls_are_one_iteration():
     assert at == [p["iterations"] for p in per]
     for pt in a + b + c + d:
         pt.close()
+
+
+def test_batch_ng_reaches_the_plain_fixed_point():
+    # Ng acceleration per model in the batch: fewer iterations, same fixed point
+    models = _models()
+    xth = cloud.legendre_all(4, 8)
+    kw = dict(tol_pop=1e-7, tol_align=1e-9)
+    out = []
+    for ng in (0, 1):
+        pts = []
+        for sp, g in models:
+            pt = Portal(0)
+            pt.upload_grid(g)
+            pt.upload_species(sp)
+            pt.set_levpop(g.lev_p)
+            pts.append(pt)
+        per, tot = batch_lambda_iterate(pts, [xth] * 3, [8] * 3, cloud.DX_INWARD, 300, ng_accel=ng, **kw)
+        assert tot["converged"]
+        out.append((tot, [pt.get_levpop() for pt in pts]))
+        for pt in pts:
+            pt.close()
+    (t0, lev0), (t1, lev1) = out
+    print(f"batch: plain {t0['iterations']} iterations, Ng {t1['iterations']} ({t1['ng_steps']} extrapolations)")
+    assert t1["ng_steps"] > 0 and t1["iterations"] < t0["iterations"]
+    for (sp, g), a, b in zip(models, lev0, lev1):
+        idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+        big = a[:, idx] > 1e-8 * a[:, idx].max(axis=0)
+        assert (np.abs(b[:, idx] - a[:, idx])[big] / a[:, idx][big]).max() < 2e-5
