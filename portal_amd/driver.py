@@ -28,8 +28,15 @@ def rescale_pol_pops(pol_pops, lev_p, lev_g):
     return pp
 
 
-def image_lines(sp, l1, l2):
-    """[(l1, l2, nu0, Aji, kd)] for the imaged lines (1-based levels)."""
+def image_lines(sp, l1, l2, nu0_overwrite=True, aji_by_level=False):
+    """[(l1, l2, nu0, Aji, kd)] for the imaged lines (1-based levels).
+
+    The defaults reproduce the image driver: the frequency loop of para_finish.f90:173-175
+    leaves every line with the last line's frequency.  nu0_overwrite=False gives each
+    line its own frequency instead.  aji_by_level=True takes A as the reference's pix = 0
+    branch does, t_Aji(l1(j)), indexing the transition list by the lower level
+    (para_finish.f90:303; the grid branch uses the line's own A, 422).
+    """
     nu0_last = sp.e_levs_hz[l2[-1] - 1] - sp.e_levs_hz[l1[-1] - 1]
     out = []
     for a, b in zip(l1, l2):
@@ -38,7 +45,10 @@ def image_lines(sp, l1, l2):
             if sp.r_trans[1, j] == b and sp.r_trans[0, j] == a:
                 A, kd = sp.t_Aji[j], sp.kd_v[j]
                 break
-        out.append((a, b, nu0_last, A, kd))
+        if aji_by_level:
+            A = sp.t_Aji[a - 1]
+        nu0 = nu0_last if nu0_overwrite else sp.e_levs_hz[b - 1] - sp.e_levs_hz[a - 1]
+        out.append((a, b, nu0, A, kd))
     return out
 
 

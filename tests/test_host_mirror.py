@@ -71,6 +71,15 @@ def test_image_lines_keep_the_nu0_overwrite():
     assert lines[0][3] == sp.t_Aji[0] and lines[1][3] == sp.t_Aji[1]
     v = driver.channel_velocities(2, 10.0, 1.0)
     assert list(v) == [-19.0, -9.0, 1.0, 11.0, 21.0]
+    # the two policies: each line its own frequency; A indexed by the lower level as the
+    # pix = 0 branch does (para_finish.f90:303)
+    own = driver.image_lines(sp, [1, 3], [2, 4], nu0_overwrite=False)
+    assert own[0][2] == sp.e_levs_hz[1] - sp.e_levs_hz[0] and own[1][2] == sp.e_levs_hz[3] - sp.e_levs_hz[2]
+    assert own[1][3] == sp.t_Aji[2]
+    bylev = driver.image_lines(sp, [1, 3], [2, 4], aji_by_level=True)
+    assert bylev[0][3] == sp.t_Aji[0] and bylev[1][3] == sp.t_Aji[2]
+    shifted = driver.image_lines(sp, [2], [3], aji_by_level=True)
+    assert shifted[0][3] == sp.t_Aji[1]
 
 
 def test_box_delaunay_equals_whole_set():
