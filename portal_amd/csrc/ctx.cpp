@@ -959,8 +959,8 @@ static int surrogate_step(Ctx &c, int *n_fallback) {
 }
 
 // the autotuner's timings of the inward-integration candidates (variant/waves: global 4,
-// staged 4, staged 2, cooperative 2, 4, 8; two unused slots) for one precision; -1 for a
-// candidate that could not run
+// staged 4, staged 2, cooperative 2, 4, 5, 8; one unused slot) for one precision; -1 for
+// a candidate that could not run
 int portal_debug_tune_ms(const portal_ctx *x, int precision, double *out8) {
     if (!x || !out8 || precision < 0 || precision > 1) return PORTAL_ERR_ARG;
     for (int k = 0; k < 8; k++) out8[k] = x->c.tune_ms[precision][k];
@@ -976,11 +976,6 @@ int portal_debug_iter_trace(const portal_ctx *x, double *out, int max_iter) {
     return n;
 }
 
-// test hook: run the inward integration of one precision with a fixed kernel variant
-// (0 global, 1 staged, 2 cooperative) and wave count; variant -1 returns to autotuning
-// Test hook: the chunk store of the single-pass walk is sized at scale x the sampled
-// estimate (default 1.1); a small scale forces the overflow fallback, scale <= 0 the
-// two-pass walk.  The next path build uses it.
 // Test hook: the Ng schedule of the single-model iteration, the first iteration
 // (0-based, at least 3) and the least gap between extrapolations (at least 3).
 int portal_debug_ng_schedule(portal_ctx *x, int first, int gap) {
@@ -990,6 +985,9 @@ int portal_debug_ng_schedule(portal_ctx *x, int first, int gap) {
     return PORTAL_OK;
 }
 
+// Test hook: the chunk store of the single-pass walk is sized at scale x the sampled
+// estimate (default 1.1); a small scale forces the overflow fallback, scale <= 0 the
+// two-pass walk.  The next path build uses it.
 int portal_debug_walk_chunks(portal_ctx *x, double scale) {
     if (!x) return PORTAL_ERR_ARG;
     x->c.walk_chunk_scale = scale;
@@ -997,6 +995,8 @@ int portal_debug_walk_chunks(portal_ctx *x, double scale) {
     return PORTAL_OK;
 }
 
+// Test hook: run the inward integration of one precision with a fixed kernel variant
+// (0 global, 1 staged, 2 cooperative) and wave count; variant -1 returns to autotuning.
 int portal_debug_force_rt(portal_ctx *x, int precision, int variant, int waves) {
     if (!x || precision < 0 || precision > 1 || variant < -1 || variant > 2 || waves < 1 || waves > 8)
         return PORTAL_ERR_ARG;
