@@ -168,6 +168,10 @@ class _Core:
         n = self.lib.portal_debug_iter_trace(self.ctx, _d(out), max_iter)
         return out[: 2 * n].reshape(n, 2)
 
+    def coop_unit(self, cells=0):
+        """Test hook: cells per staged unit of the cooperative kernel (0: the default)."""
+        self._check(self.lib.portal_debug_coop_unit(self.ctx, int(cells)))
+
     def ng_schedule(self, first=3, gap=3):
         """Test hook: first iteration and least gap of the Ng extrapolations."""
         self._check(self.lib.portal_debug_ng_schedule(self.ctx, int(first), int(gap)))

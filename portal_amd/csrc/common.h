@@ -136,6 +136,7 @@ struct Ctx {
     int forced_variant[2] = {-1, -1}, forced_waves[2] = {4, 4};  // test hook: bypass the autotuner
     double walk_chunk_scale = 1.1;  // chunk store of the single-pass walk / sampled estimate
     int ng_first = 3, ng_gap = 3;   // Ng schedule: first iteration, least gap between extrapolations
+    int coop_unit = 0;              // cells per staged unit of the cooperative kernel (0: COOP_UNIT)
     // d_pol holds the SE solution from which the current d_levp was formed (set by the
     // lambda iteration; cleared by anything that changes the state otherwise), so that
     // a lambda iteration continued by a further call measures its first change exactly
@@ -190,6 +191,7 @@ struct WalkArgs {
 // (counts and entries in chunks, one pass)
 enum WalkMode : int { WALK_COUNT = 0, WALK_WRITE = 1, WALK_CHUNKED = 2 };
 constexpr int WALK_CHUNK = 16;  // entries per chunk
+constexpr int COOP_UNIT = 64;   // cells per staged unit of the cooperative inward integration
 hipError_t launch_walk(const WalkArgs &a, int mode, hipStream_t s);
 hipError_t launch_walk_compact(const WalkArgs &a, long long n_chunks, hipStream_t s);
 int build_star_dirs(const std::vector<int> &work, const double *XYZ, const double *bvec_all, const double *gvec,
@@ -246,6 +248,7 @@ struct RtArgs {
     // the node's own cell in its J^0_0 per unit source function, sum over rays and channels
     // of w phi0 d_v (1 - e^-tau_own) (R x N), or null
     double *LAM;
+    int unit;               // cooperative kernel: cells per staged unit (0: COOP_UNIT)
 };
 hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s);
 hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s);

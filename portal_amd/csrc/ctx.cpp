@@ -432,6 +432,7 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
         dfree(d_Istar);
         return rc;
     }
+    ra.unit = c.coop_unit;
     autotune_rt(c, ra, mixed);
     ra.variant = c.tuned_variant[mixed ? 1 : 0];
     ra.waves = c.tuned_waves[mixed ? 1 : 0];
@@ -992,6 +993,14 @@ int portal_debug_walk_chunks(portal_ctx *x, double scale) {
     if (!x) return PORTAL_ERR_ARG;
     x->c.walk_chunk_scale = scale;
     x->c.paths.valid = false;
+    return PORTAL_OK;
+}
+
+// Test hook: cells per staged unit of the cooperative kernel (0: the default, 64).
+int portal_debug_coop_unit(portal_ctx *x, int cells) {
+    if (!x || cells < 0) return PORTAL_ERR_ARG;
+    x->c.coop_unit = cells;
+    x->c.tuned_key[0] = x->c.tuned_key[1] = 0;
     return PORTAL_OK;
 }
 
@@ -1638,7 +1647,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
                 classes.back().slots = sl;
             }
             CoopClass &k = classes[cls_of[m]];
-            k.cap = std::max(k.cap, rts[m].max_nc);
+            k.cap = std::max(k.cap, std::min(rts[m].max_nc, COOP_UNIT));
             k.R_max = std::max(k.R_max, rts[m].R);
             k.C_max = std::max(k.C_max, rts[m].C);
         }

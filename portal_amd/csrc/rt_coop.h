@@ -36,26 +36,29 @@ __host__ __device__ constexpr size_t coop_buf_bytes(int cap, int R, bool dust) {
            ~(size_t)15;
 }
 
+// the two staging buffers, or the reduction of J0, J2 (and the own-cell operator) per
+// (line, channel), whichever is larger
 template <typename Real>
-size_t coop_lds_bytes(int cap, int R, int npairs, bool dust) {
+size_t coop_lds_bytes(int cap, int R, int npairs, bool dust, bool lam = true) {
     const size_t stage = 2 * coop_buf_bytes<Real>(cap, R, dust);
-    const size_t red = sizeof(double) * 3 * (size_t)npairs;
+    const size_t red = sizeof(double) * (lam ? 3 : 2) * (size_t)npairs;
     return stage > red ? stage : red;
 }
 
-// All threads of the workgroup stage ray r of work item bid into buf.
+// All threads of the workgroup stage cells c0 .. c0+cnt-1 of ray r of work item bid
+// into buf (cnt <= cap).
 template <typename Real, bool DUST>
-__device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const int r, const int cap, char *buf) {
+__device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const int r, const int c0, const int cnt,
+                                           const int cap, char *buf) {
     typedef Arith<Real> A;
     typedef typename A::R2 R2;
     const int R = a.R;
     const long long off = a.offsets[(long long)bid * a.D + r];
-    const int nc = (int)(a.offsets[(long long)bid * a.D + r + 1] - off);
-    const PathEntry *P = a.paths + off;
+    const PathEntry *P = a.paths + off + c0;
     CellRec<Real> *cr = reinterpret_cast<CellRec<Real> *>(buf);
     R2 *ks = reinterpret_cast<R2 *>(cr + cap);
     const Real dls = (Real)a.dl_scale;
-    const int n = nc * R;
+    const int n = cnt * R;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const int i = j / R, l = j - i * R;
         const PathEntry e = P[i];
@@ -116,111 +119,149 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
         S0[q] = C0[q] = S2[q] = C2[q] = Lw[q] = (Real)0;
     }
 
-    coop_stage<Real, DUST>(a, bid, 0, cap, smem);
+    // Rays are staged in units of at most cap cells, so that the staging area does not
+    // grow with the longest path; unit u of ray r holds its cells u cap .. u cap + cap - 1.
+    auto ray_cells = [&](int r) {
+        const long long o = (long long)bid * D + r;
+        return (int)(a.offsets[o + 1] - a.offsets[o]);
+    };
+    auto stage_unit = [&](int r, int u, char *buf) {
+        const int nc = ray_cells(r);
+        const int c0 = u * cap;
+        coop_stage<Real, DUST>(a, bid, r, c0, min(cap, nc - c0), cap, buf);
+    };
+    // per-ray state, carried across the units of a ray
+    Real Tr[SLOTS], Em[SLOTS], v0s[SLOTS];
+    Real w = (Real)0, wc = (Real)0, iin = (Real)0;
+    bool ray_done = false;
+    stage_unit(0, 0, smem);
     __syncthreads();
-    for (int r = 0; r < D; r++) {
-        char *cur = smem + (r & 1) * bb;
-        if (r + 1 < D) coop_stage<Real, DUST>(a, bid, r + 1, cap, smem + ((r + 1) & 1) * bb);
-        // star mode: the node's own directions; rays towards the star start with B_nu(T_star)
-        const double *xr = a.xthn ? a.xthn + 3 * ((size_t)bid * D + r) : a.xth + 3 * r;
-        const Real mu = (Real)xr[0], w = (Real)xr[2];
-        const Real iin = r < n_star ? iin_star : iin_bg;
-        const Real wc = w * (((Real)3 * (mu * mu) - (Real)1) / (Real)2);
-        const long long off = a.offsets[(long long)bid * D + r];
-        const int nc = (int)(a.offsets[(long long)bid * D + r + 1] - off);
+    int r = 0, u = 0, half = 0;
+    for (;;) {
+        const char *cur = smem + half * bb;
+        const int nc = ray_cells(r);
+        const int nu = max(1, (nc + cap - 1) / cap);
+        int rn = r, un = u + 1;
+        if (un >= nu) {
+            rn = r + 1;
+            un = 0;
+        }
+        if (rn < D) stage_unit(rn, un, smem + (half ^ 1) * bb);
         const CellRec<Real> *cr = reinterpret_cast<const CellRec<Real> *>(cur);
         const R2 *ks = reinterpret_cast<const R2 *>(cr + cap);
         const R2 *kds = ks + (size_t)cap * R;
-        const Real vel_node = cr[0].pad;
-        const Real dl0 = cr[0].dl;
-        Real Tr[SLOTS], Em[SLOTS], v0s[SLOTS];
-#pragma unroll
-        for (int q = 0; q < SLOTS; q++) {
-            if (DUST) {
-                // S = (e phi + j)/(k phi + alpha), tau = (k phi + alpha) dl (inverse_rt.f90:289-297)
-                const Real kp = k0.x * e0[q];
-                const Real tau0 = (kp + d0.x) * dl0;
-                const Real Sc = (k0.y * kp + d0.y * d0.x) / (kp + d0.x);
-                Tr[q] = A::ex(-tau0);
-                const Real om0 = A::one_minus(tau0, Tr[q]);
-                Em[q] = Sc * om0;
-                if (lam_on) Lw[q] = Lw[q] + w * om0;
-            } else {
-                const Real tau0 = k0.x * e0[q] * dl0;
-                Tr[q] = A::ex(-tau0);
-                const Real om0 = A::one_minus(tau0, Tr[q]);
-                Em[q] = k0.y * om0;
-                // the own cell's share of J per unit of its source function
-                if (lam_on) Lw[q] = Lw[q] + w * om0;
-            }
-            v0s[q] = v0[q] + vel_node;
-            if (!on[q]) Tr[q] = (Real)0;
-        }
-        // the next cell's staged data are read one cell ahead
-        CellRec<Real> cn = cr[nc > 1 ? 1 : 0];
-        R2 kn = ks[(size_t)(nc > 1 ? 1 : 0) * R + lo], dn;
-        if (DUST) dn = kds[(size_t)(nc > 1 ? 1 : 0) * R + lo];
-        for (int i = 1; i < nc; i++) {
-            if ((i & 3) == 0) {
-                bool live = false;
-#pragma unroll
-                for (int q = 0; q < SLOTS; q++) {
-                    if (!(Tr[q] > (Real)A::tr_floor)) Tr[q] = (Real)0;
-                    live = live || (Tr[q] != (Real)0);
-                }
-                if (!__any(live)) break;
-            }
-            const CellRec<Real> c = cn;
-            const R2 k = kn;
-            R2 d;
-            if (DUST) d = dn;
-            const int inext = i + 1 < nc ? i + 1 : i;
-            cn = cr[inext];
-            kn = ks[(size_t)inext * R + lo];
-            if (DUST) dn = kds[(size_t)inext * R + lo];
+        const int c0 = u * cap, cnt = min(cap, nc - c0);
+        if (u == 0) {
+            // star mode: the node's own directions; rays towards the star start with B_nu(T_star)
+            const double *xr = a.xthn ? a.xthn + 3 * ((size_t)bid * D + r) : a.xth + 3 * r;
+            const Real mu = (Real)xr[0];
+            w = (Real)xr[2];
+            iin = r < n_star ? iin_star : iin_bg;
+            wc = w * (((Real)3 * (mu * mu) - (Real)1) / (Real)2);
+            const Real vel_node = cr[0].pad;
+            const Real dl0 = cr[0].dl;
+            ray_done = false;
 #pragma unroll
             for (int q = 0; q < SLOTS; q++) {
-                const Real u = fma(v0s[q], c.inv_b, -c.vb);
-                Real tau = A::tau_of(k.x, u), S = k.y;
                 if (DUST) {
-                    // tau = tau_line + alpha dl; S the opacity-weighted mean of the line's
-                    // and the dust's source functions (inverse_rt.f90:273-287)
-                    const Real tl = tau;
-                    tau = tl + d.x;
-                    S = (k.y * tl + d.y * d.x) * A::rcp(tau);
-                    if (!A::skip_test && !(tau > (Real)0)) S = (Real)0;
-                }
-                const Real et = A::att(tau);
-                const Real em = Em[q] + Tr[q] * S * A::om(tau, et);
-                const Real tr = Tr[q] * et;
-                if constexpr (A::skip_test) {
-                    const bool ok = tau > A::thr();
-                    Em[q] = ok ? em : Em[q];
-                    Tr[q] = ok ? tr : Tr[q];
+                    // S = (e phi + j)/(k phi + alpha), tau = (k phi + alpha) dl (inverse_rt.f90:289-297)
+                    const Real kp = k0.x * e0[q];
+                    const Real tau0 = (kp + d0.x) * dl0;
+                    const Real Sc = (k0.y * kp + d0.y * d0.x) / (kp + d0.x);
+                    Tr[q] = A::ex(-tau0);
+                    const Real om0 = A::one_minus(tau0, Tr[q]);
+                    Em[q] = Sc * om0;
+                    if (lam_on) Lw[q] = Lw[q] + w * om0;
                 } else {
-                    Em[q] = em;
-                    Tr[q] = tr;
+                    const Real tau0 = k0.x * e0[q] * dl0;
+                    Tr[q] = A::ex(-tau0);
+                    const Real om0 = A::one_minus(tau0, Tr[q]);
+                    Em[q] = k0.y * om0;
+                    // the own cell's share of J per unit of its source function
+                    if (lam_on) Lw[q] = Lw[q] + w * om0;
+                }
+                v0s[q] = v0[q] + vel_node;
+                if (!on[q]) Tr[q] = (Real)0;
+            }
+        }
+        if (!ray_done) {
+            // cells max(c0, 1) .. c0 + cnt - 1 of the ray (local index j = i - c0); the
+            // next cell's staged data are read one cell ahead, within the unit
+            const int jb = c0 == 0 ? 1 : 0;
+            CellRec<Real> cn = cr[jb < cnt ? jb : 0];
+            R2 kn = ks[(size_t)(jb < cnt ? jb : 0) * R + lo], dn;
+            if (DUST) dn = kds[(size_t)(jb < cnt ? jb : 0) * R + lo];
+            for (int j = jb; j < cnt; j++) {
+                if (((c0 + j) & 3) == 0) {
+                    bool live = false;
+#pragma unroll
+                    for (int q = 0; q < SLOTS; q++) {
+                        if (!(Tr[q] > (Real)A::tr_floor)) Tr[q] = (Real)0;
+                        live = live || (Tr[q] != (Real)0);
+                    }
+                    if (!__any(live)) {
+                        ray_done = true;
+                        break;
+                    }
+                }
+                const CellRec<Real> c = cn;
+                const R2 k = kn;
+                R2 d;
+                if (DUST) d = dn;
+                const int jn = j + 1 < cnt ? j + 1 : j;
+                cn = cr[jn];
+                kn = ks[(size_t)jn * R + lo];
+                if (DUST) dn = kds[(size_t)jn * R + lo];
+#pragma unroll
+                for (int q = 0; q < SLOTS; q++) {
+                    const Real uu = fma(v0s[q], c.inv_b, -c.vb);
+                    Real tau = A::tau_of(k.x, uu), S = k.y;
+                    if (DUST) {
+                        // tau = tau_line + alpha dl; S the opacity-weighted mean of the line's
+                        // and the dust's source functions (inverse_rt.f90:273-287)
+                        const Real tl = tau;
+                        tau = tl + d.x;
+                        S = (k.y * tl + d.y * d.x) * A::rcp(tau);
+                        if (!A::skip_test && !(tau > (Real)0)) S = (Real)0;
+                    }
+                    const Real et = A::att(tau);
+                    const Real em = Em[q] + Tr[q] * S * A::om(tau, et);
+                    const Real tr = Tr[q] * et;
+                    if constexpr (A::skip_test) {
+                        const bool ok = tau > A::thr();
+                        Em[q] = ok ? em : Em[q];
+                        Tr[q] = ok ? tr : Tr[q];
+                    } else {
+                        Em[q] = em;
+                        Tr[q] = tr;
+                    }
                 }
             }
         }
+        if (u == nu - 1) {
 #pragma unroll
-        for (int q = 0; q < SLOTS; q++) {
-            const Real Iout = Em[q] + Tr[q] * iin;
-            if (A::kahan) {
-                Real y = w * Iout - C0[q];
-                Real t = S0[q] + y;
-                C0[q] = (t - S0[q]) - y;
-                S0[q] = t;
-                y = wc * Iout - C2[q];
-                t = S2[q] + y;
-                C2[q] = (t - S2[q]) - y;
-                S2[q] = t;
-            } else {
-                S0[q] = S0[q] + w * Iout;
-                S2[q] = S2[q] + wc * Iout;
+            for (int q = 0; q < SLOTS; q++) {
+                const Real Iout = Em[q] + Tr[q] * iin;
+                if (A::kahan) {
+                    Real y = w * Iout - C0[q];
+                    Real t2 = S0[q] + y;
+                    C0[q] = (t2 - S0[q]) - y;
+                    S0[q] = t2;
+                    y = wc * Iout - C2[q];
+                    t2 = S2[q] + y;
+                    C2[q] = (t2 - S2[q]) - y;
+                    S2[q] = t2;
+                } else {
+                    S0[q] = S0[q] + w * Iout;
+                    S2[q] = S2[q] + wc * Iout;
+                }
             }
         }
-        __syncthreads();  // ray r+1 staged; buffer of ray r free for ray r+2
+        __syncthreads();  // the next unit staged; this unit's buffer free for the one after
+        if (rn >= D) break;
+        r = rn;
+        u = un;
+        half ^= 1;
     }
 
     double *red0 = reinterpret_cast<double *>(smem);
@@ -328,8 +369,10 @@ template <typename Real>
 hipError_t launch_coop(const RtArgs &a, int cap, hipStream_t s) {
     const int W = a.waves;
     if (W < 1 || W > 8 || a.R < 1) return hipErrorInvalidValue;
+    const int unit = a.unit > 0 ? a.unit : COOP_UNIT;
+    cap = cap < unit ? (cap < 1 ? 1 : cap) : unit;  // cells per staged unit
     const int npairs = a.R * a.C;
-    const size_t lds = coop_lds_bytes<Real>(cap, a.R, npairs, a.dust != 0);
+    const size_t lds = coop_lds_bytes<Real>(cap, a.R, npairs, a.dust != 0, a.LAM != nullptr);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const int per_line = (64 * W) / a.R;
     if (per_line < 1) return hipErrorInvalidValue;

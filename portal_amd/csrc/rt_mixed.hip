@@ -169,7 +169,8 @@ hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int
 int coop_slots_host(int R, int C, int W) { return coop_slots(R, C, W); }
 
 size_t rt_coop_persistent_lds(bool mixed, int cap, int R_max, int C) {
-    return mixed ? coop_lds_bytes<float>(cap, R_max, R_max * C, false) : coop_lds_bytes<double>(cap, R_max, R_max * C, false);
+    return mixed ? coop_lds_bytes<float>(cap, R_max, R_max * C, false, false)
+                 : coop_lds_bytes<double>(cap, R_max, R_max * C, false, false);
 }
 
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves) {

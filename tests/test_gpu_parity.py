@@ -91,6 +91,30 @@ def test_walk_forms_agree():
         assert nc == out[0][2] and mp == out[0][3]
 
 
+@pytest.mark.parametrize("precision", [PORTAL_FP64, PORTAL_MIXED])
+def test_cooperative_units_agree(precision):
+    # the cooperative kernel stages rays in units of at most 64 cells; the unit size
+    # changes where the staging breaks a ray, not the arithmetic: J is identical
+    sp = cloud.co_like_rotor(6)
+    g = cloud.make_cloud(sp, 20000, seed=9)
+    xth = cloud.legendre_all(4, 8)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    pt.force_rt_kernel(precision, 2, 4)
+    out = []
+    for unit in (0, 8, 4096):
+        pt.coop_unit(unit)
+        J0, J2, st = pt.inverse_rt(xth, 16, cloud.DX_INWARD, precision=precision)
+        out.append((J0, J2))
+    assert st["max_path"] > 8  # the 8-cell unit splits most rays
+    pt.close()
+    for J0, J2 in out[1:]:
+        np.testing.assert_array_equal(J0, out[0][0])
+        np.testing.assert_array_equal(J2, out[0][1])
+
+
 def test_se_pass_fp64_parity(portal1, cfg1, oracle_cfg1):
     sp, g, xth = cfg1
     ppo, _, _, rates_o, jj_o = oracle_cfg1
