@@ -177,10 +177,11 @@ int portal_lambda_iterate(portal_ctx *ctx, const portal_se_params *p, const port
 
 /* Batch of independent models (no reference counterpart): one context per model,
  * all on one device, all with the same precision.  Every iteration is one replay of
- * a captured hipGraph holding each model's line preparation, the persistent inward
- * integration over all (model, node) items ordered by cost (one launch per class of
- * models with the same slot count), ONE persistent SE solve over all items and each
- * model's closure.  it->ng_accel = 1 applies Ng acceleration per model between
+ * a captured hipGraph holding each model's line preparation, the inward integration
+ * over all (model, node) items ordered by cost (one launch per class of models with the
+ * same slot count; persistent workgroups or one workgroup per item, whichever timed
+ * faster on the batch), ONE persistent SE solve over all items and each model's
+ * closure.  it->ng_accel = 1 applies Ng acceleration per model between
  * replays; Anderson acceleration, the own-cell operator and the surrogate are for
  * single models (PORTAL_ERR_UNSUPPORTED).  Iterates until every model meets the
  * tolerances (or max_iter); per_model (n, may be NULL) gets each model's iteration

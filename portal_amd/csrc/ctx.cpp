@@ -1688,6 +1688,37 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     HIP_TRY(&c0, hipMemset(c0.d_flags, 0, sizeof(int) * 4));
     int n_cu = 256;
     hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c0.device);
+    // the cooperative form: persistent workgroups or one workgroup per item, timed on the
+    // largest class's first items
+    bool coop_flat = false;
+    if (coop) {
+        size_t kb = 0;
+        for (size_t k = 1; k < classes.size(); k++)
+            if (classes[k].items.size() > classes[kb].items.size()) kb = k;
+        const CoopClass &cl = classes[kb];
+        const int ns = (int)std::min<size_t>(cl.items.size(), 8192);
+        for (int m = 0; m < n; m++) {
+            Ctx &c = xs[m]->c;
+            PrepArgs pa{c.g.N, c.sp.r_num, c.sp.e_num, c.d_levp, c.g.d_velb, c.sp.d_l1, c.sp.d_l2, c.sp.d_c1B,
+                        c.sp.d_c2A, nullptr, c.sp.d_g1, c.sp.d_g2, c.sp.d_cnu_spi, mixed ? nullptr : c.d_KS,
+                        mixed ? c.d_KS32 : nullptr};
+            HIP_TRY(&c0, launch_line_prep(pa, s));
+        }
+        double ms_form[2] = {0, 0};
+        for (int f = 0; f < 2; f++) {
+            HIP_TRY(&c0, launch_fill_zero(d_cnt, sizeof(int) * n_cnt, s));
+            const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / std::max<size_t>(cl.lds, 1))));
+            Timer t(s);
+            hipError_t e = launch_rt_coop_persistent(mixed, d_rt, cl.d_items, ns, d_cnt + 2, cl.cap, cl.slots,
+                                                     COOP_W, cl.lds, f == 0 ? n_cu * bpc : 0, s);
+            ms_form[f] = t.stop();
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                ms_form[f] = 1e30;
+            }
+        }
+        coop_flat = ms_form[1] < ms_form[0];
+    }
     // autotune the staged persistent integration's waves per workgroup on a slice of the
     // items (when the cooperative form serves the batch, it runs with COOP_W waves)
     int best_w = coop ? COOP_W : 4;
@@ -1741,7 +1772,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
             const CoopClass &cl = classes[k];
             const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / std::max<size_t>(cl.lds, 1))));
             launch_rt_coop_persistent(mixed, d_rt, cl.d_items, (int)cl.items.size(), d_cnt + 2 + k, cl.cap, cl.slots,
-                                      COOP_W, cl.lds, n_cu * bpc, s);
+                                      COOP_W, cl.lds, coop_flat ? 0 : n_cu * bpc, s);
         }
     } else {
         launch_rt_persistent(mixed, d_rt, d_ri, n_items, d_cnt, cap, R_max, best_w, rt_blocks, s);
@@ -1880,7 +1911,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         total->iterations = iters;
         total->n_cross = ncross;
         total->rt_waves = best_w;
-        total->rt_variant = coop ? 2 : 1;
+        total->rt_variant = coop ? (coop_flat ? 3 : 2) : 1;  // 3: the cooperative form, one workgroup per item
         total->ng_steps = n_ng;
         total->max_path = cap;
         total->converged = all_conv ? 1 : 0;

@@ -24,6 +24,8 @@
 // add exactly zero.  Per pair the sum over rays is in ray order; the channel sum
 // with phi0 d_v is done per line in channel order.
 #pragma once
+#include <type_traits>
+
 #include "rt_body.h"
 
 namespace portal {
@@ -79,7 +81,7 @@ __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const
     }
 }
 
-template <typename Real, int SLOTS, bool DUST>
+template <typename Real, int SLOTS, bool DUST, bool LAM>
 __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, const int cap, char *smem) {
     typedef Arith<Real> A;
     typedef typename A::R2 R2;
@@ -99,10 +101,10 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     const bool lane_on = t < G * R;
     const int lo = lane_on ? t % R : 0;      // this lane's line
     const int cb = (t / R) * SLOTS;          // its first channel
-    Real v0[SLOTS], e0[SLOTS];
-    bool on[SLOTS];
-    Real S0[SLOTS], C0[SLOTS], S2[SLOTS], C2[SLOTS], Lw[SLOTS];
-    const bool lam_on = a.LAM != nullptr;
+    // slot q of this lane holds channel cb + q when on(q); the channel's velocity offset
+    // and profile weight at the node are recomputed per ray, which keeps registers free
+    auto on = [&](int q) { return lane_on && cb + q < C; };
+    Real S0[SLOTS], C0[SLOTS], S2[SLOTS], C2[SLOTS], Lw[LAM ? SLOTS : 1];
     const R2 k0 = A::ks(a, (size_t)node * R + lo);  // the node's own cell, no skip
     R2 d0;
     if (DUST) d0 = A::ksd(a, (size_t)node * R + lo);
@@ -110,14 +112,10 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     const Real iin_star = a.xthn ? (Real)a.I_star[lo] : iin_bg;
     const int n_star = a.xthn ? a.nstar[bid] : 0;
 #pragma unroll
-    for (int q = 0; q < SLOTS; q++) {
-        on[q] = lane_on && cb + q < C;
-        const int nv = on[q] ? cb + q - nvel : 0;
-        v0[q] = ((Real)2 * b) * (Real)nv / (Real)nvel;
-        const Real aa = v0[q] * inv_bn;
-        e0[q] = A::ex(-(aa * aa));
-        S0[q] = C0[q] = S2[q] = C2[q] = Lw[q] = (Real)0;
-    }
+    for (int q = 0; q < SLOTS; q++) S0[q] = C0[q] = S2[q] = C2[q] = (Real)0;
+    if (LAM)
+#pragma unroll
+        for (int q = 0; q < SLOTS; q++) Lw[q] = (Real)0;
 
     // Rays are staged in units of at most cap cells, so that the staging area does not
     // grow with the longest path; unit u of ray r holds its cells u cap .. u cap + cap - 1.
@@ -163,25 +161,29 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             ray_done = false;
 #pragma unroll
             for (int q = 0; q < SLOTS; q++) {
+                const int nv = on(q) ? cb + q - nvel : 0;
+                const Real v0 = ((Real)2 * b) * (Real)nv / (Real)nvel;
+                const Real aa = v0 * inv_bn;
+                const Real e0 = A::ex(-(aa * aa));
                 if (DUST) {
                     // S = (e phi + j)/(k phi + alpha), tau = (k phi + alpha) dl (inverse_rt.f90:289-297)
-                    const Real kp = k0.x * e0[q];
+                    const Real kp = k0.x * e0;
                     const Real tau0 = (kp + d0.x) * dl0;
                     const Real Sc = (k0.y * kp + d0.y * d0.x) / (kp + d0.x);
                     Tr[q] = A::ex(-tau0);
                     const Real om0 = A::one_minus(tau0, Tr[q]);
                     Em[q] = Sc * om0;
-                    if (lam_on) Lw[q] = Lw[q] + w * om0;
+                    if (LAM) Lw[q] = Lw[q] + w * om0;
                 } else {
-                    const Real tau0 = k0.x * e0[q] * dl0;
+                    const Real tau0 = k0.x * e0 * dl0;
                     Tr[q] = A::ex(-tau0);
                     const Real om0 = A::one_minus(tau0, Tr[q]);
                     Em[q] = k0.y * om0;
                     // the own cell's share of J per unit of its source function
-                    if (lam_on) Lw[q] = Lw[q] + w * om0;
+                    if (LAM) Lw[q] = Lw[q] + w * om0;
                 }
-                v0s[q] = v0[q] + vel_node;
-                if (!on[q]) Tr[q] = (Real)0;
+                v0s[q] = v0 + vel_node;
+                if (!on(q)) Tr[q] = (Real)0;
             }
         }
         if (!ray_done) {
@@ -270,10 +272,10 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
 #pragma unroll
     for (int q = 0; q < SLOTS; q++) {
         const int p = lo * C + cb + q;
-        if (on[q]) {
+        if (on(q)) {
             red0[p] = (double)S0[q] - (double)C0[q];
             red2[p] = (double)S2[q] - (double)C2[q];
-            red3[p] = (double)Lw[q];
+            if (LAM) red3[p] = (double)Lw[q];
         }
     }
     __syncthreads();
@@ -288,26 +290,51 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             const double phi0 = (1.0 / (spi * bd)) * exp(-(aa * aa));
             j0 = j0 + phi0 * d_v * red0[l * C + c];
             j2 = j2 + phi0 * d_v * red2[l * C + c];
-            if (lam_on) lam = lam + phi0 * d_v * red3[l * C + c];
+            if (LAM) lam = lam + phi0 * d_v * red3[l * C + c];
         }
         a.J0[(size_t)node * R + l] = j0;
         a.J2[(size_t)node * R + l] = j2 / 1.4142135623730951;
-        if (lam_on) a.LAM[(size_t)node * R + l] = lam;
+        if (LAM) a.LAM[(size_t)node * R + l] = lam;
     }
 }
 
-template <typename Real, int SLOTS, bool DUST>
+template <typename Real, int SLOTS, bool DUST, bool LAM>
 __global__ void __launch_bounds__(512) rt_coop_kernel(RtArgs a, int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    rt_coop_node<Real, SLOTS, DUST>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+    rt_coop_node<Real, SLOTS, DUST, LAM>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+}
+
+// The FP32 forms with up to 6 slots are compiled for 7 waves per SIMD (at most 72
+// VGPRs).  The FP32 update is bound partly by the latency of its dependent exponentials,
+// which more resident waves hide: 723 ms per production pass against 781 ms at the 6
+// waves the kernel's natural 80 VGPRs give (one box, same inputs; 12 dwords spill).
+// With more slots, or in the persistent form, the budget would spill far more.
+template <int SLOTS, bool DUST, bool LAM>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8)))
+rt_coop_kernel_f32(RtArgs a, int cap) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    rt_coop_node<float, SLOTS, DUST, LAM>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+}
+
+template <typename Real, int SLOTS, bool DUST, bool LAM>
+void launch_coop_k(dim3 g, dim3 b, size_t lds, hipStream_t s, const RtArgs &a, int cap) {
+    if constexpr (std::is_same<Real, float>::value && SLOTS <= 6)
+        hipLaunchKernelGGL((rt_coop_kernel_f32<SLOTS, DUST, LAM>), g, b, lds, s, a, cap);
+    else
+        hipLaunchKernelGGL((rt_coop_kernel<Real, SLOTS, DUST, LAM>), g, b, lds, s, a, cap);
 }
 
 template <typename Real, int SLOTS>
 hipError_t launch_coop_s(const RtArgs &a, int cap, size_t lds, hipStream_t s) {
+    const dim3 g(a.n_work), b(64 * a.waves);
+    const bool lam = a.LAM != nullptr;
+    if (a.dust && lam) return hipErrorInvalidValue;  // the own-cell operator covers the line alone
     if (a.dust)
-        hipLaunchKernelGGL((rt_coop_kernel<Real, SLOTS, true>), dim3(a.n_work), dim3(64 * a.waves), lds, s, a, cap);
+        launch_coop_k<Real, SLOTS, true, false>(g, b, lds, s, a, cap);
+    else if (lam)
+        launch_coop_k<Real, SLOTS, false, true>(g, b, lds, s, a, cap);
     else
-        hipLaunchKernelGGL((rt_coop_kernel<Real, SLOTS, false>), dim3(a.n_work), dim3(64 * a.waves), lds, s, a, cap);
+        launch_coop_k<Real, SLOTS, false, false>(g, b, lds, s, a, cap);
     return hipGetLastError();
 }
 
@@ -315,20 +342,31 @@ hipError_t launch_coop_s(const RtArgs &a, int cap, size_t lds, hipStream_t s) {
 // and channels need the same slot count: workgroups take items from a counter until the
 // list is exhausted (every workgroup reaches the exit test after each item).
 template <typename Real, int SLOTS>
-__global__ void __launch_bounds__(512) rt_coop_persistent_kernel(const RtArgs *models, const int2 *items,
-                                                                 int n_items, int *counter, int cap) {
+__device__ __forceinline__ void coop_persistent_body(const RtArgs *__restrict__ models,
+                                                     const int2 *__restrict__ items, int n_items, int *counter,
+                                                     int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ int s_item;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
-        const int it = s_item;
+        // the item is uniform over the workgroup: scalar registers hold it and its model
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (it >= n_items) break;
         const int2 mi = items[it];
-        rt_coop_node<Real, SLOTS, false>(models[mi.x], mi.y, cap, smem);
+        const int m = __builtin_amdgcn_readfirstlane(mi.x), w = __builtin_amdgcn_readfirstlane(mi.y);
+        const RtArgs am = models[m];
+        rt_coop_node<Real, SLOTS, false, false>(am, w, cap, smem);
         __syncthreads();
     }
+}
+
+template <typename Real, int SLOTS>
+__global__ void __launch_bounds__(512) rt_coop_persistent_kernel(const RtArgs *__restrict__ models,
+                                                                 const int2 *__restrict__ items, int n_items,
+                                                                 int *counter, int cap) {
+    coop_persistent_body<Real, SLOTS>(models, items, n_items, counter, cap);
 }
 
 // Slots per lane: the fewest consecutive channels per lane with which every line's
@@ -345,21 +383,74 @@ inline int coop_slots(int R, int C, int W) {
     return 0;
 }
 
+// Flat form over the same items: one workgroup per item, in the items' (cost) order,
+// the hardware's dispatch doing the balancing.  It needs no loop state, so it keeps the
+// register budget of rt_coop_kernel.
+template <typename Real, int SLOTS>
+__device__ __forceinline__ void coop_flat_body(const RtArgs *__restrict__ models, const int2 *__restrict__ items,
+                                               int cap) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int2 mi = items[blockIdx.x];
+    const int m = __builtin_amdgcn_readfirstlane(mi.x), w = __builtin_amdgcn_readfirstlane(mi.y);
+    rt_coop_node<Real, SLOTS, false, false>(models[m], w, cap, smem);
+}
+
+template <typename Real, int SLOTS>
+__global__ void __launch_bounds__(512) rt_coop_flat_kernel(const RtArgs *__restrict__ models,
+                                                           const int2 *__restrict__ items, int cap) {
+    coop_flat_body<Real, SLOTS>(models, items, cap);
+}
+
+template <int SLOTS>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8)))
+rt_coop_flat_kernel_f32(const RtArgs *__restrict__ models, const int2 *__restrict__ items, int cap) {
+    coop_flat_body<float, SLOTS>(models, items, cap);
+}
+
+// n_blocks > 0: the persistent form on n_blocks workgroups; n_blocks == 0: the flat form
 template <typename Real>
 hipError_t launch_coop_persistent(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
                                   int slots, int waves, size_t lds, int n_blocks, hipStream_t s) {
     if (lds > 160 * 1024 || waves < 1 || waves > 8) return hipErrorInvalidValue;
+    if (n_items <= 0) return hipSuccess;
+    if (n_blocks == 0) {
+        const dim3 g(n_items), b(64 * waves);
+        auto flat = [&](auto slots_c) {
+            constexpr int S = decltype(slots_c)::value;
+            if constexpr (std::is_same<Real, float>::value && S <= 6)
+                hipLaunchKernelGGL((rt_coop_flat_kernel_f32<S>), g, b, lds, s, models, items, cap);
+            else
+                hipLaunchKernelGGL((rt_coop_flat_kernel<Real, S>), g, b, lds, s, models, items, cap);
+        };
+        switch (slots) {
+            case 1: flat(std::integral_constant<int, 1>{}); break;
+            case 2: flat(std::integral_constant<int, 2>{}); break;
+            case 3: flat(std::integral_constant<int, 3>{}); break;
+            case 4: flat(std::integral_constant<int, 4>{}); break;
+            case 5: flat(std::integral_constant<int, 5>{}); break;
+            case 6: flat(std::integral_constant<int, 6>{}); break;
+            case 8: flat(std::integral_constant<int, 8>{}); break;
+            case 10: flat(std::integral_constant<int, 10>{}); break;
+            case 12: flat(std::integral_constant<int, 12>{}); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     const dim3 g(n_blocks), b(64 * waves);
+    auto go = [&](auto slots_c) {
+        constexpr int S = decltype(slots_c)::value;
+        hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, S>), g, b, lds, s, models, items, n_items, counter, cap);
+    };
     switch (slots) {
-        case 1: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 1>), g, b, lds, s, models, items, n_items, counter, cap); break;
-        case 2: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 2>), g, b, lds, s, models, items, n_items, counter, cap); break;
-        case 3: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 3>), g, b, lds, s, models, items, n_items, counter, cap); break;
-        case 4: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 4>), g, b, lds, s, models, items, n_items, counter, cap); break;
-        case 5: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 5>), g, b, lds, s, models, items, n_items, counter, cap); break;
-        case 6: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 6>), g, b, lds, s, models, items, n_items, counter, cap); break;
-        case 8: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 8>), g, b, lds, s, models, items, n_items, counter, cap); break;
-        case 10: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 10>), g, b, lds, s, models, items, n_items, counter, cap); break;
-        case 12: hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, 12>), g, b, lds, s, models, items, n_items, counter, cap); break;
+        case 1: go(std::integral_constant<int, 1>{}); break;
+        case 2: go(std::integral_constant<int, 2>{}); break;
+        case 3: go(std::integral_constant<int, 3>{}); break;
+        case 4: go(std::integral_constant<int, 4>{}); break;
+        case 5: go(std::integral_constant<int, 5>{}); break;
+        case 6: go(std::integral_constant<int, 6>{}); break;
+        case 8: go(std::integral_constant<int, 8>{}); break;
+        case 10: go(std::integral_constant<int, 10>{}); break;
+        case 12: go(std::integral_constant<int, 12>{}); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -122,7 +122,7 @@ def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True):
     t = time.perf_counter()
     # calls of at most 10 iterations, each continuing the last (the same iteration as one
     # call), with a line per call
-    done, iters_at, dev_ms, rt_ms, se_ms = 0, [0] * n_models, 0.0, 0.0, 0.0
+    done, iters_at, dev_ms, rt_ms = 0, [0] * n_models, 0.0, 0.0
     while True:
         per, tot = batch_lambda_iterate(pts, [xth] * n_models, [nvel] * n_models, cloud.DX_INWARD, 10,
                                         tol_pop=TOL_POP, tol_align=TOL_ALIGN, precision=PORTAL_MIXED,
@@ -133,7 +133,6 @@ def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True):
         done += tot["iterations"]
         dev_ms += tot["t_total_ms"]
         rt_ms += tot["t_rt_ms"]
-        se_ms += tot["t_se_ms"]
         _log(f"batch: {done} iterations, {sum(1 for v in iters_at if v)} of {n_models} models converged, "
              f"{time.perf_counter() - t:.1f}s")
         if tot["converged"] or done >= 400:
@@ -147,7 +146,9 @@ def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True):
            "iterations_max": done, "all_converged": bool(all(iters_at)),
            "iterations_per_model": [v if v else done for v in iters_at],
            "levels_per_model": [s[1] for s in sizes], "nodes_per_model": [s[2] for s in sizes],
-           "device_ms": dev_ms, "iteration_ms": rt_ms, "se_ms": se_ms}
+           "device_ms": dev_ms, "iteration_ms": rt_ms,
+           "inward_form": {1: "staged persistent", 2: "cooperative persistent",
+                           3: "cooperative, one workgroup per item"}.get(tot["rt_variant"], "?")}
     out["unconverged"] = [{"model": m, "levels": sizes[m][1], "nodes": sizes[m][2],
                            "pop_change": per[m]["max_rel_change"], "align_change": per[m]["max_align_change"]}
                           for m in range(n_models) if not iters_at[m]]
