@@ -127,6 +127,10 @@ typedef struct {
     int ng_steps;            /* Ng extrapolations applied by the lambda iteration */
     int converged;           /* 1: the lambda iteration met both tolerances */
     double max_align_change; /* last iteration's largest change of rho^2_0/rho^0_0 */
+    int se_form;             /* SE solve: 0 one launch (one persistent launch in a batch),
+                                1 a batch's launch per class of system sizes */
+    int se_waves;            /* waves per SE system; for a launch per class, the bitwise
+                                OR of the classes' wave counts (1, 2, 4) */
 } portal_stats;
 
 /* Context -------------------------------------------------------------------- */

@@ -186,6 +186,12 @@ class _Core:
         and its waves per workgroup for one precision; variant -1 returns to autotuning."""
         self._check(self.lib.portal_debug_force_rt(self.ctx, int(precision), int(variant), int(waves)))
 
+    def force_se(self, form=-1, waves=0):
+        """Test hook: waves per system of this context's SE solve (1, 2, 4; 0 resets to 1);
+        for a batch this context leads, the SE form (-1 autotuned, 0 one persistent launch,
+        1 a launch per class of system sizes) and the classes' waves (0 autotuned)."""
+        self._check(self.lib.portal_debug_force_se(self.ctx, int(form), int(waves)))
+
     def get_pol_pops(self):
         """pol_pops of the last SE solve (N x 2L) without recomputing."""
         g, sp = self.grid, self.species

@@ -137,6 +137,10 @@ struct Ctx {
     double walk_chunk_scale = 1.1;  // chunk store of the single-pass walk / sampled estimate
     int ng_first = 3, ng_gap = 3;   // Ng schedule: first iteration, least gap between extrapolations
     int coop_unit = 0;              // cells per staged unit of the cooperative kernel (0: COOP_UNIT)
+    int se_waves = 1;               // waves per system of the single-model SE solve
+    // test hook for a batch led by this context: SE form (-1 autotuned, 0 one persistent
+    // launch, 1 a launch per class) and the waves per system of the classes (0 autotuned)
+    int se_forced_form = -1, se_forced_waves = 0;
     // d_pol holds the SE solution from which the current d_levp was formed (set by the
     // lambda iteration; cleared by anything that changes the state otherwise), so that
     // a lambda iteration continued by a further call measures its first change exactly
@@ -303,6 +307,9 @@ hipError_t launch_rt_persistent(bool mixed, const RtArgs *models, const int2 *it
                                 int cap, int R_max, int waves, int n_blocks, hipStream_t s);
 hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd_max,
                                 int nt_max, int n_blocks, hipStream_t s);
+// one workgroup of `waves` (1, 2 or 4) waves per item, LDS sized for (cd_max, nt_max)
+hipError_t launch_se_items(const SeArgs *models, const int2 *items, int n_items, int cd_max, int nt_max, int waves,
+                           hipStream_t s);
 size_t se_lds_bytes(int c_dim, int n_trans);
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves);
 // the cooperative form of the persistent integration, one launch per slot count

@@ -513,6 +513,7 @@ int run_se(Ctx &c, int extra_info, portal_stats *st) {
     a.rates = extra_info ? c.d_rates : nullptr;
     a.flags = c.d_flags;
     a.extra_info = extra_info;
+    a.waves = c.se_waves;
     HIP_TRY(&c, hipMemsetAsync(c.d_pol, 0, sizeof(double) * c.g.N * 2 * s.e_num, c.stream));
     if (extra_info) HIP_TRY(&c, hipMemsetAsync(c.d_rates, 0, sizeof(double) * c.g.N * 6 * s.e_num, c.stream));
     Timer t(c.stream);
@@ -520,7 +521,11 @@ int run_se(Ctx &c, int extra_info, portal_stats *st) {
     const double ms = t.stop();
     int flags = 0;
     HIP_TRY(&c, hipMemcpy(&flags, c.d_flags, sizeof(int), hipMemcpyDeviceToHost));
-    if (st) st->t_se_ms += ms;
+    if (st) {
+        st->t_se_ms += ms;
+        st->se_form = 0;
+        st->se_waves = a.waves;
+    }
     if (flags & FLAG_TEMP)
         return fail(c, PORTAL_ERR_TEMP, "kinetic temperature above the collision-rate table (miscellaneous.f90:331-360)");
     if (flags & FLAG_LU) return fail(c, PORTAL_ERR_LU, "Density-matrix calculation: failed (singular SE matrix)");
@@ -1012,6 +1017,17 @@ int portal_debug_force_rt(portal_ctx *x, int precision, int variant, int waves) 
     x->c.forced_variant[precision] = variant;
     x->c.forced_waves[precision] = waves;
     x->c.tuned_key[precision] = 0;
+    return PORTAL_OK;
+}
+
+// Test hook: waves per system of this context's SE solve (1, 2 or 4), and for a batch
+// that this context leads, the SE form (-1 autotuned, 0 one persistent launch, 1 a
+// launch per class of system sizes) and the classes' waves (0 autotuned).
+int portal_debug_force_se(portal_ctx *x, int form, int waves) {
+    if (!x || form < -1 || form > 1 || !(waves == 0 || waves == 1 || waves == 2 || waves == 4)) return PORTAL_ERR_ARG;
+    x->c.se_waves = waves > 0 ? waves : 1;
+    x->c.se_forced_form = form;
+    x->c.se_forced_waves = waves;
     return PORTAL_OK;
 }
 
@@ -1755,6 +1771,84 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     const size_t se_lds = se_lds_bytes(cd_max, nt_max);
     if (se_lds > 160 * 1024) return fail(c0, PORTAL_ERR_ARG, "SE system too large for one workgroup's LDS");
     const int se_blocks = n_cu * std::max(1, std::min(16, (int)((160 * 1024) / se_lds)));
+    // The SE solves: one persistent launch whose LDS fits the largest system (form 0), or
+    // a launch per class of models with one system size, each with its own LDS and its
+    // own waves per system (form 1).  The autotuner times both on every k-th item.
+    struct SeClass {
+        int cd = 0, nt = 1, waves = 1;
+        std::vector<int2> items, sample;
+        int2 *d_items = nullptr, *d_sample = nullptr;
+    };
+    std::vector<SeClass> se_cls;
+    int se_form = 1, se_waves_or = 0;
+    auto free_se_cls = [&]() {
+        for (auto &k : se_cls) {
+            dfree(k.d_items);
+            dfree(k.d_sample);
+        }
+    };
+    {
+        std::vector<int> cls_of(n, -1);
+        for (int m = 0; m < n; m++) {
+            for (size_t k = 0; k < se_cls.size(); k++)
+                if (se_cls[k].cd == ses[m].c_dim) cls_of[m] = (int)k;
+            if (cls_of[m] < 0) {
+                cls_of[m] = (int)se_cls.size();
+                se_cls.emplace_back();
+                se_cls.back().cd = ses[m].c_dim;
+            }
+            se_cls[cls_of[m]].nt = std::max(se_cls[cls_of[m]].nt, ses[m].n_trans);
+        }
+        for (const int2 &itm : si) se_cls[cls_of[itm.x]].items.push_back(itm);
+        const size_t stride = std::max<size_t>(1, si.size() / 65536);
+        std::vector<int2> gsample;
+        for (size_t i = 0; i < si.size(); i += stride) gsample.push_back(si[i]);
+        for (auto &k : se_cls) {
+            for (size_t i = 0; i < k.items.size(); i += stride) k.sample.push_back(k.items[i]);
+            HIP_TRY(&c0, upload(k.d_items, k.items));
+            HIP_TRY(&c0, upload(k.d_sample, k.sample));
+        }
+        const int forced_form = c0.se_forced_form, forced_w = c0.se_forced_waves;
+        double ms_cls = 0.0;
+        for (auto &k : se_cls) {
+            double best = 1e30;
+            for (int w : {1, 2, 4}) {
+                if (forced_w > 0 && w != forced_w) continue;
+                if (forced_w == 0 && forced_form == 0) break;  // the classes are not used
+                Timer t(s);
+                hipError_t e = launch_se_items(d_se, k.d_sample, (int)k.sample.size(), k.cd, k.nt, w, s);
+                const double ms = t.stop();
+                if (e != hipSuccess) {
+                    (void)hipGetLastError();
+                    continue;
+                }
+                if (ms < best) {
+                    best = ms;
+                    k.waves = w;
+                }
+            }
+            ms_cls += best;
+            se_waves_or |= k.waves;
+        }
+        double ms_one = 1e30;
+        if (forced_form != 1) {
+            int2 *d_gs = nullptr;
+            HIP_TRY(&c0, upload(d_gs, gsample));
+            HIP_TRY(&c0, launch_fill_zero(d_cnt, sizeof(int) * n_cnt, s));
+            Timer t(s);
+            hipError_t e = launch_se_persistent(d_se, d_gs, (int)gsample.size(), d_cnt + 1, cd_max, nt_max, se_blocks, s);
+            ms_one = t.stop();
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                ms_one = 1e30;
+            }
+            dfree(d_gs);
+        }
+        se_form = forced_form >= 0 ? forced_form : (ms_one <= ms_cls ? 0 : 1);
+        if (se_form == 0) se_waves_or = 1;
+        // the timing launches solved systems of partial fields: their flags do not count
+        HIP_TRY(&c0, hipMemset(c0.d_flags, 0, sizeof(int) * 4));
+    }
     // capture one iteration
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1777,7 +1871,10 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     } else {
         launch_rt_persistent(mixed, d_rt, d_ri, n_items, d_cnt, cap, R_max, best_w, rt_blocks, s);
     }
-    launch_se_persistent(d_se, d_si, (int)si.size(), d_cnt + 1, cd_max, nt_max, se_blocks, s);
+    if (se_form == 0)
+        launch_se_persistent(d_se, d_si, (int)si.size(), d_cnt + 1, cd_max, nt_max, se_blocks, s);
+    else
+        for (const SeClass &k : se_cls) launch_se_items(d_se, k.d_items, (int)k.items.size(), k.cd, k.nt, k.waves, s);
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
         launch_lambda_update((int)c.g.work.size(), c.g.N, c.sp.e_num, c.g.d_work, c.g.d_ab, c.g.d_h2, c.sp.d_jlev,
@@ -1885,6 +1982,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     for (auto *p : prev) hipFree(p);
     free_xths();
     free_classes();
+    free_se_cls();
     const double ms = ttot.stop();
     if (flags & FLAG_TEMP) return fail(c0, PORTAL_ERR_TEMP, "kinetic temperature above the collision-rate table");
     if (flags & FLAG_LU) return fail(c0, PORTAL_ERR_LU, "Density-matrix calculation: failed (singular SE matrix)");
@@ -1913,6 +2011,8 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         total->rt_waves = best_w;
         total->rt_variant = coop ? (coop_flat ? 3 : 2) : 1;  // 3: the cooperative form, one workgroup per item
         total->ng_steps = n_ng;
+        total->se_form = se_form;
+        total->se_waves = se_waves_or;
         total->max_path = cap;
         total->converged = all_conv ? 1 : 0;
     }

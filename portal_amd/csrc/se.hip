@@ -1,4 +1,4 @@
-// Statistical-equilibrium assembly and solve, one wavefront per node (FP64).
+// Statistical-equilibrium assembly and solve, one workgroup of 1, 2 or 4 wavefronts per node (FP64).
 //
 // Restates make_AC (miscellaneous.f90:288-392), setup_stateq (pop_eqs.f90:361-653)
 // and get_rhovec (em_coefs.f90:115-201) with the packing of trace_node
@@ -15,8 +15,8 @@
 // A&A 245, 171, eq. 2-4), and at a fixed point J0* + lam S = J0, so the solution is
 // that of the plain iteration.
 // Row 1 is replaced by sqrt(2J+1) at every level's K=0 slot and A rho = e1 is
-// solved by LU with partial pivoting (first maximal pivot, as DGETRF) in LDS, one
-// row per lane, followed by one step of iterative refinement with a compensated
+// solved by LU with partial pivoting (first maximal pivot, as DGETRF) in LDS,
+// followed by one step of iterative refinement with a compensated
 // residual.  The reference takes column 1 of DGETRI's explicit inverse, which is the
 // same vector up to rounding.  Compiled with -ffp-contract=off.
 #include "common.h"
@@ -28,8 +28,14 @@ __device__ __forceinline__ void two_prod(double a, double b, double &p, double &
     e = fma(a, b, -p);
 }
 
+// A system is solved by W wavefronts together (64W threads, "waves per system"): the
+// elementwise loops stride over all of them, the trailing update of each LU step splits
+// its columns between the waves (column j to wave j mod W), and every wave runs the
+// pivot search on its own, so the factors and the solution do not depend on W.
+template <int W>
 __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *sm) {
-    const int lane = threadIdx.x;
+    constexpr int NT = 64 * W;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int node = a.work[wi];
     const int cd = a.c_dim, nt = a.n_trans, L = a.L, R = a.R;
     double *x = sm;                          // 10*nt scalars
@@ -48,10 +54,10 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
         if (a.col_temps[i] > T) { ib = i; break; }
     const bool below = T <= a.col_temps[0];
     bool temp_ok = (ib >= 0) || (below && a.c_temp == 1);
-    if (lane == 0) s_info = 0;
-    if (!temp_ok && lane == 0) atomicOr(a.flags, FLAG_TEMP);
+    if (tid == 0) s_info = 0;
+    if (!temp_ok && tid == 0) atomicOr(a.flags, FLAG_TEMP);
 
-    for (int t = lane; t < nt; t += 64) {
+    for (int t = tid; t < nt; t += NT) {
         const int rad = a.t_rad[t], col = a.t_col[t];
         const double A = a.t_A[t], B21 = a.t_B21[t], nu = a.t_nu[t], gul = a.t_gul[t];
         const double J0v = rad >= 0 ? a.J0[(size_t)node * R + rad] : 0.0;
@@ -102,23 +108,24 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
         x[10 * t + 8] = B12 * Jp;
         x[10 * t + 9] = Ap;
     }
-    for (int i = lane; i < cd * cd; i += 64) M[i] = 0.0;
+    for (int i = tid; i < cd * cd; i += NT) M[i] = 0.0;
     __syncthreads();
-    for (int e = lane; e < a.n_ent; e += 64) {
+    for (int e = tid; e < a.n_ent; e += NT) {
         double s = 0.0;
         for (int k = a.ent_ptr[e]; k < a.ent_ptr[e + 1]; k++) s = s + a.term_coef[k] * x[a.term_q[k]];
         M[a.ent_idx[e]] = s;
     }
     __syncthreads();
     // first row <- normalisation sum_i sqrt(2J_i+1) rho^0_0(i) = 1
-    for (int j = lane; j < cd; j += 64) M[(size_t)cd * j] = 0.0;
+    for (int j = tid; j < cd; j += NT) M[(size_t)cd * j] = 0.0;
     __syncthreads();
-    for (int i = lane; i < L; i += 64) M[(size_t)cd * a.count_array[i]] = sqrt(2.0 * a.j_lev[i] + 1.0);
+    for (int i = tid; i < L; i += NT) M[(size_t)cd * a.count_array[i]] = sqrt(2.0 * a.j_lev[i] + 1.0);
     __syncthreads();
-    for (int i = lane; i < cd * cd; i += 64) M0[i] = M[i];
+    for (int i = tid; i < cd * cd; i += NT) M0[i] = M[i];
     __syncthreads();
 
-    // LU with partial pivoting; lane owns rows lane, lane+64, ...
+    // LU with partial pivoting; in the trailing update a lane owns rows k+1+lane,
+    // k+65+lane, ... of its wave's columns
     for (int k = 0; k < cd; k++) {
         double best = -1.0;
         int bi = cd;
@@ -132,39 +139,42 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
             if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
         }
         if (!(best > 0.0)) {
-            if (lane == 0) s_info = k + 1;
+            if (tid == 0) s_info = k + 1;
             break;
         }
-        if (lane == 0) piv[k] = bi;
+        if (tid == 0) piv[k] = bi;
+        // every wave has read column k before any wave swaps rows
+        if (W > 1) __syncthreads();
         if (bi != k)
-            for (int j = lane; j < cd; j += 64) {
+            for (int j = tid; j < cd; j += NT) {
                 const double t = M[k + (size_t)cd * j];
                 M[k + (size_t)cd * j] = M[bi + (size_t)cd * j];
                 M[bi + (size_t)cd * j] = t;
             }
         __syncthreads();
         const double pv = M[k + (size_t)cd * k];
-        for (int i = k + 1 + lane; i < cd; i += 64) M[i + (size_t)cd * k] = M[i + (size_t)cd * k] / pv;
+        for (int i = k + 1 + tid; i < cd; i += NT) M[i + (size_t)cd * k] = M[i + (size_t)cd * k] / pv;
         __syncthreads();
         for (int i = k + 1 + lane; i < cd; i += 64) {
             const double li = M[i + (size_t)cd * k];
-            for (int j = k + 1; j < cd; j++) M[i + (size_t)cd * j] = M[i + (size_t)cd * j] - li * M[k + (size_t)cd * j];
+            for (int j = k + 1 + wv; j < cd; j += W)
+                M[i + (size_t)cd * j] = M[i + (size_t)cd * j] - li * M[k + (size_t)cd * j];
         }
         __syncthreads();
     }
     __syncthreads();
     const int info = s_info;
     if (info) {
-        if (lane == 0) atomicOr(a.flags, FLAG_LU);
+        if (tid == 0) atomicOr(a.flags, FLAG_LU);
         return;
     }
     // solve with rhs e1, then one refinement step
     for (int it = 0; it < 2; it++) {
         if (it == 0) {
-            for (int i = lane; i < cd; i += 64) y[i] = (i == 0) ? 1.0 : 0.0;
+            for (int i = tid; i < cd; i += NT) y[i] = (i == 0) ? 1.0 : 0.0;
         } else {
             // compensated residual r = e1 - M0 v
-            for (int i = lane; i < cd; i += 64) {
+            for (int i = tid; i < cd; i += NT) {
                 double s = (i == 0) ? 1.0 : 0.0, c = 0.0;
                 for (int j = 0; j < cd; j++) {
                     double p, e;
@@ -178,7 +188,7 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
             }
         }
         __syncthreads();
-        if (lane == 0)
+        if (tid == 0)
             for (int k = 0; k < cd; k++) {
                 const int p = piv[k];
                 if (p != k) { const double t = y[k]; y[k] = y[p]; y[p] = t; }
@@ -186,22 +196,22 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
         __syncthreads();
         for (int k = 0; k < cd; k++) {
             const double yk = y[k];
-            for (int i = k + 1 + lane; i < cd; i += 64) y[i] = y[i] - M[i + (size_t)cd * k] * yk;
+            for (int i = k + 1 + tid; i < cd; i += NT) y[i] = y[i] - M[i + (size_t)cd * k] * yk;
             __syncthreads();
         }
         for (int k = cd - 1; k >= 0; k--) {
-            if (lane == 0) y[k] = y[k] / M[k + (size_t)cd * k];
+            if (tid == 0) y[k] = y[k] / M[k + (size_t)cd * k];
             __syncthreads();
             const double yk = y[k];
-            for (int i = lane; i < k; i += 64) y[i] = y[i] - M[i + (size_t)cd * k] * yk;
+            for (int i = tid; i < k; i += NT) y[i] = y[i] - M[i + (size_t)cd * k] * yk;
             __syncthreads();
         }
-        for (int i = lane; i < cd; i += 64) v[i] = (it == 0) ? y[i] : v[i] + y[i];
+        for (int i = tid; i < cd; i += NT) v[i] = (it == 0) ? y[i] : v[i] + y[i];
         __syncthreads();
     }
     // pack (rho00, rho20) per level, NaN -> 0 (main_sub.f90:112-137)
     const int N = a.N;
-    for (int i = lane; i < L; i += 64) {
+    for (int i = tid; i < L; i += NT) {
         const int ca = a.count_array[i];
         double r0 = v[ca], r2 = a.j_lev[i] != 0 ? v[ca + 1] : 0.0;
         if (isnan(r0)) { r0 = 0.0; r2 = 0.0; }
@@ -213,22 +223,22 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
         // replacement) times rho, em_coefs.f90:188-199
         for (int cls = 0; cls < 3; cls++) {
             __syncthreads();
-            for (int i = lane; i < cd * cd; i += 64) M[i] = 0.0;
+            for (int i = tid; i < cd * cd; i += NT) M[i] = 0.0;
             __syncthreads();
-            for (int e = lane; e < a.n_ent; e += 64) {
+            for (int e = tid; e < a.n_ent; e += NT) {
                 double s = 0.0;
                 for (int k = a.ent_ptr[e]; k < a.ent_ptr[e + 1]; k++)
                     if (a.term_class[k] == cls) s = s + a.term_coef[k] * x[a.term_q[k]];
                 M[a.ent_idx[e]] = s;
             }
             __syncthreads();
-            for (int i = lane; i < cd; i += 64) {
+            for (int i = tid; i < cd; i += NT) {
                 double s = 0.0;
                 for (int j = 0; j < cd; j++) s = s + fabs(M[i + (size_t)cd * j]) * v[j];
                 y[i] = s;
             }
             __syncthreads();
-            for (int i = lane; i < L; i += 64) {
+            for (int i = tid; i < L; i += NT) {
                 const int ca = a.count_array[i];
                 const bool nanr = isnan(v[ca]);
                 a.rates[node + (size_t)N * (cls + 3 * (size_t)(2 * i))] = y[ca];
@@ -239,13 +249,15 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
     }
 }
 
-__global__ void __launch_bounds__(64) se_kernel(SeArgs a) {
+template <int W>
+__global__ void __launch_bounds__(64 * W) se_kernel(SeArgs a) {
     extern __shared__ double sm[];
     if ((int)blockIdx.x >= a.n_work) return;
-    se_node(a, blockIdx.x, sm);
+    se_node<W>(a, blockIdx.x, sm);
 }
 
-// Persistent form over the (model, node) items of a batch of models.
+// Persistent form over the (model, node) items of a batch of models, one wave per
+// system, with the LDS of the largest system.
 __global__ void __launch_bounds__(64) se_persistent_kernel(const SeArgs *models, const int2 *items, int n_items,
                                                            int *counter) {
     extern __shared__ double sm[];
@@ -257,9 +269,19 @@ __global__ void __launch_bounds__(64) se_persistent_kernel(const SeArgs *models,
         __syncthreads();
         if (it >= n_items) break;
         const int2 mi = items[it];
-        se_node(models[mi.x], mi.y, sm);
+        se_node<1>(models[mi.x], mi.y, sm);
         __syncthreads();
     }
+}
+
+// One workgroup of W waves per (model, node) item: a class of models whose systems
+// share one LDS size.
+template <int W>
+__global__ void __launch_bounds__(64 * W) se_items_kernel(const SeArgs *models, const int2 *items, int n_items) {
+    extern __shared__ double sm[];
+    if ((int)blockIdx.x >= n_items) return;
+    const int2 mi = items[blockIdx.x];
+    se_node<W>(models[mi.x], mi.y, sm);
 }
 
 size_t se_lds_bytes(int c_dim, int n_trans) {
@@ -274,11 +296,30 @@ hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_i
     return hipGetLastError();
 }
 
+hipError_t launch_se_items(const SeArgs *models, const int2 *items, int n_items, int cd_max, int nt_max, int waves,
+                           hipStream_t s) {
+    if (n_items == 0) return hipSuccess;
+    const size_t lds = se_lds_bytes(cd_max, nt_max);
+    if (lds > 160 * 1024 || n_items < 0) return hipErrorInvalidValue;
+    switch (waves) {
+        case 1: hipLaunchKernelGGL(se_items_kernel<1>, dim3(n_items), dim3(64), lds, s, models, items, n_items); break;
+        case 2: hipLaunchKernelGGL(se_items_kernel<2>, dim3(n_items), dim3(128), lds, s, models, items, n_items); break;
+        case 4: hipLaunchKernelGGL(se_items_kernel<4>, dim3(n_items), dim3(256), lds, s, models, items, n_items); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_se(const SeArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
     const size_t lds = se_lds_bytes(a.c_dim, a.n_trans);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(se_kernel, dim3(a.n_work), dim3(64), lds, s, a);
+    switch (a.waves <= 1 ? 1 : a.waves) {
+        case 1: hipLaunchKernelGGL(se_kernel<1>, dim3(a.n_work), dim3(64), lds, s, a); break;
+        case 2: hipLaunchKernelGGL(se_kernel<2>, dim3(a.n_work), dim3(128), lds, s, a); break;
+        case 4: hipLaunchKernelGGL(se_kernel<4>, dim3(a.n_work), dim3(256), lds, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

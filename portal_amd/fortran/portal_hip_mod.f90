@@ -60,6 +60,7 @@ module portal_hip
     real(c_double) :: max_rel_change
     integer(c_int) :: rt_variant, rt_waves, max_path, ng_steps, converged
     real(c_double) :: max_align_change
+    integer(c_int) :: se_form, se_waves
   end type
 
   interface

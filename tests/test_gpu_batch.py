@@ -140,3 +140,70 @@ def test_batch_ng_reaches_the_plain_fixed_point():
         idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
         big = a[:, idx] > 1e-8 * a[:, idx].max(axis=0)
         assert (np.abs(b[:, idx] - a[:, idx])[big] / a[:, idx][big]).max() < 2e-5
+
+
+def _open(models):
+    pts = []
+    for sp, g in models:
+        pt = Portal(0)
+        pt.upload_grid(g)
+        pt.upload_species(sp)
+        pt.set_levpop(g.lev_p)
+        pts.append(pt)
+    return pts
+
+
+def test_se_waves_per_system_give_the_same_bits():
+    # one, two and four waves per SE system in the single-model solve; the 20-level
+    # rotor's 74-dimensional system gives some lanes two rows and the waves a column
+    # split that is not even
+    sp = cloud.co_like_rotor(20)
+    g = cloud.make_cloud(sp, 500, seed=44, n_exp=-1.5)
+    xth = cloud.legendre_all(4, 8)
+    out = []
+    for waves in (1, 2, 4):
+        pt = Portal(0)
+        pt.upload_grid(g)
+        pt.upload_species(sp)
+        pt.set_levpop(g.lev_p)
+        pt.force_se(-1, waves)
+        pp, lev, st = pt.lambda_iterate(xth, 8, cloud.DX_INWARD, 3, tol_pop=0.0, tol_align=0.0,
+                                        precision=PORTAL_FP64)
+        assert st["se_waves"] == waves and st["se_form"] == 0
+        out.append((pp, lev))
+        pt.close()
+    assert np.isfinite(out[0][0]).all() and np.abs(out[0][0][:, 0]).max() > 0
+    for pp, lev in out[1:]:
+        assert np.array_equal(pp, out[0][0])
+        assert np.array_equal(lev, out[0][1])
+
+
+@pytest.mark.parametrize("precision", [PORTAL_FP64, PORTAL_MIXED])
+def test_batch_se_forms_give_the_same_bits(precision):
+    # the one persistent SE launch and the launch per class at 1, 2 and 4 waves per
+    # system (and whatever the autotuner picks) solve each model's systems identically
+    specs = [(500, 4, 41), (700, 12, 42), (600, 20, 43)]
+    models = []
+    for n, L, seed in specs:
+        sp = cloud.co_like_rotor(L)
+        models.append((sp, cloud.make_cloud(sp, n, seed=seed, n_exp=-1.4)))
+    xth = cloud.legendre_all(4, 8)
+    runs = []
+    for form, waves in [(0, 0), (1, 1), (1, 2), (1, 4), (-1, 0)]:
+        pts = _open(models)
+        pts[0].force_se(form, waves)
+        per, tot = batch_lambda_iterate(pts, [xth] * 3, [8] * 3, cloud.DX_INWARD, 3, tol_pop=0.0, tol_align=0.0,
+                                        precision=precision)
+        print(form, waves, tot["se_form"], tot["se_waves"])
+        assert tot["iterations"] == 3
+        if form >= 0:
+            assert tot["se_form"] == form
+        if form == 1 and waves:
+            assert tot["se_waves"] == waves
+        runs.append([(pt.get_pol_pops(), pt.get_levpop()) for pt in pts])
+        for pt in pts:
+            pt.close()
+    for r in runs[1:]:
+        for (pa, la), (pb, lb) in zip(runs[0], r):
+            assert np.array_equal(pa, pb)
+            assert np.array_equal(la, lb)
