@@ -307,10 +307,12 @@ hipError_t launch_rt_persistent(bool mixed, const RtArgs *models, const int2 *it
                                 int cap, int R_max, int waves, int n_blocks, hipStream_t s);
 hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd_max,
                                 int nt_max, int n_blocks, hipStream_t s);
-// one workgroup of `waves` (1, 2 or 4) waves per item, LDS sized for (cd_max, nt_max)
-hipError_t launch_se_items(const SeArgs *models, const int2 *items, int n_items, int cd_max, int nt_max, int waves,
-                           hipStream_t s);
-size_t se_lds_bytes(int c_dim, int n_trans);
+// persistent workgroups of `waves` (1, 2 or 4) waves over the items of one system size
+// (cd, nt), each with a slab of cd*cd doubles at slabs + blockIdx.x*cd*cd
+hipError_t launch_se_slab(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd, int nt,
+                          int waves, int n_blocks, double *slabs, hipStream_t s);
+int se_slab_blocks_per_cu(int waves, int cd, int nt);
+size_t se_lds_bytes(int c_dim, int n_trans, bool m0_in_lds = true);
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves);
 // the cooperative form of the persistent integration, one launch per slot count
 hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,

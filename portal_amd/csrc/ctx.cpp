@@ -1768,24 +1768,29 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     const size_t rt_lds = rt_persistent_lds(mixed, cap, R_max, best_w);
     if (!coop && rt_lds > 160 * 1024) return fail(c0, PORTAL_ERR_ARG, "batch paths too long for LDS staging");
     const int rt_blocks = n_cu * std::max(1, std::min(8 / std::max(1, best_w / 2), (int)((160 * 1024) / rt_lds)));
-    const size_t se_lds = se_lds_bytes(cd_max, nt_max);
-    if (se_lds > 160 * 1024) return fail(c0, PORTAL_ERR_ARG, "SE system too large for one workgroup's LDS");
+    const size_t se_lds = se_lds_bytes(cd_max, nt_max, true);
     const int se_blocks = n_cu * std::max(1, std::min(16, (int)((160 * 1024) / se_lds)));
     // The SE solves: one persistent launch whose LDS fits the largest system (form 0), or
-    // a launch per class of models with one system size, each with its own LDS and its
-    // own waves per system (form 1).  The autotuner times both on every k-th item.
+    // a persistent launch per class of models with one system size (form 1), each with
+    // its own waves per system and LDS; there the residual's copy of the matrix lives in
+    // a global slab per workgroup, which leaves room for more systems per CU.  The
+    // autotuner times both forms on every k-th item.
     struct SeClass {
-        int cd = 0, nt = 1, waves = 1;
+        int cd = 0, nt = 1, waves = 1, blocks = 0;
         std::vector<int2> items, sample;
         int2 *d_items = nullptr, *d_sample = nullptr;
     };
     std::vector<SeClass> se_cls;
     int se_form = 1, se_waves_or = 0;
+    int *d_se_cnt = nullptr;
+    double *d_slabs = nullptr;
     auto free_se_cls = [&]() {
         for (auto &k : se_cls) {
             dfree(k.d_items);
             dfree(k.d_sample);
         }
+        dfree(d_se_cnt);
+        dfree(d_slabs);
     };
     {
         std::vector<int> cls_of(n, -1);
@@ -1803,35 +1808,51 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         const size_t stride = std::max<size_t>(1, si.size() / 65536);
         std::vector<int2> gsample;
         for (size_t i = 0; i < si.size(); i += stride) gsample.push_back(si[i]);
+        size_t slab_doubles = 1;
         for (auto &k : se_cls) {
             for (size_t i = 0; i < k.items.size(); i += stride) k.sample.push_back(k.items[i]);
             HIP_TRY(&c0, upload(k.d_items, k.items));
             HIP_TRY(&c0, upload(k.d_sample, k.sample));
+            for (int w : {1, 2, 4})
+                slab_doubles = std::max(slab_doubles, (size_t)n_cu * se_slab_blocks_per_cu(w, k.cd, k.nt) * k.cd * k.cd);
         }
+        HIP_TRY(&c0, hipMalloc((void **)&d_se_cnt, sizeof(int) * std::max<size_t>(1, se_cls.size())));
+        HIP_TRY(&c0, hipMalloc((void **)&d_slabs, sizeof(double) * slab_doubles));
         const int forced_form = c0.se_forced_form, forced_w = c0.se_forced_waves;
         double ms_cls = 0.0;
         for (auto &k : se_cls) {
             double best = 1e30;
             for (int w : {1, 2, 4}) {
                 if (forced_w > 0 && w != forced_w) continue;
-                if (forced_w == 0 && forced_form == 0) break;  // the classes are not used
+                const int bpc = se_slab_blocks_per_cu(w, k.cd, k.nt);
+                if (bpc < 1) continue;
+                if (forced_form == 0 && k.blocks > 0) break;  // the classes are not used: any runnable choice
+                HIP_TRY(&c0, launch_fill_zero(d_se_cnt, sizeof(int), s));
                 Timer t(s);
-                hipError_t e = launch_se_items(d_se, k.d_sample, (int)k.sample.size(), k.cd, k.nt, w, s);
+                hipError_t e = launch_se_slab(d_se, k.d_sample, (int)k.sample.size(), d_se_cnt, k.cd, k.nt, w,
+                                              n_cu * bpc, d_slabs, s);
                 const double ms = t.stop();
                 if (e != hipSuccess) {
                     (void)hipGetLastError();
                     continue;
                 }
-                if (ms < best) {
+                if (ms < best || k.blocks == 0) {
                     best = ms;
                     k.waves = w;
+                    k.blocks = n_cu * bpc;
                 }
+            }
+            if (k.blocks == 0 && forced_form != 0) {
+                free_se_cls();
+                free_xths();
+                return fail(c0, PORTAL_ERR_ARG, "SE system of dimension " + std::to_string(k.cd) +
+                                                    " does not fit one workgroup");
             }
             ms_cls += best;
             se_waves_or |= k.waves;
         }
         double ms_one = 1e30;
-        if (forced_form != 1) {
+        if (forced_form != 1 && se_lds <= 160 * 1024) {
             int2 *d_gs = nullptr;
             HIP_TRY(&c0, upload(d_gs, gsample));
             HIP_TRY(&c0, launch_fill_zero(d_cnt, sizeof(int) * n_cnt, s));
@@ -1845,6 +1866,11 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
             dfree(d_gs);
         }
         se_form = forced_form >= 0 ? forced_form : (ms_one <= ms_cls ? 0 : 1);
+        if (se_form == 0 && se_lds > 160 * 1024) {
+            free_se_cls();
+            free_xths();
+            return fail(c0, PORTAL_ERR_ARG, "SE system too large for one workgroup's LDS");
+        }
         if (se_form == 0) se_waves_or = 1;
         // the timing launches solved systems of partial fields: their flags do not count
         HIP_TRY(&c0, hipMemset(c0.d_flags, 0, sizeof(int) * 4));
@@ -1854,6 +1880,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     hipGraphExec_t exec = nullptr;
     HIP_TRY(&c0, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     launch_fill_zero(d_cnt, sizeof(int) * n_cnt, s);
+    launch_fill_zero(d_se_cnt, sizeof(int) * std::max<size_t>(1, se_cls.size()), s);
     launch_fill_zero(d_max, sizeof(unsigned long long) * 2 * n, s);
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
@@ -1874,7 +1901,11 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     if (se_form == 0)
         launch_se_persistent(d_se, d_si, (int)si.size(), d_cnt + 1, cd_max, nt_max, se_blocks, s);
     else
-        for (const SeClass &k : se_cls) launch_se_items(d_se, k.d_items, (int)k.items.size(), k.cd, k.nt, k.waves, s);
+        for (size_t k = 0; k < se_cls.size(); k++) {
+            const SeClass &cl = se_cls[k];
+            launch_se_slab(d_se, cl.d_items, (int)cl.items.size(), d_se_cnt + k, cl.cd, cl.nt, cl.waves, cl.blocks,
+                           d_slabs, s);
+        }
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
         launch_lambda_update((int)c.g.work.size(), c.g.N, c.sp.e_num, c.g.d_work, c.g.d_ab, c.g.d_h2, c.sp.d_jlev,

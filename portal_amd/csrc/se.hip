@@ -28,22 +28,32 @@ __device__ __forceinline__ void two_prod(double a, double b, double &p, double &
     e = fma(a, b, -p);
 }
 
+// lane l's x to the whole wave (l wave-uniform)
+__device__ __forceinline__ double lane_bcast(double x, int l) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // A system is solved by W wavefronts together (64W threads, "waves per system"): the
 // elementwise loops stride over all of them, the trailing update of each LU step splits
 // its columns between the waves (column j to wave j mod W), and every wave runs the
 // pivot search on its own, so the factors and the solution do not depend on W.
-template <int W>
-__device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *sm) {
+// G: the copy of the assembled matrix that the refinement residual reads lives in
+// global memory at M0g (a slab of the workgroup's own) rather than in LDS.
+template <int W, bool G>
+__device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *sm, double *M0g) {
     constexpr int NT = 64 * W;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int node = a.work[wi];
     const int cd = a.c_dim, nt = a.n_trans, L = a.L, R = a.R;
     double *x = sm;                          // 10*nt scalars
     double *M = x + 10 * nt;                 // LU in place, column-major cd x cd
-    double *M0 = M + cd * cd;                // assembled matrix (for the residual)
-    double *v = M0 + cd * cd;                // solution
+    double *M0 = G ? M0g : M + cd * cd;     // assembled matrix (for the residual)
+    double *v = G ? M + cd * cd : M0 + cd * cd;  // solution
     double *y = v + cd;                      // rhs / work
-    int *piv = (int *)(y + cd);              // pivots
+    int *perm = (int *)(y + cd);             // row permutation of the factorisation
     __shared__ int s_info;
 
     const double T = a.tkin[node];
@@ -124,41 +134,67 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
     for (int i = tid; i < cd * cd; i += NT) M0[i] = M[i];
     __syncthreads();
 
-    // LU with partial pivoting; in the trailing update a lane owns rows k+1+lane,
-    // k+65+lane, ... of its wave's columns
+    for (int i = tid; i < cd; i += NT) perm[i] = i;
+    __syncthreads();
+
+    // LU with partial pivoting, three barriers per column.  A: every wave finds the
+    // pivot in column k and the rows k and bi are swapped in every column but k.
+    // B: column k is scaled, taking the swap of its entries k and bi into account.
+    // C: the trailing update, a lane owning rows k+1+lane, k+65+lane of its wave's
+    // columns; M[k][k] receives the pivot.  perm records the swaps for the solve.
     for (int k = 0; k < cd; k++) {
-        double best = -1.0;
+        double best = -1.0, bv = 0.0;
         int bi = cd;
         for (int i = k + lane; i < cd; i += 64) {
-            const double vv = fabs(M[i + (size_t)cd * k]);
-            if (vv > best) { best = vv; bi = i; }
+            const double mv = M[i + (size_t)cd * k], vv = fabs(mv);
+            if (vv > best) { best = vv; bi = i; bv = mv; }
         }
         for (int o = 32; o > 0; o >>= 1) {
             const double ob = __shfl_xor(best, o, 64);
             const int oi = __shfl_xor(bi, o, 64);
-            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            const double ov = __shfl_xor(bv, o, 64);
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; bv = ov; }
         }
         if (!(best > 0.0)) {
             if (tid == 0) s_info = k + 1;
             break;
         }
-        if (tid == 0) piv[k] = bi;
-        // every wave has read column k before any wave swaps rows
-        if (W > 1) __syncthreads();
-        if (bi != k)
+        const double pv = bv;
+        if (bi != k) {
+            if (tid == 0) {
+                const int t = perm[k];
+                perm[k] = perm[bi];
+                perm[bi] = t;
+            }
             for (int j = tid; j < cd; j += NT) {
+                if (j == k) continue;
                 const double t = M[k + (size_t)cd * j];
                 M[k + (size_t)cd * j] = M[bi + (size_t)cd * j];
                 M[bi + (size_t)cd * j] = t;
             }
+        }
         __syncthreads();
-        const double pv = M[k + (size_t)cd * k];
-        for (int i = k + 1 + tid; i < cd; i += NT) M[i + (size_t)cd * k] = M[i + (size_t)cd * k] / pv;
+        for (int i = k + 1 + tid; i < cd; i += NT) {
+            const double mv = (i == bi) ? M[k + (size_t)cd * k] : M[i + (size_t)cd * k];
+            M[i + (size_t)cd * k] = mv / pv;
+        }
         __syncthreads();
+        if (tid == 0) M[k + (size_t)cd * k] = pv;
         for (int i = k + 1 + lane; i < cd; i += 64) {
             const double li = M[i + (size_t)cd * k];
-            for (int j = k + 1 + wv; j < cd; j += W)
-                M[i + (size_t)cd * j] = M[i + (size_t)cd * j] - li * M[k + (size_t)cd * j];
+            int j = k + 1 + wv;
+            // eight columns' loads in flight before their stores (rows i and k differ)
+            for (; j + 7 * W < cd; j += 8 * W) {
+                double u[8], m[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    u[q] = M[k + (size_t)cd * (j + q * W)];
+                    m[q] = M[i + (size_t)cd * (j + q * W)];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) M[i + (size_t)cd * (j + q * W)] = m[q] - li * u[q];
+            }
+            for (; j < cd; j += W) M[i + (size_t)cd * j] = M[i + (size_t)cd * j] - li * M[k + (size_t)cd * j];
         }
         __syncthreads();
     }
@@ -168,47 +204,105 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
         if (tid == 0) atomicOr(a.flags, FLAG_LU);
         return;
     }
-    // solve with rhs e1, then one refinement step
-    for (int it = 0; it < 2; it++) {
-        if (it == 0) {
-            for (int i = tid; i < cd; i += NT) y[i] = (i == 0) ? 1.0 : 0.0;
-        } else {
-            // compensated residual r = e1 - M0 v
-            for (int i = tid; i < cd; i += NT) {
-                double s = (i == 0) ? 1.0 : 0.0, c = 0.0;
-                for (int j = 0; j < cd; j++) {
-                    double p, e;
-                    two_prod(-M0[i + (size_t)cd * j], v[j], p, e);
-                    const double t = s + p;
-                    const double z = t - s;
-                    c += (s - (t - z)) + (p - z) + e;
-                    s = t;
+    // Solve for e1, then one refinement step with a compensated residual, in wave 0's
+    // registers with no barrier: lane l holds rows l and l + 64 (cd <= 128), the
+    // permutation is a gather by perm, and each substitution step broadcasts one
+    // element by readlane.
+    if (wv == 0) {
+        const int i0 = lane, i1 = lane + 64;
+        const bool h0 = i0 < cd, h1 = i1 < cd;
+        const int p0 = h0 ? perm[i0] : 0, p1 = h1 ? perm[i1] : 0;
+        double b0 = (i0 == 0) ? 1.0 : 0.0, b1 = 0.0, v0 = 0.0, v1 = 0.0;
+        for (int it = 0; it < 2; it++) {
+            if (it == 1) {
+                // r = e1 - M0 v
+                double s0 = (i0 == 0) ? 1.0 : 0.0, c0 = 0.0, s1 = 0.0, c1 = 0.0;
+                for (int j0 = 0; j0 < cd; j0 += 8) {
+                    double m0[8], m1[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        m0[q] = (h0 && j0 + q < cd) ? M0[i0 + (size_t)cd * (j0 + q)] : 0.0;
+                        m1[q] = (h1 && j0 + q < cd) ? M0[i1 + (size_t)cd * (j0 + q)] : 0.0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const int j = j0 + q;
+                        if (j >= cd) break;
+                        const double vj = lane_bcast(j < 64 ? v0 : v1, j & 63);
+                        if (h0) {
+                            double p, e;
+                            two_prod(-m0[q], vj, p, e);
+                            const double t = s0 + p;
+                            const double z = t - s0;
+                            c0 += (s0 - (t - z)) + (p - z) + e;
+                            s0 = t;
+                        }
+                        if (h1) {
+                            double p, e;
+                            two_prod(-m1[q], vj, p, e);
+                            const double t = s1 + p;
+                            const double z = t - s1;
+                            c1 += (s1 - (t - z)) + (p - z) + e;
+                            s1 = t;
+                        }
+                    }
                 }
-                y[i] = s + c;
+                b0 = s0 + c0;
+                b1 = s1 + c1;
+            }
+            const double g00 = __shfl(b0, p0 & 63, 64), g01 = __shfl(b1, p0 & 63, 64);
+            const double g10 = __shfl(b0, p1 & 63, 64), g11 = __shfl(b1, p1 & 63, 64);
+            double y0 = p0 < 64 ? g00 : g01, y1 = p1 < 64 ? g10 : g11;
+            // the factors' entries are loaded eight steps at a time
+            for (int k0 = 0; k0 < cd; k0 += 8) {
+                double l0[8], l1[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    l0[q] = (h0 && k0 + q < cd) ? M[i0 + (size_t)cd * (k0 + q)] : 0.0;
+                    l1[q] = (h1 && k0 + q < cd) ? M[i1 + (size_t)cd * (k0 + q)] : 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int k = k0 + q;
+                    if (k >= cd) break;
+                    const double yk = lane_bcast(k < 64 ? y0 : y1, k & 63);
+                    if (h0 && i0 > k) y0 = y0 - l0[q] * yk;
+                    if (h1 && i1 > k) y1 = y1 - l1[q] * yk;
+                }
+            }
+            for (int k0 = (cd - 1) & ~7; k0 >= 0; k0 -= 8) {
+                double l0[8], l1[8], dg[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int kk = k0 + q;
+                    l0[q] = (h0 && kk < cd) ? M[i0 + (size_t)cd * kk] : 0.0;
+                    l1[q] = (h1 && kk < cd) ? M[i1 + (size_t)cd * kk] : 0.0;
+                    dg[q] = kk < cd ? M[kk + (size_t)cd * kk] : 1.0;
+                }
+#pragma unroll
+                for (int q = 7; q >= 0; q--) {
+                    const int k = k0 + q;
+                    if (k >= cd) continue;
+                    const double dk = dg[q];
+                    if (i0 == k) y0 = y0 / dk;
+                    if (i1 == k) y1 = y1 / dk;
+                    const double yk = lane_bcast(k < 64 ? y0 : y1, k & 63);
+                    if (i0 < k) y0 = y0 - l0[q] * yk;
+                    if (h1 && i1 < k) y1 = y1 - l1[q] * yk;
+                }
+            }
+            if (it == 0) {
+                v0 = y0;
+                v1 = y1;
+            } else {
+                v0 = v0 + y0;
+                v1 = v1 + y1;
             }
         }
-        __syncthreads();
-        if (tid == 0)
-            for (int k = 0; k < cd; k++) {
-                const int p = piv[k];
-                if (p != k) { const double t = y[k]; y[k] = y[p]; y[p] = t; }
-            }
-        __syncthreads();
-        for (int k = 0; k < cd; k++) {
-            const double yk = y[k];
-            for (int i = k + 1 + tid; i < cd; i += NT) y[i] = y[i] - M[i + (size_t)cd * k] * yk;
-            __syncthreads();
-        }
-        for (int k = cd - 1; k >= 0; k--) {
-            if (tid == 0) y[k] = y[k] / M[k + (size_t)cd * k];
-            __syncthreads();
-            const double yk = y[k];
-            for (int i = tid; i < k; i += NT) y[i] = y[i] - M[i + (size_t)cd * k] * yk;
-            __syncthreads();
-        }
-        for (int i = tid; i < cd; i += NT) v[i] = (it == 0) ? y[i] : v[i] + y[i];
-        __syncthreads();
+        if (h0) v[i0] = v0;
+        if (h1) v[i1] = v1;
     }
+    __syncthreads();
     // pack (rho00, rho20) per level, NaN -> 0 (main_sub.f90:112-137)
     const int N = a.N;
     for (int i = tid; i < L; i += NT) {
@@ -253,7 +347,7 @@ template <int W>
 __global__ void __launch_bounds__(64 * W) se_kernel(SeArgs a) {
     extern __shared__ double sm[];
     if ((int)blockIdx.x >= a.n_work) return;
-    se_node<W>(a, blockIdx.x, sm);
+    se_node<W, false>(a, blockIdx.x, sm, nullptr);
 }
 
 // Persistent form over the (model, node) items of a batch of models, one wave per
@@ -269,42 +363,77 @@ __global__ void __launch_bounds__(64) se_persistent_kernel(const SeArgs *models,
         __syncthreads();
         if (it >= n_items) break;
         const int2 mi = items[it];
-        se_node<1>(models[mi.x], mi.y, sm);
+        se_node<1, false>(models[mi.x], mi.y, sm, nullptr);
         __syncthreads();
     }
 }
 
-// One workgroup of W waves per (model, node) item: a class of models whose systems
-// share one LDS size.
+// Persistent form for a class of models with one system size: W waves per system, the
+// residual's matrix copy in the workgroup's slab of `slab` doubles, so that LDS holds
+// only the transition scalars, the factors and two vectors.
 template <int W>
-__global__ void __launch_bounds__(64 * W) se_items_kernel(const SeArgs *models, const int2 *items, int n_items) {
+__global__ void __launch_bounds__(64 * W) se_slab_kernel(const SeArgs *models, const int2 *items, int n_items,
+                                                        int *counter, double *slabs, int slab) {
     extern __shared__ double sm[];
-    if ((int)blockIdx.x >= n_items) return;
-    const int2 mi = items[blockIdx.x];
-    se_node<W>(models[mi.x], mi.y, sm);
+    __shared__ int s_item;
+    double *M0 = slabs + (size_t)blockIdx.x * slab;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const int2 mi = items[it];
+        se_node<W, true>(models[mi.x], mi.y, sm, M0);
+        __syncthreads();
+    }
 }
 
-size_t se_lds_bytes(int c_dim, int n_trans) {
-    return sizeof(double) * (10 * (size_t)n_trans + 2 * (size_t)c_dim * c_dim + 2 * c_dim) + sizeof(int) * c_dim + 16;
+size_t se_lds_bytes(int c_dim, int n_trans, bool m0_in_lds) {
+    return sizeof(double) * (10 * (size_t)n_trans + (m0_in_lds ? 2 : 1) * (size_t)c_dim * c_dim + 2 * c_dim) +
+           sizeof(int) * c_dim + 16;
 }
 
 hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd_max,
                                 int nt_max, int n_blocks, hipStream_t s) {
-    const size_t lds = se_lds_bytes(cd_max, nt_max);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const size_t lds = se_lds_bytes(cd_max, nt_max, true);
+    if (lds > 160 * 1024 || cd_max > 128) return hipErrorInvalidValue;
     hipLaunchKernelGGL(se_persistent_kernel, dim3(n_blocks), dim3(64), lds, s, models, items, n_items, counter);
     return hipGetLastError();
 }
 
-hipError_t launch_se_items(const SeArgs *models, const int2 *items, int n_items, int cd_max, int nt_max, int waves,
-                           hipStream_t s) {
-    if (n_items == 0) return hipSuccess;
-    const size_t lds = se_lds_bytes(cd_max, nt_max);
-    if (lds > 160 * 1024 || n_items < 0) return hipErrorInvalidValue;
+int se_slab_blocks_per_cu(int waves, int cd, int nt) {
+    const size_t lds = se_lds_bytes(cd, nt, false);
+    if (lds > 160 * 1024 || cd > 128) return 0;
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
     switch (waves) {
-        case 1: hipLaunchKernelGGL(se_items_kernel<1>, dim3(n_items), dim3(64), lds, s, models, items, n_items); break;
-        case 2: hipLaunchKernelGGL(se_items_kernel<2>, dim3(n_items), dim3(128), lds, s, models, items, n_items); break;
-        case 4: hipLaunchKernelGGL(se_items_kernel<4>, dim3(n_items), dim3(256), lds, s, models, items, n_items); break;
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, se_slab_kernel<1>, 64, lds); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, se_slab_kernel<2>, 128, lds); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, se_slab_kernel<4>, 256, lds); break;
+        default: break;
+    }
+    return e == hipSuccess ? nb : 0;
+}
+
+hipError_t launch_se_slab(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd, int nt,
+                          int waves, int n_blocks, double *slabs, hipStream_t s) {
+    const size_t lds = se_lds_bytes(cd, nt, false);
+    if (lds > 160 * 1024 || cd > 128 || n_items < 0 || n_blocks < 1) return hipErrorInvalidValue;
+    const int slab = cd * cd;
+    switch (waves) {
+        case 1:
+            hipLaunchKernelGGL(se_slab_kernel<1>, dim3(n_blocks), dim3(64), lds, s, models, items, n_items, counter,
+                               slabs, slab);
+            break;
+        case 2:
+            hipLaunchKernelGGL(se_slab_kernel<2>, dim3(n_blocks), dim3(128), lds, s, models, items, n_items, counter,
+                               slabs, slab);
+            break;
+        case 4:
+            hipLaunchKernelGGL(se_slab_kernel<4>, dim3(n_blocks), dim3(256), lds, s, models, items, n_items, counter,
+                               slabs, slab);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -312,8 +441,8 @@ hipError_t launch_se_items(const SeArgs *models, const int2 *items, int n_items,
 
 hipError_t launch_se(const SeArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
-    const size_t lds = se_lds_bytes(a.c_dim, a.n_trans);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const size_t lds = se_lds_bytes(a.c_dim, a.n_trans, true);
+    if (lds > 160 * 1024 || a.c_dim > 128) return hipErrorInvalidValue;
     switch (a.waves <= 1 ? 1 : a.waves) {
         case 1: hipLaunchKernelGGL(se_kernel<1>, dim3(a.n_work), dim3(64), lds, s, a); break;
         case 2: hipLaunchKernelGGL(se_kernel<2>, dim3(a.n_work), dim3(128), lds, s, a); break;

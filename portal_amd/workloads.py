@@ -148,7 +148,9 @@ def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True):
            "levels_per_model": [s[1] for s in sizes], "nodes_per_model": [s[2] for s in sizes],
            "device_ms": dev_ms, "iteration_ms": rt_ms,
            "inward_form": {1: "staged persistent", 2: "cooperative persistent",
-                           3: "cooperative, one workgroup per item"}.get(tot["rt_variant"], "?")}
+                           3: "cooperative, one workgroup per item"}.get(tot["rt_variant"], "?"),
+           "se_form": {0: "one persistent launch", 1: "a launch per system size"}.get(tot["se_form"], "?"),
+           "se_waves_used": [w for w in (1, 2, 4) if tot["se_waves"] & w]}
     out["unconverged"] = [{"model": m, "levels": sizes[m][1], "nodes": sizes[m][2],
                            "pop_change": per[m]["max_rel_change"], "align_change": per[m]["max_align_change"]}
                           for m in range(n_models) if not iters_at[m]]
