@@ -184,8 +184,9 @@ int portal_lambda_iterate(portal_ctx *ctx, const portal_se_params *p, const port
  * a captured hipGraph holding each model's line preparation, the inward integration
  * over all (model, node) items ordered by cost (one launch per class of models with the
  * same slot count; persistent workgroups or one workgroup per item, whichever timed
- * faster on the batch), ONE persistent SE solve over all items and each model's
- * closure.  it->ng_accel = 1 applies Ng acceleration per model between
+ * faster on the batch), the SE solves (one persistent launch per system size with
+ * 1, 2 or 4 waves per system, or one persistent launch over all items, whichever timed
+ * faster; the same bits either way) and each model's closure.  it->ng_accel = 1 applies Ng acceleration per model between
  * replays; Anderson acceleration, the own-cell operator and the surrogate are for
  * single models (PORTAL_ERR_UNSUPPORTED).  Iterates until every model meets the
  * tolerances (or max_iter); per_model (n, may be NULL) gets each model's iteration
