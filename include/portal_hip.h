@@ -40,7 +40,7 @@ extern "C" {
 #define PORTAL_ERR_TEMP (-6)         /* T_kin above the collision table, miscellaneous.f90:331-360 */
 #define PORTAL_ERR_STATE (-7)        /* grid/species not uploaded */
 #define PORTAL_ERR_NOMEM (-8)        /* device allocation failed */
-#define PORTAL_ERR_UNSUPPORTED (-9)  /* dust or star mode in a batch of models */
+#define PORTAL_ERR_UNSUPPORTED (-9)  /* a batch option that is for single models */
 
 #define PORTAL_FP64 0   /* every stage in FP64 */
 #define PORTAL_MIXED 1  /* FP32 ray transfer + compensated FP32 J sums; FP64 SE */
@@ -186,9 +186,11 @@ int portal_lambda_iterate(portal_ctx *ctx, const portal_se_params *p, const port
  * same slot count; persistent workgroups or one workgroup per item, whichever timed
  * faster on the batch), the SE solves (one persistent launch per system size with
  * 1, 2 or 4 waves per system, or one persistent launch over all items, whichever timed
- * faster; the same bits either way) and each model's closure.  it->ng_accel = 1 applies Ng acceleration per model between
- * replays; Anderson acceleration, the own-cell operator and the surrogate are for
- * single models (PORTAL_ERR_UNSUPPORTED).  Iterates until every model meets the
+ * faster; the same bits either way) and each model's closure.  Models with dust
+ * (dust classes of the cooperative kernel) or in star mode (their per-node direction
+ * sets) are integrated as in the single-model pass.  it->ng_accel = 1 applies Ng
+ * acceleration per model between replays; Anderson acceleration, the own-cell
+ * operator and the surrogate are for single models (PORTAL_ERR_UNSUPPORTED).  Iterates until every model meets the
  * tolerances (or max_iter); per_model (n, may be NULL) gets each model's iteration
  * count at convergence and whether it converged, total the device time of the whole
  * batch.  A further call continues each model's iteration.  Results stay in each

@@ -314,10 +314,11 @@ hipError_t launch_se_slab(const SeArgs *models, const int2 *items, int n_items, 
 int se_slab_blocks_per_cu(int waves, int cd, int nt);
 size_t se_lds_bytes(int c_dim, int n_trans, bool m0_in_lds = true);
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves);
-// the cooperative form of the persistent integration, one launch per slot count
+// the cooperative form of the persistent integration, one launch per (slot count, dust)
 hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,
-                                     int cap, int slots, int waves, size_t lds, int n_blocks, hipStream_t s);
-size_t rt_coop_persistent_lds(bool mixed, int cap, int R_max, int C);
+                                     int cap, int slots, int waves, size_t lds, int n_blocks, bool dust,
+                                     hipStream_t s);
+size_t rt_coop_persistent_lds(bool mixed, int cap, int R_max, int C, bool dust);
 int coop_slots_host(int R, int C, int W);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 hipError_t launch_ng_dots(int n_work, int L, const int *work, const double *x0, const double *x1, const double *x2,

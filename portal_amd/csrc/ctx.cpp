@@ -1515,9 +1515,6 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         Ctx &c = xs[m]->c;
         int rc = check_se_params(c, &params[m]);
         if (rc) return fail(c0, rc, "model " + std::to_string(m) + ": " + c.err);
-        if (c.g.has_dust || params[m].star_mod)
-            return fail(c0, PORTAL_ERR_UNSUPPORTED,
-                        "model " + std::to_string(m) + ": dust or star mode in a batch is not supported");
         chained[m] = c.pol_chain ? 1 : 0;
         c.pol_chain = false;
     }
@@ -1529,9 +1526,26 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     std::vector<RtArgs> rts(n);
     std::vector<SeArgs> ses(n);
     std::vector<std::pair<double, int2>> rt_items, se_items;
-    std::vector<double *> d_xths(n, nullptr);
+    std::vector<double *> d_xths(n, nullptr), d_istars(n, nullptr);
+    std::vector<char> dusty(n, 0);
     auto free_xths = [&]() {
         for (auto *&p : d_xths) dfree(p);
+        for (auto *&p : d_istars) dfree(p);
+    };
+    // line coefficients of model m, with its dust coefficients when it has dust
+    auto prep_of = [&](int m) {
+        Ctx &c = xs[m]->c;
+        PrepArgs pa{c.g.N, c.sp.r_num, c.sp.e_num, c.d_levp, c.g.d_velb, c.sp.d_l1, c.sp.d_l2, c.sp.d_c1B, c.sp.d_c2A,
+                    nullptr, c.sp.d_g1, c.sp.d_g2, c.sp.d_cnu_spi, mixed ? nullptr : c.d_KS, mixed ? c.d_KS32 : nullptr};
+        if (dusty[m]) {
+            pa.dust = c.g.d_dust;
+            pa.tdust = c.g.d_tdust;
+            pa.kd = c.sp.d_kd;
+            pa.nu = c.sp.d_nu;
+            pa.KSD = mixed ? nullptr : c.d_KSD;
+            pa.KSD32 = mixed ? c.d_KSD32 : nullptr;
+        }
+        return pa;
     };
     int cap = 8, R_max = 1, cd_max = 1, nt_max = 1;
     for (int m = 0; m < n; m++) {
@@ -1545,7 +1559,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         }
         if (total) total->t_walk_ms += st.t_walk_ms;
         const Species &sp = c.sp;
-        const int D = params[m].n_sphere, nw = (int)c.g.work.size();
+        const int D = c.paths.D, nw = (int)c.g.work.size();
         RtArgs &ra = rts[m];
         ra = RtArgs{};
         ra.n_work = nw;
@@ -1567,10 +1581,32 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         ra.variant = 1;
         ra.max_nc = (c.paths.max_nc + 7) & ~7;
         {
-            std::vector<double> xv(params[m].xth, params[m].xth + 3 * (size_t)D);
+            std::vector<double> xv(params[m].xth, params[m].xth + 3 * (size_t)params[m].n_sphere);
             HIP_TRY(&c0, upload(d_xths[m], xv));
         }
         ra.xth = d_xths[m];
+        // dust, as in the single-model pass: some node with dust and some line with a dust
+        // opacity (inverse_rt.f90:273-287)
+        bool dust = false;
+        if (c.g.has_dust)
+            for (double kd : sp.kd_v) dust = dust || kd != 0.0;
+        if (dust) {
+            if (!c.d_KSD) HIP_TRY(&c0, hipMalloc((void **)&c.d_KSD, sizeof(double) * 2 * c.g.N * sp.r_num));
+            if (!c.d_KSD32) HIP_TRY(&c0, hipMalloc((void **)&c.d_KSD32, sizeof(float) * 2 * c.g.N * sp.r_num));
+        }
+        dusty[m] = dust ? 1 : 0;
+        ra.dust = dust ? 1 : 0;
+        ra.KSD = c.d_KSD;
+        ra.KSD32 = c.d_KSD32;
+        if (c.paths.star) {
+            // rays towards the star start with B_nu(T_star) (inverse_rt.f90:113-117)
+            std::vector<double> Is(sp.r_num);
+            for (int k = 0; k < sp.r_num; k++) Is[k] = black_body(params[m].T_star, sp.nu_hz[k]);
+            HIP_TRY(&c0, upload(d_istars[m], Is));
+            ra.xthn = c.paths.d_xthn;
+            ra.nstar = c.paths.d_nstar;
+            ra.I_star = d_istars[m];
+        }
         rc = check_rt_args(c, ra, mixed);
         if (rc) {
             free_xths();
@@ -1641,6 +1677,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     constexpr int COOP_W = 4;
     struct CoopClass {
         int slots = 0, cap = 8, R_max = 1, C_max = 1;
+        bool dust = false;
         std::vector<int2> items;
         size_t lds = 0;
         int2 *d_items = nullptr;
@@ -1656,11 +1693,12 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
                 break;
             }
             for (size_t k = 0; k < classes.size(); k++)
-                if (classes[k].slots == sl) cls_of[m] = (int)k;
+                if (classes[k].slots == sl && classes[k].dust == (dusty[m] != 0)) cls_of[m] = (int)k;
             if (cls_of[m] < 0) {
                 cls_of[m] = (int)classes.size();
                 classes.emplace_back();
                 classes.back().slots = sl;
+                classes.back().dust = dusty[m] != 0;
             }
             CoopClass &k = classes[cls_of[m]];
             k.cap = std::max(k.cap, std::min(rts[m].max_nc, COOP_UNIT));
@@ -1670,11 +1708,19 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         if (coop) {
             for (const int2 &itm : ri) classes[cls_of[itm.x]].items.push_back(itm);
             for (auto &k : classes) {
-                k.lds = rt_coop_persistent_lds(mixed, k.cap, k.R_max, k.C_max);
+                k.lds = rt_coop_persistent_lds(mixed, k.cap, k.R_max, k.C_max, k.dust);
                 coop = coop && k.lds <= 160 * 1024;
             }
         }
     }
+    if (!coop)
+        for (int m = 0; m < n; m++)
+            if (dusty[m] || rts[m].xthn) {
+                free_xths();
+                return fail(c0, PORTAL_ERR_UNSUPPORTED, "model " + std::to_string(m) +
+                                                            ": dust or star mode in a batch needs the cooperative "
+                                                            "kernel, and some model's lines and channels do not fit it");
+            }
     RtArgs *d_rt = nullptr;
     SeArgs *d_se = nullptr;
     int2 *d_ri = nullptr, *d_si = nullptr;
@@ -1715,10 +1761,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         const int ns = (int)std::min<size_t>(cl.items.size(), 8192);
         for (int m = 0; m < n; m++) {
             Ctx &c = xs[m]->c;
-            PrepArgs pa{c.g.N, c.sp.r_num, c.sp.e_num, c.d_levp, c.g.d_velb, c.sp.d_l1, c.sp.d_l2, c.sp.d_c1B,
-                        c.sp.d_c2A, nullptr, c.sp.d_g1, c.sp.d_g2, c.sp.d_cnu_spi, mixed ? nullptr : c.d_KS,
-                        mixed ? c.d_KS32 : nullptr};
-            HIP_TRY(&c0, launch_line_prep(pa, s));
+            HIP_TRY(&c0, launch_line_prep(prep_of(m), s));
         }
         double ms_form[2] = {0, 0};
         for (int f = 0; f < 2; f++) {
@@ -1726,7 +1769,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
             const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / std::max<size_t>(cl.lds, 1))));
             Timer t(s);
             hipError_t e = launch_rt_coop_persistent(mixed, d_rt, cl.d_items, ns, d_cnt + 2, cl.cap, cl.slots,
-                                                     COOP_W, cl.lds, f == 0 ? n_cu * bpc : 0, s);
+                                                     COOP_W, cl.lds, f == 0 ? n_cu * bpc : 0, cl.dust, s);
             ms_form[f] = t.stop();
             if (e != hipSuccess) {
                 (void)hipGetLastError();
@@ -1746,10 +1789,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         const int bpc = std::max(1, std::min(8 / (w / 2 > 0 ? w / 2 : 1), (int)((160 * 1024) / lds)));
         for (int m = 0; m < n; m++) {
             Ctx &c = xs[m]->c;
-            PrepArgs pa{c.g.N, c.sp.r_num, c.sp.e_num, c.d_levp, c.g.d_velb, c.sp.d_l1, c.sp.d_l2, c.sp.d_c1B,
-                        c.sp.d_c2A, nullptr, c.sp.d_g1, c.sp.d_g2, c.sp.d_cnu_spi, mixed ? nullptr : c.d_KS,
-                        mixed ? c.d_KS32 : nullptr};
-            HIP_TRY(&c0, launch_line_prep(pa, s));
+            HIP_TRY(&c0, launch_line_prep(prep_of(m), s));
         }
         HIP_TRY(&c0, launch_fill_zero(d_cnt, sizeof(int) * 4, s));
         Timer t(s);
@@ -1884,16 +1924,14 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     launch_fill_zero(d_max, sizeof(unsigned long long) * 2 * n, s);
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
-        PrepArgs pa{c.g.N, c.sp.r_num, c.sp.e_num, c.d_levp, c.g.d_velb, c.sp.d_l1, c.sp.d_l2, c.sp.d_c1B, c.sp.d_c2A,
-                    nullptr, c.sp.d_g1, c.sp.d_g2, c.sp.d_cnu_spi, mixed ? nullptr : c.d_KS, mixed ? c.d_KS32 : nullptr};
-        launch_line_prep(pa, s);
+        launch_line_prep(prep_of(m), s);
     }
     if (coop) {
         for (size_t k = 0; k < classes.size(); k++) {
             const CoopClass &cl = classes[k];
             const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / std::max<size_t>(cl.lds, 1))));
             launch_rt_coop_persistent(mixed, d_rt, cl.d_items, (int)cl.items.size(), d_cnt + 2 + k, cl.cap, cl.slots,
-                                      COOP_W, cl.lds, coop_flat ? 0 : n_cu * bpc, s);
+                                      COOP_W, cl.lds, coop_flat ? 0 : n_cu * bpc, cl.dust, s);
         }
     } else {
         launch_rt_persistent(mixed, d_rt, d_ri, n_items, d_cnt, cap, R_max, best_w, rt_blocks, s);

@@ -340,8 +340,9 @@ hipError_t launch_coop_s(const RtArgs &a, int cap, size_t lds, hipStream_t s) {
 
 // Persistent form over the (model, work item) items of a batch of models whose lines
 // and channels need the same slot count: workgroups take items from a counter until the
-// list is exhausted (every workgroup reaches the exit test after each item).
-template <typename Real, int SLOTS>
+// list is exhausted (every workgroup reaches the exit test after each item).  DUST: the
+// class's models carry dust (every model of a class agrees).
+template <typename Real, int SLOTS, bool DUST>
 __device__ __forceinline__ void coop_persistent_body(const RtArgs *__restrict__ models,
                                                      const int2 *__restrict__ items, int n_items, int *counter,
                                                      int cap) {
@@ -357,16 +358,16 @@ __device__ __forceinline__ void coop_persistent_body(const RtArgs *__restrict__ 
         const int2 mi = items[it];
         const int m = __builtin_amdgcn_readfirstlane(mi.x), w = __builtin_amdgcn_readfirstlane(mi.y);
         const RtArgs am = models[m];
-        rt_coop_node<Real, SLOTS, false, false>(am, w, cap, smem);
+        rt_coop_node<Real, SLOTS, DUST, false>(am, w, cap, smem);
         __syncthreads();
     }
 }
 
-template <typename Real, int SLOTS>
+template <typename Real, int SLOTS, bool DUST>
 __global__ void __launch_bounds__(512) rt_coop_persistent_kernel(const RtArgs *__restrict__ models,
                                                                  const int2 *__restrict__ items, int n_items,
                                                                  int *counter, int cap) {
-    coop_persistent_body<Real, SLOTS>(models, items, n_items, counter, cap);
+    coop_persistent_body<Real, SLOTS, DUST>(models, items, n_items, counter, cap);
 }
 
 // Slots per lane: the fewest consecutive channels per lane with which every line's
@@ -386,31 +387,31 @@ inline int coop_slots(int R, int C, int W) {
 // Flat form over the same items: one workgroup per item, in the items' (cost) order,
 // the hardware's dispatch doing the balancing.  It needs no loop state, so it keeps the
 // register budget of rt_coop_kernel.
-template <typename Real, int SLOTS>
+template <typename Real, int SLOTS, bool DUST>
 __device__ __forceinline__ void coop_flat_body(const RtArgs *__restrict__ models, const int2 *__restrict__ items,
                                                int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int2 mi = items[blockIdx.x];
     const int m = __builtin_amdgcn_readfirstlane(mi.x), w = __builtin_amdgcn_readfirstlane(mi.y);
-    rt_coop_node<Real, SLOTS, false, false>(models[m], w, cap, smem);
+    rt_coop_node<Real, SLOTS, DUST, false>(models[m], w, cap, smem);
 }
 
-template <typename Real, int SLOTS>
+template <typename Real, int SLOTS, bool DUST>
 __global__ void __launch_bounds__(512) rt_coop_flat_kernel(const RtArgs *__restrict__ models,
                                                            const int2 *__restrict__ items, int cap) {
-    coop_flat_body<Real, SLOTS>(models, items, cap);
+    coop_flat_body<Real, SLOTS, DUST>(models, items, cap);
 }
 
-template <int SLOTS>
+template <int SLOTS, bool DUST>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8)))
 rt_coop_flat_kernel_f32(const RtArgs *__restrict__ models, const int2 *__restrict__ items, int cap) {
-    coop_flat_body<float, SLOTS>(models, items, cap);
+    coop_flat_body<float, SLOTS, DUST>(models, items, cap);
 }
 
 // n_blocks > 0: the persistent form on n_blocks workgroups; n_blocks == 0: the flat form
-template <typename Real>
-hipError_t launch_coop_persistent(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
-                                  int slots, int waves, size_t lds, int n_blocks, hipStream_t s) {
+template <typename Real, bool DUST>
+hipError_t launch_coop_persistent_d(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
+                                    int slots, int waves, size_t lds, int n_blocks, hipStream_t s) {
     if (lds > 160 * 1024 || waves < 1 || waves > 8) return hipErrorInvalidValue;
     if (n_items <= 0) return hipSuccess;
     if (n_blocks == 0) {
@@ -418,9 +419,9 @@ hipError_t launch_coop_persistent(const RtArgs *models, const int2 *items, int n
         auto flat = [&](auto slots_c) {
             constexpr int S = decltype(slots_c)::value;
             if constexpr (std::is_same<Real, float>::value && S <= 6)
-                hipLaunchKernelGGL((rt_coop_flat_kernel_f32<S>), g, b, lds, s, models, items, cap);
+                hipLaunchKernelGGL((rt_coop_flat_kernel_f32<S, DUST>), g, b, lds, s, models, items, cap);
             else
-                hipLaunchKernelGGL((rt_coop_flat_kernel<Real, S>), g, b, lds, s, models, items, cap);
+                hipLaunchKernelGGL((rt_coop_flat_kernel<Real, S, DUST>), g, b, lds, s, models, items, cap);
         };
         switch (slots) {
             case 1: flat(std::integral_constant<int, 1>{}); break;
@@ -439,7 +440,8 @@ hipError_t launch_coop_persistent(const RtArgs *models, const int2 *items, int n
     const dim3 g(n_blocks), b(64 * waves);
     auto go = [&](auto slots_c) {
         constexpr int S = decltype(slots_c)::value;
-        hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, S>), g, b, lds, s, models, items, n_items, counter, cap);
+        hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, S, DUST>), g, b, lds, s, models, items, n_items, counter,
+                           cap);
     };
     switch (slots) {
         case 1: go(std::integral_constant<int, 1>{}); break;
@@ -454,6 +456,16 @@ hipError_t launch_coop_persistent(const RtArgs *models, const int2 *items, int n
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// n_blocks > 0: the persistent form on n_blocks workgroups; n_blocks == 0: the flat form
+template <typename Real>
+hipError_t launch_coop_persistent(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
+                                  int slots, int waves, size_t lds, int n_blocks, bool dust, hipStream_t s) {
+    return dust ? launch_coop_persistent_d<Real, true>(models, items, n_items, counter, cap, slots, waves, lds,
+                                                       n_blocks, s)
+                : launch_coop_persistent_d<Real, false>(models, items, n_items, counter, cap, slots, waves, lds,
+                                                        n_blocks, s);
 }
 
 template <typename Real>

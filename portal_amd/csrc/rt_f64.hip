@@ -192,8 +192,8 @@ hipError_t launch_rt_persistent_f64(const RtArgs *models, const int2 *items, int
 size_t rt_persistent_lds_f64(int cap, int R_max, int waves) { return staged_lds_bytes<double, 20>(cap, R_max, waves); }
 
 hipError_t launch_coop_persistent_f64(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
-                                      int slots, int waves, size_t lds, int n_blocks, hipStream_t s) {
-    return launch_coop_persistent<double>(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, s);
+                                      int slots, int waves, size_t lds, int n_blocks, bool dust, hipStream_t s) {
+    return launch_coop_persistent<double>(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust, s);
 }
 
 }  // namespace portal

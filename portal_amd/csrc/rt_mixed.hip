@@ -158,19 +158,21 @@ hipError_t launch_rt_persistent(bool mixed, const RtArgs *models, const int2 *it
 }
 
 hipError_t launch_coop_persistent_f64(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
-                                      int slots, int waves, size_t lds, int n_blocks, hipStream_t s);
+                                      int slots, int waves, size_t lds, int n_blocks, bool dust, hipStream_t s);
 
 hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,
-                                     int cap, int slots, int waves, size_t lds, int n_blocks, hipStream_t s) {
-    if (!mixed) return launch_coop_persistent_f64(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, s);
-    return launch_coop_persistent<float>(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, s);
+                                     int cap, int slots, int waves, size_t lds, int n_blocks, bool dust,
+                                     hipStream_t s) {
+    if (!mixed)
+        return launch_coop_persistent_f64(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust, s);
+    return launch_coop_persistent<float>(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust, s);
 }
 
 int coop_slots_host(int R, int C, int W) { return coop_slots(R, C, W); }
 
-size_t rt_coop_persistent_lds(bool mixed, int cap, int R_max, int C) {
-    return mixed ? coop_lds_bytes<float>(cap, R_max, R_max * C, false, false)
-                 : coop_lds_bytes<double>(cap, R_max, R_max * C, false, false);
+size_t rt_coop_persistent_lds(bool mixed, int cap, int R_max, int C, bool dust) {
+    return mixed ? coop_lds_bytes<float>(cap, R_max, R_max * C, dust, false)
+                 : coop_lds_bytes<double>(cap, R_max, R_max * C, dust, false);
 }
 
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves) {

@@ -207,3 +207,57 @@ def test_batch_se_forms_give_the_same_bits(precision):
         for (pa, la), (pb, lb) in zip(runs[0], r):
             assert np.array_equal(pa, pb)
             assert np.array_equal(la, lb)
+
+
+@pytest.mark.parametrize("precision,tol", [(PORTAL_FP64, 1e-10), (PORTAL_MIXED, 1e-5)])
+def test_batch_with_dust_and_star_equals_single_models(precision, tol):
+    # a model with dust, one in star mode and a plain one in one batch: every model's
+    # iterations equal its single-model iterations (dust and the star's rays take the
+    # cooperative kernel's dust variant and per-node direction sets)
+    sp1 = cloud.co_like_rotor(4)
+    g1 = cloud.make_cloud(sp1, 1200, seed=51)
+    nb = g1.bulk_node_num - 1
+    g1.dust_p[:nb] = 1e-17 * g1.h2_dens[:nb] / 1e12
+    sp1.kd_v = np.array([3000.0, 6000.0, 9000.0])
+    sp2 = cloud.co_like_rotor(5)
+    g2 = cloud.make_cloud(sp2, 1500, seed=52)
+    star = dict(nth=6, nph=11, R_star=0.15, T_star=6000.0, X_star=np.zeros(3))
+    r = np.linalg.norm(g2.XYZ, axis=0)
+    g2.ab_all[np.where(r[: g2.bulk_node_num] < 1.3 * star["R_star"])[0]] = 0.0
+    sp3 = cloud.co_like_rotor(6)
+    g3 = cloud.make_cloud(sp3, 900, seed=53)
+    models = [(sp1, g1, None), (sp2, g2, star), (sp3, g3, None)]
+    xth = cloud.legendre_all(6, 11)
+    iters = 3
+
+    def open_model(sp, g, st):
+        pt = Portal(0)
+        pt.upload_grid(g)
+        pt.upload_species(sp)
+        pt.set_levpop(g.lev_p)
+        pt.star = st
+        return pt
+
+    singles = []
+    for sp, g, st in models:
+        pt = open_model(sp, g, st)
+        pp, lev, _ = pt.lambda_iterate(xth, 8, cloud.DX_INWARD, iters, tol_pop=0.0, tol_align=0.0,
+                                       precision=precision)
+        singles.append((pp, lev))
+        pt.close()
+    pts = [open_model(*m) for m in models]
+    per, tot = batch_lambda_iterate(pts, [xth] * 3, [8] * 3, cloud.DX_INWARD, iters, tol_pop=0.0, tol_align=0.0,
+                                    precision=precision)
+    assert tot["iterations"] == iters and tot["rt_variant"] in (2, 3)
+    for (sp, g, _), pt, (pp, lev) in zip(models, pts, singles):
+        idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+        np.testing.assert_allclose(pt.get_levpop()[:, idx], lev[:, idx], rtol=tol)
+        np.testing.assert_allclose(pt.get_pol_pops()[idx][:, 0::2], pp[idx][:, 0::2], rtol=tol)
+        pt.close()
+    # the dust and the star matter: the dust model without dust iterates differently
+    sp1.kd_v = np.zeros(3)
+    pt = open_model(sp1, g1, None)
+    _, lev0, _ = pt.lambda_iterate(xth, 8, cloud.DX_INWARD, iters, tol_pop=0.0, tol_align=0.0, precision=precision)
+    idx = np.where(g1.ab_all[: g1.bulk_node_num] > 0)[0]
+    assert np.abs(lev0[:, idx] / singles[0][1][:, idx] - 1).max() > 1e-4
+    pt.close()

@@ -357,7 +357,6 @@ def test_inward_pass_with_dust(oracle, variant, waves):
 def test_star_mode(oracle):
     # star mode (main_sub.f90:56-79): per-node star_all directions, star rays ending on the
     # star with B_nu(T_star) behind them; nodes inside the star are not solved
-    from portal_amd.api import batch_lambda_iterate
     sp = cloud.co_like_rotor(4)
     g = cloud.make_cloud(sp, 2500, seed=42)
     star = dict(nth=6, nph=11, R_star=0.15, T_star=6000.0, X_star=np.zeros(3))
@@ -391,9 +390,6 @@ def test_star_mode(oracle):
     pp, st = pt.se_pass(xth, 32, cloud.DX_INWARD)
     w = np.sqrt(2 * sp.j_lev + 1.0)
     np.testing.assert_allclose(pp[idx][:, 0::2] @ w, 1.0, atol=1e-12)
-    with pytest.raises(PortalError) as e:
-        batch_lambda_iterate([pt], [xth], [32], cloud.DX_INWARD, 2)
-    assert e.value.code == -9  # PORTAL_ERR_UNSUPPORTED
     pt.close()
 
 
