@@ -129,6 +129,9 @@ int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
         return PORTAL_OK;
     P.valid = false;
     const int nw = (int)c.g.work.size();
+    // the reference's node loop would do nothing; here a grid without a node to solve
+    // is refused before any launch
+    if (nw == 0) return fail(c, PORTAL_ERR_ARG, "no bulk node has a non-zero abundance: nothing to solve");
     double *d_xth = nullptr, *d_bvec = nullptr, *d_gvec = nullptr;
     int *d_nvar = nullptr;
     dfree(P.d_xthn);

@@ -308,6 +308,10 @@ __global__ void __launch_bounds__(SCAN_BS) scan_add(long long *out, const long l
 
 hipError_t launch_scan_counts(const int *counts, long long *offsets, long long n, hipStream_t s,
                               long long *total_host) {
+    if (n <= 0) {
+        *total_host = 0;
+        return n < 0 ? hipErrorInvalidValue : hipMemsetAsync(offsets, 0, sizeof(long long), s);
+    }
     const long long nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
     long long *bsum = nullptr;
     hipError_t e = hipMalloc((void **)&bsum, sizeof(long long) * (nb + 1));

@@ -277,6 +277,14 @@ def test_error_paths(cfg1):
     with pytest.raises(PortalError) as e:
         pt.se_pass(cloud.legendre_all(4, 8), 8, cloud.DX_INWARD)
     assert e.value.code == -6  # PORTAL_ERR_TEMP
+    # no node to solve: refused before any launch
+    g3 = cloud.make_cloud(sp, 300, seed=5)
+    g3.ab_all[:] = 0.0
+    pt.upload_grid(g3)
+    pt.set_levpop(g3.lev_p)
+    with pytest.raises(PortalError) as e:
+        pt.se_pass(cloud.legendre_all(4, 8), 8, cloud.DX_INWARD)
+    assert e.value.code == -1  # PORTAL_ERR_ARG
     pt.close()
 
 
