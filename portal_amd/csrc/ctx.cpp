@@ -1495,9 +1495,10 @@ int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *p
 
 // ---------------------------------------------------------------------------------
 // Batches of independent models (one context each, same device): every iteration is
-// one hipGraph replay of {reset, line preparation per model, ONE persistent inward
-// integration over all (model, node) items, ONE persistent SE solve over all items,
-// closure per model}.
+// one hipGraph replay of {reset, line preparation per model, the inward integration of
+// all (model, node) items (one cooperative launch per slot-and-dust class, or one staged
+// persistent launch), the SE solves (one persistent launch per system size, or one over
+// all items), closure per model}.
 int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *params, const portal_iter_params *it,
                                 portal_stats *per_model, portal_stats *total) {
     if (!xs || n <= 0 || !params || !it || it->max_iter <= 0) return PORTAL_ERR_ARG;
