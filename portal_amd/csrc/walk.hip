@@ -78,12 +78,16 @@ struct ChunkWriter {
 template <int MODE>
 __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
     constexpr bool WRITE = MODE == WALK_WRITE, CHUNKED = MODE == WALK_CHUNKED;
-    const long long n_paths = (long long)((a.n_work + a.sample - 1) / a.sample) * a.D;
+    const long long n_items = (a.n_work + a.sample - 1) / a.sample;
+    const long long n_paths = n_items * a.D;
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= n_paths) return;
-    // with sampling (count mode only) path tid walks ray tid % D of work item sample*(tid / D)
-    const long long id = a.sample > 1 ? (tid / a.D) * a.sample * a.D + tid % a.D : tid;
-    const int w = (int)(id / a.D), r = (int)(id % a.D);
+    // thread tid walks ray tid / n_items of work item sample * (tid % n_items): a wave's
+    // 64 rays share a direction and start at nodes adjacent on the work list's Morton
+    // curve, so they cross neighbouring cells; the path id stays w * D + r
+    const int r = (int)(tid / n_items), w = (int)((tid % n_items) * a.sample);
+    const long long id = (long long)w * a.D + r;
+    const long long cidx = a.sample > 1 ? tid : id;  // a sampled count goes to its thread
     const int node = a.work[w];
     bool star_ray = false;
     ChunkWriter cw;
@@ -96,7 +100,7 @@ __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
         if (WRITE) a.paths[a.offsets[id]] = e;
         if (CHUNKED) cw.put(a, id, e);
         if (!WRITE) {
-            a.counts[tid] = 1;
+            a.counts[cidx] = 1;
             atomicMax(a.flags + 1, 1);
         }
         return;
@@ -195,7 +199,7 @@ __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
         if (next < 0) break;
     }
     if (!WRITE) {
-        a.counts[tid] = nc > a.n_depth ? 0 : nc;
+        a.counts[cidx] = nc > a.n_depth ? 0 : nc;
         atomicMax(a.flags + 1, nc);
     }
 }
