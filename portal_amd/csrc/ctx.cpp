@@ -7,6 +7,7 @@
 // timed with HIP events on that stream.
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstring>
 #include <numeric>
 
@@ -38,6 +39,14 @@ void dfree(T *&p) {
     if (p) hipFree((void *)p);
     p = nullptr;
 }
+
+// runs f when its scope ends, on every return path
+struct Defer {
+    std::function<void()> f;
+    ~Defer() {
+        if (f) f();
+    }
+};
 
 int fail(Ctx &c, int code, const std::string &msg) {
     c.err = msg;
@@ -1536,6 +1545,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         for (auto *&p : d_xths) dfree(p);
         for (auto *&p : d_istars) dfree(p);
     };
+    Defer xths_at_exit{[&] { free_xths(); }};
     // line coefficients of model m, with its dust coefficients when it has dust
     auto prep_of = [&](int m) {
         Ctx &c = xs[m]->c;
@@ -1734,6 +1744,16 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     auto free_classes = [&]() {
         for (auto &k : classes) dfree(k.d_items);
     };
+    Defer buffers_at_exit{[&] {
+        free_classes();
+        dfree(d_rt);
+        dfree(d_se);
+        dfree(d_ri);
+        dfree(d_si);
+        dfree(d_cnt);
+        dfree(d_max);
+        for (auto *&p : prev) dfree(p);
+    }};
     HIP_TRY(&c0, upload(d_rt, rts));
     HIP_TRY(&c0, upload(d_se, ses));
     HIP_TRY(&c0, upload(d_ri, ri));
@@ -1836,6 +1856,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         dfree(d_se_cnt);
         dfree(d_slabs);
     };
+    Defer se_at_exit{[&] { free_se_cls(); }};
     {
         std::vector<int> cls_of(n, -1);
         for (int m = 0; m < n; m++) {
@@ -1922,6 +1943,10 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     // capture one iteration
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    Defer graph_at_exit{[&] {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+    }};
     HIP_TRY(&c0, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     launch_fill_zero(d_cnt, sizeof(int) * n_cnt, s);
     launch_fill_zero(d_se_cnt, sizeof(int) * std::max<size_t>(1, se_cls.size()), s);
@@ -1974,6 +1999,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         dfree(d_ngp);
         dfree(d_ngs);
     };
+    Defer ng_at_exit{[&] { free_ng(); }};
     if (ng) {
         for (int m = 0; m < n; m++)
             for (int h = 0; h < 3; h++)
@@ -2045,14 +2071,16 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     int flags = 0;
     HIP_TRY(&c0, hipMemcpy(&flags, c0.d_flags, sizeof(int), hipMemcpyDeviceToHost));
     hipGraphExecDestroy(exec);
+    exec = nullptr;
     hipGraphDestroy(graph);
-    hipFree(d_rt);
-    hipFree(d_se);
-    hipFree(d_ri);
-    hipFree(d_si);
-    hipFree(d_cnt);
-    hipFree(d_max);
-    for (auto *p : prev) hipFree(p);
+    graph = nullptr;
+    dfree(d_rt);
+    dfree(d_se);
+    dfree(d_ri);
+    dfree(d_si);
+    dfree(d_cnt);
+    dfree(d_max);
+    for (auto *&p : prev) dfree(p);
     free_xths();
     free_classes();
     free_se_cls();
