@@ -136,6 +136,33 @@ def test_se_solve_from_oracle_J(portal1, cfg1, oracle, oracle_cfg1):
     assert_pops_close(pp, ref, sp, solved(g), 1e-11)
 
 
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_se_solve_74_dimensional_systems(oracle, waves):
+    # a 20-level rotor: 74-dimensional systems, two rows per lane in the factorisation and
+    # the substitutions, split over 1, 2 or 4 waves; same J into both solvers
+    sp = cloud.co_like_rotor(20)
+    g = cloud.make_cloud(sp, 400, seed=61, n_exp=-1.5)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    assert pt.species_info()[1] == 74
+    J0, J2, _ = pt.inverse_rt(cloud.legendre_all(4, 8), 8, cloud.DX_INWARD)
+    pt.force_se(-1, waves)
+    pp = pt.se_solve(J0, J2)
+    pt.close()
+    ref = oracle.solve_from_J(g, sp, J0, J2)
+    idx = solved(g)
+    w = np.sqrt(2 * sp.j_lev + 1.0)
+    for i in range(sp.e_num):
+        o0 = ref[idx, 2 * i]
+        big = w[i] * o0 > 1e-10  # levels that hold a share of the population
+        assert big.any() or i > 10
+        np.testing.assert_allclose(pp[idx, 2 * i][big], o0[big], rtol=1e-10, atol=0)
+        np.testing.assert_allclose(pp[idx, 2 * i][~big], o0[~big], rtol=0, atol=1e-20)
+        np.testing.assert_allclose(pp[idx, 2 * i + 1], ref[idx, 2 * i + 1], rtol=0, atol=1e-10 * np.abs(o0).max())
+
+
 def test_mixed_precision_error_bounds(portal1, cfg1, oracle_cfg1):
     sp, g, xth = cfg1
     ppo, J0o, J2o, _, _ = oracle_cfg1
