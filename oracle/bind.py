@@ -204,6 +204,14 @@ class Oracle:
             raise RuntimeError("singular SE matrix")
         return (rho, rates) if extra_info else rho
 
+    def comp_emcoefs(self, r1_0, r2_0, r1_2, r2_2, j1, j2, th, nu0, Aji):
+        """(eta_I, eta_Q, eps_I, eps_Q) of one line (em_coefs.f90:1-113)."""
+        out = [C.c_double() for _ in range(4)]
+        self.lib.orc_comp_emcoefs(C.c_double(r1_0), C.c_double(r2_0), C.c_double(r1_2), C.c_double(r2_2),
+                                  int(j1), int(j2), C.c_double(th), C.c_double(nu0), C.c_double(Aji),
+                                  *[C.byref(o) for o in out])
+        return tuple(o.value for o in out)
+
     def coef(self, name, j, k, j2, k2):
         a = C.c_double()
         b = C.c_double()

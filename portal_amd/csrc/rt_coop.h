@@ -18,7 +18,10 @@
 // The segment update is branch-free: tau and e^-tau are evaluated for every lane.  In
 // FP64 the update is selected when tau > 1e-30 (false for NaN), which is the
 // reference's skip; in mixed precision the staging makes skipped segments exact
-// no-ops (Arith<float>::stage), so no test is needed.
+// no-ops (Arith<float>::stage, fold_seg), so no test is needed.  Without dust the FP32
+// record of a (cell, line) carries the source function folded into the terms of
+// 1 - e^-tau (SegF), which leaves nine VALU instructions and two v_exp_f32 per
+// (cell, line, channel).
 // The check for exhausted transmission (every lane's Tr exactly zero, or below the
 // mixed floor) runs every fourth cell; cells visited after a lane's Tr reached zero
 // add exactly zero.  Per pair the sum over rays is in ray order; the channel sum
@@ -30,12 +33,73 @@
 
 namespace portal {
 
-// one staging buffer: cap cell records, cap x R (L, S), and with dust cap x R (alpha dl, B_nu(T_dust))
+// Per (cell, line) record of the FP32 kernel without dust.  With tau2 = tau log2 e,
+//   S (1 - e^-tau) = min(tau2 (S a0 - tau2 (S a1 - tau2 S a2)), S c - S e^-tau),
+// a0..a2 the cubic's coefficients in units of tau2 (ln 2, ln^2 2 / 2, ln^3 2 / 6) and
+// c = 1 + 2^-22.  The cubic is the Taylor sum to tau^3, which lies above 1 - e^-tau
+// for every tau > 0 (the remainder's next term is negative) and is exact to tau^4/24;
+// S c - S e^-tau lies above S (1 - e^-tau) too, because the offset 2^-22 S exceeds the
+// rounding of e^-tau (1 ulp) and of S c.  The smaller of two upper bounds is within
+// min(tau^4/24, 2.4e-7) of the value, relative error below 7e-6 at any tau, where the
+// select at tau = 1e-2 of the cubic or 1 - e^-tau left up to 9e-6.  A skipped segment
+// (K dl not > 0, or a cell whose profile argument is not finite) is staged with
+// L = -inf and S = 0, which makes its update an exact no-op.
+struct __attribute__((aligned(8))) SegF {
+    float L, S, Sc, s0, s1, s2;
+};
+
+__device__ __forceinline__ SegF fold_seg(float2 k, float dl, bool cell) {
+    const float kdl = k.x * dl;
+    const bool pos = cell && kdl > 0.0f;
+    const float S = pos ? k.y : 0.0f;
+    SegF g;
+    g.L = pos ? __log2f(kdl * 1.44269504f) : -INFINITY;
+    g.S = S;
+    g.Sc = S * 1.00000024f;  // 1 + 2^-22
+    g.s0 = S * 0.693147182f;
+    g.s1 = S * 0.240226507f;
+    g.s2 = S * 0.0555041087f;
+    return g;
+}
+
+// the staged record of a (cell, line): SegF for FP32 without dust, else (L, S) of Arith
+template <typename Real, bool DUST>
+struct CoopSeg {
+    typedef typename Arith<Real>::R2 T;
+    static constexpr bool fold = false;
+};
+template <>
+struct CoopSeg<float, false> {
+    typedef SegF T;
+    static constexpr bool fold = true;
+};
+
+template <typename Real>
+__host__ __device__ constexpr size_t coop_seg_bytes(bool dust) {
+    return dust || !std::is_same<Real, float>::value ? sizeof(typename Arith<Real>::R2) : sizeof(SegF);
+}
+
+// one staging buffer: cap cell records, cap x R per-line records, and with dust cap x R
+// (alpha dl, B_nu(T_dust))
 template <typename Real>
 __host__ __device__ constexpr size_t coop_buf_bytes(int cap, int R, bool dust) {
-    return ((size_t)cap * (stage_bytes_per_cell<Real>(R) + (dust ? (size_t)R * sizeof(typename Arith<Real>::R2) : 0)) +
+    return ((size_t)cap * (sizeof(CellRec<Real>) + (size_t)R * coop_seg_bytes<Real>(dust) +
+                           (dust ? (size_t)R * sizeof(typename Arith<Real>::R2) : 0)) +
             15) &
            ~(size_t)15;
+}
+
+// Cells per staged unit: COOP_UNIT, or fewer where the two staging buffers of a
+// workgroup would not leave room for 7 workgroups of 4 waves per CU (the FP32 kernel's
+// occupancy; LDS is 160 KiB per CU), rounded down to a multiple of 8.  Only the folded
+// FP32 record (24 bytes per line) reaches that bound at the configurations' line counts.
+template <typename Real>
+__host__ __device__ constexpr int coop_unit_cells(int R, bool dust) {
+    const size_t per_wg = (160 * 1024) / 7;
+    const size_t per_cell = 2 * (sizeof(CellRec<Real>) + (size_t)R * coop_seg_bytes<Real>(dust) +
+                                 (dust ? (size_t)R * sizeof(typename Arith<Real>::R2) : 0));
+    const int fit = (int)(per_wg / per_cell) & ~7;
+    return fit >= COOP_UNIT ? COOP_UNIT : (fit < 8 ? 8 : fit);
 }
 
 // the two staging buffers, or the reduction of J0, J2 (and the own-cell operator) per
@@ -54,11 +118,13 @@ __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const
                                            const int cap, char *buf) {
     typedef Arith<Real> A;
     typedef typename A::R2 R2;
+    typedef typename CoopSeg<Real, DUST>::T Seg;
     const int R = a.R;
     const long long off = a.offsets[(long long)bid * a.D + r];
     const PathEntry *P = a.paths + off + c0;
     CellRec<Real> *cr = reinterpret_cast<CellRec<Real> *>(buf);
-    R2 *ks = reinterpret_cast<R2 *>(cr + cap);
+    Seg *ks = reinterpret_cast<Seg *>(cr + cap);
+    R2 *kds = reinterpret_cast<R2 *>(ks + (size_t)cap * R);
     const Real dls = (Real)a.dl_scale;
     const int n = cnt * R;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
@@ -68,8 +134,11 @@ __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const
         const Real dl = (Real)e.len * dls;
         const Real vb = (Real)(e.vray * ib) * (Real)A::vscale, sib = (Real)ib * (Real)A::vscale;
         const bool ok = A::cell_ok(vb, sib);
-        ks[j] = A::stage(A::ks(a, (size_t)e.cell * R + l), dl, ok);
-        if (DUST) ks[(size_t)cap * R + j] = A::stage_dust(A::ksd(a, (size_t)e.cell * R + l), dl, ok);
+        if constexpr (CoopSeg<Real, DUST>::fold)
+            ks[j] = fold_seg(A::ks(a, (size_t)e.cell * R + l), dl, ok);
+        else
+            ks[j] = A::stage(A::ks(a, (size_t)e.cell * R + l), dl, ok);
+        if (DUST) kds[j] = A::stage_dust(A::ksd(a, (size_t)e.cell * R + l), dl, ok);
         if (l == 0) {
             CellRec<Real> c;
             c.vb = ok ? vb : (Real)0;
@@ -81,10 +150,19 @@ __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const
     }
 }
 
+// nonzero exactly when x is not +0 (transmissions are never negative)
+__device__ __forceinline__ unsigned nonzero_bits(float x) { return __float_as_uint(x); }
+__device__ __forceinline__ unsigned nonzero_bits(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    return (unsigned)(b >> 32) | (unsigned)b;
+}
+
 template <typename Real, int SLOTS, bool DUST, bool LAM>
 __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, const int cap, char *smem) {
     typedef Arith<Real> A;
     typedef typename A::R2 R2;
+    typedef typename CoopSeg<Real, DUST>::T Seg;
+    constexpr bool FOLD = CoopSeg<Real, DUST>::fold;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int W = blockDim.x >> 6;
@@ -146,8 +224,8 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
         }
         if (rn < D) stage_unit(rn, un, smem + (half ^ 1) * bb);
         const CellRec<Real> *cr = reinterpret_cast<const CellRec<Real> *>(cur);
-        const R2 *ks = reinterpret_cast<const R2 *>(cr + cap);
-        const R2 *kds = ks + (size_t)cap * R;
+        const Seg *ks = reinterpret_cast<const Seg *>(cr + cap);
+        const R2 *kds = reinterpret_cast<const R2 *>(ks + (size_t)cap * R);
         const int c0 = u * cap, cnt = min(cap, nc - c0);
         if (u == 0) {
             // star mode: the node's own directions; rays towards the star start with B_nu(T_star)
@@ -190,52 +268,73 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             // cells max(c0, 1) .. c0 + cnt - 1 of the ray (local index j = i - c0); the
             // next cell's staged data are read one cell ahead, within the unit
             const int jb = c0 == 0 ? 1 : 0;
+            // this lane's record of cell j is at byte j R sizeof(Seg) from kl: a uniform
+            // offset on one address register
+            const char *kl = reinterpret_cast<const char *>(ks + lo);
+            const int kstride = R * (int)sizeof(Seg);
             CellRec<Real> cn = cr[jb < cnt ? jb : 0];
-            R2 kn = ks[(size_t)(jb < cnt ? jb : 0) * R + lo], dn;
+            Seg kn = *reinterpret_cast<const Seg *>(kl + (jb < cnt ? jb : 0) * kstride);
+            R2 dn;
             if (DUST) dn = kds[(size_t)(jb < cnt ? jb : 0) * R + lo];
             for (int j = jb; j < cnt; j++) {
                 if (((c0 + j) & 3) == 0) {
-                    bool live = false;
+                    // mixed: a channel whose transmission fell to the floor stops (zeroed
+                    // here, so that it adds exactly zero whatever the rest of its wave does);
+                    // FP64: transmission is never negative, so only exact zeros stop
+                    unsigned bits = 0;
 #pragma unroll
                     for (int q = 0; q < SLOTS; q++) {
-                        if (!(Tr[q] > (Real)A::tr_floor)) Tr[q] = (Real)0;
-                        live = live || (Tr[q] != (Real)0);
+                        if constexpr (A::tr_floor > 0) Tr[q] = Tr[q] > (Real)A::tr_floor ? Tr[q] : (Real)0;
+                        bits |= nonzero_bits(Tr[q]);
                     }
-                    if (!__any(live)) {
+                    if (!__any(bits != 0u)) {
                         ray_done = true;
                         break;
                     }
                 }
                 const CellRec<Real> c = cn;
-                const R2 k = kn;
+                const Seg k = kn;
                 R2 d;
                 if (DUST) d = dn;
                 const int jn = j + 1 < cnt ? j + 1 : j;
                 cn = cr[jn];
-                kn = ks[(size_t)jn * R + lo];
+                kn = *reinterpret_cast<const Seg *>(kl + jn * kstride);
                 if (DUST) dn = kds[(size_t)jn * R + lo];
+                if constexpr (FOLD) {
 #pragma unroll
-                for (int q = 0; q < SLOTS; q++) {
-                    const Real uu = fma(v0s[q], c.inv_b, -c.vb);
-                    Real tau = A::tau_of(k.x, uu), S = k.y;
-                    if (DUST) {
-                        // tau = tau_line + alpha dl; S the opacity-weighted mean of the line's
-                        // and the dust's source functions (inverse_rt.f90:273-287)
-                        const Real tl = tau;
-                        tau = tl + d.x;
-                        S = (k.y * tl + d.y * d.x) * A::rcp(tau);
-                        if (!A::skip_test && !(tau > (Real)0)) S = (Real)0;
+                    for (int q = 0; q < SLOTS; q++) {
+                        const float uu = fmaf(v0s[q], c.inv_b, -c.vb);
+                        const float tau2 = __builtin_amdgcn_exp2f(fmaf(-uu, uu, k.L));
+                        const float et = __builtin_amdgcn_exp2f(-tau2);
+                        const float cub = tau2 * fmaf(-tau2, fmaf(-tau2, k.s2, k.s1), k.s0);
+                        const float dir = fmaf(-k.S, et, k.Sc);
+                        Em[q] = fmaf(Tr[q], __builtin_fminf(cub, dir), Em[q]);
+                        Tr[q] = Tr[q] * et;
                     }
-                    const Real et = A::att(tau);
-                    const Real em = Em[q] + Tr[q] * S * A::om(tau, et);
-                    const Real tr = Tr[q] * et;
-                    if constexpr (A::skip_test) {
-                        const bool ok = tau > A::thr();
-                        Em[q] = ok ? em : Em[q];
-                        Tr[q] = ok ? tr : Tr[q];
-                    } else {
-                        Em[q] = em;
-                        Tr[q] = tr;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < SLOTS; q++) {
+                        const Real uu = fma(v0s[q], c.inv_b, -c.vb);
+                        Real tau = A::tau_of(k.x, uu), S = k.y;
+                        if (DUST) {
+                            // tau = tau_line + alpha dl; S the opacity-weighted mean of the line's
+                            // and the dust's source functions (inverse_rt.f90:273-287)
+                            const Real tl = tau;
+                            tau = tl + d.x;
+                            S = (k.y * tl + d.y * d.x) * A::rcp(tau);
+                            if (!A::skip_test && !(tau > (Real)0)) S = (Real)0;
+                        }
+                        const Real et = A::att(tau);
+                        const Real em = Em[q] + Tr[q] * S * A::om(tau, et);
+                        const Real tr = Tr[q] * et;
+                        if constexpr (A::skip_test) {
+                            const bool ok = tau > A::thr();
+                            Em[q] = ok ? em : Em[q];
+                            Tr[q] = ok ? tr : Tr[q];
+                        } else {
+                            Em[q] = em;
+                            Tr[q] = tr;
+                        }
                     }
                 }
             }
@@ -472,7 +571,7 @@ template <typename Real>
 hipError_t launch_coop(const RtArgs &a, int cap, hipStream_t s) {
     const int W = a.waves;
     if (W < 1 || W > 8 || a.R < 1) return hipErrorInvalidValue;
-    const int unit = a.unit > 0 ? a.unit : COOP_UNIT;
+    const int unit = a.unit > 0 ? a.unit : coop_unit_cells<Real>(a.R, a.dust != 0);
     cap = cap < unit ? (cap < 1 ? 1 : cap) : unit;  // cells per staged unit
     const int npairs = a.R * a.C;
     const size_t lds = coop_lds_bytes<Real>(cap, a.R, npairs, a.dust != 0, a.LAM != nullptr);
