@@ -171,6 +171,19 @@ int portal_inverse_rt(portal_ctx *ctx, const portal_se_params *p, double *J0, do
 int portal_se_solve(portal_ctx *ctx, const double *J0, const double *J2, double *pol_pops,
                     double *pol_rates, portal_stats *stats);
 
+/* No counterpart in the reference: portal_inverse_rt for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  d_J0, d_J2 are
+ * device buffers of r_num x node_num that the caller owns; the direction table (and
+ * in star mode the star's intensities) go to a static device array of 256 KiB per
+ * device instead of buffers allocated per call, so n_sphere is at most 10,901
+ * (PORTAL_ERR_ARG beyond).  Nothing is allocated: the pass buffers, the inward paths
+ * for exactly these directions and field, and the launch shape must come from an
+ * earlier portal_inverse_rt or portal_se_pass on this context (PORTAL_ERR_STATE
+ * otherwise).  The results equal portal_inverse_rt's bit for bit.  Returns when the
+ * outputs are written.  Calls that overlap on one device share the static array and
+ * race. */
+int portal_inverse_rt_device(portal_ctx *ctx, const portal_se_params *p, double *d_J0, double *d_J2);
+
 /* pol_pops (node_num x 2L) of the last SE solve on the device, without recomputing. */
 int portal_get_pol_pops(portal_ctx *ctx, double *pol_pops);
 

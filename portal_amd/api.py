@@ -131,6 +131,13 @@ class _Core:
         self._check(self.lib.portal_inverse_rt(self.ctx, C.byref(p), _d(J0), _d(J2), C.byref(st)))
         return J0, J2, st.as_dict()
 
+    def inverse_rt_device(self, xth, nvel, dx, d_J0, d_J2, n_depth=4096, precision=L.PORTAL_FP64):
+        """portal_inverse_rt_device: J0/J2 into caller-owned device buffers (addresses of
+        r_num x node_num doubles), allocating nothing; needs an earlier inverse_rt or
+        se_pass with the same directions on this context."""
+        p = self._se_params(xth, nvel, dx, n_depth, 0, precision)
+        self._check(self.lib.portal_inverse_rt_device(self.ctx, C.byref(p), C.c_void_p(d_J0), C.c_void_p(d_J2)))
+
     def se_solve(self, J0, J2, rates=False):
         g, sp = self.grid, self.species
         pp = np.zeros((g.node_num, 2 * sp.e_num), order="F")

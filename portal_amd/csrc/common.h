@@ -137,6 +137,7 @@ struct Ctx {
     double walk_chunk_scale = 1.1;  // chunk store of the single-pass walk / sampled estimate
     int ng_first = 3, ng_gap = 3;   // Ng schedule: first iteration, least gap between extrapolations
     int coop_unit = 0;              // cells per staged unit of the cooperative kernel (0: COOP_UNIT)
+    std::string scratch_err;        // why the allocation-free entry point's scratch is unavailable
     int se_waves = 1;               // waves per system of the single-model SE solve
     // test hook for a batch led by this context: SE form (-1 autotuned, 0 one persistent
     // launch, 1 a launch per class) and the waves per system of the classes (0 autotuned)

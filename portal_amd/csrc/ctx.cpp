@@ -9,12 +9,25 @@
 #include <cmath>
 #include <functional>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <numeric>
 
 #include "common.h"
 #include "image.h"
 
 extern "C" double portal_debug_wig6jj(int a, int b, int c, int d, int e, int f);
+
+// Device memory of the allocation-free inward pass (portal_inverse_rt_device): the
+// direction table (3 x n_sphere doubles) and, in star mode, the star's intensity per
+// line, staged here instead of in buffers allocated per call.  Every device holds its
+// own copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for that device (g_scratch below), never through this
+// name.  Calls that overlap on one device share it and race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 256 * 1024;
+constexpr size_t NOALLOC_STAR_OFFSET = NOALLOC_SCRATCH_BYTES - 64 * sizeof(double);  // r_num <= 64 intensities
+constexpr int NOALLOC_MAX_DIRS = (int)(NOALLOC_STAR_OFFSET / (3 * sizeof(double)));  // 10,901 directions
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
 namespace portal {
 int build_species_tables(Species &s);
@@ -51,6 +64,27 @@ struct Defer {
 int fail(Ctx &c, int code, const std::string &msg) {
     c.err = msg;
     return code;
+}
+
+// the device address of `scratch` on each device, resolved once per device by the
+// first context created there
+constexpr int MAX_DEVICES = 64;
+void *g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// with the context's device current
+int resolve_scratch(Ctx &c) {
+    if (c.device >= MAX_DEVICES) return fail(c, PORTAL_ERR_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[c.device]) return PORTAL_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();  // leave no error behind for the context's other calls
+        return fail(c, PORTAL_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch[c.device] = ptr;
+    return PORTAL_OK;
 }
 
 uint64_t spread3(uint32_t v) {
@@ -90,10 +124,18 @@ struct Timer {
     }
 };
 
+size_t node_buffer_key(const Ctx &c) {
+    const size_t N = c.g.N, R = c.sp.r_num, L = c.sp.e_num;
+    return N * std::max<size_t>(R, 1) * 1000 + L;
+}
+
+// the per-pass node buffers exist for the uploaded grid and species
+bool node_buffers_ready(const Ctx &c) { return c.g.valid && c.sp.valid && c.cap_nodes == node_buffer_key(c); }
+
 int ensure_node_buffers(Ctx &c) {
     const size_t N = c.g.N, R = c.sp.r_num, L = c.sp.e_num;
     if (!c.g.valid || !c.sp.valid) return fail(c, PORTAL_ERR_STATE, "grid and species must be uploaded first");
-    const size_t need = N * std::max<size_t>(R, 1) * 1000 + L;
+    const size_t need = node_buffer_key(c);
     if (c.cap_nodes == need) return PORTAL_OK;
     dfree(c.d_KS);
     dfree(c.d_KS32);
@@ -124,18 +166,25 @@ bool same(const std::vector<double> &v, const double *p, size_t n) {
     return v.size() == n && std::memcmp(v.data(), p, n * sizeof(double)) == 0;
 }
 
-// Build (or reuse) the inward paths for this direction set and field geometry.
-int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
-    PathSet &P = c.paths;
+// The built inward paths are those of this direction set and field geometry.
+bool paths_match(const Ctx &c, const portal_se_params *p) {
+    const PathSet &P = c.paths;
     const bool star = p->star_mod == 1;
     const int N = c.g.N, D = star ? p->nth * p->nph : p->n_sphere;
     const bool same_star = !star ? P.star == 0
                                  : (P.star == 1 && P.nth == p->nth && P.nph == p->nph && P.R_star == p->R_star &&
                                     same(P.X_star, p->X_star, 3));
-    if (P.valid && same_star && P.D == D && P.dx == p->dx && P.n_depth == p->n_depth &&
-        (star || same(P.xth, p->xth, 3 * (size_t)D)) && same(P.gvec, p->gvec, 3) &&
-        same(P.bvec, p->bvec_all, 3 * (size_t)N))
-        return PORTAL_OK;
+    return P.valid && same_star && P.D == D && P.dx == p->dx && P.n_depth == p->n_depth &&
+           (star || same(P.xth, p->xth, 3 * (size_t)D)) && same(P.gvec, p->gvec, 3) &&
+           same(P.bvec, p->bvec_all, 3 * (size_t)N);
+}
+
+// Build (or reuse) the inward paths for this direction set and field geometry.
+int ensure_paths(Ctx &c, const portal_se_params *p, portal_stats *st) {
+    PathSet &P = c.paths;
+    const bool star = p->star_mod == 1;
+    const int N = c.g.N, D = star ? p->nth * p->nph : p->n_sphere;
+    if (paths_match(c, p)) return PORTAL_OK;
     P.valid = false;
     const int nw = (int)c.g.work.size();
     // the reference's node loop would do nothing; here a grid without a node to solve
@@ -323,10 +372,20 @@ constexpr int N_RT_CANDS = 7;
 constexpr struct RtCand { int variant, waves; } RT_CANDS[N_RT_CANDS] = {{0, 4}, {1, 4}, {1, 2}, {2, 2}, {2, 4},
                                                                       {2, 5}, {2, 8}};
 
+// the autotuner's key: the paths and the launch's shape
+int rt_tune_key(const Ctx &c, const RtArgs &ra) {
+    return (int)((c.paths.total % 1000003) ^ (ra.C * 131 + ra.R * 7 + ra.D)) * 4 + ra.dust + (ra.xthn ? 2 : 0);
+}
+
+// a launch shape is chosen for these paths and this shape (forced, or timed earlier)
+bool rt_tuned(const Ctx &c, const RtArgs &ra, bool mixed) {
+    const int pi = mixed ? 1 : 0;
+    return c.forced_variant[pi] >= 0 || c.tuned_key[pi] == rt_tune_key(c, ra);
+}
+
 int autotune_rt(Ctx &c, RtArgs ra, bool mixed) {
     const int pi = mixed ? 1 : 0;
-    const int key = (int)((c.paths.total % 1000003) ^ (ra.C * 131 + ra.R * 7 + ra.D)) * 4 + ra.dust +
-                    (ra.xthn ? 2 : 0);
+    const int key = rt_tune_key(c, ra);
     if (c.forced_variant[pi] >= 0) {
         c.tuned_variant[pi] = c.forced_variant[pi];
         c.tuned_waves[pi] = c.forced_waves[pi];
@@ -363,12 +422,15 @@ int autotune_rt(Ctx &c, RtArgs ra, bool mixed) {
     return PORTAL_OK;
 }
 
-int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
-    c.pol_valid = false;
-    int rc = ensure_paths(c, p, st);
-    if (rc) return rc;
+// One inward pass over built paths: the line coefficients, J0/J2 cleared, the
+// integration launch.  xth (3 x n_sphere) and I_star (star mode: r_num) are on the
+// device; J0/J2 (r_num x node_num) are the outputs.  may_alloc = false is the
+// allocation-free form: no buffer is created (the dust coefficients must exist), no
+// launch shape is timed (one must have been chosen for these paths), no event is
+// recorded (st must be null).
+int inward_pass(Ctx &c, const portal_se_params *p, const double *xth, const double *I_star, double *J0, double *J2,
+                bool may_alloc, portal_stats *st) {
     const Species &s = c.sp;
-    Timer tp(c.stream);
     PrepArgs pa{};
     pa.N = c.g.N;
     pa.R = s.r_num;
@@ -390,6 +452,8 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
     if (c.g.has_dust)
         for (double kd : s.kd_v) dust = dust || kd != 0.0;
     if (dust) {
+        if (!may_alloc && (!c.d_KSD || !c.d_KSD32))
+            return fail(c, PORTAL_ERR_STATE, "dust coefficients not set up: run the allocating pass once first");
         if (!c.d_KSD) HIP_TRY(&c, hipMalloc((void **)&c.d_KSD, sizeof(double) * 2 * c.g.N * s.r_num));
         if (!c.d_KSD32) HIP_TRY(&c, hipMalloc((void **)&c.d_KSD32, sizeof(float) * 2 * c.g.N * s.r_num));
         pa.dust = c.g.d_dust;
@@ -399,10 +463,14 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
         pa.KSD = mixed ? nullptr : c.d_KSD;
         pa.KSD32 = mixed ? c.d_KSD32 : nullptr;
     }
-    HIP_TRY(&c, launch_line_prep(pa, c.stream));
-    const double t_prep = tp.stop();
-    HIP_TRY(&c, hipMemsetAsync(c.d_J0, 0, sizeof(double) * c.g.N * s.r_num, c.stream));
-    HIP_TRY(&c, hipMemsetAsync(c.d_J2, 0, sizeof(double) * c.g.N * s.r_num, c.stream));
+    double t_prep = 0.0;
+    {
+        std::unique_ptr<Timer> tp(st ? new Timer(c.stream) : nullptr);
+        HIP_TRY(&c, launch_line_prep(pa, c.stream));
+        if (tp) t_prep = tp->stop();
+    }
+    HIP_TRY(&c, hipMemsetAsync(J0, 0, sizeof(double) * c.g.N * s.r_num, c.stream));
+    HIP_TRY(&c, hipMemsetAsync(J2, 0, sizeof(double) * c.g.N * s.r_num, c.stream));
     RtArgs ra{};
     ra.n_work = (int)c.g.work.size();
     ra.D = c.paths.D;
@@ -414,38 +482,30 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
     ra.inv_b = c.g.d_inv_b;
     ra.KS = c.d_KS;
     ra.KS32 = c.d_KS32;
-    ra.xth = nullptr;
+    ra.xth = xth;
     ra.offsets = c.paths.offsets;
     ra.paths = c.paths.entries;
     ra.I_in = s.d_Iin;
     ra.dl_scale = c.g.tx * p->dx;
-    ra.J0 = c.d_J0;
-    ra.J2 = c.d_J2;
+    ra.J0 = J0;
+    ra.J2 = J2;
     ra.max_nc = (c.paths.max_nc + 7) & ~7;
     ra.dust = dust ? 1 : 0;
     ra.KSD = c.d_KSD;
     ra.KSD32 = c.d_KSD32;
-    double *d_xth = nullptr, *d_Istar = nullptr;
-    std::vector<double> xv(p->xth, p->xth + 3 * (size_t)p->n_sphere);
-    HIP_TRY(&c, upload(d_xth, xv));
-    ra.xth = d_xth;
     if (c.paths.star) {
-        // rays towards the star start with B_nu(T_star) (inverse_rt.f90:113-117)
-        std::vector<double> Is(s.r_num);
-        for (int k = 0; k < s.r_num; k++) Is[k] = black_body(p->T_star, s.nu_hz[k]);
-        HIP_TRY(&c, upload(d_Istar, Is));
+        if (!I_star) return fail(c, PORTAL_ERR_STATE, "star-mode paths without the star's intensities");
         ra.xthn = c.paths.d_xthn;
         ra.nstar = c.paths.d_nstar;
-        ra.I_star = d_Istar;
+        ra.I_star = I_star;
     }
-    rc = check_rt_args(c, ra, mixed);
-    if (rc) {
-        dfree(d_xth);
-        dfree(d_Istar);
-        return rc;
-    }
+    int rc = check_rt_args(c, ra, mixed);
+    if (rc) return rc;
     ra.unit = c.coop_unit;
-    autotune_rt(c, ra, mixed);
+    if (may_alloc)
+        autotune_rt(c, ra, mixed);
+    else if (!rt_tuned(c, ra, mixed))
+        return fail(c, PORTAL_ERR_STATE, "no launch shape chosen for these paths: run the allocating pass once first");
     ra.variant = c.tuned_variant[mixed ? 1 : 0];
     ra.waves = c.tuned_waves[mixed ? 1 : 0];
     if (c.ali) {
@@ -462,11 +522,13 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
             ra.waves = best < 0 ? 4 : RT_CANDS[best].waves;
         }
     }
-    Timer tr(c.stream);
-    hipError_t e = mixed ? launch_rt_mixed(ra, c.stream) : launch_rt_f64(ra, c.stream);
-    const double t_rt = tr.stop();
-    dfree(d_xth);
-    dfree(d_Istar);
+    double t_rt = 0.0;
+    hipError_t e;
+    {
+        std::unique_ptr<Timer> tr(st ? new Timer(c.stream) : nullptr);
+        e = mixed ? launch_rt_mixed(ra, c.stream) : launch_rt_f64(ra, c.stream);
+        if (tr) t_rt = tr->stop();
+    }
     HIP_TRY(&c, e);
     if (st) {
         st->t_prep_ms += t_prep;
@@ -479,6 +541,30 @@ int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
         st->max_path = c.paths.max_nc;
     }
     return PORTAL_OK;
+}
+
+// B_nu(T_star) per line: the incoming intensity of rays towards the star (inverse_rt.f90:113-117)
+void star_intensities(const Species &s, double T_star, double *out) {
+    for (int k = 0; k < s.r_num; k++) out[k] = black_body(T_star, s.nu_hz[k]);
+}
+
+int run_inward(Ctx &c, const portal_se_params *p, portal_stats *st) {
+    c.pol_valid = false;
+    int rc = ensure_paths(c, p, st);
+    if (rc) return rc;
+    double *d_xth = nullptr, *d_Istar = nullptr;
+    Defer release{[&] {
+        dfree(d_xth);
+        dfree(d_Istar);
+    }};
+    std::vector<double> xv(p->xth, p->xth + 3 * (size_t)p->n_sphere);
+    HIP_TRY(&c, upload(d_xth, xv));
+    if (c.paths.star) {
+        std::vector<double> Is(c.sp.r_num);
+        star_intensities(c.sp, p->T_star, Is.data());
+        HIP_TRY(&c, upload(d_Istar, Is));
+    }
+    return inward_pass(c, p, d_xth, d_Istar, c.d_J0, c.d_J2, true, st);
 }
 
 int run_se(Ctx &c, int extra_info, portal_stats *st) {
@@ -545,7 +631,8 @@ int run_se(Ctx &c, int extra_info, portal_stats *st) {
     return PORTAL_OK;
 }
 
-int check_se_params(Ctx &c, const portal_se_params *p) {
+// the arguments alone (no device state)
+int check_se_args(Ctx &c, const portal_se_params *p) {
     if (!p || !p->xth || !p->gvec || !p->bvec_all) return fail(c, PORTAL_ERR_ARG, "null parameter");
     if (p->star_mod != 0 && p->star_mod != 1) return fail(c, PORTAL_ERR_ARG, "star_mod must be 0 or 1");
     if (p->star_mod == 1 && (p->nth <= 0 || p->nph <= 0 || !p->X_star || !(p->R_star > 0) || !(p->T_star >= 0)))
@@ -553,7 +640,12 @@ int check_se_params(Ctx &c, const portal_se_params *p) {
     if (p->n_sphere <= 0 || p->nvel <= 0 || p->n_depth <= 0 || !(p->dx > 0))
         return fail(c, PORTAL_ERR_ARG, "n_sphere, nvel, n_depth and dx must be positive");
     if (p->precision != PORTAL_FP64 && p->precision != PORTAL_MIXED) return fail(c, PORTAL_ERR_ARG, "bad precision");
-    return ensure_node_buffers(c);
+    return PORTAL_OK;
+}
+
+int check_se_params(Ctx &c, const portal_se_params *p) {
+    const int rc = check_se_args(c, p);
+    return rc ? rc : ensure_node_buffers(c);
 }
 
 }  // namespace
@@ -572,6 +664,9 @@ int portal_ctx_create(portal_ctx **out, int device) {
         delete x;
         return PORTAL_ERR_DEVICE;
     }
+    // the allocation-free entry point's scratch; only that entry point fails without it
+    // (it reports this message)
+    if (resolve_scratch(x->c) != PORTAL_OK) x->c.scratch_err = x->c.err;
     *out = x;
     return PORTAL_OK;
 }
@@ -875,6 +970,45 @@ int portal_inverse_rt(portal_ctx *x, const portal_se_params *p, double *J0, doub
     if (J2) HIP_TRY(&c, hipMemcpyAsync(J2, c.d_J2, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
     const double ms = tt.stop();
     if (st) st->t_total_ms = ms;
+    return PORTAL_OK;
+}
+
+int portal_inverse_rt_device(portal_ctx *x, const portal_se_params *p, double *d_J0, double *d_J2) {
+    if (!x) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    if (!d_J0 || !d_J2) return fail(c, PORTAL_ERR_ARG, "J0 and J2 must be device buffers of r_num x node_num");
+    if (hipSetDevice(c.device) != hipSuccess) return fail(c, PORTAL_ERR_DEVICE, "hipSetDevice failed");
+    int rc = check_se_args(c, p);
+    if (rc) return rc;
+    if (c.sp.valid && c.sp.r_num > 64) return fail(c, PORTAL_ERR_ARG, "more than 64 lines");
+    if (p->n_sphere > NOALLOC_MAX_DIRS)
+        return fail(c, PORTAL_ERR_ARG, "n_sphere = " + std::to_string(p->n_sphere) +
+                                           " exceeds the allocation-free pass's scratch (at most " +
+                                           std::to_string(NOALLOC_MAX_DIRS) + " directions)");
+    if (!node_buffers_ready(c))
+        return fail(c, PORTAL_ERR_STATE, "no pass buffers for this grid and species: call portal_inverse_rt "
+                                         "(or portal_se_pass) once first");
+    if (!paths_match(c, p))
+        return fail(c, PORTAL_ERR_STATE, "no inward paths for these directions and this field: call "
+                                         "portal_inverse_rt (or portal_se_pass) once with them first");
+    unsigned char *scr = static_cast<unsigned char *>(g_scratch[c.device]);
+    if (!scr) return fail(c, PORTAL_ERR_DEVICE, "scratch address not resolved for this device: " + c.scratch_err);
+    c.pol_valid = false;
+    double *d_xth = reinterpret_cast<double *>(scr);
+    double *d_Istar = nullptr;
+    // host copies into the scratch; the stack buffer outlives the copy (synchronised below)
+    double Is[64];
+    HIP_TRY(&c, hipMemcpyAsync(d_xth, p->xth, sizeof(double) * 3 * (size_t)p->n_sphere, hipMemcpyHostToDevice,
+                               c.stream));
+    if (c.paths.star) {
+        star_intensities(c.sp, p->T_star, Is);
+        d_Istar = reinterpret_cast<double *>(scr + NOALLOC_STAR_OFFSET);
+        HIP_TRY(&c, hipMemcpyAsync(d_Istar, Is, sizeof(double) * c.sp.r_num, hipMemcpyHostToDevice, c.stream));
+    }
+    rc = inward_pass(c, p, d_xth, d_Istar, d_J0, d_J2, false, nullptr);
+    const hipError_t e = hipStreamSynchronize(c.stream);
+    if (rc) return rc;
+    HIP_TRY(&c, e);
     return PORTAL_OK;
 }
 

@@ -25,7 +25,8 @@ STATUS = {0: "PORTAL_OK", -1: "PORTAL_ERR_ARG", -2: "PORTAL_ERR_DEVICE", -3: "PO
 # every entry point declared in include/portal_hip.h
 EXPORTS = ["portal_ctx_create", "portal_ctx_destroy", "portal_last_error", "portal_upload_grid",
            "portal_upload_species", "portal_species_info", "portal_set_levpop", "portal_get_levpop",
-           "portal_se_pass", "portal_inverse_rt", "portal_se_solve", "portal_lambda_iterate",
+           "portal_se_pass", "portal_inverse_rt", "portal_inverse_rt_device", "portal_se_solve",
+           "portal_lambda_iterate",
            "portal_image_pass", "portal_mlp_create", "portal_mlp_destroy", "portal_mlp_num_params",
            "portal_mlp_set_params", "portal_mlp_get_params", "portal_mlp_forward", "portal_mlp_train_step",
            "portal_mlp_train", "portal_surrogate_features", "portal_get_pol_pops",
@@ -84,6 +85,7 @@ def load(path: str = LIB_PATH):
         f = getattr(lib, n)
         if n not in ("portal_last_error",):
             f.restype = C.c_int
+    lib.portal_inverse_rt_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.portal_mlp_num_params.restype = C.c_longlong
     lib.portal_mlp_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong]
     lib.portal_mlp_train_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p,
