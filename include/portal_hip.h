@@ -70,7 +70,14 @@ typedef struct {
     double tol_pop;          /* stop when max |d(sqrt(g) rho00)| / (sqrt(g) rho00) <= tol_pop */
     double tol_align;        /* ... and max |d(rho20/rho00)| <= tol_align */
     int surrogate;           /* 1: the trained MLP supplies the first iterate */
-    double fallback_tol;     /* reserved (the surrogate iterate is rejected per node on the validity tests) */
+    double fallback_tol;     /* the surrogate's residual test (>= 0; +inf turns it off): one lambda
+                                iteration after the prediction, a node whose largest relative
+                                level-population change in that iteration exceeds both fallback_tol
+                                and the change its plain first iterate made (from the starting
+                                populations) reverts to that plain iterate.  Predictions that fail
+                                the validity tests (non-finite, populations summing outside
+                                [0.5, 2], |rho2/rho0| >= 0.5) are rejected when applied.
+                                stats.fallbacks counts both kinds */
     int rebuild_paths;       /* 1: rebuild the inward paths even if the geometry is unchanged */
     int ng_accel;            /* 0: plain iteration; 1: Ng acceleration of the level densities from the
                                 last four iterates, every third iteration from the fourth on;

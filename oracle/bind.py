@@ -317,6 +317,48 @@ class Reference:
             cd(0.0), cd(0.0), ci(0), ci(0), _d(Xs), ci(n_depth))
         return J0, J2
 
+    def inverse_nodes(self, g, sp, xth, nvel, dx, nodes, n_depth=4096, nthreads=0, lev_p=None):
+        """inverse_rt.f90:1-137 for each node in `nodes` (1-based), as the reference's node
+        loop (main_para.f90:210-257) runs it: one call per node, nodes spread over
+        nthreads host threads (0: all cores).  The arrays are converted once; the Fortran
+        routine keeps its locals on the stack, so concurrent calls are independent.
+        Returns J0, J2 (r_num x len(nodes))."""
+        from concurrent.futures import ThreadPoolExecutor
+        N, R = g.node_num, sp.r_num
+        ne = _i32(g.del_ne())
+        keep = dict(ne=ne, nnum=_i32(g.del_num_ne), X=_f64(g.XYZ), lp=_f64(g.lev_p if lev_p is None else lev_p),
+                    xt=_f64(xth), gv=_f64(g.gvec), rt=_i32(sp.r_trans), nu=_f64(sp.nu_hz), lg=_i32(sp.lev_g),
+                    b=_f64(g.b_all), v=_f64(g.vel_all), A=_f64(sp.t_Aji), B=_f64(sp.t_Bij), dp=_f64(g.dust_p),
+                    kd=_f64(sp.kd_v), td=_f64(g.temp_all[1]), bv=_f64(g.bvec_all))
+        ints = {k: C.c_int(int(v)) for k, v in dict(N=N, maxne=ne.shape[1], nvel=nvel, ns=keep["xt"].shape[1],
+                                                      zero=0, R=R, L=sp.e_num, nd=n_depth).items()}
+        dbl = {k: C.c_double(float(v)) for k, v in dict(dx=dx, tx=g.tx, z=0.0).items()}
+        nodes = [int(n) for n in nodes]
+        J0 = np.zeros((R, len(nodes)), order="F")
+        J2 = np.zeros((R, len(nodes)), order="F")
+        Xs = np.zeros(3)
+        k_ = keep
+
+        def one(t):
+            node = C.c_int(nodes[t])
+            j0 = np.zeros(R)
+            j2 = np.zeros(R)
+            bvec = np.ascontiguousarray(k_["bv"][:, nodes[t] - 1])
+            self.lib.inverse_rt_(
+                _i(k_["ne"]), _i(k_["nnum"]), C.byref(ints["N"]), C.byref(ints["maxne"]), C.byref(node), _d(k_["X"]),
+                C.byref(ints["nvel"]), C.byref(ints["ns"]), C.byref(ints["zero"]), _d(bvec), _d(k_["gv"]),
+                C.byref(dbl["dx"]), C.byref(dbl["tx"]), C.byref(ints["R"]), _i(k_["rt"]), _d(k_["nu"]),
+                C.byref(ints["L"]), _d(k_["lp"]), _i(k_["lg"]), _d(k_["b"]), _d(k_["v"]), _d(k_["A"]), _d(k_["B"]),
+                _d(k_["xt"]), _d(k_["dp"]), _d(k_["kd"]), _d(k_["td"]), _d(j0), _d(j2), C.byref(ints["zero"]),
+                C.byref(ints["zero"]), C.byref(dbl["z"]), C.byref(dbl["z"]), C.byref(ints["zero"]),
+                C.byref(ints["zero"]), _d(Xs), C.byref(ints["nd"]))
+            J0[:, t] = j0
+            J2[:, t] = j2
+
+        with ThreadPoolExecutor(max_workers=nthreads or os.cpu_count() or 1) as ex:
+            list(ex.map(one, range(len(nodes))))
+        return J0, J2
+
     def star_all(self, nth, nph, kvec, gvec, bvec, dOm):
         """star_all (miscellaneous.f90:1139-1226) -> (n_star, xth 3 x n_sphere)."""
         xth = np.zeros((3, nth * nph), order="F")

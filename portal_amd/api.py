@@ -193,6 +193,20 @@ class _Core:
         and its waves per workgroup for one precision; variant -1 returns to autotuning."""
         self._check(self.lib.portal_debug_force_rt(self.ctx, int(precision), int(variant), int(waves)))
 
+    def surrogate_perturb(self, stride, factor=1e3):
+        """Test hook: every stride-th work item's surrogate prediction gets its first level's
+        population scaled by factor, renormalised (stride 0: off)."""
+        self._check(self.lib.portal_debug_surrogate_perturb(self.ctx, int(stride), C.c_double(factor)))
+
+    def fallback_map(self):
+        """Per work item of the last surrogate run: 0 kept, 1 failed the validity tests,
+        2 reverted by the residual test."""
+        out = np.zeros(max(self.grid.node_num, 1), dtype=np.int32)  # work items <= nodes
+        m = self.lib.portal_debug_fallback_map(self.ctx, out.ctypes.data_as(C.POINTER(C.c_int)), len(out))
+        if m < 0:
+            self._check(m)
+        return out[:m]
+
     def force_se(self, form=-1, waves=0):
         """Test hook: waves per system of this context's SE solve (1, 2, 4; 0 resets to 1);
         for a batch this context leads, the SE form (-1 autotuned, 0 one persistent launch,

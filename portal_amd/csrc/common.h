@@ -138,6 +138,9 @@ struct Ctx {
     int ng_first = 3, ng_gap = 3;   // Ng schedule: first iteration, least gap between extrapolations
     int coop_unit = 0;              // cells per staged unit of the cooperative kernel (0: COOP_UNIT)
     std::string scratch_err;        // why the allocation-free entry point's scratch is unavailable
+    int surr_perturb_stride = 0;    // test hook of the surrogate fallback (0: off)
+    double surr_perturb_factor = 1.0;
+    std::vector<int> fallback_map;  // per work item of the last surrogate run: 0 kept, 1 invalid, 2 residual
     int se_waves = 1;               // waves per system of the single-model SE solve
     // test hook for a batch led by this context: SE form (-1 autotuned, 0 one persistent
     // launch, 1 a launch per class) and the waves per system of the classes (0 autotuned)
@@ -294,15 +297,29 @@ struct SurrArgs {
     const float *out;        // n_work x n_out (network outputs, physical units)
     double *levp, *pol;      // updated for accepted nodes
     int *n_fallback;
+    int *flags;              // n_work: 1 where the prediction failed the validity tests, else 0 (may be null)
+    int perturb_stride;      // test hook (0: off): items w with w % stride == 0 get a wrong prediction
+    double perturb_factor;
+};
+struct SurrRevertArgs {
+    int n_work, N, L;
+    const int *work;
+    const unsigned long long *res_s, *res_p;  // per item, double bits
+    double tol;
+    const double *plain_lev, *plain_pol;
+    double *levp, *pol;
+    int *flags, *n_fallback;
 };
 hipError_t launch_surr_features(const SurrArgs &a, hipStream_t s);
 hipError_t launch_surr_apply(const SurrArgs &a, hipStream_t s);
+hipError_t launch_surr_revert(const SurrRevertArgs &a, hipStream_t s);
 hipError_t mlp_infer_device(Ctx &c, int B, const float *feats, float *out);
 int mlp_dims(const Ctx &c, int *n_in, int *n_out);
 
 hipError_t launch_lambda_update(int n_work, int N, int L, const int *work, const double *ab, const double *h2,
                                 const int *j_lev, const double *pol, const double *pol_prev, double *levp,
-                                unsigned long long *maxes, hipStream_t s, double *pol_save = nullptr);
+                                unsigned long long *maxes, hipStream_t s, double *pol_save = nullptr,
+                                unsigned long long *node_res = nullptr);
 // batched (persistent) launches over (model, node) items
 hipError_t launch_rt_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,
                                 int cap, int R_max, int waves, int n_blocks, hipStream_t s);

@@ -1064,7 +1064,7 @@ int portal_se_pass(portal_ctx *x, const portal_se_params *p, double *pol_pops, d
     return PORTAL_OK;
 }
 
-static int surrogate_step(Ctx &c, int *n_fallback) {
+static int surrogate_step(Ctx &c, int *n_fallback, int *d_item_flags) {
     int n_in = 0, n_out = 0;
     if (!mlp_dims(c, &n_in, &n_out)) return fail(c, PORTAL_ERR_STATE, "no surrogate network (portal_mlp_create)");
     const Species &s = c.sp;
@@ -1075,6 +1075,11 @@ static int surrogate_step(Ctx &c, int *n_fallback) {
     const int nw = (int)c.g.work.size();
     float *d_f = nullptr, *d_o = nullptr;
     int *d_n = nullptr;
+    Defer release{[&] {
+        dfree(d_f);
+        dfree(d_o);
+        dfree(d_n);
+    }};
     HIP_TRY(&c, hipMalloc((void **)&d_f, sizeof(float) * (size_t)nw * n_in + 16));
     HIP_TRY(&c, hipMalloc((void **)&d_o, sizeof(float) * (size_t)nw * n_out + 16));
     HIP_TRY(&c, hipMalloc((void **)&d_n, sizeof(int)));
@@ -1099,14 +1104,14 @@ static int surrogate_step(Ctx &c, int *n_fallback) {
     a.levp = c.d_levp;
     a.pol = c.d_pol;
     a.n_fallback = d_n;
+    a.flags = d_item_flags;
+    a.perturb_stride = c.surr_perturb_stride;
+    a.perturb_factor = c.surr_perturb_factor;
     HIP_TRY(&c, launch_surr_features(a, c.stream));
     HIP_TRY(&c, mlp_infer_device(c, nw, d_f, d_o));
     HIP_TRY(&c, launch_surr_apply(a, c.stream));
     HIP_TRY(&c, hipMemcpyAsync(n_fallback, d_n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(&c, hipStreamSynchronize(c.stream));
-    hipFree(d_f);
-    hipFree(d_o);
-    hipFree(d_n);
     return PORTAL_OK;
 }
 
@@ -1117,6 +1122,24 @@ int portal_debug_tune_ms(const portal_ctx *x, int precision, double *out8) {
     if (!x || !out8 || precision < 0 || precision > 1) return PORTAL_ERR_ARG;
     for (int k = 0; k < 8; k++) out8[k] = x->c.tune_ms[precision][k];
     return PORTAL_OK;
+}
+
+// Test hooks of the surrogate's residual fallback: every stride-th work item's
+// prediction gets its first level's population scaled by factor (stride 0: off); and
+// the per-item outcome of the last surrogate run (0 kept, 1 failed the validity tests,
+// 2 reverted by the residual test), out holding n_work ints (returns n_work).
+int portal_debug_surrogate_perturb(portal_ctx *x, int stride, double factor) {
+    if (!x || stride < 0 || !(factor > 0)) return PORTAL_ERR_ARG;
+    x->c.surr_perturb_stride = stride;
+    x->c.surr_perturb_factor = factor;
+    return PORTAL_OK;
+}
+
+int portal_debug_fallback_map(const portal_ctx *x, int *out, int n) {
+    if (!x || !out || n < 0) return PORTAL_ERR_ARG;
+    const int m = (int)std::min<size_t>(x->c.fallback_map.size(), (size_t)n);
+    for (int i = 0; i < m; i++) out[i] = x->c.fallback_map[i];
+    return (int)x->c.fallback_map.size();
 }
 
 // the population and alignment changes of each iteration of the last lambda call
@@ -1234,17 +1257,48 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     if (rc) return rc;
     if (it->max_iter <= 0) return fail(c, PORTAL_ERR_ARG, "max_iter must be positive");
     if (it->surrogate && !c.mlp) return fail(c, PORTAL_ERR_STATE, "no surrogate network (portal_mlp_create)");
-    if (st) std::memset(st, 0, sizeof(*st));
-    if (it->rebuild_paths) c.paths.valid = false;
-    const size_t L = c.sp.e_num, N = c.g.N;
-    double *d_prev = nullptr;
-    unsigned long long *d_max = nullptr;
-    HIP_TRY(&c, hipMalloc((void **)&d_prev, sizeof(double) * N * 2 * L));
-    HIP_TRY(&c, hipMalloc((void **)&d_max, sizeof(unsigned long long) * 2));
     // acceleration: 1 Ng (the three previous level-density iterates), 2 Anderson (the
     // window of (x, G(x)) pairs, compact over solved nodes)
     if (it->ng_accel < 0 || it->ng_accel > 2) return fail(c, PORTAL_ERR_ARG, "ng_accel must be 0, 1 or 2");
     if (it->ali < 0 || it->ali > 1) return fail(c, PORTAL_ERR_ARG, "ali must be 0 or 1");
+    if (it->surrogate && !(it->fallback_tol >= 0.0))
+        return fail(c, PORTAL_ERR_ARG, "fallback_tol must be >= 0 (+inf turns the residual test off)");
+    if (st) std::memset(st, 0, sizeof(*st));
+    if (it->rebuild_paths) c.paths.valid = false;
+    const size_t L = c.sp.e_num, N = c.g.N;
+    const int nw = (int)c.g.work.size();
+    double *d_prev = nullptr, *d_hist = nullptr, *d_ngp = nullptr, *d_aa = nullptr;
+    unsigned long long *d_max = nullptr;
+    // the surrogate's residual test: per item the changes of the plain first iterate and
+    // of the iteration after the prediction, the plain first iterate itself, item flags
+    unsigned long long *d_res = nullptr;
+    double *d_plain_lev = nullptr, *d_plain_pol = nullptr;
+    int *d_iflags = nullptr, *d_nrev = nullptr;
+    Defer release{[&] {
+        dfree(d_prev);
+        dfree(d_max);
+        dfree(d_hist);
+        dfree(d_ngp);
+        dfree(d_aa);
+        dfree(d_res);
+        dfree(d_plain_lev);
+        dfree(d_plain_pol);
+        dfree(d_iflags);
+        dfree(d_nrev);
+    }};
+    HIP_TRY(&c, hipMalloc((void **)&d_prev, sizeof(double) * N * 2 * L));
+    HIP_TRY(&c, hipMalloc((void **)&d_max, sizeof(unsigned long long) * 2));
+    if (it->surrogate) {
+        HIP_TRY(&c, hipMalloc((void **)&d_res, sizeof(unsigned long long) * 2 * std::max(nw, 1)));
+        HIP_TRY(&c, hipMalloc((void **)&d_plain_lev, sizeof(double) * N * L));
+        HIP_TRY(&c, hipMalloc((void **)&d_plain_pol, sizeof(double) * N * 2 * L));
+        HIP_TRY(&c, hipMalloc((void **)&d_iflags, sizeof(int) * std::max(nw, 1)));
+        HIP_TRY(&c, hipMalloc((void **)&d_nrev, sizeof(int)));
+        HIP_TRY(&c, hipMemsetAsync(d_res, 0, sizeof(unsigned long long) * 2 * std::max(nw, 1), c.stream));
+        HIP_TRY(&c, hipMemsetAsync(d_iflags, 0, sizeof(int) * std::max(nw, 1), c.stream));
+        HIP_TRY(&c, hipMemsetAsync(d_nrev, 0, sizeof(int), c.stream));
+    }
+    unsigned long long *res_p = d_res, *res_s = d_res ? d_res + std::max(nw, 1) : nullptr;
     const bool ng = it->ng_accel == 1, aa = it->ng_accel == 2;
     // the own-cell operator covers the line alone: with dust in the inward pass the
     // iteration stays plain
@@ -1256,7 +1310,6 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
         ~AliScope() { c.ali = false; }
     } ali_scope{c};
     c.ali = it->ali == 1 && !dust_on;
-    double *d_hist = nullptr, *d_ngp = nullptr, *d_aa = nullptr;
     if (ng) {
         HIP_TRY(&c, hipMalloc((void **)&d_hist, sizeof(double) * N * L * 3));
         HIP_TRY(&c, hipMalloc((void **)&d_ngp, sizeof(double) * ng_partial_doubles()));
@@ -1272,7 +1325,6 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     int aa_cnt = 0, aa_pos = 0;
     double *hist[3] = {d_hist, d_hist ? d_hist + N * L : nullptr, d_hist ? d_hist + 2 * N * L : nullptr};
     int nhist = 0, n_ng = 0, last_ng = -1, conv = 0;
-    const int nw = (int)c.g.work.size();
     Timer tt(c.stream);
     int iters = 0, nfb = 0;
     double dpop = 0, dal = 0;
@@ -1288,14 +1340,44 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
         if (!rc) rc = run_se(c, 0, st);
         if (rc) break;
         HIP_TRY(&c, hipMemsetAsync(d_max, 0, sizeof(unsigned long long) * 2, c.stream));
+        // with the surrogate, the per-item population changes of the first two iterations
+        unsigned long long *node_res = !it->surrogate ? nullptr : k == 0 ? res_p : k == 1 ? res_s : nullptr;
         HIP_TRY(&c, launch_lambda_update((int)c.g.work.size(), (int)N, (int)L, c.g.d_work, c.g.d_ab, c.g.d_h2,
                                          c.sp.d_jlev, c.d_pol, (k > 0 || chained) ? d_prev : nullptr, c.d_levp, d_max,
-                                         c.stream));
+                                         c.stream, nullptr, node_res));
         if (k == 0 && it->surrogate) {
-            // the network's prediction of the converged state replaces the plain first
-            // iterate wherever it passes the validity checks
-            rc = surrogate_step(c, &nfb);
+            // keep the plain first iterate, then the network's prediction of the converged
+            // state replaces it wherever it passes the validity checks
+            HIP_TRY(&c, hipMemcpyAsync(d_plain_lev, c.d_levp, sizeof(double) * N * L, hipMemcpyDeviceToDevice,
+                                       c.stream));
+            HIP_TRY(&c, hipMemcpyAsync(d_plain_pol, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToDevice,
+                                       c.stream));
+            rc = surrogate_step(c, &nfb, d_iflags);
             if (rc) break;
+        }
+        if (k == 1 && it->surrogate) {
+            // the residual test: an item whose change in this first iteration after the
+            // prediction exceeds fallback_tol and the change of its plain first iterate
+            // goes back to that plain iterate
+            SurrRevertArgs ra{};
+            ra.n_work = nw;
+            ra.N = (int)N;
+            ra.L = (int)L;
+            ra.work = c.g.d_work;
+            ra.res_s = res_s;
+            ra.res_p = res_p;
+            ra.tol = it->fallback_tol;
+            ra.plain_lev = d_plain_lev;
+            ra.plain_pol = d_plain_pol;
+            ra.levp = c.d_levp;
+            ra.pol = c.d_pol;
+            ra.flags = d_iflags;
+            ra.n_fallback = d_nrev;
+            HIP_TRY(&c, launch_surr_revert(ra, c.stream));
+            int nrev = 0;
+            HIP_TRY(&c, hipMemcpyAsync(&nrev, d_nrev, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+            HIP_TRY(&c, hipStreamSynchronize(c.stream));
+            nfb += nrev;
         }
         unsigned long long mx[2];
         HIP_TRY(&c, hipMemcpyAsync(mx, d_max, sizeof(mx), hipMemcpyDeviceToHost, c.stream));
@@ -1407,12 +1489,12 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
         }
     }
     const double ms = tt.stop();
-    hipFree(d_prev);
-    hipFree(d_max);
-    if (d_hist) hipFree(d_hist);
-    if (d_ngp) hipFree(d_ngp);
-    if (d_aa) hipFree(d_aa);
     if (rc) return rc;
+    if (it->surrogate) {
+        c.fallback_map.assign(std::max(nw, 1), 0);
+        HIP_TRY(&c, hipMemcpy(c.fallback_map.data(), d_iflags, sizeof(int) * std::max(nw, 1), hipMemcpyDeviceToHost));
+        c.fallback_map.resize(nw);
+    }
     // the Ng history is not carried over, and an extrapolation leaves d_levp ahead of d_pol
     c.pol_chain = n_ng == 0 || last_ng < iters - 1;
     if (st) {
@@ -1894,7 +1976,9 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     HIP_TRY(&c0, upload(d_si, si));
     if (coop)
         for (auto &k : classes) HIP_TRY(&c0, upload(k.d_items, k.items));
-    const int n_cnt = 2 + (int)classes.size();
+    // [0] staged items, [1] SE items, [2..] one per cooperative class; the staged
+    // form's timing clears four
+    const int n_cnt = std::max(4, 2 + (int)classes.size());
     HIP_TRY(&c0, hipMalloc((void **)&d_cnt, sizeof(int) * n_cnt));
     HIP_TRY(&c0, hipMalloc((void **)&d_max, sizeof(unsigned long long) * 2 * n));
     for (int m = 0; m < n; m++) {

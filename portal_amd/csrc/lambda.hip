@@ -23,6 +23,8 @@ struct LambdaArgs {
     double *levp;            // L x N, updated in place
     unsigned long long *maxes;  // [0] pop change, [1] alignment change (double bits)
     double *pol_save;        // when set, receives this pass's pol (the next pass's pol_prev)
+    unsigned long long *node_res;  // when set, per work item the largest relative population change
+                                   // over its levels (double bits, ORed in with atomicMax)
 };
 
 __global__ void __launch_bounds__(256) lambda_update_kernel(LambdaArgs a) {
@@ -49,6 +51,7 @@ __global__ void __launch_bounds__(256) lambda_update_kernel(LambdaArgs a) {
         if (!(dal == dal)) dal = 1.0;
         if (sqrt(2.0 * a.j_lev[i] + 1.0) * r0 < 1e-10) dpop = dal = 0.0;  // false for NaN
         a.levp[(size_t)node * a.L + i] = nv;
+        if (a.node_res) atomicMax(a.node_res + w, (unsigned long long)__double_as_longlong(dpop));
         if (a.pol_save) {
             a.pol_save[node + N * (2 * i)] = r0;
             a.pol_save[node + N * (2 * i + 1)] = r2;
@@ -67,8 +70,9 @@ __global__ void __launch_bounds__(256) lambda_update_kernel(LambdaArgs a) {
 
 hipError_t launch_lambda_update(int n_work, int N, int L, const int *work, const double *ab, const double *h2,
                                 const int *j_lev, const double *pol, const double *pol_prev, double *levp,
-                                unsigned long long *maxes, hipStream_t s, double *pol_save) {
-    LambdaArgs a{n_work, N, L, work, ab, h2, j_lev, pol, pol_prev, levp, maxes, pol_save};
+                                unsigned long long *maxes, hipStream_t s, double *pol_save,
+                                unsigned long long *node_res) {
+    LambdaArgs a{n_work, N, L, work, ab, h2, j_lev, pol, pol_prev, levp, maxes, pol_save, node_res};
     const long long n = (long long)n_work * L;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(lambda_update_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
@@ -353,22 +357,59 @@ __global__ void __launch_bounds__(256) surr_apply_kernel(SurrArgs a) {
         const double rat = (r0 != 0.0 ? r2 / r0 : 0.0) + 1e-3 * (double)o[q++];
         ok = ok && isfinite(rat) && fabs(rat) < 0.5;
     }
+    if (a.flags) a.flags[w] = ok ? 0 : 1;
     if (!ok) {
         atomicAdd(a.n_fallback, 1);
         return;
+    }
+    // test hook: every perturb_stride-th item gets level 1's population scaled by
+    // perturb_factor, renormalised (a wrong prediction that passes the checks above)
+    const bool pert = a.perturb_stride > 0 && w % a.perturb_stride == 0;
+    double p0 = 0.0;
+    if (pert) {
+        p0 = exp10(surr_logpop(sqrt(2.0 * a.j_lev[0] + 1.0) * a.pol[node]) + (double)o[0]);
+        s += (a.perturb_factor - 1.0) * p0;
     }
     const double nab = a.ab[node] * a.h2[node];
     q = a.L;
     for (int i = 0; i < a.L; i++) {
         const double sg = sqrt(2.0 * a.j_lev[i] + 1.0);
         const double r0_1 = a.pol[node + N * (2 * i)], r2_1 = a.pol[node + N * (2 * i + 1)];
-        const double p = exp10(surr_logpop(sg * r0_1) + (double)o[i]) / s;
+        const double p = exp10(surr_logpop(sg * r0_1) + (double)o[i]) * (pert && i == 0 ? a.perturb_factor : 1.0) / s;
         const double r0 = p / sg;
         const double rat = a.j_lev[i] != 0 ? (r0_1 != 0.0 ? r2_1 / r0_1 : 0.0) + 1e-3 * (double)o[q++] : 0.0;
         a.levp[(size_t)node * a.L + i] = nab * p;
         a.pol[node + N * (2 * i)] = r0;
         a.pol[node + N * (2 * i + 1)] = rat * r0;
     }
+}
+
+// The residual test of the surrogate iterate, one lambda iteration after it was applied:
+// an accepted item whose largest relative level-population change in that iteration
+// (res_s) exceeds both tol and the change its plain first iterate made (res_p) reverts
+// to that plain iterate (plain_lev / plain_pol); flags[w] = 2 marks it.  A NaN change
+// counts as exceeding.
+__global__ void __launch_bounds__(256) surr_revert_kernel(SurrRevertArgs a) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= a.n_work || a.flags[w] != 0) return;
+    const double rs = __longlong_as_double((long long)a.res_s[w]);
+    const double rp = __longlong_as_double((long long)a.res_p[w]);
+    if (rs <= fmax(a.tol, rp)) return;  // false for NaN
+    a.flags[w] = 2;
+    atomicAdd(a.n_fallback, 1);
+    const int node = a.work[w];
+    const size_t N = a.N;
+    for (int i = 0; i < a.L; i++) {
+        a.levp[(size_t)node * a.L + i] = a.plain_lev[(size_t)node * a.L + i];
+        a.pol[node + N * (2 * i)] = a.plain_pol[node + N * (2 * i)];
+        a.pol[node + N * (2 * i + 1)] = a.plain_pol[node + N * (2 * i + 1)];
+    }
+}
+
+hipError_t launch_surr_revert(const SurrRevertArgs &a, hipStream_t s) {
+    if (a.n_work == 0) return hipSuccess;
+    hipLaunchKernelGGL(surr_revert_kernel, dim3((a.n_work + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_surr_features(const SurrArgs &a, hipStream_t s) {
