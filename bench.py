@@ -229,10 +229,21 @@ def main():
     secondary = {"se_solve": {"bound": "fp64", "achieved": se_tfs, "peak": 78.6, "unit": "TFLOP/s",
                               "frac": se_tfs / 78.6, "ms_per_pass": se_ms, "c_dim": c_dim,
                               "flops_per_node": se_flops}}
+    # the dominant kernel's HBM traffic and instruction issue, from its rocprofv3 counter
+    # runs on this workload (profiles/)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "rt_mixed_traffic.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+        prof = json.load(open(tf))
+        traffic = prof.get("bytes_per_launch")
+        if "valu_instr_per_launch" in prof:
+            secondary["rt_valu_issue"] = {
+                "bound": "valu", "frac": prof["valu_issue_quad_cycle_frac"],
+                "dual_issue_frac": prof["valu_dual_issue_quad_cycle_frac"],
+                "valu_instr_per_launch": prof["valu_instr_per_launch"],
+                "valu_instr_per_s": prof["valu_instr_per_launch"] / (avg_ms * 1e-3),
+                "note": "frac: SIMD quad-cycles that issue a VALU instruction (rocprofv3 counters, "
+                        "profiles/" + prof.get("counters", "") + ")"}
 
     # CPU baseline: the reference's own inverse_rt (inverse_rt.f90:1-137, compiled from its
     # Fortran sources at its own -O1 into oracle/_ref), one call per node over the host's

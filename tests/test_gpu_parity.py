@@ -48,10 +48,16 @@ def solved(g):
 
 
 def assert_pops_close(pp, ref, sp, idx, rtol):
+    """rho^0_0 per node and level to rtol; rho^2_0 per node and level to rtol of that
+    level's own rho^0_0 at that node (of 1e-6 of the node's largest rho^0_0 for levels
+    holding less: their rho^2_0 is a rounding-level remainder of the solve)."""
+    node_max = np.max(np.abs(ref[idx][:, 0::2]), axis=1)
     for i in range(sp.e_num):
         r0, o0 = pp[idx, 2 * i], ref[idx, 2 * i]
         np.testing.assert_allclose(r0, o0, rtol=rtol, atol=0)
-        np.testing.assert_allclose(pp[idx, 2 * i + 1], ref[idx, 2 * i + 1], rtol=0, atol=rtol * np.abs(o0).max())
+        scale = np.maximum(np.abs(o0), 1e-6 * node_max)
+        d2 = np.abs(pp[idx, 2 * i + 1] - ref[idx, 2 * i + 1])
+        assert np.all(d2 <= rtol * scale), f"level {i}: max |d rho20| / rho00 = {np.max(d2 / scale):.2e}"
 
 
 def test_inverse_rt_fp64_parity(portal1, cfg1, oracle_cfg1):
@@ -329,10 +335,11 @@ def test_image_pass_fp64_parity(portal1, cfg1, oracle, oracle_cfg1):
     Imax = np.abs(Io).max()
     assert Imax > 0
     np.testing.assert_allclose(I, Io, rtol=1e-10, atol=1e-10 * Imax)
-    np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * Imax)
-    np.testing.assert_allclose(U, Uo, rtol=0, atol=1e-10 * Imax)
+    # Q and U on their own scales (max |Q| is a few 1e-3 of max |I| here)
+    np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * np.abs(Qo).max())
+    np.testing.assert_allclose(U, Uo, rtol=0, atol=1e-10 * np.abs(Uo).max())
     np.testing.assert_allclose(T, To, rtol=1e-10, atol=1e-12)
-    assert np.abs(Qo).max() > 0  # polarisation is present
+    assert np.abs(Qo).max() > 1e-6 * Imax and np.abs(Uo).max() > 0  # polarisation is present
 
 
 def test_image_pass_mixed_error(portal1, cfg1, oracle_cfg1):
@@ -457,8 +464,8 @@ def test_image_pass_star(oracle):
     assert 20 < on.sum() < len(on) // 4  # pixels on the star's disc
     Imax = np.abs(Io).max()
     np.testing.assert_allclose(I, Io, rtol=1e-10, atol=1e-10 * Imax)
-    np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * Imax)
-    np.testing.assert_allclose(U, Uo, rtol=0, atol=1e-10 * Imax)
+    np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * np.abs(Qo).max())
+    np.testing.assert_allclose(U, Uo, rtol=0, atol=1e-10 * np.abs(Uo).max())
     np.testing.assert_allclose(T, To, rtol=1e-10, atol=1e-12)
     # FP32 transfer: the star's light in units of its own B_nu, the line on the CMB's
     s_on, s_off = np.abs(I[on]).max(), np.abs(I[~on]).max()
@@ -495,8 +502,8 @@ def test_image_pass_node_pixels(oracle):
         assert I.shape == (g.bulk_node_num, 2, 13)
         Imax = np.abs(Io).max()
         np.testing.assert_allclose(I, Io, rtol=1e-10, atol=1e-10 * Imax)
-        np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * Imax)
-        np.testing.assert_allclose(U, Uo, rtol=0, atol=1e-10 * Imax)
+        np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * np.abs(Qo).max())
+        np.testing.assert_allclose(U, Uo, rtol=0, atol=1e-10 * np.abs(Uo).max())
         np.testing.assert_allclose(T, To, rtol=1e-10, atol=1e-12)
         e = max(np.abs(Im - I).max(), np.abs(Qm - Q).max(), np.abs(Um - U).max()) / Imax
         print(f"node pixels (star {star is not None}): mixed max err {e:.2e} of max |I|")
@@ -528,7 +535,7 @@ def test_image_pass_dust_branch(cfg1, oracle):
     sp.kd_v = np.zeros(sp.r_num)
     Imax = np.abs(Io).max()
     np.testing.assert_allclose(I, Io, rtol=1e-10, atol=1e-10 * Imax)
-    np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * Imax)
+    np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-10 * np.abs(Qo).max())
     dmax = np.abs(dusto[0]).max()
     assert dmax > 0
     np.testing.assert_allclose(dust, dusto, rtol=1e-9, atol=1e-10 * dmax)
