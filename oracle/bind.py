@@ -273,6 +273,42 @@ class Reference:
     def _r(x, t=C.c_double):
         return C.byref(t(x))
 
+    def eat_lime(self, pop, grid, mol, dust, max_ne_temp=300):
+        """eat_lime_dim + eat_lime_no_delaunay (eat_lime.f90:1-313) on the four files: the
+        arrays they fill, by name.  Fortran character(len=100) arguments get 100-character
+        blank-padded buffers and their hidden lengths after the other arguments."""
+        def s100(x):
+            b = x.encode()
+            assert len(b) <= 100
+            return C.create_string_buffer(b.ljust(100), 100)
+        fp, fg, fm, fd = (s100(x) for x in (pop, grid, mol, dust))
+        L100 = C.c_size_t(100)
+        dims = [C.c_int(0) for _ in range(7)]
+        self.lib.eat_lime_dim_(fg, fm, *[C.byref(d) for d in dims], L100, L100)
+        N, ndel, E, R, Cn, Ct, Cp = (d.value for d in dims)
+        out = dict(node_num=N, del_num=ndel, e_num=E, r_num=R, c_num=Cn, c_temp=Ct, c_par=Cp)
+        arr = dict(XYZ=np.zeros((3, N), order="F"), vel_all=np.zeros((3, N), order="F"),
+                   temp_all=np.zeros((2, N), order="F"), lev_p=np.zeros((E, N), order="F"), e_levs=np.zeros(E),
+                   t_Aji=np.zeros(R), nu=np.zeros(R), t_Cji=np.zeros((Ct, Cn, Cp), order="F"),
+                   col_temps=np.zeros(Ct), h2_dens=np.zeros(N), b_all=np.zeros(N), ab_all=np.zeros(N),
+                   j_lev=np.zeros(E, dtype=np.int32), r_trans=np.zeros((2, R), dtype=np.int32, order="F"),
+                   c_trans=np.zeros((2, Cn), dtype=np.int32, order="F"), del_num_ne=np.zeros(N, dtype=np.int32),
+                   del_ne=np.zeros((N, max_ne_temp), dtype=np.int32, order="F"), kd_v=np.zeros(R),
+                   dust_p=np.zeros(N))
+        maxne, bulk = C.c_int(0), C.c_int(0)
+        rmax, dii, mass = C.c_double(0), C.c_double(0), C.c_double(0)
+        ci = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+        p = lambda k: arr[k].ctypes.data_as(C.c_void_p)  # noqa: E731
+        self.lib.eat_lime_no_delaunay_(
+            fp, fg, fm, fd, ci(N), ci(ndel), ci(E), ci(R), ci(Cn), ci(Ct), p("XYZ"), p("vel_all"), p("temp_all"),
+            p("lev_p"), p("e_levs"), p("t_Aji"), p("nu"), p("t_Cji"), p("col_temps"), ci(Cp), p("h2_dens"),
+            p("b_all"), p("ab_all"), p("j_lev"), p("r_trans"), p("c_trans"), ci(max_ne_temp), C.byref(maxne),
+            p("del_num_ne"), p("del_ne"), C.byref(rmax), C.byref(bulk), p("kd_v"), C.byref(dii), p("dust_p"),
+            C.byref(mass), L100, L100, L100, L100)
+        out.update(arr)
+        out.update(maxne=maxne.value, rmax=rmax.value, bulk_node_num=bulk.value, mass_mol=mass.value)
+        return out
+
     def legendre_all(self, nth, nph):
         xth = np.zeros((3, nth * nph), order="F")
         n = C.c_int(0)

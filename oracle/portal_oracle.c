@@ -18,12 +18,18 @@
  *                              compiled reference (_ref), bit-level and <=1e-12
  *   make_ac                    compiled reference (_ref)
  *   t_xxx_const, r_xxx_const   exact rational re-evaluation (sympy) + selection rules
- *   setup_stateq / get_rhovec  invariants: normalisation, J2=0 => rho^K>0 = 0,
- *                              detailed balance => Boltzmann, collision limit => LTE,
- *                              numpy.linalg.solve on the assembled matrix
+ *   setup_stateq / get_rhovec  an independent rate-equation model in the magnetic-
+ *                              sublevel basis (every rank K <= 6, Delta J = 0/+-1 lines,
+ *                              stimulated emission, collisions) to 1e-10; the two-level
+ *                              J = 0 -> 1 weak-radiation limit rho20/rho00 = J20/J00;
+ *                              normalisation, detailed balance => Boltzmann, collision
+ *                              limit => LTE
  *   rotate_all_save / trace_out_raytrace / polarize_prop / rot_stokes /
  *   ray_in_raytrace            compiled reference (_ref) + expm closed form
- *   comp_emcoefs               closed forms (rho2=0 => eta_Q=eps_Q=0; exact 6j)
+ *   comp_emcoefs               the Zeeman-component emission/absorption sums of the same
+ *                              sublevel model to 1e-12; w^(2) against the closed forms of
+ *                              Landi Degl'Innocenti & Landolfi (2004) Table 10.1; pure pi /
+ *                              sigma light gives Q/I = +1 / -1; Kirchhoff's law (CODATA)
  */
 #include "portal_oracle.h"
 

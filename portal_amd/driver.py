@@ -9,6 +9,10 @@ Python interface, the tests and the benchmark can run a whole pass:
                         line gets the last line's frequency)
 * ``channel_velocities`` para_finish.f90:124-126
 * ``cube``              para_finish.f90:451-484 (x, y, v cube times I_to_T)
+* ``ang_scale``         para_finish.f90:188 (pixel size in degrees)
+* ``write_fits``        writeimage (fits_writing.f90:1-158): a FITS primary array of
+                        the float32 cube with the reference's header keywords, as
+                        para_finish.f90:497-511 calls it
 """
 from __future__ import annotations
 
@@ -63,3 +67,52 @@ def cube(all_X, line, im_pix, nu0=None):
     if nu0 is not None:
         a = a * (3.255e39 / (nu0 * nu0))
     return a
+
+
+def ang_scale(im_size, tx, distance, im_pix):
+    """Pixel size in degrees (para_finish.f90:188); distance in the units of tx."""
+    import math
+    return -180.0 * math.asin(2.0 * im_size * tx / distance) / (math.pi * (im_pix * 2 + 1))
+
+
+def _card(key, value=None, raw=None):
+    if key == "END":
+        return "END".ljust(80)
+    if raw is None:
+        if isinstance(value, bool):
+            raw = ("T" if value else "F").rjust(20)
+        elif isinstance(value, str):
+            raw = ("'" + value.ljust(8) + "'").ljust(20)
+        elif isinstance(value, int):
+            raw = str(value).rjust(20)
+        else:  # a real*4 keyword with 8 decimals (ftpkye(..., 8, ...))
+            raw = f"{float(np.float32(value)):.8E}".rjust(20)
+    return (key.ljust(8) + "= " + raw).ljust(80)
+
+
+def write_fits(path, cube, restfreq, cdelt1, crpix1, cdelt2, crpix2, cdelt3, crpix3, crval1=0.0, crval2=0.0,
+               crval3=0.0):
+    """writeimage (fits_writing.f90:1-158): cube is (npix, npix, nvel) indexed [x, y, v]
+    as the reference's array(npix, npix, nvel); it is stored as big-endian float32 with
+    x varying fastest, after the keywords the reference writes, in its order.  The
+    image driver passes cdelt1 = ang_scale, cdelt2 = -ang_scale, crpix1 = crpix2 =
+    im_pix + 1, cdelt3 = dvel, crpix3 = nvel + 1 (para_finish.f90:497-511)."""
+    npix, npix2, nvel = cube.shape
+    assert npix == npix2
+    cards = [_card("SIMPLE", True), _card("BITPIX", -32), _card("NAXIS", 3), _card("NAXIS1", npix),
+             _card("NAXIS2", npix), _card("NAXIS3", nvel), _card("EXTEND", True),
+             _card("OBJECT", "PORTAL"), _card("EPOCH", 2.0e3), _card("LONPOLE", 1.8e2), _card("EQUINOX", 2.0e3),
+             _card("SPECSYS", "LSRK"), _card("RESTFREQ", restfreq), _card("VELREF", 257),
+             _card("CTYPE1", "RA---SIN"), _card("CDELT1", cdelt1), _card("CRPIX1", crpix1),
+             _card("CRVAL1", crval1), _card("CUNIT1", "DEG"), _card("CTYPE2", "DEC--SIN"),
+             _card("CDELT2", cdelt2), _card("CRPIX2", crpix2), _card("CRVAL2", crval2), _card("CUNIT2", "DEG"),
+             _card("CTYPE3", "VELO-LSR"), _card("CDELT3", cdelt3), _card("CRPIX3", crpix3),
+             _card("CRVAL3", crval3), _card("CUNIT3", "M/S"), _card("BSCALE", 1.0), _card("BZERO", 0.0),
+             _card("BUNIT", "K"), _card("END")]
+    head = "".join(cards).encode("ascii")
+    head += b" " * (-len(head) % 2880)
+    data = np.asarray(cube, dtype=">f4").ravel(order="F").tobytes()
+    data += b"\0" * (-len(data) % 2880)
+    with open(path, "wb") as f:
+        f.write(head)
+        f.write(data)

@@ -5,14 +5,15 @@ Two sources:
 * the reference's own routines, compiled from its Fortran sources by
   ``make -C oracle ref`` into ``oracle/_ref/libportal_ref.so``: quadrature,
   find_dir, black_body, trace_out, inverse_rt, make_AC, rotate_all_save,
-  trace_out_raytrace, polarize_prop, rot_stokes(_back), ray_in_raytrace;
+  trace_out_raytrace, polarize_prop, rot_stokes(_back), ray_in_raytrace, and the
+  input stage eat_lime_dim / eat_lime_no_delaunay on the model files in lime_model/;
 * exact arithmetic (sympy) for what the reference cannot be built for here
   (pop_eqs.f90 / em_coefs.f90 need the WIGXJPF library, which this image lacks):
   Wigner 3j/6j symbols and the L&L04 7.20 coupling constants re-evaluated as
   exact sums.
 
 The fixtures hold inputs (or the seed that regenerates them) and outputs only.
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [star | star_image | node_pixels | eat_lime]
 """
 from __future__ import annotations
 
@@ -377,9 +378,49 @@ def make_node_pixels(R):
     print("node-pixel fixtures written to", OUT)
 
 
+LIME_DIR = os.path.join(OUT, "lime_model")
+EAT_LIME_KEYS = ("XYZ", "vel_all", "temp_all", "lev_p", "e_levs", "t_Aji", "nu", "t_Cji", "col_temps", "h2_dens",
+                 "b_all", "ab_all", "j_lev", "r_trans", "c_trans", "del_num_ne", "del_ne", "kd_v", "dust_p")
+
+
+def lime_model_files():
+    """A small model in LIME/LAMDA/dust-table formats: 200 interior nodes and their sinks,
+    a 5-level rotor with two collision partners, a 67-row dust table, and a gas-to-dust
+    ratio on about half of the nodes.  Returns the four paths (written once)."""
+    from scipy.spatial import Delaunay
+    from portal_amd import lime_io
+    sp = cloud.co_like_rotor(5)
+    sp2 = cloud.Species(e_num=sp.e_num, j_lev=sp.j_lev, e_levs_cm=sp.e_levs_cm, mass_mol=sp.mass_mol,
+                        r_trans=sp.r_trans, t_Aji=sp.t_Aji, nu_ghz=sp.nu_ghz, c_trans=sp.c_trans,
+                        col_temps=sp.col_temps,
+                        t_Cji=np.asfortranarray(np.concatenate([sp.t_Cji, 0.7 * sp.t_Cji], axis=2)), kmax=6)
+    g = cloud.make_cloud(sp2, 200, seed=7)
+    pos = (g.XYZ * g.tx).T.copy()
+    tets = Delaunay(pos).simplices
+    lam = np.logspace(0, 4, 67)
+    rng = np.random.default_rng(3)
+    gd = np.where(rng.random(g.bulk_node_num) < 0.5, 100.0 * rng.uniform(0.5, 2, g.bulk_node_num), 0.0)
+    return lime_io.write_model(LIME_DIR, g, sp2, tets, pos, dust_table=np.array([lam, 1e3 * lam ** -1.5]),
+                               gas_to_dust=gd)
+
+
+def make_eat_lime(R):
+    """The reference's eat_lime_dim + eat_lime_no_delaunay on the committed model files."""
+    paths = lime_model_files()
+    rel = {k: os.path.relpath(v, OUT) for k, v in paths.items()}
+    ref = R.eat_lime(paths["pop"], paths["grid"], paths["mol"], paths["dust"])
+    scal = {k: np.array(ref[k]) for k in ("node_num", "del_num", "e_num", "r_num", "c_num", "c_temp", "c_par",
+                                          "maxne", "rmax", "bulk_node_num", "mass_mol")}
+    np.savez_compressed(os.path.join(OUT, "eat_lime.npz"), files=json.dumps(rel),
+                        **{k: ref[k] for k in EAT_LIME_KEYS}, **scal)
+    print("eat_lime fixtures written to", OUT)
+
+
 if __name__ == "__main__":
     import sys
-    if sys.argv[1:] == ["star"]:
+    if sys.argv[1:] == ["eat_lime"]:
+        make_eat_lime(Reference())
+    elif sys.argv[1:] == ["star"]:
         make_star(Reference())
     elif sys.argv[1:] == ["star_image"]:
         make_star_image(Reference())
