@@ -328,23 +328,49 @@ __global__ void __launch_bounds__(256) stokes_kernel(ImgArgs a) {
     }
 }
 
-// (1 - e^-x)/x, by its series below |x| = 1e-2 (relative error < 1e-8)
+// (1 - e^-x)/x, by its series below |x| = 0.1 (relative error < 1e-9 there, and the
+// direct form loses at most 6e-7 to the rounding of e^-x above)
 __device__ __forceinline__ float phi1_f(float x, float ex) {
-    const float series = fmaf(x, fmaf(x, 0.16666667f, -0.5f), 1.0f);
-    return fabsf(x) < 1e-2f ? series : __fdividef(1.0f - ex, x);
+    const float series =
+        fmaf(x, fmaf(x, fmaf(x, fmaf(x, fmaf(x, -1.3888889e-3f, 8.3333333e-3f), -4.1666668e-2f), 0.16666667f), -0.5f),
+             1.0f);
+    return fabsf(x) < 0.1f ? series : __fdividef(1.0f - ex, x);
+}
+
+// (1 - e^-t (1 + t)) / t^2, by its series below t = 1 (terms to t^10), directly above
+__device__ __forceinline__ float g_f(float t, float et) {
+    float s = 2.2964459e-8f;  // 1/43545600
+    s = fmaf(s, t, -2.5052108e-7f);
+    s = fmaf(s, t, 2.4801587e-6f);
+    s = fmaf(s, t, -2.2045855e-5f);
+    s = fmaf(s, t, 1.7361111e-4f);
+    s = fmaf(s, t, -1.1904762e-3f);
+    s = fmaf(s, t, 6.9444444e-3f);
+    s = fmaf(s, t, -3.3333335e-2f);
+    s = fmaf(s, t, 0.125f);
+    s = fmaf(s, t, -0.33333334f);
+    s = fmaf(s, t, 0.5f);
+    return t < 1.0f ? s : __fdividef(fmaf(-et, 1.0f + t, 1.0f), t * t);
 }
 
 // Mixed-precision Stokes transfer: the same cell walk, profile, rotations and skip
-// test as stokes_kernel, evaluated in FP32.  The 2x2 propagator of polarize_prop
-// (raytrace.f90:1-56) is applied in the basis X = I + Q, Y = I - Q, where it is two
-// scalar transfer problems with absorption eta_I +- eta_Q:
-//   X' = X e^-m + (eps_I + eps_Q) ds (1 - e^-m)/m,  m = (eta_I + eta_Q) ds,
-//   Y' = Y e^-p + (eps_I - eps_Q) ds (1 - e^-p)/p,  p = (eta_I - eta_Q) ds,
-//   U' = U e^-(eta_I ds);
-// this is the closed form of the reference in exact arithmetic, without its
-// cancellation 1 - (...) e^-tau at small optical depth.  Intensities are carried in
-// units of the line's background intensity I_in, so that emissivities of weak lines
-// stay in the FP32 normal range; the optional dust branch stays FP64.
+// test as stokes_kernel, evaluated in FP32.  Over a segment with t = eta_I ds,
+// q = eta_Q ds, a = eps_I ds, b = eps_Q ds, the 2x2 propagator of polarize_prop
+// (raytrace.f90:1-56) is, in exact arithmetic,
+//   I' = I C - Q S + a A + b D,   Q' = Q C - I S + a D + b A,   U' = U e^-t,
+//   C = e^-t cosh q,  S = e^-t sinh q,
+//   A = (phi(t+q) + phi(t-q)) / 2,  D = (phi(t+q) - phi(t-q)) / 2,  phi(x) = (1 - e^-x)/x.
+// Q is small beside I (a few 1e-4 of it in these clouds), so it is updated from these
+// terms, each formed without cancellation, rather than as the difference of the
+// transported I + Q and I - Q (which would leave Q an FP32 rounding of I, 1e-3 of Q
+// itself).  With r = q/t (|r| < 1: eta_Q is a fraction of eta_I),
+//   A = (phi(t) - e^-t (q r c + r^2 (1 + q^2 s))) / (1 - r^2),
+//   D = q (-g(t) + r e^-t (q s + r c)) / (1 - r^2),
+// where s = (sinh q / q - 1)/q^2, c = (cosh q - 1)/q^2 and g(t) = (1 - e^-t (1 + t))/t^2,
+// each by its series where small arguments would cancel.  Near total polarisation
+// (|r| >= 0.999) the transported I + Q and I - Q are used instead.  Intensities are
+// carried in units of the line's background intensity I_in, so that emissivities of weak
+// lines stay in the FP32 normal range; the optional dust branch stays FP64.
 __global__ void __launch_bounds__(256) stokes_f32_kernel(ImgArgs a) {
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)a.n_rt * a.nlines * a.C;
@@ -385,12 +411,37 @@ __global__ void __launch_bounds__(256) stokes_f32_kernel(ImgArgs a) {
         const double delta_x = dxl * (double)(steps[i + 1] - steps[i]);
         const float ds = (float)delta_x;
         if (ds * eta_I > 1e-30f) {
-            const float m = (eta_I + eta_Q) * ds, p = (eta_I - eta_Q) * ds, t = eta_I * ds;
-            const float em = __expf(-m), ep = __expf(-p), et = __expf(-t);
-            const float X = fmaf(I + Q_out, em, (eps_I + eps_Q) * ds * phi1_f(m, em));
-            const float Y = fmaf(I - Q_out, ep, (eps_I - eps_Q) * ds * phi1_f(p, ep));
-            I = 0.5f * (X + Y);
-            Q_out = 0.5f * (X - Y);
+            const float t = eta_I * ds, q = eta_Q * ds, sa = eps_I * ds, sb = eps_Q * ds;
+            const float et = __expf(-t);
+            const float r = __fdividef(q, t);
+            if (fabsf(r) < 0.999f) {
+                const float q2 = q * q;
+                float sn, cn;
+                if (fabsf(q) < 1.0f) {
+                    sn = fmaf(q2, fmaf(q2, fmaf(q2, 2.7557319e-6f, 1.9841270e-4f), 8.3333333e-3f), 0.16666667f);
+                    cn = fmaf(q2, fmaf(q2, fmaf(q2, 2.4801587e-5f, 1.3888889e-3f), 4.1666668e-2f), 0.5f);
+                } else {
+                    const float eq = __expf(q), emq = __expf(-q);
+                    sn = __fdividef(__fdividef(eq - emq, 2.0f * q) - 1.0f, q2);
+                    cn = __fdividef(fmaf(eq + emq, 0.5f, -1.0f), q2);
+                }
+                const float s1 = q2 * sn, c1 = q2 * cn;
+                const float Cc = et * (1.0f + c1), Sc = et * q * (1.0f + s1);
+                const float om = fmaf(-r, r, 1.0f);
+                const float A = __fdividef(fmaf(-et, fmaf(q * r, cn, r * r * (1.0f + s1)), phi1_f(t, et)), om);
+                const float D = __fdividef(q * fmaf(r * et, fmaf(q, sn, r * cn), -g_f(t, et)), om);
+                const float In = fmaf(I, Cc, fmaf(-Q_out, Sc, fmaf(sa, A, sb * D)));
+                Q_out = fmaf(Q_out, Cc, fmaf(-I, Sc, fmaf(sa, D, sb * A)));
+                I = In;
+            } else {
+                // near total polarisation: the transported I + Q and I - Q
+                const float m = t + q, p = t - q;
+                const float em = __expf(-m), ep = __expf(-p);
+                const float X = fmaf(I + Q_out, em, (sa + sb) * phi1_f(m, em));
+                const float Y = fmaf(I - Q_out, ep, (sa - sb) * phi1_f(p, ep));
+                I = 0.5f * (X + Y);
+                Q_out = 0.5f * (X - Y);
+            }
             U_out = U_out * et;
             tau = tau + t;
         }

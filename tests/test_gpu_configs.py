@@ -199,3 +199,27 @@ def test_million_node_surrogate_run(million):
     assert rel[big].max() < 5e-5
     assert da.max() < 5e-6
     assert max(e) < 1e-4
+
+
+def test_lime_model_files_through_the_c_abi(oracle):
+    """A model read from LIME/LAMDA/dust-table files (tests/golden/lime_model/, read as
+    the reference's eat_lime reads them: dust on half of the nodes, two collision
+    partners) through the FP64 SE pass, against the oracle on the same arrays."""
+    import json
+    import os
+    from portal_amd import lime_io
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    files = {k: os.path.join(gold, v) for k, v in json.loads(str(np.load(os.path.join(gold, "eat_lime.npz"))["files"])).items()}
+    g, sp, _ = lime_io.read_model(files["pop"], files["grid"], files["mol"], files["dust"],
+                                  bvec=np.array([0.36, -0.48, 0.8]), gvec=np.array([0.6, 0.8, 0.0]))
+    xth = cloud.legendre_all(4, 8)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    pp, st = pt.se_pass(xth, 16, cloud.DX_INWARD)
+    pt.close()
+    ref = oracle.se_pass(g, sp, xth, 16, cloud.DX_INWARD, nthreads=NTH)[0]
+    idx = solved(g)
+    assert (g.dust_p[idx] > 0).sum() > 50 and sp.c_par == 2
+    assert_pol_close(pp, ref, idx, sp.e_num, 1e-10)
