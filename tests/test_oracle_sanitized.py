@@ -1,0 +1,98 @@
+"""The oracle under AddressSanitizer + UndefinedBehaviorSanitizer (CPU only).
+
+oracle/sanitize_main.c drives the oracle's SE pass (every bulk node) and one image
+pass from a model this test writes; `make -C oracle sanitize` builds it with gcc's
+-fsanitize=address,undefined and -fno-sanitize-recover, so any out-of-bounds access,
+use after free, leak, signed overflow or misaligned load in the oracle ends the run
+with a non-zero status.  Its outputs must equal the regular build's (liboracle.so) on
+the same inputs: the instrumented run computes what the checker computes.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from portal_amd import cloud, driver
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SAN = os.path.join(ROOT, "oracle", "_san", "oracle_san")
+
+
+def _put(f, a, kind):
+    a = np.asarray(a, dtype=np.int32 if kind == 0 else np.float64).ravel(order="F")
+    f.write(np.array([a.size, kind], dtype=np.int32).tobytes())
+    f.write(a.tobytes())
+
+
+def _read(path):
+    out = []
+    with open(path, "rb") as f:
+        while True:
+            h = np.frombuffer(f.read(8), dtype=np.int32)
+            if h.size == 0:
+                return out
+            out.append(np.frombuffer(f.read(8 * int(h[0])), dtype=np.float64))
+
+
+@pytest.fixture(scope="module")
+def sanitized():
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not present")
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "sanitize"], check=True)
+    return SAN
+
+
+@pytest.mark.parametrize("nlev,seed,dust", [(4, 5, False), (6, 8, True)])
+def test_oracle_clean_under_asan_ubsan(sanitized, oracle, tmp_path, nlev, seed, dust):
+    sp = cloud.co_like_rotor(nlev)
+    g = cloud.make_cloud(sp, 250, seed=seed)
+    if dust:
+        g.dust_p[: g.bulk_node_num] = 1e-3
+        sp.kd_v[:] = 2e-2
+    xth = cloud.legendre_all(4, 7)
+    nvel, img_nvel, im_pix, im_size = 16, 12, 10, 0.7
+    lines = driver.image_lines(sp, [1, 2], [2, 3])
+    vel_out = driver.channel_velocities(img_nvel, 93.75)
+    th_inc, ph_az, alpha = np.pi / 3, 0.4, 0.15
+    dims = [g.node_num, g.bulk_node_num, sp.e_num, sp.r_num, sp.c_num, sp.c_temp, sp.c_par, sp.n_trans,
+            sp.kmax, sp.c_dim, xth.shape[1], nvel, 4096, len(lines), im_pix, img_nvel, 4096]
+    src = tmp_path / "in.bin"
+    with open(src, "wb") as f:
+        _put(f, dims, 0)
+        for a in (g.XYZ, g.vel_all, g.temp_all, g.b_all, g.h2_dens, g.ab_all, g.dust_p, g.lev_p):
+            _put(f, a, 1)
+        _put(f, g.nbr_ptr, 0)
+        _put(f, g.nbr_idx, 0)
+        _put(f, [g.tx], 1)
+        for a in (sp.j_lev, sp.lev_g, sp.r_trans, sp.c_trans, sp.all_trans, sp.count_array):
+            _put(f, a, 0)
+        for a in (sp.e_levs_hz, sp.t_Aji, sp.nu_hz, sp.t_Bij, sp.kd_v, sp.t_Cji, sp.col_temps, xth):
+            _put(f, a, 1)
+        _put(f, [cloud.DX_INWARD, cloud.DX_IMAGE, th_inc, ph_az, im_size, alpha], 1)
+        _put(f, g.gvec, 1)
+        _put(f, g.bvec_all, 1)
+        _put(f, [x[0] for x in lines], 0)
+        _put(f, [x[1] for x in lines], 0)
+        for k in (2, 3, 4):
+            _put(f, [x[k] for x in lines], 1)
+        _put(f, vel_out, 1)
+    dst = tmp_path / "out.bin"
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0:verify_asan_link_order=0",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    r = subprocess.run([sanitized, str(src), str(dst)], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, f"sanitizer run failed ({r.returncode}):\n{r.stderr[-4000:]}"
+    assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr[-4000:]
+    pp_s, J0_s, J2_s, img_s = _read(dst)
+
+    pp, J0, J2 = oracle.se_pass(g, sp, xth, nvel, cloud.DX_INWARD, nthreads=1)
+    outs = oracle.image_pass(g, sp, pp, lines, th_inc, ph_az, img_nvel, vel_out, im_pix, im_size, cloud.DX_IMAGE,
+                             alpha=alpha, n_depth_rt=4096, nthreads=1)
+    assert np.abs(J0).max() > 0 and np.abs(J2).max() > 0
+    np.testing.assert_allclose(pp_s, pp.ravel(order="F"), rtol=1e-13, atol=0)
+    np.testing.assert_allclose(J0_s, J0.ravel(order="F"), rtol=1e-13, atol=0)
+    np.testing.assert_allclose(J2_s, J2.ravel(order="F"), rtol=1e-12, atol=1e-13 * np.abs(J2).max())
+    img = np.concatenate([o.ravel(order="F") for o in outs])
+    assert np.abs(outs[0]).max() > 0
+    np.testing.assert_allclose(img_s, img, rtol=1e-12, atol=1e-13 * np.abs(img).max())
