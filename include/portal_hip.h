@@ -138,6 +138,8 @@ typedef struct {
                                 1 a batch's launch per class of system sizes */
     int se_waves;            /* waves per SE system; for a launch per class, the bitwise
                                 OR of the classes' wave counts (1, 2, 4) */
+    double t_mlp_ms;         /* surrogate network inference in the surrogate step (input
+                                normalisation and the BF16 GEMMs), 0 without it */
 } portal_stats;
 
 /* Context -------------------------------------------------------------------- */

@@ -1064,7 +1064,7 @@ int portal_se_pass(portal_ctx *x, const portal_se_params *p, double *pol_pops, d
     return PORTAL_OK;
 }
 
-static int surrogate_step(Ctx &c, int *n_fallback, int *d_item_flags) {
+static int surrogate_step(Ctx &c, int *n_fallback, int *d_item_flags, double *t_mlp_ms) {
     int n_in = 0, n_out = 0;
     if (!mlp_dims(c, &n_in, &n_out)) return fail(c, PORTAL_ERR_STATE, "no surrogate network (portal_mlp_create)");
     const Species &s = c.sp;
@@ -1075,11 +1075,16 @@ static int surrogate_step(Ctx &c, int *n_fallback, int *d_item_flags) {
     const int nw = (int)c.g.work.size();
     float *d_f = nullptr, *d_o = nullptr;
     int *d_n = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
     Defer release{[&] {
         dfree(d_f);
         dfree(d_o);
         dfree(d_n);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
     }};
+    HIP_TRY(&c, hipEventCreate(&ev[0]));
+    HIP_TRY(&c, hipEventCreate(&ev[1]));
     HIP_TRY(&c, hipMalloc((void **)&d_f, sizeof(float) * (size_t)nw * n_in + 16));
     HIP_TRY(&c, hipMalloc((void **)&d_o, sizeof(float) * (size_t)nw * n_out + 16));
     HIP_TRY(&c, hipMalloc((void **)&d_n, sizeof(int)));
@@ -1108,10 +1113,15 @@ static int surrogate_step(Ctx &c, int *n_fallback, int *d_item_flags) {
     a.perturb_stride = c.surr_perturb_stride;
     a.perturb_factor = c.surr_perturb_factor;
     HIP_TRY(&c, launch_surr_features(a, c.stream));
+    HIP_TRY(&c, hipEventRecord(ev[0], c.stream));
     HIP_TRY(&c, mlp_infer_device(c, nw, d_f, d_o));
+    HIP_TRY(&c, hipEventRecord(ev[1], c.stream));
     HIP_TRY(&c, launch_surr_apply(a, c.stream));
     HIP_TRY(&c, hipMemcpyAsync(n_fallback, d_n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(&c, hipStreamSynchronize(c.stream));
+    float ms = 0.0f;
+    HIP_TRY(&c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    if (t_mlp_ms) *t_mlp_ms += ms;
     return PORTAL_OK;
 }
 
@@ -1327,6 +1337,7 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
     int nhist = 0, n_ng = 0, last_ng = -1, conv = 0;
     Timer tt(c.stream);
     int iters = 0, nfb = 0;
+    double t_mlp = 0.0;
     double dpop = 0, dal = 0;
     // continuing an earlier call: its last SE solution is this call's first comparison
     const bool chained = c.pol_chain && !it->surrogate;
@@ -1352,7 +1363,7 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
                                        c.stream));
             HIP_TRY(&c, hipMemcpyAsync(d_plain_pol, c.d_pol, sizeof(double) * N * 2 * L, hipMemcpyDeviceToDevice,
                                        c.stream));
-            rc = surrogate_step(c, &nfb, d_iflags);
+            rc = surrogate_step(c, &nfb, d_iflags, &t_mlp);
             if (rc) break;
         }
         if (k == 1 && it->surrogate) {
@@ -1502,6 +1513,7 @@ int portal_lambda_iterate(portal_ctx *x, const portal_se_params *p, const portal
         st->iterations = iters;
         st->max_rel_change = dpop;
         st->fallbacks = nfb;
+        st->t_mlp_ms = t_mlp;
         st->ng_steps = n_ng;
         st->converged = conv;
         st->max_align_change = dal;

@@ -7,8 +7,12 @@
 // it as its initial iterate (ctx.cpp) and the FP64 SE solve judges and corrects it.
 //
 // GEMM: C[M x N] = A[M x K] B[K x N] with bf16 operands read through arbitrary
-// strides (so the transposed products of back-propagation need no copies), a 64x64
-// block tile of four 32x32 wavefront tiles, K staged through LDS 32 at a time, and
+// strides (so the transposed products of back-propagation need no copies).  A block of
+// four waves owns a TM x TN tile (64 x 64, or 128 x 128 when that still gives a few
+// blocks per CU), each wave a (TM/2) x (TN/2) quarter of 16 x 16 fragments; K is staged
+// through LDS 32 at a time, the next slice's global loads in flight while the MFMAs
+// consume the current one.  An operand whose k (or whose m / n) is the unit stride is
+// read 8 elements (16 bytes) per load; other strides fall back to element loads.
 // v_mfma_f32_16x16x32_bf16: each lane feeds 8 consecutive k of one row of A and one
 // column of B and receives 4 rows of one column of the 16x16 result.  Epilogues fuse
 // bias, SiLU, the SiLU derivative and split-K partial stores.
@@ -42,55 +46,127 @@ struct GemmArgs {
     long long ldz;
 };
 
-constexpr int BM = 64, BN = 64, BK = 32, PAD = 8;
+constexpr int BK = 32, PAD = 8;
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
 
-__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) bf16 As[BM][BK + PAD];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[BN][BK + PAD];  // transposed: Bs[n][k]
+// how an operand's TR x BK slice (TR rows of m or n, BK of k) is fetched: 8 consecutive k
+// of one row per load (KVEC), 8 consecutive rows at one k (RVEC), or element by element
+enum { LD_ELEM = 0, LD_KVEC = 1, LD_RVEC = 2 };
+
+template <int TR>
+struct Stage {
+    static constexpr int NV = TR * BK / 8 / 256;  // 8-element groups per thread
+    uint4 v[NV];
+};
+
+// load thread tid's groups of the slice rows r0.., k0.. of X (row stride sr, k stride sk)
+template <int TR>
+__device__ __forceinline__ void stage_load(Stage<TR> &st, int mode, const bf16 *X, long long sr, long long sk,
+                                           int r0, int rlim, int k0, int klim, int tid) {
+#pragma unroll
+    for (int q = 0; q < Stage<TR>::NV; q++) {
+        const int e = tid + 256 * q;
+        int r, c;
+        if (mode == LD_RVEC) {
+            r = (e % (TR / 8)) * 8;
+            c = e / (TR / 8);
+        } else {
+            r = e / (BK / 8);
+            c = (e % (BK / 8)) * 8;
+        }
+        const int rr = r0 + r, kk = k0 + c;
+        union {
+            uint4 u;
+            bf16 h[8];
+        } x;
+        const bool full = mode == LD_RVEC ? (rr + 7 < rlim && kk < klim) : (rr < rlim && kk + 7 < klim);
+        if (mode != LD_ELEM && full) {
+            x.u = *reinterpret_cast<const uint4 *>(X + (long long)rr * sr + (long long)kk * sk);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int ri = mode == LD_RVEC ? rr + i : rr, ki = mode == LD_RVEC ? kk : kk + i;
+                x.h[i] = (ri < rlim && ki < klim) ? X[(long long)ri * sr + (long long)ki * sk] : (bf16)0.0f;
+            }
+        }
+        st.v[q] = x.u;
+    }
+}
+
+// store the groups into S[row][k] (k contiguous, row stride BK + PAD)
+template <int TR>
+__device__ __forceinline__ void stage_store(const Stage<TR> &st, int mode, bf16 (*S)[BK + PAD], int tid) {
+#pragma unroll
+    for (int q = 0; q < Stage<TR>::NV; q++) {
+        const int e = tid + 256 * q;
+        if (mode == LD_RVEC) {
+            const int r = (e % (TR / 8)) * 8, c = e / (TR / 8);
+            union {
+                uint4 u;
+                bf16 h[8];
+            } x;
+            x.u = st.v[q];
+#pragma unroll
+            for (int i = 0; i < 8; i++) S[r + i][c] = x.h[i];
+        } else {
+            const int r = e / (BK / 8), c = (e % (BK / 8)) * 8;
+            *reinterpret_cast<uint4 *>(&S[r][c]) = st.v[q];
+        }
+    }
+}
+
+template <int TM, int TN>
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g, int amode, int bmode) {
+    constexpr int FI = TM / 32, FJ = TN / 32;  // 16x16 fragments per wave along m and n
+    __shared__ __attribute__((aligned(16))) bf16 As[TM][BK + PAD];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[TN][BK + PAD];  // transposed: Bs[n][k]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int kchunk = ((g.K + g.ksplit - 1) / g.ksplit + BK - 1) / BK * BK;
     const int kbeg = blockIdx.z * kchunk;
     const int kend = min(g.K, kbeg + kchunk);
-    f32x4 acc[2][2];
+    f32x4 acc[FI][FJ];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < FI; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const bf16 zero = (bf16)0.0f;
+        for (int j = 0; j < FJ; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    Stage<TM> sa;
+    Stage<TN> sb;
+    if (kbeg < kend) {
+        stage_load<TM>(sa, amode, g.A, g.sam, g.sak, m0, g.M, kbeg, kend, tid);
+        stage_load<TN>(sb, bmode, g.B, g.sbn, g.sbk, n0, g.N, kbeg, kend, tid);
+    }
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        for (int e = tid; e < BM * BK; e += 256) {
-            const int r = e / BK, c = e % BK, m = m0 + r, k = k0 + c;
-            As[r][c] = (m < g.M && k < kend) ? g.A[(long long)m * g.sam + (long long)k * g.sak] : zero;
-        }
-        for (int e = tid; e < BN * BK; e += 256) {
-            const int r = e % BN, c = e / BN, n = n0 + r, k = k0 + c;
-            Bs[r][c] = (n < g.N && k < kend) ? g.B[(long long)k * g.sbk + (long long)n * g.sbn] : zero;
-        }
+        stage_store<TM>(sa, amode, As, tid);
+        stage_store<TN>(sb, bmode, Bs, tid);
         __syncthreads();
+        if (k0 + BK < kend) {
+            stage_load<TM>(sa, amode, g.A, g.sam, g.sak, m0, g.M, k0 + BK, kend, tid);
+            stage_load<TN>(sb, bmode, g.B, g.sbn, g.sbk, n0, g.N, k0 + BK, kend, tid);
+        }
         const int kk = (lane >> 4) * 8;
+        bf16x8 b[FJ];
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[wm * 32 + i * 16 + (lane & 15)][kk]);
+        for (int j = 0; j < FJ; j++)
+            b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[wn * (TN / 2) + j * 16 + (lane & 15)][kk]);
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Bs[wn * 32 + j * 16 + (lane & 15)][kk]);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
-            }
+        for (int i = 0; i < FI; i++) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[wm * (TM / 2) + i * 16 + (lane & 15)][kk]);
+#pragma unroll
+            for (int j = 0; j < FJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < FI; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < FJ; j++)
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + t;
-                const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+                const int m = m0 + wm * (TM / 2) + i * 16 + (lane >> 4) * 4 + t;
+                const int n = n0 + wn * (TN / 2) + j * 16 + (lane & 15);
                 if (m >= g.M || n >= g.N) continue;
                 const float v = acc[i][j][t];
                 switch (g.epi) {
@@ -118,9 +194,27 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
             }
 }
 
+// the 16-byte form for an operand with k stride sk and row (m or n) stride sr: its base
+// must be 16-byte aligned and the stride that is not 1 a multiple of 8 elements
+static int load_mode(const bf16 *X, long long sk, long long sr) {
+    const bool aligned = (reinterpret_cast<uintptr_t>(X) & 15) == 0;
+    if (aligned && sk == 1 && sr % 8 == 0) return LD_KVEC;
+    if (aligned && sr == 1 && sk % 8 == 0) return LD_RVEC;
+    return LD_ELEM;
+}
+
 static hipError_t gemm(const GemmArgs &g, hipStream_t s) {
-    dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, g.ksplit);
-    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, s, g);
+    const int am = load_mode(g.A, g.sak, g.sam);
+    const int bm = load_mode(g.B, g.sbk, g.sbn);
+    // the large tile only where it still leaves two blocks per CU
+    const long long big = (long long)((g.N + 127) / 128) * ((g.M + 127) / 128) * g.ksplit;
+    if (big >= 512) {
+        dim3 grid((g.N + 127) / 128, (g.M + 127) / 128, g.ksplit);
+        hipLaunchKernelGGL((gemm_bf16_kernel<128, 128>), grid, dim3(256), 0, s, g, am, bm);
+    } else {
+        dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, g.ksplit);
+        hipLaunchKernelGGL((gemm_bf16_kernel<64, 64>), grid, dim3(256), 0, s, g, am, bm);
+    }
     return hipGetLastError();
 }
 

@@ -61,6 +61,7 @@ module portal_hip
     integer(c_int) :: rt_variant, rt_waves, max_path, ng_steps, converged
     real(c_double) :: max_align_change
     integer(c_int) :: se_form, se_waves
+    real(c_double) :: t_mlp_ms
   end type
 
   interface

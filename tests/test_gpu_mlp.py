@@ -34,7 +34,7 @@ def test_forward_matches_oracle(net):
     yo, _, _ = M.forward(p.astype(np.float64), x, nm.astype(np.float64), dims, bf16=True)
     rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
     print("forward rel err", rel)
-    assert rel < 2e-2
+    assert rel < 2e-3
 
 
 def test_gradients_match_oracle(net):
@@ -44,18 +44,39 @@ def test_gradients_match_oracle(net):
     p, nm = pt.mlp_get_params()
     loss, g = pt.mlp_train_step(x, t, 0.0, want_grads=True)
     lo, go, _ = M.loss_and_grad(p.astype(np.float64), x, t, nm.astype(np.float64), dims, bf16=True)
-    assert loss == pytest.approx(lo, rel=2e-2)
+    assert loss == pytest.approx(lo, rel=2e-3)
     o = 0
     for a, b in M.layer_dims(*dims):
         for n in (a * b, b):
             gg, oo = g[o:o + n], go[o:o + n]
             rel = np.linalg.norm(gg - oo) / max(np.linalg.norm(oo), 1e-30)
             print(f"block of {n}: grad rel err {rel:.3e}")
-            assert rel < 6e-2
+            assert rel < 5e-3
             o += n
     # the parameters did not move with lr = 0
     p2, _ = pt.mlp_get_params()
     np.testing.assert_array_equal(p, p2)
+
+
+def test_large_batch_forward_and_gradients(net):
+    """65,536 rows: the forward products and the back-propagated dZ take the 128 x 128 tile
+    (the small batches above take the 64 x 64 one), with 16-byte, row-vector and element
+    operand loads all in use; errors to the same bounds."""
+    pt, dims = net
+    x, t, norm = _data(dims, 65536, 4)
+    pt.mlp_set_params(M.init_params(dims, seed=9).astype(np.float32), norm)
+    p, nm = pt.mlp_get_params()
+    y = pt.mlp_forward(x)
+    yo, _, _ = M.forward(p.astype(np.float64), x, nm.astype(np.float64), dims, bf16=True)
+    rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
+    print("65,536-row forward rel err", rel)
+    assert rel < 2e-3
+    loss, g = pt.mlp_train_step(x, t, 0.0, want_grads=True)
+    lo, go, _ = M.loss_and_grad(p.astype(np.float64), x, t, nm.astype(np.float64), dims, bf16=True)
+    assert loss == pytest.approx(lo, rel=2e-3)
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    print("65,536-row gradient rel err", rel)
+    assert rel < 5e-3
 
 
 def test_training_reduces_loss(net):
