@@ -54,6 +54,51 @@ def stokes_error(ref, got):
             "max_Q_over_max_I": float(sq / s), "max_U_over_max_I": float(su / s)}
 
 
+def cpu_baseline(g, sp, xth, args, log):
+    """The CPU baseline: the reference's own inverse_rt (inverse_rt.f90:1-137, compiled from
+    its Fortran sources into oracle/_ref, at its own -O1 or, with --ref-build O3, at -O3), one
+    call per node over the host's threads as its OpenMP node loop runs it, on a bounded node
+    sample of the same cloud and one FP64 pass; the project's C restatement is timed on the
+    same sample beside it.  The reference's SE solve needs a library this image lacks, so the
+    sample covers the inward transfer, which is nearly all of the reference's SE pass."""
+    if args.cpu_sample <= 0:
+        log("cpu baseline skipped: --cpu-sample 0")
+        return None
+    try:
+        from oracle.bind import LIB_REF, Oracle, Reference
+        from portal_amd import cloud
+        nthr = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
+        rng = np.random.default_rng(1)
+        idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+        sub = np.sort(rng.choice(idx, min(args.cpu_sample, len(idx)), replace=False)) + 1
+        tp = time.perf_counter()
+        Oracle().inverse_nodes(g, sp, xth, args.nvel, cloud.DX_INWARD, sub, nthreads=nthr)
+        tp = time.perf_counter() - tp
+        lib = LIB_REF if args.ref_build == "O1" else os.path.join(os.path.dirname(LIB_REF), "O3", "libportal_ref.so")
+        try:
+            ref = Reference(lib)
+            kind = "reference"
+        except FileNotFoundError:
+            ref, kind = None, "port"
+        tc = tp
+        if ref is not None:
+            tc = time.perf_counter()
+            ref.inverse_nodes(g, sp, xth, args.nvel, cloud.DX_INWARD, sub, nthreads=nthr)
+            tc = time.perf_counter() - tc
+        per_pass = len(sub) / tc
+        what = (f"the reference's inverse_rt (inverse_rt.f90:1-137, its Fortran built with amdflang -{args.ref_build})"
+                if kind == "reference" else "the C restatement of inverse_rt")
+        log(f"CPU {kind}: {per_pass:.1f} nodes/s per pass on {nthr} threads (C restatement {len(sub) / tp:.1f})")
+        return {"value": per_pass / args.iters, "unit": "converged grid nodes/s", "cores": nthr, "kind": kind,
+                "sample": f"{what}: the inward transfer of {len(sub)} seeded nodes of the same cloud, one FP64 "
+                          f"pass, {tc:.1f}s on {nthr} threads; converged rate = per-pass rate / {args.iters} "
+                          f"iterations (the SE solve, a small part of a pass, is not in the sample)",
+                "nodes_per_pass_per_s": per_pass, "port_nodes_per_pass_per_s": len(sub) / tp}
+    except Exception as e:  # checkers not built
+        log(f"cpu baseline skipped: {e}")
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +117,11 @@ def main():
     ap.add_argument("--im-pix", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (default: up to 16)")
+    ap.add_argument("--ref-build", choices=("O1", "O3"), default="O1",
+                    help="the reference build timed as the CPU baseline: its own -O1 or -O3")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="build the cloud, time only the CPU baseline, print its JSON and exit (no GPU)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (HBM counter calibration) and exit")
@@ -87,8 +137,8 @@ def main():
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a design "
-              "described in DESIGN.md and is not run", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
+              "a design only and is not run", file=sys.stderr)
         sys.exit(2)
 
     from portal_amd import cloud, driver
@@ -103,6 +153,11 @@ def main():
     N_b = int((g.ab_all[: g.bulk_node_num] > 0).sum())
     log(f"cloud: {g.node_num} nodes ({N_b} solved), {xth.shape[1]} directions, {2 * args.nvel + 1} channels, "
         f"{sp.e_num} levels; built in {time.time() - t0:.1f}s")
+    if args.cpu_only:
+        cpu = cpu_baseline(g, sp, xth, args, log)
+        print(json.dumps({"cpu_baseline": cpu, "nodes": args.nodes, "levels": args.levels,
+                          "directions": int(xth.shape[1]), "channels": 2 * args.nvel + 1}), flush=True)
+        return
 
     pt = Portal(0)
     pt.upload_grid(g)
@@ -229,6 +284,17 @@ def main():
     secondary = {"se_solve": {"bound": "fp64", "achieved": se_tfs, "peak": 78.6, "unit": "TFLOP/s",
                               "frac": se_tfs / 78.6, "ms_per_pass": se_ms, "c_dim": c_dim,
                               "flops_per_node": se_flops}}
+    # the surrogate network's inference (one forward of every solved node, BF16 MFMA with
+    # FP32 accumulation): 2 din dout flops per node and layer over the GEMMs, against the
+    # dense BF16 MFMA peak; its time is the device time of the normalisation + GEMMs
+    if surr and stm.get("t_mlp_ms", 0.0) > 0.0:
+        n_in, width, depth, n_out = pt.mlp_dims
+        dims = [n_in] + [width] * depth + [n_out]
+        mlp_flops = 2.0 * sum(a * b for a, b in zip(dims[:-1], dims[1:]))
+        mlp_tfs = N_b * mlp_flops / (stm["t_mlp_ms"] * 1e-3) / 1e12
+        secondary["mlp_inference"] = {"bound": "mfma_bf16", "achieved": mlp_tfs, "peak": 2500.0, "unit": "TFLOP/s",
+                                      "frac": mlp_tfs / 2500.0, "ms": stm["t_mlp_ms"], "rows": N_b,
+                                      "flops_per_row": mlp_flops, "layers": dims}
     # the dominant kernel's HBM traffic and instruction issue, from its rocprofv3 counter
     # runs on this workload (profiles/)
     traffic = None
@@ -245,45 +311,7 @@ def main():
                 "note": "frac: SIMD quad-cycles that issue a VALU instruction (rocprofv3 counters, "
                         "profiles/" + prof.get("counters", "") + ")"}
 
-    # CPU baseline: the reference's own inverse_rt (inverse_rt.f90:1-137, compiled from its
-    # Fortran sources at its own -O1 into oracle/_ref), one call per node over the host's
-    # threads as its OpenMP node loop runs it, on a bounded node sample of the same cloud and
-    # one FP64 pass; the project's C restatement is timed on the same sample beside it.  The
-    # reference's SE solve needs a library this image lacks, so the sample covers the inward
-    # transfer, which is nearly all of the reference's SE pass.
-    cpu = None
-    try:
-        if args.cpu_sample <= 0:
-            raise RuntimeError("--cpu-sample 0")
-        from oracle.bind import Oracle, Reference
-        nthr = min(16, os.cpu_count() or 1)
-        rng = np.random.default_rng(1)
-        idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
-        sub = np.sort(rng.choice(idx, min(args.cpu_sample, len(idx)), replace=False)) + 1
-        tp = time.perf_counter()
-        Oracle().inverse_nodes(g, sp, xth, args.nvel, cloud.DX_INWARD, sub, nthreads=nthr)
-        tp = time.perf_counter() - tp
-        try:
-            ref = Reference()
-            kind = "reference"
-        except FileNotFoundError:
-            ref, kind = None, "port"
-        tc = tp
-        if ref is not None:
-            tc = time.perf_counter()
-            ref.inverse_nodes(g, sp, xth, args.nvel, cloud.DX_INWARD, sub, nthreads=nthr)
-            tc = time.perf_counter() - tc
-        per_pass = len(sub) / tc
-        what = ("the reference's inverse_rt (inverse_rt.f90:1-137, its Fortran built with amdflang -O1)"
-                if kind == "reference" else "the C restatement of inverse_rt")
-        cpu = {"value": per_pass / args.iters, "unit": "converged grid nodes/s", "cores": nthr, "kind": kind,
-               "sample": f"{what}: the inward transfer of {len(sub)} seeded nodes of the same cloud, one FP64 pass, "
-                         f"{tc:.1f}s on {nthr} threads; converged rate = per-pass rate / {args.iters} iterations "
-                         f"(the SE solve, a small part of a pass, is not in the sample)",
-               "nodes_per_pass_per_s": per_pass, "port_nodes_per_pass_per_s": len(sub) / tp}
-        log(f"CPU {kind}: {per_pass:.1f} nodes/s per pass on {nthr} threads (C restatement {len(sub) / tp:.1f})")
-    except Exception as e:  # checkers not built
-        log(f"cpu baseline skipped: {e}")
+    cpu = cpu_baseline(g, sp, xth, args, log)
 
     res = {
         "metric": "converged grid nodes/s: mixed+surrogate vs FP64 GPU at <=1e-4 rel Stokes err",

@@ -9,7 +9,10 @@ on seeded node subsets (needs an MI355X):
 * the 10^6-node cloud through the surrogate path: first-pass FP64 J of 32 seeded nodes
   against the oracle, then the mixed surrogate run against the plain mixed run (fewer
   iterations, the same fixed point, every node normalised, the fallback count) and the
-  Stokes image of each.
+  Stokes image of each;
+* the 200,000-node, 10-level cloud's 513 x 513 x 129 image: FP64 against the oracle on
+  seeded pixel bands, mixed against FP64 on every pixel;
+* a model read from LIME/LAMDA/dust-table files through the FP64 SE pass.
 """
 import numpy as np
 import pytest
@@ -198,6 +201,48 @@ def test_million_node_surrogate_run(million):
           f"Stokes I/Q/U max error {e[0]:.1e} {e[1]:.1e} {e[2]:.1e} of max |I|")
     assert rel[big].max() < 5e-5
     assert da.max() < 5e-6
+    assert max(e) < 1e-4
+
+
+def test_production_cloud_image_against_the_oracle(oracle):
+    """The 200,000-node, 10-level cloud's 513 x 513 x 129 image of J=1-0 after one FP64 SE
+    pass: the FP64 cube against the oracle on three seeded bands of 171 pixels (every
+    channel, Q and U on their own scales), and the mixed cube against the FP64 one."""
+    sp = cloud.co_like_rotor(10)
+    g = cloud.make_cloud(sp, 200_000, seed=2024)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    pp, _ = pt.se_pass(cloud.legendre_all(9, 32), 64, cloud.DX_INWARD)
+    rp = driver.rescale_pol_pops(pp, g.lev_p, sp.lev_g)
+    lines = driver.image_lines(sp, [1], [2])
+    vel_out = driver.channel_velocities(64, 46.875)
+    im_pix, n_side = 256, 513
+    args = (rp, lines, np.pi / 4, 0.0, 64, vel_out, im_pix, 0.7, cloud.DX_IMAGE)
+    (I, Q, U, T), _ = pt.image_pass(*args, n_depth_rt=2048, precision=PORTAL_FP64)
+    (Im, Qm, Um, Tm), _ = pt.image_pass(*args, n_depth_rt=2048, precision=PORTAL_MIXED)
+    pt.close()
+    assert I.shape == (n_side * n_side, 1, 129)
+    rng = np.random.default_rng(43)
+    for ii in (int(x) for x in rng.integers(-200, 200, 3)):
+        lo = (ii + im_pix) * n_side + int(rng.integers(0, n_side - 171)) + 1
+        hi = lo + 170
+        Io, Qo, Uo, To = oracle.image_pass(g, sp, rp, lines, np.pi / 4, 0.0, 64, vel_out, im_pix, 0.7,
+                                           cloud.DX_IMAGE, n_depth_rt=2048, pix_lo=lo, pix_hi=hi, nthreads=NTH)
+        s = slice(lo - 1, hi)
+        Imax, Qmax, Umax = np.abs(Io[s]).max(), np.abs(Qo[s]).max(), np.abs(Uo[s]).max()
+        print(f"row {ii:+d}, pixels {lo}..{hi}: max |Q|/max |I| {Qmax / Imax:.1e}; errors I "
+              f"{np.abs(I[s] - Io[s]).max() / Imax:.1e} Q {np.abs(Q[s] - Qo[s]).max() / Qmax:.1e} "
+              f"U {np.abs(U[s] - Uo[s]).max() / Umax:.1e} (own scales)")
+        assert Imax > 0 and Qmax > 0 and Umax > 0
+        np.testing.assert_allclose(I[s], Io[s], rtol=1e-10, atol=1e-10 * Imax)
+        np.testing.assert_allclose(Q[s], Qo[s], rtol=0, atol=1e-10 * Qmax)
+        np.testing.assert_allclose(U[s], Uo[s], rtol=0, atol=1e-10 * Umax)
+        np.testing.assert_allclose(T[s], To[s], rtol=1e-10, atol=1e-12)
+    e = [np.abs(Im - I).max() / np.abs(I).max(), np.abs(Qm - Q).max() / np.abs(Q).max(),
+         np.abs(Um - U).max() / np.abs(U).max()]
+    print(f"mixed 513^2 x 129 cube: max |dI|/max |I| {e[0]:.1e}, |dQ|/max |Q| {e[1]:.1e}, |dU|/max |U| {e[2]:.1e}")
     assert max(e) < 1e-4
 
 
