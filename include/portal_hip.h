@@ -165,6 +165,20 @@ int portal_upload_species(portal_ctx *ctx, int e_num, const int *j_lev, const do
                           int kmax);
 int portal_species_info(const portal_ctx *ctx, int *n_trans, int *c_dim);
 
+/* Collisional depolarisation D^(K) (depol.f90:242-359; the slot pop_eqs.f90:642-646),
+ * which the reference computes but switches off (trace_node passes D0 = 0,
+ * main_sub.f90:96).  portal_depol_table fills D0_all (c_temp x e_num x (kmax-1),
+ * column-major, the reference's D0(nT, nlev, 2:kmax)) from the uploaded species as
+ * de_polarization does (depol.f90:242-317: Morris 1985 B11, sudden approximation; k_m is
+ * the rate between the J = 0 level and the level with J = m, where the reference's code
+ * reads level index m).  portal_set_depol(ctx, D0_all) makes every later SE solve of the
+ * context subtract D^(K)(T) n_H2 1e-6 (make_depolarization_temp, depol.f90:319-359,
+ * bracketed in T as make_AC brackets the collision rates) on the diagonal of each K >= 2
+ * row; NULL switches it off again (the reference's behaviour).  A new species upload
+ * clears it. */
+int portal_depol_table(const portal_ctx *ctx, double *D0_all);
+int portal_set_depol(portal_ctx *ctx, const double *D0_all);
+
 /* Level densities lev_p (e_num x node_num), the populations the inward rays see
  * (inverse_rt.f90:121-124). */
 int portal_set_levpop(portal_ctx *ctx, const double *lev_p);

@@ -48,7 +48,7 @@ class OrcSeParams(C.Structure):
     _fields_ = [("n_sphere", C.c_int), ("xth", dp), ("nvel", C.c_int), ("dx", C.c_double),
                 ("gvec", dp), ("bvec_all", dp), ("n_depth", C.c_int), ("extra_info", C.c_int),
                 ("star_mod", C.c_int), ("nth", C.c_int), ("nph", C.c_int), ("R_star", C.c_double),
-                ("T_star", C.c_double), ("X_star", dp)]
+                ("T_star", C.c_double), ("X_star", dp), ("D0_all", dp)]
 
 
 def _f64(a):
@@ -101,14 +101,17 @@ class Oracle:
         self._keep.append(arrs)
         return s
 
-    def params(self, xth, nvel, dx, gvec, bvec_all, n_depth, extra_info=0, star=None):
-        """star: None, or dict(nth, nph, R_star, T_star, X_star) for star mode."""
+    def params(self, xth, nvel, dx, gvec, bvec_all, n_depth, extra_info=0, star=None, depol=None):
+        """star: None, or dict(nth, nph, R_star, T_star, X_star) for star mode; depol: None,
+        or the D0_all table of depol_table (collisional depolarisation)."""
         arrs = dict(xth=_f64(xth), g=_f64(gvec), b=_f64(bvec_all),
                     xs=_f64(star["X_star"] if star else np.zeros(3)))
+        if depol is not None:
+            arrs["d"] = _f64(depol)
         s = OrcSeParams(arrs["xth"].shape[1], _d(arrs["xth"]), nvel, dx, _d(arrs["g"]), _d(arrs["b"]),
                         n_depth, extra_info, 1 if star else 0, star["nth"] if star else 0,
                         star["nph"] if star else 0, star["R_star"] if star else 0.0,
-                        star["T_star"] if star else 0.0, _d(arrs["xs"]))
+                        star["T_star"] if star else 0.0, _d(arrs["xs"]), _d(arrs["d"]) if depol is not None else None)
         self._keep.append(arrs)
         return s
 
@@ -130,14 +133,14 @@ class Oracle:
         return self.lib.orc_wig6jj(*[int(x) for x in a])
 
     def se_pass(self, g, sp, xth, nvel, dx, n_depth=4096, node_lo=1, node_hi=None, lev_p=None,
-                extra_info=0, nthreads=0, bvec_all=None, gvec=None, star=None):
+                extra_info=0, nthreads=0, bvec_all=None, gvec=None, star=None, depol=None):
         """Nodes node_lo..node_hi of the SE pass.  Returns (pol_pops N x 2L, J0 R x N, J2 R x N[, rates, JJ])."""
         if node_hi is None:
             node_hi = g.bulk_node_num
         G = self.grid(g, lev_p)
         S = self.species(sp)
         P = self.params(xth, nvel, dx, g.gvec if gvec is None else gvec,
-                        g.bvec_all if bvec_all is None else bvec_all, n_depth, extra_info, star)
+                        g.bvec_all if bvec_all is None else bvec_all, n_depth, extra_info, star, depol)
         N, L, R = g.node_num, sp.e_num, sp.r_num
         pp = np.zeros((N, 2 * L), order="F")
         J0 = np.zeros((R, N), order="F")
@@ -194,12 +197,31 @@ class Oracle:
         self.lib.orc_make_ac(C.byref(S), C.c_double(nH2), C.c_double(Temp), _d(J0), _d(J2), _d(AC))
         return AC
 
-    def get_rhovec(self, sp, AC, Temp, extra_info=0):
+    def depol_table(self, sp):
+        """D0_all (c_temp x e_num x (kmax - 1)) of orc_depol_table (depol.f90:242-317)."""
+        S = self.species(sp)
+        D = np.zeros((sp.c_temp, sp.e_num, max(sp.kmax - 1, 1)), order="F")
+        if self.lib.orc_depol_table(C.byref(S), _d(D)):
+            raise RuntimeError("no J = 0 level")
+        return D
+
+    def depol_at(self, sp, D0_all, Temp, nH2):
+        """D0 (e_num x (kmax - 1)) of orc_depol_at (depol.f90:319-359)."""
+        S = self.species(sp)
+        D = np.zeros((sp.e_num, max(sp.kmax - 1, 1)), order="F")
+        D0_all = _f64(D0_all)
+        self.lib.orc_depol_at(C.byref(S), _d(D0_all), C.c_double(Temp), C.c_double(nH2), _d(D))
+        return D
+
+    def get_rhovec(self, sp, AC, Temp, extra_info=0, D0=None):
+        """D0: None, or the depolarisation rates (e_num x (kmax - 1)) on the K >= 2 diagonal."""
         S = self.species(sp)
         rho = np.zeros(sp.c_dim)
         rates = np.zeros((3, sp.c_dim), order="F")
         AC = _f64(AC)
-        rc = self.lib.orc_get_rhovec(C.byref(S), _d(AC), C.c_double(Temp), _d(rho), extra_info, _d(rates))
+        D = _f64(D0) if D0 is not None else None
+        rc = self.lib.orc_get_rhovec_d(C.byref(S), _d(AC), C.c_double(Temp), _d(D) if D is not None else None,
+                                       _d(rho), extra_info, _d(rates))
         if rc:
             raise RuntimeError("singular SE matrix")
         return (rho, rates) if extra_info else rho

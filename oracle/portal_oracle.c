@@ -693,10 +693,92 @@ void orc_make_ac(const orc_species *s, double nH2, double Temp, const double *J0
     free(C_r);
 }
 
+/* Collisional depolarisation (depol.f90:242-359).  The reference computes D^(K) but
+ * trace_node passes D0 = 0 (main_sub.f90:96), so these are the option's restatement,
+ * with the index policy below where the reference's code cannot be followed.
+ *
+ * orc_depol_table, de_polarization (depol.f90:242-317): Morris (1985) Eq. B11 in the
+ * sudden approximation,
+ *   D^(K)(J) = (2J+1)^2 sum_{m=1}^{2J} (-1)^m (2m+1) (J J m; 0 0 0)^2
+ *              [{J J m; J J 0} - {J J m; J J K}] k_m(T),
+ * for even K = 2..min(2J, kmax) at every tabulated collision temperature.  k_m is the
+ * tabulated rate coefficient of the collisional transition between the J = 0 level and
+ * the level whose J is m, with make_AC's partner mix (0.25 / 0.75 for two partners), and
+ * 0 where the species has no such transition.  The reference reads C_j0 at level index
+ * m, the level with J = m - 1 when j_lev(i) = i - 1 (and past its array for m >= e_num);
+ * this takes the level with J = m.  D0_all is c_temp x e_num x (kmax - 1) column-major,
+ * the reference's D0(nT, nlev, 2:kmax); odd K, K > 2J and levels with J = 0 hold 0.
+ * Returns 0, or -1 without a J = 0 level. */
+int orc_depol_table(const orc_species *s, double *D0_all) {
+    const int nT = s->c_temp, L = s->e_num, nk = s->kmax - 1, nC = s->c_num;
+    if (nk <= 0) return 0;
+    for (size_t i = 0; i < (size_t)nT * L * nk; i++) D0_all[i] = 0.0;
+    int l0 = -1;
+    for (int i = 0; i < L; i++)
+        if (s->j_lev[i] == 0) { l0 = i; break; }
+    if (l0 < 0) return -1;
+#define CC(t, c, p) s->t_Cji[(t) + (size_t)nT * ((c) + (size_t)nC * (p))]
+    for (int i = 0; i < L; i++) {
+        const int J = s->j_lev[i];
+        const double lj2 = (2.0 * J + 1.0) * (2.0 * J + 1.0);
+        for (int K = 2; K <= s->kmax && K <= 2 * J; K += 2)
+            for (int m = 1; m <= 2 * J; m++) {
+                const double w1 = orc_wig3jj(2 * J, 2 * J, 2 * m, 0, 0, 0);
+                if (w1 == 0.0) continue;
+                const double fac = ((m & 1) ? -1.0 : 1.0) * (2.0 * m + 1.0);
+                const double fac2 = orc_wig6jj(2 * J, 2 * J, 2 * m, 2 * J, 2 * J, 0) -
+                                    orc_wig6jj(2 * J, 2 * J, 2 * m, 2 * J, 2 * J, 2 * K);
+                int lm = -1, col = -1;
+                for (int q = 0; q < L; q++)
+                    if (q != l0 && s->j_lev[q] == m) { lm = q; break; }
+                if (lm >= 0)
+                    for (int c = 0; c < nC; c++) {
+                        const int a = s->c_trans[2 * c] - 1, b = s->c_trans[2 * c + 1] - 1;
+                        if ((a == l0 && b == lm) || (a == lm && b == l0)) { col = c; break; }
+                    }
+                if (col < 0) continue;
+                for (int t = 0; t < nT; t++) {
+                    const double k = s->c_par == 2 ? 0.25 * CC(t, col, 0) + 0.75 * CC(t, col, 1) : CC(t, col, 0);
+                    D0_all[t + (size_t)nT * (i + (size_t)L * (K - 2))] += lj2 * fac * w1 * w1 * fac2 * k;
+                }
+            }
+    }
+#undef CC
+    return 0;
+}
+
+/* make_depolarization_temp, depol.f90:319-359: D0 (e_num x (kmax - 1)) at Temp, times
+ * n_H2 [m^-3] * 1e-6.  The reference's bracket loop has no exit, so it takes the last
+ * interval whose upper temperature exceeds Temp, and leaves D0 unset above the table;
+ * this brackets as make_AC does: the first interval with col_temps(i) > Temp, so below
+ * col_temps(1) the first interval's line extrapolated, and NaN above the table. */
+void orc_depol_at(const orc_species *s, const double *D0_all, double Temp, double nH2, double *D0) {
+    const int nT = s->c_temp, L = s->e_num, nk = s->kmax - 1;
+    const double *ct = s->col_temps;
+    for (int e = 0; e < L * nk; e++) {
+        const double *f = D0_all + (size_t)nT * e;
+        double v = NAN;
+        if (Temp <= ct[0]) v = f[0];
+        for (int i = 1; i < nT; i++)
+            if (ct[i] > Temp) {
+                const double ddT = (Temp - ct[i - 1]) / (ct[i] - ct[i - 1]);
+                v = f[i - 1] + ddT * (f[i] - f[i - 1]);
+                break;
+            }
+        D0[e] = v * nH2 * 1e-6;
+    }
+}
+
 /* setup_stateq, pop_eqs.f90:361-653.  Matrices are c_dim x c_dim column-major.
- * The uninitialised col_dep of pop_eqs.f90:642-648 is taken as 0 (D0 = 0). */
-void orc_setup_stateq(const orc_species *s, const double *AC, double Temp, double *Amat, double *A_mat,
-                      double *B_mat, double *C_mat) {
+ * D0 (e_num x (kmax - 1), the depolarisation rates of make_depolarization_temp, or NULL
+ * for none, as trace_node passes) enters the diagonal of every K >= 2 row, as
+ * pop_eqs.f90:642-646 does.  The reference's col_dep is not reset between entries, so a
+ * nonzero D0 there would also enter every later entry's Amat sum (and is uninitialised
+ * before the first K >= 2 diagonal); here it enters the diagonal only, the relaxation
+ * term -D^(K) rho^K_0 of Landi Degl'Innocenti & Landolfi (2004) Eq. 7.101.  With D0 = 0
+ * the two agree. */
+void orc_setup_stateq_d(const orc_species *s, const double *AC, double Temp, const double *D0, double *Amat,
+                        double *A_mat, double *B_mat, double *C_mat) {
     const int nlev = s->e_num, nt = s->n_trans, cd = s->c_dim, kmax = s->kmax;
     const double a_to_b = 6.781962e49;
     size_t nn = (size_t)cd * cd;
@@ -805,8 +887,9 @@ void orc_setup_stateq(const orc_species *s, const double *AC, double Temp, doubl
                         M(C_mat, c1, c2) = M(C_mat, c1, c2) - C21;
                     }
                 }
+                col_dep = 0.0;
                 if (k1 == k2 && k2 >= 2) {
-                    col_dep = 0.0; /* D0(i1,k1) = 0 (main_sub.f90:96) */
+                    col_dep = D0 ? D0[(i1 - 1) + (size_t)nlev * (k1 - 2)] : 0.0;
                     M(C_mat, c1, c2) = M(C_mat, c1, c2) - col_dep;
                 }
                 M(Amat, c1, c2) = -1.0 * (rad_abs + rad_stim + col_abs + col_stim + col_dep);
@@ -818,6 +901,11 @@ void orc_setup_stateq(const orc_species *s, const double *AC, double Temp, doubl
     free(dn);
     free(acu);
     free(acd);
+}
+
+void orc_setup_stateq(const orc_species *s, const double *AC, double Temp, double *Amat, double *A_mat,
+                      double *B_mat, double *C_mat) {
+    orc_setup_stateq_d(s, AC, Temp, NULL, Amat, A_mat, B_mat, C_mat);
 }
 
 /* Dense solve A x = b by LU with partial pivoting (first maximal pivot, as
@@ -884,11 +972,18 @@ static int solve_ld(int n, const double *A, const double *b, double *x) {
  * LU failure code (the reference stops the run there). */
 int orc_get_rhovec(const orc_species *s, const double *AC, double Temp, double *rhovec, int extra_info,
                    double *rates) {
+    return orc_get_rhovec_d(s, AC, Temp, NULL, rhovec, extra_info, rates);
+}
+
+/* get_rhovec with the depolarisation rates D0 (e_num x (kmax - 1), or NULL) that the
+ * reference's signature carries (em_coefs.f90:115,130). */
+int orc_get_rhovec_d(const orc_species *s, const double *AC, double Temp, const double *D0, double *rhovec,
+                     int extra_info, double *rates) {
     int cd = s->c_dim;
     size_t nn = (size_t)cd * cd;
     double *mats = (double *)malloc(sizeof(double) * nn * 4);
     double *Am = mats, *Rm = mats + nn, *Bm = mats + 2 * nn, *Cm = mats + 3 * nn;
-    orc_setup_stateq(s, AC, Temp, Am, Rm, Bm, Cm);
+    orc_setup_stateq_d(s, AC, Temp, D0, Am, Rm, Bm, Cm);
     for (int j = 0; j < cd; j++) Am[(size_t)cd * j] = 0.0;
     for (int i = 0; i < s->e_num; i++) Am[(size_t)cd * s->count_array[i]] = sqrt(2.0 * s->j_lev[i] + 1.0);
     double *b = (double *)calloc(cd, sizeof(double));
@@ -940,7 +1035,13 @@ int orc_trace_node(int node, const orc_grid *g, const orc_species *s, const orc_
     double *rho = (double *)malloc(sizeof(double) * s->c_dim * 4);
     double *rates = rho + s->c_dim;
     orc_make_ac(s, g->h2_dens[node - 1], g->temp[2 * (size_t)(node - 1)], J0, J2, AC);
-    int info = orc_get_rhovec(s, AC, g->temp[2 * (size_t)(node - 1)], rho, p->extra_info, rates);
+    double *D0 = NULL;
+    if (p->D0_all && s->kmax >= 2) {
+        D0 = (double *)malloc(sizeof(double) * s->e_num * (s->kmax - 1));
+        orc_depol_at(s, p->D0_all, g->temp[2 * (size_t)(node - 1)], g->h2_dens[node - 1], D0);
+    }
+    int info = orc_get_rhovec_d(s, AC, g->temp[2 * (size_t)(node - 1)], D0, rho, p->extra_info, rates);
+    free(D0);
     if (!info) pack_node(s, rho, rates, p->extra_info, node_pops, node_rates);
     if (p->extra_info && JJ_20)
         for (int k = 0; k < s->r_num; k++) JJ_20[k] = J2[k] / J0[k];

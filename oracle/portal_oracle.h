@@ -61,6 +61,9 @@ typedef struct {
     int star_mod, nth, nph;
     double R_star, T_star;
     const double *X_star;
+    /* collisional depolarisation: the D0_all table of orc_depol_table (c_temp x e_num x
+     * (kmax - 1)), or NULL for none (main_sub.f90:96 passes D0 = 0) */
+    const double *D0_all;
 } orc_se_params;
 
 /* Wigner symbols, arguments are 2j / 2m like WIGXJPF's fwig3jj/fwig6jj. */
@@ -101,6 +104,12 @@ void orc_setup_stateq(const orc_species *s, const double *AC, double Temp, doubl
                       double *A_mat, double *B_mat, double *C_mat);
 int  orc_get_rhovec(const orc_species *s, const double *AC, double Temp, double *rhovec,
                     int extra_info, double *rates);
+void orc_setup_stateq_d(const orc_species *s, const double *AC, double Temp, const double *D0, double *Amat,
+                        double *A_mat, double *B_mat, double *C_mat);
+int  orc_get_rhovec_d(const orc_species *s, const double *AC, double Temp, const double *D0, double *rhovec,
+                      int extra_info, double *rates);
+int  orc_depol_table(const orc_species *s, double *D0_all);
+void orc_depol_at(const orc_species *s, const double *D0_all, double Temp, double nH2, double *D0);
 int  orc_trace_node(int node, const orc_grid *g, const orc_species *s, const orc_se_params *p,
                     double *node_pops, double *node_rates, double *JJ_20, double *J0, double *J2);
 int  orc_se_pass(const orc_grid *g, const orc_species *s, const orc_se_params *p, int node_lo,

@@ -64,8 +64,8 @@ int main(int argc, char **argv) {
     if (!f) return 2;
     int n;
     int *dims = ri(f, &n); /* node_num bulk e_num r_num c_num c_temp c_par n_trans kmax c_dim n_sphere nvel
-                              n_depth nlevels im_pix img_nvel n_depth_rt */
-    if (n != 17) return 2;
+                              n_depth nlevels im_pix img_nvel n_depth_rt depolarisation */
+    if (n != 18) return 2;
     const int N = dims[0], L = dims[2];
     orc_grid g = {dims[0], dims[1], dims[2]};
     g.XYZ = rd(f, &n);
@@ -104,6 +104,12 @@ int main(int argc, char **argv) {
     p.bvec_all = rd(f, &n);
     p.n_depth = dims[12];
     p.extra_info = 1;
+    if (dims[17]) {
+        /* the depolarisation option: the table of orc_depol_table in every node's SE solve */
+        double *D0_all = keep(calloc((size_t)s.c_temp * s.e_num * (s.kmax > 1 ? s.kmax - 1 : 1), sizeof(double)));
+        if (orc_depol_table(&s, D0_all)) return 3;
+        p.D0_all = D0_all;
+    }
     const int nl = dims[13];
     int *l1 = ri(f, &n), *l2 = ri(f, &n);
     double *nu0 = rd(f, &n), *Aji = rd(f, &n), *kd = rd(f, &n), *vel_out = rd(f, &n);

@@ -94,6 +94,25 @@ class _Core:
         self._check(self.lib.portal_species_info(self.ctx, C.byref(nt), C.byref(cd)))
         return nt.value, cd.value
 
+    # collisional depolarisation (depol.f90:242-359), off by default as in the reference
+    def depol_table(self):
+        """D0_all (c_temp x e_num x (kmax - 1)) of de_polarization for the uploaded species."""
+        sp = self.species
+        D = np.zeros((sp.c_temp, sp.e_num, max(sp.kmax - 1, 1)), order="F")
+        self._check(self.lib.portal_depol_table(self.ctx, D.ctypes.data_as(C.c_void_p)))
+        return D
+
+    def set_depol(self, D0_all):
+        """Subtract D^(K)(T) n_H2 1e-6 on the K >= 2 diagonal in every later SE solve (None: off)."""
+        if D0_all is None:
+            self._check(self.lib.portal_set_depol(self.ctx, None))
+            return
+        D = np.asfortranarray(D0_all, dtype=np.float64)
+        sp = self.species
+        if D.shape != (sp.c_temp, sp.e_num, sp.kmax - 1):
+            raise ValueError(f"depolarisation table shape {D.shape}, expected {(sp.c_temp, sp.e_num, sp.kmax - 1)}")
+        self._check(self.lib.portal_set_depol(self.ctx, D.ctypes.data_as(C.c_void_p)))
+
     # star mode (main_sub.f90:56-79): None, or dict(nth, nph, R_star, T_star, X_star) used by
     # every later pass; the directions then come from star_all instead of xth
     star = None

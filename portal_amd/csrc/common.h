@@ -78,8 +78,11 @@ struct Species {
     int *d_tlo = nullptr, *d_tup = nullptr;               // per transition, 0-based levels
     double *d_Cji = nullptr, *d_coltemps = nullptr;
     int *d_count = nullptr, *d_jlev = nullptr;
+    // collisional depolarisation table (c_temp x e_num x (kmax - 1)), or null for none
+    double *d_depol = nullptr;
     bool valid = false;
 };
+int depol_table(const Species &s, std::vector<double> &out);
 
 struct Grid {
     int N = 0, N_b = 0;
@@ -279,6 +282,9 @@ struct SeArgs {
     // densities (L x N) that produced J, and each transition's lower/upper level (0-based)
     const double *lam, *levp;
     const int *t_lo, *t_up;
+    // collisional depolarisation D^(K) (c_temp x L x (kmax - 1) on the collision
+    // temperatures), or null: -D^(K)(T) n_H2 1e-6 on the diagonal of each K >= 2 row
+    const double *depol;
     double *pol;                                // N x 2L
     double *rates;                              // N x 3 x 2L or null
     int *flags;

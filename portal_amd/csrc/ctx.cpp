@@ -607,6 +607,7 @@ int run_se(Ctx &c, int extra_info, portal_stats *st) {
         a.lam = c.d_LAM;
         a.levp = c.d_levp;
     }
+    a.depol = s.d_depol;
     a.pol = c.d_pol;
     a.rates = extra_info ? c.d_rates : nullptr;
     a.flags = c.d_flags;
@@ -713,6 +714,7 @@ int portal_ctx_destroy(portal_ctx *x) {
     dfree(s.d_coltemps);
     dfree(s.d_count);
     dfree(s.d_jlev);
+    dfree(s.d_depol);
     dfree(s.se.d_ent_idx);
     dfree(s.se.d_ent_ptr);
     dfree(s.se.d_term_q);
@@ -924,6 +926,35 @@ int portal_upload_species(portal_ctx *x, int e_num, const int *j_lev, const doub
     dfree(c.d_levp);
     c.cap_nodes = 0;
     s.valid = true;
+    return PORTAL_OK;
+}
+
+// Collisional depolarisation (depol.f90:242-359).  The reference computes D^(K) in
+// de_polarization but trace_node passes D0 = 0 (main_sub.f90:96); here the table is an
+// option of the uploaded species.
+int portal_depol_table(const portal_ctx *x, double *D0_all) {
+    if (!x || !D0_all) return PORTAL_ERR_ARG;
+    const Species &s = x->c.sp;
+    if (!s.valid) return PORTAL_ERR_STATE;
+    std::vector<double> t;
+    if (depol_table(s, t)) return PORTAL_ERR_ARG;
+    std::copy(t.begin(), t.end(), D0_all);
+    return PORTAL_OK;
+}
+
+int portal_set_depol(portal_ctx *x, const double *D0_all) {
+    if (!x) return PORTAL_ERR_ARG;
+    Ctx &c = x->c;
+    Species &s = c.sp;
+    if (!s.valid) return fail(c, PORTAL_ERR_STATE, "no species (portal_upload_species)");
+    dfree(s.d_depol);
+    if (!D0_all) return PORTAL_OK;
+    if (s.kmax < 2) return fail(c, PORTAL_ERR_ARG, "depolarisation needs kmax >= 2");
+    const size_t n = (size_t)s.c_temp * s.e_num * (s.kmax - 1);
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(D0_all[i])) return fail(c, PORTAL_ERR_ARG, "depolarisation table holds a non-finite value");
+    std::vector<double> v(D0_all, D0_all + n);
+    HIP_TRY(&c, upload(s.d_depol, v));
     return PORTAL_OK;
 }
 
@@ -1890,6 +1921,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         a.term_coef = sp.se.d_term_coef;
         a.count_array = sp.d_count;
         a.j_lev = sp.d_jlev;
+        a.depol = sp.d_depol;
         a.pol = c.d_pol;
         a.rates = nullptr;
         a.flags = c0.d_flags;

@@ -36,6 +36,29 @@ __device__ __forceinline__ double lane_bcast(double x, int l) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Collisional depolarisation: subtract D^(K)(T) n_H2 1e-6 from the diagonal of every
+// K >= 2 row of M (K = 2 sl for level i's slot sl), the table interpolated with the
+// collision rates' bracket (ib, below); the threads of the workgroup split the levels.
+__device__ __forceinline__ void add_depol(const SeArgs &a, double *M, int cd, int ib, bool below, double T,
+                                          double nH2, int tid, int nt) {
+    const int L = a.L;
+    for (int i = tid; i < L; i += nt) {
+        const int c0 = a.count_array[i], c1 = i + 1 < L ? a.count_array[i + 1] : cd;
+        for (int sl = 1; sl < c1 - c0; sl++) {
+            const double *f = a.depol + (size_t)a.c_temp * (i + (size_t)L * (2 * sl - 2));
+            double D;
+            if (ib >= 0) {
+                const double ddT = (T - a.col_temps[ib - 1]) / (a.col_temps[ib] - a.col_temps[ib - 1]);
+                D = f[ib - 1] + ddT * (f[ib] - f[ib - 1]);
+            } else {
+                D = below ? f[0] : __builtin_nan("");
+            }
+            const int d = c0 + sl;
+            M[d + (size_t)cd * d] = M[d + (size_t)cd * d] - D * nH2 * 1e-6;
+        }
+    }
+}
+
 // A system is solved by W wavefronts together (64W threads, "waves per system"): the
 // elementwise loops stride over all of them, the trailing update of each LU step splits
 // its columns between the waves (column j to wave j mod W), and every wave runs the
@@ -126,8 +149,12 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
         M[a.ent_idx[e]] = s;
     }
     __syncthreads();
-    // first row <- normalisation sum_i sqrt(2J_i+1) rho^0_0(i) = 1
+    // first row <- normalisation sum_i sqrt(2J_i+1) rho^0_0(i) = 1; collisional
+    // depolarisation on the diagonal of the K >= 2 rows (pop_eqs.f90:642-646; never row 0),
+    // interpolated in T as the collision rates are, times n_H2 1e-6
+    // (make_depolarization_temp, depol.f90:319-359)
     for (int j = tid; j < cd; j += NT) M[(size_t)cd * j] = 0.0;
+    if (a.depol) add_depol(a, M, cd, ib, below, T, nH2, tid, NT);
     __syncthreads();
     for (int i = tid; i < L; i += NT) M[(size_t)cd * a.count_array[i]] = sqrt(2.0 * a.j_lev[i] + 1.0);
     __syncthreads();
@@ -326,6 +353,10 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
                 M[a.ent_idx[e]] = s;
             }
             __syncthreads();
+            if (cls == 2 && a.depol) {  // the collisional matrix carries -D^(K) too (pop_eqs.f90:645)
+                add_depol(a, M, cd, ib, below, T, nH2, tid, NT);
+                __syncthreads();
+            }
             for (int i = tid; i < cd; i += NT) {
                 double s = 0.0;
                 for (int j = 0; j < cd; j++) s = s + fabs(M[i + (size_t)cd * j]) * v[j];

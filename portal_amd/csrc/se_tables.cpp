@@ -288,4 +288,49 @@ int build_species_tables(Species &s) {
     return 0;
 }
 
+// de_polarization (depol.f90:242-317): Morris (1985) Eq. B11 in the sudden
+// approximation, the rank-K relaxation rate coefficient of a level with J,
+//   D^(K)(J) = (2J+1)^2 sum_{m=1}^{2J} (-1)^m (2m+1) (J J m; 0 0 0)^2
+//              [{J J m; J J 0} - {J J m; J J K}] k_m(T),
+// for even K = 2..min(2J, kmax) at each tabulated collision temperature.  k_m is the
+// tabulated rate coefficient between the J = 0 level and the level whose J is m (0.25 /
+// 0.75 partner mix with two partners, as make_AC), 0 where the species has none.  The
+// reference reads its rate array at level index m (J = m - 1); this takes J = m.  out
+// is c_temp x e_num x (kmax - 1) column-major, the reference's D0(nT, nlev, 2:kmax).
+// Returns 0, or -1 if the species has no J = 0 level.
+int depol_table(const Species &s, std::vector<double> &out) {
+    const int nT = s.c_temp, L = s.e_num, nk = std::max(s.kmax - 1, 0), nC = s.c_num;
+    out.assign((size_t)nT * L * nk, 0.0);
+    int l0 = -1;
+    for (int i = 0; i < L && l0 < 0; i++)
+        if (s.j_lev[i] == 0) l0 = i;
+    if (l0 < 0) return -1;
+    for (int i = 0; i < L; i++) {
+        const int J = s.j_lev[i];
+        const double lj2 = (2.0 * J + 1.0) * (2.0 * J + 1.0);
+        for (int K = 2; K <= s.kmax && K <= 2 * J; K += 2)
+            for (int m = 1; m <= 2 * J; m++) {
+                const double w1 = wig3jj(2 * J, 2 * J, 2 * m, 0, 0, 0);
+                if (w1 == 0.0) continue;
+                int lm = -1, col = -1;
+                for (int q = 0; q < L && lm < 0; q++)
+                    if (q != l0 && s.j_lev[q] == m) lm = q;
+                for (int c = 0; c < nC && lm >= 0 && col < 0; c++) {
+                    const int a = s.c_trans[2 * c] - 1, b = s.c_trans[2 * c + 1] - 1;
+                    if ((a == l0 && b == lm) || (a == lm && b == l0)) col = c;
+                }
+                if (col < 0) continue;
+                const double f = lj2 * sgn(m) * (2.0 * m + 1.0) * w1 * w1 *
+                                 (wig6jj(2 * J, 2 * J, 2 * m, 2 * J, 2 * J, 0) - wig6jj(2 * J, 2 * J, 2 * m, 2 * J, 2 * J, 2 * K));
+                for (int t = 0; t < nT; t++) {
+                    const double *C = s.t_Cji.data();
+                    const double k = s.c_par == 2 ? 0.25 * C[t + (size_t)nT * col] + 0.75 * C[t + (size_t)nT * (col + nC)]
+                                                  : C[t + (size_t)nT * col];
+                    out[t + (size_t)nT * (i + (size_t)L * (K - 2))] += f * k;
+                }
+            }
+    }
+    return 0;
+}
+
 }  // namespace portal

@@ -106,6 +106,20 @@ module portal_hip
       real(c_double), intent(in) :: lev_p(*)
     end function
 
+    ! collisional depolarisation (depol.f90:242-359): the table of de_polarization,
+    ! D0_all(nT, nlev, 2:kmax), and switching it on (c_null_ptr: off, as main_sub.f90:96)
+    integer(c_int) function portal_depol_table(ctx, D0_all) bind(C, name="portal_depol_table")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: D0_all(*)
+    end function
+
+    integer(c_int) function portal_set_depol(ctx, D0_all) bind(C, name="portal_set_depol")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: D0_all
+    end function
+
     integer(c_int) function portal_get_levpop(ctx, lev_p) bind(C, name="portal_get_levpop")
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

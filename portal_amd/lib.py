@@ -30,7 +30,7 @@ EXPORTS = ["portal_ctx_create", "portal_ctx_destroy", "portal_last_error", "port
            "portal_image_pass", "portal_mlp_create", "portal_mlp_destroy", "portal_mlp_num_params",
            "portal_mlp_set_params", "portal_mlp_get_params", "portal_mlp_forward", "portal_mlp_train_step",
            "portal_mlp_train", "portal_surrogate_features", "portal_get_pol_pops",
-           "portal_batch_lambda_iterate"]
+           "portal_batch_lambda_iterate", "portal_depol_table", "portal_set_depol"]
 
 
 class SeParams(C.Structure):

@@ -53,6 +53,40 @@ def test_batch_equals_single_model_iterations(precision, tol):
         pt.close()
 
 
+def test_batch_with_depolarisation_equals_single_models():
+    """Models with collisional depolarisation switched on (two of three; the third keeps
+    the reference's D = 0) iterate in the batch as they do alone (FP64, 1e-10)."""
+    models = _models()
+    xth = cloud.legendre_all(4, 8)
+
+    def ctx(sp, g, m):
+        pt = Portal(0)
+        pt.upload_grid(g)
+        pt.upload_species(sp)
+        pt.set_levpop(g.lev_p)
+        if m != 1:
+            pt.set_depol(30.0 * pt.depol_table())
+        return pt
+
+    singles = []
+    for m, (sp, g) in enumerate(models):
+        pt = ctx(sp, g, m)
+        singles.append(pt.lambda_iterate(xth, 8, cloud.DX_INWARD, 3, tol_pop=0.0, tol_align=0.0,
+                                         precision=PORTAL_FP64)[:2])
+        pt.close()
+    pts = [ctx(sp, g, m) for m, (sp, g) in enumerate(models)]
+    batch_lambda_iterate(pts, [xth] * 3, [8] * 3, cloud.DX_INWARD, 3, tol_pop=0.0, tol_align=0.0,
+                         precision=PORTAL_FP64)
+    for (sp, g), pt, (pp, lev) in zip(models, pts, singles):
+        idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+        np.testing.assert_allclose(pt.get_levpop()[:, idx], lev[:, idx], rtol=1e-10)
+        pb = pt.get_pol_pops()
+        np.testing.assert_allclose(pb[idx][:, 0::2], pp[idx][:, 0::2], rtol=1e-10)
+        scale = np.repeat(np.abs(pp[idx][:, 0::2]), 1, axis=1)
+        assert np.all(np.abs(pb[idx][:, 1::2] - pp[idx][:, 1::2]) <= 1e-10 * scale + 1e-300)
+        pt.close()
+
+
 def test_batch_converges_each_model():
     models = _models()
     xth = cloud.legendre_all(4, 8)

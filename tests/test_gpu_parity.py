@@ -134,6 +134,38 @@ def test_se_pass_fp64_parity(portal1, cfg1, oracle_cfg1):
     np.testing.assert_allclose(rates[idx], rates_o[idx], rtol=1e-9, atol=1e-9 * np.abs(rates_o[idx]).max())
 
 
+def test_se_pass_with_collisional_depolarisation(oracle):
+    """The depolarisation option (depol.f90:242-359; the reference switches it off at
+    main_sub.f90:96): the library's D^(K) table equals the oracle's, the FP64 SE pass with
+    it equals the oracle's to 1e-10 (populations, alignment of every level, the collisional
+    rates that carry -D^(K)), and switching it off again gives the plain pass."""
+    sp = cloud.co_like_rotor(6)
+    g = cloud.make_cloud(sp, 1500, seed=21)
+    xth = cloud.legendre_all(4, 8)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    D = pt.depol_table()
+    np.testing.assert_allclose(D, oracle.depol_table(sp), rtol=1e-13, atol=0)
+    assert D[:, 1:, 0].min() > 0
+    D = 30.0 * D  # stronger than these rotors' rates give, so that the alignment changes visibly
+    pt.set_depol(D)
+    pp, rates, _, _ = pt.se_pass(xth, 16, cloud.DX_INWARD, extra_info=1)
+    ppo, _, _, rates_o, _ = oracle.se_pass(g, sp, xth, 16, cloud.DX_INWARD, extra_info=1, nthreads=NTH, depol=D)
+    idx = solved(g)
+    assert_pops_close(pp, ppo, sp, idx, 1e-10)
+    np.testing.assert_allclose(rates[idx], rates_o[idx], rtol=1e-9, atol=1e-9 * np.abs(rates_o[idx]).max())
+    pt.set_depol(None)
+    pp0, _ = pt.se_pass(xth, 16, cloud.DX_INWARD)
+    ppo0 = oracle.se_pass(g, sp, xth, 16, cloud.DX_INWARD, nthreads=NTH)[0]
+    pt.close()
+    assert_pops_close(pp0, ppo0, sp, idx, 1e-10)
+    al = lambda p: np.abs(p[idx][:, 3::2]) / p[idx][:, 2::2]  # noqa: E731  (levels with J > 0)
+    print(f"mean |rho20/rho00|: {al(pp0).mean():.3e} without, {al(pp).mean():.3e} with depolarisation")
+    assert al(pp).mean() < 0.8 * al(pp0).mean()
+
+
 def test_se_solve_from_oracle_J(portal1, cfg1, oracle, oracle_cfg1):
     sp, g, xth = cfg1
     _, J0o, J2o, _, _ = oracle_cfg1

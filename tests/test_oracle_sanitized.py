@@ -1,7 +1,7 @@
 """The oracle under AddressSanitizer + UndefinedBehaviorSanitizer (CPU only).
 
 oracle/sanitize_main.c drives the oracle's SE pass (every bulk node) and one image
-pass from a model this test writes; `make -C oracle sanitize` builds it with gcc's
+pass from a model this test writes (one case with dust and collisional depolarisation); `make -C oracle sanitize` builds it with gcc's
 -fsanitize=address,undefined and -fno-sanitize-recover, so any out-of-bounds access,
 use after free, leak, signed overflow or misaligned load in the oracle ends the run
 with a non-zero status.  Its outputs must equal the regular build's (liboracle.so) on
@@ -44,6 +44,7 @@ def sanitized():
     return SAN
 
 
+# the second case also switches on dust and the depolarisation option
 @pytest.mark.parametrize("nlev,seed,dust", [(4, 5, False), (6, 8, True)])
 def test_oracle_clean_under_asan_ubsan(sanitized, oracle, tmp_path, nlev, seed, dust):
     sp = cloud.co_like_rotor(nlev)
@@ -57,7 +58,7 @@ def test_oracle_clean_under_asan_ubsan(sanitized, oracle, tmp_path, nlev, seed, 
     vel_out = driver.channel_velocities(img_nvel, 93.75)
     th_inc, ph_az, alpha = np.pi / 3, 0.4, 0.15
     dims = [g.node_num, g.bulk_node_num, sp.e_num, sp.r_num, sp.c_num, sp.c_temp, sp.c_par, sp.n_trans,
-            sp.kmax, sp.c_dim, xth.shape[1], nvel, 4096, len(lines), im_pix, img_nvel, 4096]
+            sp.kmax, sp.c_dim, xth.shape[1], nvel, 4096, len(lines), im_pix, img_nvel, 4096, int(dust)]
     src = tmp_path / "in.bin"
     with open(src, "wb") as f:
         _put(f, dims, 0)
@@ -86,7 +87,8 @@ def test_oracle_clean_under_asan_ubsan(sanitized, oracle, tmp_path, nlev, seed, 
     assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr[-4000:]
     pp_s, J0_s, J2_s, img_s = _read(dst)
 
-    pp, J0, J2 = oracle.se_pass(g, sp, xth, nvel, cloud.DX_INWARD, nthreads=1)
+    pp, J0, J2 = oracle.se_pass(g, sp, xth, nvel, cloud.DX_INWARD, nthreads=1,
+                                depol=oracle.depol_table(sp) if dust else None)
     outs = oracle.image_pass(g, sp, pp, lines, th_inc, ph_az, img_nvel, vel_out, im_pix, im_size, cloud.DX_IMAGE,
                              alpha=alpha, n_depth_rt=4096, nthreads=1)
     assert np.abs(J0).max() > 0 and np.abs(J2).max() > 0

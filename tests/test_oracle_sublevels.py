@@ -71,9 +71,13 @@ def make_species(J, E_cm, rad, kmax=6):
                          t_Cji=np.asfortranarray(np.full((len(ct), len(pairs), 1), 1e-11)), kmax=kmax)
 
 
-def sublevel_se(sp, AC, T):
+def sublevel_se(sp, AC, T, depol=None):
     """Steady state of the sublevel rate equations, normalised to total population 1,
-    returned as rho^K_0 in the SE vector's layout (count_array slots)."""
+    returned as rho^K_0 in the SE vector's layout (count_array slots).  depol: None, or
+    {level index: {K: D}}, an isotropic elastic process that relaxes rank K of that level
+    at rate D and leaves its other ranks alone: the operator -sum_K D v_K v_K^T on the
+    level's sublevels, v_K(M) = (-1)^(J-M) sqrt(2K+1) (J J K; M -M 0) (orthonormal, and
+    rho^K_0 = v_K . N)."""
     J = [int(j) for j in sp.j_lev]
     states = [(i, M) for i in range(sp.e_num) for M in range(-J[i], J[i] + 1)]
     at = {s: k for k, s in enumerate(states)}
@@ -95,6 +99,12 @@ def sublevel_se(sp, AC, T):
                 R[b, a] += B12 * gl * s * Jp + C12 / gu
                 R[a, b] += A * gu * s + B21 * gu * s * Jp + C21 / gl
     M = R - np.diag(R.sum(axis=0))
+    for i, ranks in (depol or {}).items():
+        idx = [at[(i, m)] for m in range(-J[i], J[i] + 1)]
+        for K, D in ranks.items():
+            v = np.array([(-1) ** (J[i] - m) * math.sqrt(2 * K + 1) * w3(J[i], J[i], K, m, -m, 0)
+                          for m in range(-J[i], J[i] + 1)])
+            M[np.ix_(idx, idx)] -= D * np.outer(v, v)
     M[0, :] = 1.0
     rhs = np.zeros(n)
     rhs[0] = 1.0
@@ -291,3 +301,131 @@ def test_emcoefs_kirchhoff_and_einstein_relations(oracle, Jl, Ju, T):
     assert eI / aI == pytest.approx(planck, rel=1e-6)
     assert eI == pytest.approx(H_SI / (4 * math.pi) * nu * A * Nu, rel=5e-5)
     assert aQ == 0.0 and eQ == 0.0
+
+
+# --------------------------------------------------------------------------------------
+# collisional depolarisation (depol.f90:242-359; the slot pop_eqs.f90:642-646)
+
+def _depol_arrays(sp, J, rng, scale):
+    """Random rank-dependent rates: the sublevel model's dict and the oracle's D0
+    (e_num x (kmax - 1), column K - 2)."""
+    D0 = np.zeros((sp.e_num, sp.kmax - 1), order="F")
+    ranks = {}
+    for i in range(sp.e_num):
+        for K in range(2, min(2 * J[i], sp.kmax) + 1, 2):
+            d = scale * 10 ** rng.uniform(-1, 1)
+            D0[i, K - 2] = d
+            ranks.setdefault(i, {})[K] = d
+    return D0, ranks
+
+
+@pytest.mark.parametrize("case", range(len(LADDERS)))
+def test_depolarisation_relaxes_each_rank(oracle, case):
+    """setup_stateq with D^(K) on the diagonal of the K >= 2 rows against the sublevel
+    equations with a process that relaxes exactly rank K at rate D^(K): every rank of
+    every level, with rates of each rank chosen independently (a D placed in the wrong
+    row, with the wrong sign or on the K = 0 row fails)."""
+    J, rad = LADDERS[case]
+    rng = np.random.default_rng(900 + case)
+    E = np.concatenate([[0.0], np.sort(rng.uniform(1.0, 60.0, len(J) - 1))])
+    sp = make_species(J, E, rad)
+    AC = random_AC(sp, rad, E, rng)
+    T = float(rng.uniform(10, 80))
+    base = sublevel_se(sp, AC, T)
+    A_scale = float(np.max(AC[0]))
+    D0, ranks = _depol_arrays(sp, J, rng, A_scale)
+    got = oracle.get_rhovec(sp, AC, T, D0=D0)
+    ref = sublevel_se(sp, AC, T, depol=ranks)
+    for i in range(sp.e_num):
+        c0 = sp.count_array[i]
+        n = len(range(0, min(2 * J[i], sp.kmax) + 1, 2))
+        np.testing.assert_allclose(got[c0:c0 + n], ref[c0:c0 + n], rtol=0, atol=1e-10 * abs(ref[c0]))
+    # the rates matter here: the alignment moved
+    moved = max(abs(ref[sp.count_array[i] + 1] - base[sp.count_array[i] + 1]) / ref[sp.count_array[i]]
+                for i in range(sp.e_num) if J[i] > 0)
+    assert moved > 1e-4
+    # D0 = 0 is the reference's own case (main_sub.f90:96): bit for bit the plain solve
+    np.testing.assert_array_equal(oracle.get_rhovec(sp, AC, T, D0=0 * D0), oracle.get_rhovec(sp, AC, T))
+
+
+def test_strong_depolarisation_removes_alignment(oracle):
+    """rho^2_0 / rho^0_0 falls as 1/D once D dominates the radiative rates of a level."""
+    J, rad = LADDERS[2]
+    rng = np.random.default_rng(5)
+    E = np.concatenate([[0.0], np.sort(rng.uniform(1.0, 60.0, len(J) - 1))])
+    sp = make_species(J, E, rad, kmax=2)
+    AC = random_AC(sp, rad, E, rng)
+    al = []
+    for D in (1e2, 1e3, 1e4):
+        D0 = np.zeros((sp.e_num, 1), order="F")
+        D0[[i for i in range(sp.e_num) if J[i] > 0], 0] = D
+        rho = oracle.get_rhovec(sp, AC, 30.0, D0=D0)
+        al.append(np.array([rho[sp.count_array[i] + 1] / rho[sp.count_array[i]] for i in range(sp.e_num) if J[i] > 0]))
+    np.testing.assert_allclose(al[1] * 1e3, al[0] * 1e2, rtol=2e-2)
+    np.testing.assert_allclose(al[2] * 1e4, al[1] * 1e3, rtol=2e-3)
+
+
+def _rotor_with_rates(nlev, rng, c_par=1):
+    sp = cloud.co_like_rotor(nlev)
+    assert sp.kmax == 6
+    ct = sp.col_temps
+    C = np.zeros((len(ct), sp.c_num, c_par), order="F")
+    for c in range(sp.c_num):
+        for p in range(c_par):
+            C[:, c, p] = 10 ** rng.uniform(-12, -10) * (ct / ct[0]) ** rng.uniform(-0.5, 0.5)
+    sp.t_Cji = C
+    return sp
+
+
+def test_depolarisation_table_closed_forms(oracle):
+    """de_polarization's Morris (1985) B11 sum: J = 1 has only m = 2, (1 1 2; 0 0 0)^2 =
+    2/15, {1 1 2; 1 1 0} = 1/3, {1 1 2; 1 1 2} = 1/30, so D^(2)(1) = 9 * 5 * (2/15) *
+    (1/3 - 1/30) k_2 = (9/5) k_2 with k_2 the J = 2 <-> 0 rate; J = 2, 3 against the sum
+    with sympy's exact 3j/6j; zero for J = 0, odd K and K > 2J; the two-partner mix."""
+    from sympy.physics.wigner import wigner_6j
+    rng = np.random.default_rng(11)
+    for c_par in (1, 2):
+        sp = _rotor_with_rates(5, rng, c_par)
+        D = oracle.depol_table(sp)
+        assert D.shape == (sp.c_temp, sp.e_num, sp.kmax - 1)
+
+        def k(Jm):
+            col = [c for c in range(sp.c_num) if set(sp.c_trans[:, c]) == {1, Jm + 1}][0]
+            t = sp.t_Cji[:, col, :]
+            return t[:, 0] if c_par == 1 else 0.25 * t[:, 0] + 0.75 * t[:, 1]
+
+        np.testing.assert_allclose(D[:, 1, 0], 1.8 * k(2), rtol=1e-13)
+        for Jl in (2, 3):
+            for K in range(2, 2 * Jl + 1, 2):
+                exp = np.zeros(sp.c_temp)
+                for m in range(2, 2 * Jl + 1, 2):
+                    if m > 4:
+                        continue  # the 5-level rotor has no J = 6 level: no such transition
+                    w = float(wigner_3j(Jl, Jl, m, 0, 0, 0)) ** 2
+                    f = float(wigner_6j(Jl, Jl, m, Jl, Jl, 0) - wigner_6j(Jl, Jl, m, Jl, Jl, K))
+                    exp += (2 * Jl + 1) ** 2 * (2 * m + 1) * w * f * k(m)
+                np.testing.assert_allclose(D[:, Jl, K - 2], exp, rtol=1e-12, atol=1e-30)
+        assert not D[:, 0, :].any()           # J = 0
+        assert not D[:, :, 1::2].any()         # odd K
+        assert not D[:, 1, 2:].any()           # J = 1: K = 4, 6 > 2J
+        assert np.all(D[:, 1:, 0] > 0)         # rank 2 of every J > 0 level relaxes
+
+
+def test_depolarisation_temperature_interpolation(oracle):
+    """make_depolarization_temp's linear interpolation in T, bracketed as make_AC brackets
+    the collision rates (so the first interval's line is extrapolated below the table),
+    times n_H2 [m^-3] 1e-6; NaN above the table."""
+    rng = np.random.default_rng(12)
+    sp = _rotor_with_rates(4, rng)
+    D = oracle.depol_table(sp)
+    ct = sp.col_temps
+    nH2 = 3e11
+    for T in (0.5 * (ct[1] + ct[2]), ct[3], 0.3 * ct[0], ct[0]):
+        got = oracle.depol_at(sp, D, T, nH2)
+        for i in range(sp.e_num):
+            for K in range(sp.kmax - 1):
+                f = D[:, i, K]
+                v = np.interp(T, ct, f) if T >= ct[0] else f[0] + (T - ct[0]) / (ct[1] - ct[0]) * (f[1] - f[0])
+                ref = v * nH2 * 1e-6
+                assert got[i, K] == pytest.approx(ref, rel=1e-14, abs=1e-300)
+    assert np.isnan(oracle.depol_at(sp, D, ct[-1] * 2, nH2)).all()
