@@ -337,6 +337,7 @@ hipError_t launch_se_slab(const SeArgs *models, const int2 *items, int n_items, 
                           int waves, int n_blocks, double *slabs, hipStream_t s);
 int se_slab_blocks_per_cu(int waves, int cd, int nt);
 size_t se_lds_bytes(int c_dim, int n_trans, bool m0_in_lds = true);
+size_t se_slab_doubles(int c_dim, int n_trans);
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves);
 // the cooperative form of the persistent integration, one launch per (slot count, dust)
 hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,

@@ -2141,7 +2141,8 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
             HIP_TRY(&c0, upload(k.d_items, k.items));
             HIP_TRY(&c0, upload(k.d_sample, k.sample));
             for (int w : {1, 2, 4})
-                slab_doubles = std::max(slab_doubles, (size_t)n_cu * se_slab_blocks_per_cu(w, k.cd, k.nt) * k.cd * k.cd);
+                slab_doubles = std::max(slab_doubles, (size_t)n_cu * se_slab_blocks_per_cu(w, k.cd, k.nt) *
+                                                          se_slab_doubles(k.cd, k.nt));
         }
         HIP_TRY(&c0, hipMalloc((void **)&d_se_cnt, sizeof(int) * std::max<size_t>(1, se_cls.size())));
         HIP_TRY(&c0, hipMalloc((void **)&d_slabs, sizeof(double) * slab_doubles));

@@ -22,10 +22,10 @@
 // record of a (cell, line) carries the source function folded into the terms of
 // 1 - e^-tau (SegF), which leaves nine VALU instructions and two v_exp_f32 per
 // (cell, line, channel).
-// The check for exhausted transmission (every lane's Tr exactly zero, or below the
-// mixed floor) runs every fourth cell; cells visited after a lane's Tr reached zero
-// add exactly zero.  Per pair the sum over rays is in ray order; the channel sum
-// with phi0 d_v is done per line in channel order.
+// The check for exhausted transmission (every lane's Tr exactly zero in FP64, at or
+// below the floor in mixed precision) runs every fourth cell; in FP64, cells visited
+// after a lane's Tr reached zero add exactly zero.  Per pair the sum over rays is in
+// ray order; the channel sum with phi0 d_v is done per line in channel order.
 #pragma once
 #include <type_traits>
 
@@ -278,16 +278,24 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             if (DUST) dn = kds[(size_t)(jb < cnt ? jb : 0) * R + lo];
             for (int j = jb; j < cnt; j++) {
                 if (((c0 + j) & 3) == 0) {
-                    // mixed: a channel whose transmission fell to the floor stops (zeroed
-                    // here, so that it adds exactly zero whatever the rest of its wave does);
-                    // FP64: transmission is never negative, so only exact zeros stop
-                    unsigned bits = 0;
+                    // mixed: the wave leaves the ray once every channel's transmission is at
+                    // or below the floor (the dropped tail is below the floor times the
+                    // remaining source function; a channel under the floor still adds its
+                    // share until then); FP64: transmission is never negative, so only
+                    // exact zeros stop
+                    bool live;
+                    if constexpr (A::tr_floor > 0) {
+                        Real m = Tr[0];
 #pragma unroll
-                    for (int q = 0; q < SLOTS; q++) {
-                        if constexpr (A::tr_floor > 0) Tr[q] = Tr[q] > (Real)A::tr_floor ? Tr[q] : (Real)0;
-                        bits |= nonzero_bits(Tr[q]);
+                        for (int q = 1; q < SLOTS; q++) m = fmaxf(m, Tr[q]);
+                        live = m > (Real)A::tr_floor;
+                    } else {
+                        unsigned bits = 0;
+#pragma unroll
+                        for (int q = 0; q < SLOTS; q++) bits |= nonzero_bits(Tr[q]);
+                        live = bits != 0u;
                     }
-                    if (!__any(bits != 0u)) {
+                    if (!__any(live)) {
                         ray_done = true;
                         break;
                     }

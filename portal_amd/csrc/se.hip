@@ -63,16 +63,19 @@ __device__ __forceinline__ void add_depol(const SeArgs &a, double *M, int cd, in
 // elementwise loops stride over all of them, the trailing update of each LU step splits
 // its columns between the waves (column j to wave j mod W), and every wave runs the
 // pivot search on its own, so the factors and the solution do not depend on W.
-// G: the copy of the assembled matrix that the refinement residual reads lives in
-// global memory at M0g (a slab of the workgroup's own) rather than in LDS.
+// G: the copy of the assembled matrix that the refinement residual reads, and the
+// transition scalars, live in global memory at M0g (a slab of the workgroup's own,
+// cd^2 + 10 nt doubles) rather than in LDS.
 template <int W, bool G>
 __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *sm, double *M0g) {
     constexpr int NT = 64 * W;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int node = a.work[wi];
     const int cd = a.c_dim, nt = a.n_trans, L = a.L, R = a.R;
-    double *x = sm;                          // 10*nt scalars
-    double *M = x + 10 * nt;                 // LU in place, column-major cd x cd
+    // the slab form (G) keeps the scalars next to the matrix copy in global memory, so
+    // that LDS holds only the factors and two vectors
+    double *x = G ? M0g + (size_t)cd * cd : sm;  // 10*nt scalars
+    double *M = G ? sm : x + 10 * nt;        // LU in place, column-major cd x cd
     double *M0 = G ? M0g : M + cd * cd;     // assembled matrix (for the residual)
     double *v = G ? M + cd * cd : M0 + cd * cd;  // solution
     double *y = v + cd;                      // rhs / work
@@ -400,8 +403,8 @@ __global__ void __launch_bounds__(64) se_persistent_kernel(const SeArgs *models,
 }
 
 // Persistent form for a class of models with one system size: W waves per system, the
-// residual's matrix copy in the workgroup's slab of `slab` doubles, so that LDS holds
-// only the transition scalars, the factors and two vectors.
+// residual's matrix copy and the transition scalars in the workgroup's slab of `slab`
+// doubles, so that LDS holds only the factors and two vectors.
 template <int W>
 __global__ void __launch_bounds__(64 * W) se_slab_kernel(const SeArgs *models, const int2 *items, int n_items,
                                                         int *counter, double *slabs, int slab) {
@@ -420,10 +423,16 @@ __global__ void __launch_bounds__(64 * W) se_slab_kernel(const SeArgs *models, c
     }
 }
 
+// LDS of one system: with m0_in_lds the scalars, the factors, the matrix copy and two
+// vectors; without (the slab form) the factors and two vectors
 size_t se_lds_bytes(int c_dim, int n_trans, bool m0_in_lds) {
-    return sizeof(double) * (10 * (size_t)n_trans + (m0_in_lds ? 2 : 1) * (size_t)c_dim * c_dim + 2 * c_dim) +
+    return sizeof(double) * ((m0_in_lds ? 10 * (size_t)n_trans + 2 * (size_t)c_dim * c_dim : (size_t)c_dim * c_dim) +
+                             2 * c_dim) +
            sizeof(int) * c_dim + 16;
 }
+
+// doubles of one workgroup's slab: the matrix copy and the transition scalars
+size_t se_slab_doubles(int c_dim, int n_trans) { return (size_t)c_dim * c_dim + 10 * (size_t)n_trans; }
 
 hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd_max,
                                 int nt_max, int n_blocks, hipStream_t s) {
@@ -451,7 +460,7 @@ hipError_t launch_se_slab(const SeArgs *models, const int2 *items, int n_items, 
                           int waves, int n_blocks, double *slabs, hipStream_t s) {
     const size_t lds = se_lds_bytes(cd, nt, false);
     if (lds > 160 * 1024 || cd > 128 || n_items < 0 || n_blocks < 1) return hipErrorInvalidValue;
-    const int slab = cd * cd;
+    const int slab = (int)se_slab_doubles(cd, nt);
     switch (waves) {
         case 1:
             hipLaunchKernelGGL(se_slab_kernel<1>, dim3(n_blocks), dim3(64), lds, s, models, items, n_items, counter,
