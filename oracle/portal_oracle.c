@@ -30,6 +30,15 @@
  *                              sublevel model to 1e-12; w^(2) against the closed forms of
  *                              Landi Degl'Innocenti & Landolfi (2004) Table 10.1; pure pi /
  *                              sigma light gives Q/I = +1 / -1; Kirchhoff's law (CODATA)
+ *   depol_table / depol_at / setup_stateq_d (collisional depolarisation, an option the
+ *                              reference never runs: D0 = 0 at main_sub.f90:96)
+ *                              the sublevel model with a process relaxing exactly rank K
+ *                              at rate D^(K), every rank of every level, to 1e-10;
+ *                              D^(2)(J=1) = (9/5) k_2 by hand; J = 2, 3 with exact 3j/6j;
+ *                              zeros for J = 0, odd K, K > 2J; make_AC's bracket in T
+ *
+ * The same file also runs under AddressSanitizer + UBSan (oracle/sanitize_main.c,
+ * make -C oracle sanitize), which the CPU tests require to be clean.
  */
 #include "portal_oracle.h"
 
