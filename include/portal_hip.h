@@ -225,12 +225,16 @@ int portal_lambda_iterate(portal_ctx *ctx, const portal_se_params *p, const port
  * faster; the same bits either way) and each model's closure.  Models with dust
  * (dust classes of the cooperative kernel) or in star mode (their per-node direction
  * sets) are integrated as in the single-model pass.  it->ng_accel = 1 applies Ng
- * acceleration per model between replays; Anderson acceleration, the own-cell
- * operator and the surrogate are for single models (PORTAL_ERR_UNSUPPORTED).  Iterates until every model meets the
- * tolerances (or max_iter); per_model (n, may be NULL) gets each model's iteration
- * count at convergence and whether it converged, total the device time of the whole
- * batch.  A further call continues each model's iteration.  Results stay in each
- * context (portal_get_pol_pops / portal_get_levpop). */
+ * acceleration per model between replays; Anderson acceleration and the own-cell
+ * operator are for single models (PORTAL_ERR_UNSUPPORTED).  it->surrogate = 1 gives
+ * every model whose context holds a network (portal_mlp_create + weights) the
+ * network's prediction after the first replay and the residual test (fallback_tol)
+ * after the second, as portal_lambda_iterate does; models without one iterate
+ * plainly.  Iterates until every model meets the tolerances (or max_iter); per_model
+ * (n, may be NULL) gets each model's iteration count at convergence, whether it
+ * converged and its surrogate fallbacks, total the device time of the whole batch.  A
+ * further call continues each model's iteration (not with the surrogate, which starts
+ * afresh).  Results stay in each context (portal_get_pol_pops / portal_get_levpop). */
 int portal_batch_lambda_iterate(portal_ctx **ctxs, int n_models, const portal_se_params *params,
                                 const portal_iter_params *it, portal_stats *per_model, portal_stats *total);
 

@@ -338,13 +338,15 @@ class Portal(SurrogateMixin, _Core):
 
 
 def batch_lambda_iterate(portals, xths, nvels, dx, max_iter, tol_pop=1e-6, tol_align=1e-8, n_depth=4096,
-                         precision=L.PORTAL_MIXED, rebuild_paths=0, ng_accel=0):
-    """portal_batch_lambda_iterate over several Portal contexts (one model each)."""
+                         precision=L.PORTAL_MIXED, rebuild_paths=0, ng_accel=0, surrogate=0, fallback_tol=1e-8):
+    """portal_batch_lambda_iterate over several Portal contexts (one model each).  With
+    surrogate = 1, every model whose context holds a network gets its prediction after the
+    first pass (the others iterate plainly)."""
     n = len(portals)
     lib = portals[0].lib
     arr = (C.c_void_p * n)(*[p.ctx.value for p in portals])
     ps = (L.SeParams * n)(*[p._se_params(x, nv, dx, n_depth, 0, precision) for p, x, nv in zip(portals, xths, nvels)])
-    it = L.IterParams(max_iter, tol_pop, tol_align, 0, 0.0, rebuild_paths, ng_accel)
+    it = L.IterParams(max_iter, tol_pop, tol_align, surrogate, fallback_tol, rebuild_paths, ng_accel)
     per = (L.Stats * n)()
     tot = L.Stats()
     rc = lib.portal_batch_lambda_iterate(arr, n, ps, C.byref(it), per, C.byref(tot))

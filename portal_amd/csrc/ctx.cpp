@@ -1772,7 +1772,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     if (!xs || n <= 0 || !params || !it || it->max_iter <= 0) return PORTAL_ERR_ARG;
     Ctx &c0 = xs[0]->c;
     hipSetDevice(c0.device);
-    if (it->surrogate) return fail(c0, PORTAL_ERR_UNSUPPORTED, "the surrogate iterate is not available for batches");
+    if (it->surrogate && !(it->fallback_tol >= 0.0)) return fail(c0, PORTAL_ERR_ARG, "fallback_tol must be >= 0");
     if (it->ng_accel < 0 || it->ng_accel > 1)
         return fail(c0, it->ng_accel == 2 ? PORTAL_ERR_UNSUPPORTED : PORTAL_ERR_ARG,
                     "a batch takes ng_accel 0 or 1 (Anderson acceleration is for single models)");
@@ -1787,8 +1787,18 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         Ctx &c = xs[m]->c;
         int rc = check_se_params(c, &params[m]);
         if (rc) return fail(c0, rc, "model " + std::to_string(m) + ": " + c.err);
-        chained[m] = c.pol_chain ? 1 : 0;
+        chained[m] = c.pol_chain && !it->surrogate ? 1 : 0;
         c.pol_chain = false;
+    }
+    // with the surrogate, every model that holds a network gets its prediction after the
+    // first pass and the residual test after the second, as a single model does; a model
+    // without a network iterates plainly
+    std::vector<char> use_surr(n, 0);
+    for (int m = 0; m < n; m++) {
+        Ctx &c = xs[m]->c;
+        int ni = 0, no = 0;
+        use_surr[m] = it->surrogate && mlp_dims(c, &ni, &no) ? 1 : 0;
+        c.fallback_map.clear();
     }
     if (total) std::memset(total, 0, sizeof(*total));
     if (per_model) std::memset(per_model, 0, sizeof(portal_stats) * n);
@@ -2025,6 +2035,38 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     const int n_cnt = std::max(4, 2 + (int)classes.size());
     HIP_TRY(&c0, hipMalloc((void **)&d_cnt, sizeof(int) * n_cnt));
     HIP_TRY(&c0, hipMalloc((void **)&d_max, sizeof(unsigned long long) * 2 * n));
+    // per model with the surrogate: the per-item change of the current iteration (res_cur,
+    // written by the captured population update), that of the plain first iterate (res_p),
+    // the plain first iterate itself, per-item outcomes and the revert count
+    struct SurrBufs {
+        unsigned long long *res_cur = nullptr, *res_p = nullptr;
+        double *plain_lev = nullptr, *plain_pol = nullptr;
+        int *iflags = nullptr, *nrev = nullptr;
+    };
+    std::vector<SurrBufs> sb(n);
+    Defer free_sb{[&] {
+        for (SurrBufs &b : sb) {
+            dfree(b.res_cur);
+            dfree(b.res_p);
+            dfree(b.plain_lev);
+            dfree(b.plain_pol);
+            dfree(b.iflags);
+            dfree(b.nrev);
+        }
+    }};
+    for (int m = 0; m < n; m++) {
+        if (!use_surr[m]) continue;
+        Ctx &c = xs[m]->c;
+        const size_t nw = std::max<size_t>(c.g.work.size(), 1), NL = (size_t)c.g.N * c.sp.e_num;
+        HIP_TRY(&c0, hipMalloc((void **)&sb[m].res_cur, sizeof(unsigned long long) * nw));
+        HIP_TRY(&c0, hipMalloc((void **)&sb[m].res_p, sizeof(unsigned long long) * nw));
+        HIP_TRY(&c0, hipMalloc((void **)&sb[m].plain_lev, sizeof(double) * NL));
+        HIP_TRY(&c0, hipMalloc((void **)&sb[m].plain_pol, sizeof(double) * 2 * NL));
+        HIP_TRY(&c0, hipMalloc((void **)&sb[m].iflags, sizeof(int) * nw));
+        HIP_TRY(&c0, hipMalloc((void **)&sb[m].nrev, sizeof(int)));
+        HIP_TRY(&c0, hipMemset(sb[m].iflags, 0, sizeof(int) * nw));
+        HIP_TRY(&c0, hipMemset(sb[m].nrev, 0, sizeof(int)));
+    }
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
         HIP_TRY(&c0, hipMalloc((void **)&prev[m], sizeof(double) * c.g.N * 2 * c.sp.e_num));
@@ -2238,8 +2280,10 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         }
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
+        if (use_surr[m])
+            launch_fill_zero(sb[m].res_cur, sizeof(unsigned long long) * std::max<size_t>(c.g.work.size(), 1), s);
         launch_lambda_update((int)c.g.work.size(), c.g.N, c.sp.e_num, c.g.d_work, c.g.d_ab, c.g.d_h2, c.sp.d_jlev,
-                             c.d_pol, prev[m], c.d_levp, d_max + 2 * m, s, prev[m]);
+                             c.d_pol, prev[m], c.d_levp, d_max + 2 * m, s, prev[m], sb[m].res_cur);
     }
     hipError_t ce = hipStreamEndCapture(s, &graph);
     if (ce != hipSuccess) {
@@ -2273,12 +2317,59 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     }
     int nhist = 0, n_ng = 0, last_ng = -1;
     std::vector<double> sums(8 * (size_t)n);
+    std::vector<int> nfb(n, 0);
+    double t_mlp = 0.0;
     for (int k = 0; k < it->max_iter; k++) {
         Timer ti(s);
         HIP_TRY(&c0, hipGraphLaunch(exec, s));
         HIP_TRY(&c0, hipMemcpyAsync(mx.data(), d_max, sizeof(unsigned long long) * 2 * n, hipMemcpyDeviceToHost, s));
         t_iter += ti.stop();
         iters = k + 1;
+        if (it->surrogate && k <= 1) {
+            HIP_TRY(&c0, hipStreamSynchronize(s));
+            for (int m = 0; m < n; m++) {
+                if (!use_surr[m]) continue;
+                Ctx &c = xs[m]->c;
+                const int nw = (int)c.g.work.size();
+                const size_t NL = (size_t)c.g.N * c.sp.e_num;
+                if (k == 0) {
+                    // keep the plain first iterate and its per-item change, then the
+                    // network's prediction replaces it wherever it passes the validity tests
+                    HIP_TRY(&c0, hipMemcpyAsync(sb[m].res_p, sb[m].res_cur, sizeof(unsigned long long) * std::max(nw, 1),
+                                                hipMemcpyDeviceToDevice, s));
+                    HIP_TRY(&c0, hipMemcpyAsync(sb[m].plain_lev, c.d_levp, sizeof(double) * NL, hipMemcpyDeviceToDevice, s));
+                    HIP_TRY(&c0, hipMemcpyAsync(sb[m].plain_pol, c.d_pol, sizeof(double) * 2 * NL, hipMemcpyDeviceToDevice, s));
+                    HIP_TRY(&c0, hipStreamSynchronize(s));
+                    int f = 0;
+                    int rc = surrogate_step(c, &f, sb[m].iflags, &t_mlp);
+                    if (rc) return fail(c0, rc, "model " + std::to_string(m) + ": " + c.err);
+                    nfb[m] += f;
+                } else {
+                    // the residual test of the single-model iteration
+                    SurrRevertArgs ra{};
+                    ra.n_work = nw;
+                    ra.N = c.g.N;
+                    ra.L = c.sp.e_num;
+                    ra.work = c.g.d_work;
+                    ra.res_s = sb[m].res_cur;
+                    ra.res_p = sb[m].res_p;
+                    ra.tol = it->fallback_tol;
+                    ra.plain_lev = sb[m].plain_lev;
+                    ra.plain_pol = sb[m].plain_pol;
+                    ra.levp = c.d_levp;
+                    ra.pol = c.d_pol;
+                    ra.flags = sb[m].iflags;
+                    ra.n_fallback = sb[m].nrev;
+                    HIP_TRY(&c0, launch_surr_revert(ra, s));
+                    int nrev = 0;
+                    HIP_TRY(&c0, hipMemcpyAsync(&nrev, sb[m].nrev, sizeof(int), hipMemcpyDeviceToHost, s));
+                    HIP_TRY(&c0, hipStreamSynchronize(s));
+                    nfb[m] += nrev;
+                }
+                // the next pass is compared with the state it starts from
+                HIP_TRY(&c0, hipMemcpyAsync(prev[m], c.d_pol, sizeof(double) * 2 * NL, hipMemcpyDeviceToDevice, s));
+            }
+        }
         bool all = true;
         for (int m = 0; m < n; m++) {
             double dp, da;
@@ -2358,7 +2449,15 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         // d_pol formed d_levp in the last pass, unless an extrapolation followed it
         c.pol_chain = last_ng < iters - 1;
         all_conv = all_conv && conv_at[m];
+        if (use_surr[m]) {
+            const int nw = (int)c.g.work.size();
+            c.fallback_map.assign(std::max(nw, 1), 0);
+            HIP_TRY(&c0, hipMemcpy(c.fallback_map.data(), sb[m].iflags, sizeof(int) * std::max(nw, 1),
+                                   hipMemcpyDeviceToHost));
+            c.fallback_map.resize(nw);
+        }
         if (per_model) {
+            per_model[m].fallbacks = nfb[m];
             per_model[m].converged = conv_at[m] ? 1 : 0;
             per_model[m].iterations = conv_at[m] ? conv_at[m] : iters;
             per_model[m].max_rel_change = last[m];
@@ -2379,6 +2478,8 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         total->se_waves = se_waves_or;
         total->max_path = cap;
         total->converged = all_conv ? 1 : 0;
+        for (int m = 0; m < n; m++) total->fallbacks += nfb[m];
+        total->t_mlp_ms = t_mlp;
     }
     return PORTAL_OK;
 }

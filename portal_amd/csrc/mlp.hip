@@ -29,7 +29,9 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-enum { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_SILU = 2, EPI_DSILU = 3 };
+// epilogues: split-K partial store; + bias; + bias, keeping Z and H = SiLU(Z) (training);
+// dZ = dH SiLU'(Z); + bias, H = SiLU(Z) only (inference, Z is not kept)
+enum { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_SILU = 2, EPI_DSILU = 3, EPI_BIAS_SILU_H = 4 };
 
 struct GemmArgs {
     int M, N, K, ksplit, epi;
@@ -182,6 +184,9 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g, int amode, i
                         g.H[(long long)m * g.ldh + n] = (bf16)silu(z);
                         break;
                     }
+                    case EPI_BIAS_SILU_H:
+                        g.H[(long long)m * g.ldh + n] = (bf16)silu(v + g.bias[n]);
+                        break;
                     default: {  // EPI_DSILU: dZ = dH * silu'(Z)
                         const float z = g.Z[(long long)m * g.ldz + n];
                         const float s = 1.0f / (1.0f + __expf(-z));
@@ -396,8 +401,9 @@ static hipError_t mlp_reserve(Mlp &m, int B) {
 
 static unsigned blocks(long long n) { return (unsigned)((n + 255) / 256); }
 
-// forward on m.Xd (fp32 features, B x n_in): normalised output in m.Z[nl-1]
-static hipError_t mlp_forward_dev(Mlp &m, int B, hipStream_t s) {
+// forward on m.Xd (fp32 features, B x n_in): normalised output in m.Z[nl-1]; keep_z
+// keeps every layer's pre-activation for back-propagation (training)
+static hipError_t mlp_forward_dev(Mlp &m, int B, hipStream_t s, bool keep_z = true) {
     const float *mu = m.norm, *sd = m.norm + m.n_in;
     hipLaunchKernelGGL(normalise_kernel, dim3(blocks((long long)B * m.n_in)), dim3(256), 0, s, m.Xd, mu, sd, m.X16, B,
                        m.n_in);
@@ -418,7 +424,7 @@ static hipError_t mlp_forward_dev(Mlp &m, int B, hipStream_t s) {
         g.ldc = m.dout[l];
         g.H = m.H[l];
         g.ldh = m.dout[l];
-        g.epi = (l < m.nl - 1) ? EPI_BIAS_SILU : EPI_BIAS;
+        g.epi = (l < m.nl - 1) ? (keep_z ? EPI_BIAS_SILU : EPI_BIAS_SILU_H) : EPI_BIAS;
         hipError_t e = gemm(g, s);
         if (e) return e;
     }
@@ -521,7 +527,7 @@ hipError_t mlp_infer_device(Ctx &c, int B, const float *feats, float *out) {
     if (e) return e;
     if ((e = hipMemcpyAsync(m.Xd, feats, sizeof(float) * (size_t)B * m.n_in, hipMemcpyDeviceToDevice, c.stream)))
         return e;
-    if ((e = mlp_forward_dev(m, B, c.stream))) return e;
+    if ((e = mlp_forward_dev(m, B, c.stream, false))) return e;
     hipLaunchKernelGGL(denorm_kernel, dim3(blocks((long long)B * m.n_out)), dim3(256), 0, c.stream,
                        m.Z[m.nl - 1], m.norm + 2 * m.n_in, m.norm + 2 * m.n_in + m.n_out, out, B, m.n_out);
     return hipGetLastError();
@@ -653,7 +659,7 @@ int portal_mlp_forward(portal_ctx *x, int batch, const float *feats, float *out)
     Mlp &m = *(Mlp *)c.mlp;
     HIP_TRY(&c, mlp_reserve(m, batch));
     HIP_TRY(&c, hipMemcpyAsync(m.Xd, feats, sizeof(float) * (size_t)batch * m.n_in, hipMemcpyHostToDevice, c.stream));
-    HIP_TRY(&c, mlp_forward_dev(m, batch, c.stream));
+    HIP_TRY(&c, mlp_forward_dev(m, batch, c.stream, false));
     hipLaunchKernelGGL(denorm_kernel, dim3(blocks((long long)batch * m.n_out)), dim3(256), 0, c.stream,
                        m.Z[m.nl - 1], m.norm + 2 * m.n_in, m.norm + 2 * m.n_in + m.n_out, m.Yd, batch, m.n_out);
     HIP_TRY(&c, hipMemcpyAsync(out, m.Yd, sizeof(float) * (size_t)batch * m.n_out, hipMemcpyDeviceToHost, c.stream));

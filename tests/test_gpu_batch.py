@@ -335,3 +335,50 @@ def test_batch_staged_form_when_a_model_does_not_fit_the_cooperative_kernel():
         pb = pt.get_pol_pops()
         np.testing.assert_allclose(pb[idx][:, 0::2], pp[idx][:, 0::2], rtol=1e-10)
         pt.close()
+
+
+def test_batch_surrogate_equals_single_models():
+    """The surrogate iterate in a batch: models whose context holds a network (the 4- and
+    10-level rotors) get the prediction after the first replay and the residual test
+    after the second, as they do alone; a model without a network iterates plainly.
+    FP64, four iterations, against single-model runs to 1e-10, with the same fallbacks."""
+    from portal_amd import surrogate
+    specs = [(3000, 4, 41), (2000, 10, 42), (1500, 6, 43)]
+    models = []
+    for n, L, seed in specs:
+        sp = cloud.co_like_rotor(L)
+        models.append((sp, cloud.make_cloud(sp, n, seed=seed)))
+    xth = cloud.legendre_all(4, 8)
+
+    def ctx(sp, g):
+        pt = Portal(0)
+        pt.upload_grid(g)
+        pt.upload_species(sp)
+        pt.set_levpop(g.lev_p)
+        return pt, surrogate.load_into(pt, sp)
+
+    singles, has = [], []
+    for sp, g in models:
+        pt, h = ctx(sp, g)
+        has.append(h)
+        pp, lev, st = pt.lambda_iterate(xth, 16, cloud.DX_INWARD, 4, tol_pop=0.0, tol_align=0.0,
+                                        precision=PORTAL_FP64, surrogate=1 if h else 0)
+        singles.append((pp, lev, st["fallbacks"], pt.fallback_map() if h else None))
+        pt.close()
+    if not (has[0] and has[1]):
+        pytest.skip("no trained weights for the 4- and 10-level rotors")
+    assert not has[2]
+    pts = [ctx(sp, g)[0] for sp, g in models]
+    per, tot = batch_lambda_iterate(pts, [xth] * 3, [16] * 3, cloud.DX_INWARD, 4, tol_pop=0.0, tol_align=0.0,
+                                    precision=PORTAL_FP64, surrogate=1)
+    print("fallbacks per model:", [p["fallbacks"] for p in per], "single:", [s[2] for s in singles])
+    for m, ((sp, g), pt, (pp, lev, nfb, fmap)) in enumerate(zip(models, pts, singles)):
+        idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+        np.testing.assert_allclose(pt.get_levpop()[:, idx], lev[:, idx], rtol=1e-10)
+        pb = pt.get_pol_pops()
+        np.testing.assert_allclose(pb[idx][:, 0::2], pp[idx][:, 0::2], rtol=1e-10)
+        assert per[m]["fallbacks"] == nfb
+        if fmap is not None:
+            np.testing.assert_array_equal(pt.fallback_map(), fmap)
+        pt.close()
+    assert tot["fallbacks"] == sum(s[2] for s in singles)
