@@ -99,10 +99,13 @@ def small(seed=1):
     return out
 
 
-def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True):
+def build_batch(n_models=64, seed=4000):
+    """The batch configuration's models: n_models seeded clouds with node counts
+    log-uniform in [5e3, 1e5], 4..20-level rotors, density slopes uniform in [-2, -1] and
+    random fields, each uploaded to its own context at LTE.  Returns [(species, grid,
+    Portal)]."""
     rng = np.random.default_rng(seed)
-    xth = cloud.legendre_all(nth, nph)
-    pts, sizes = [], []
+    out = []
     t0 = time.perf_counter()
     for m in range(n_models):
         n = int(round(np.exp(rng.uniform(np.log(5e3), np.log(1e5)))))
@@ -114,11 +117,19 @@ def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True):
         pt.upload_grid(g)
         pt.upload_species(sp)
         pt.set_levpop(g.lev_p)
-        pts.append(pt)
-        sizes.append((n, L, int((g.ab_all[: g.bulk_node_num] > 0).sum())))
+        out.append((sp, g, pt))
         if (m + 1) % 8 == 0:
             _log(f"batch: {m + 1} clouds built, {time.perf_counter() - t0:.1f}s")
-    _log(f"batch: {n_models} clouds, {sum(s[0] for s in sizes)} nodes, built in {time.perf_counter() - t0:.1f}s")
+    _log(f"batch: {n_models} clouds, {sum(g.node_num for _, g, _ in out)} nodes, built in "
+         f"{time.perf_counter() - t0:.1f}s")
+    return out
+
+
+def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True):
+    xth = cloud.legendre_all(nth, nph)
+    models = build_batch(n_models, seed)
+    pts = [pt for _, _, pt in models]
+    sizes = [(g.node_num, sp.e_num, int((g.ab_all[: g.bulk_node_num] > 0).sum())) for sp, g, _ in models]
     t = time.perf_counter()
     # calls of at most 10 iterations, each continuing the last (the same iteration as one
     # call), with a line per call

@@ -12,7 +12,9 @@ on seeded node subsets (needs an MI355X):
   Stokes image of each;
 * the 200,000-node, 10-level cloud's 513 x 513 x 129 image: FP64 against the oracle on
   seeded pixel bands, mixed against FP64 on every pixel;
-* a model read from LIME/LAMDA/dust-table files through the FP64 SE pass.
+* a model read from LIME/LAMDA/dust-table files through the FP64 SE pass;
+* the 64-cloud batch at its full size, converged, with property checks and an oracle
+  fixed-point check on two of its models.
 """
 import numpy as np
 import pytest
@@ -268,3 +270,56 @@ def test_lime_model_files_through_the_c_abi(oracle):
     idx = solved(g)
     assert (g.dust_p[idx] > 0).sum() > 50 and sp.c_par == 2
     assert_pol_close(pp, ref, idx, sp.e_num, 1e-10)
+
+
+def test_the_64_cloud_batch_at_full_size(oracle):
+    """The batch configuration at its full size: 64 seeded clouds (about 2e6 nodes,
+    log-uniform in [5e3, 1e5], 4..20-level rotors), mixed precision with Ng, to 1e-6 /
+    1e-8.  Every model converges; every solved node is normalised with finite, positive
+    populations; and on 8 nodes of the smallest and of a 20-level model the converged
+    state is a fixed point of the oracle's FP64 SE pass to the mixed path's accuracy."""
+    from portal_amd import workloads
+    models = workloads.build_batch()
+    xth = cloud.legendre_all(6, 11)
+    pts = [pt for _, _, pt in models]
+    n = len(pts)
+    done, conv_at = 0, [0] * n
+    while done < 400:
+        per, tot = batch_lambda_iterate(pts, [xth] * n, [32] * n, cloud.DX_INWARD, 10, tol_pop=1e-6, tol_align=1e-8,
+                                        precision=PORTAL_MIXED, rebuild_paths=1 if done == 0 else 0, ng_accel=1)
+        for m, p in enumerate(per):
+            if not conv_at[m] and p["converged"]:
+                conv_at[m] = done + p["iterations"]
+        done += tot["iterations"]
+        print(f"64-cloud batch: {done} iterations, {sum(1 for v in conv_at if v)} of {n} converged", flush=True)
+        if tot["converged"]:
+            break
+    print(f"iterations per model: min {min(conv_at)}, max {max(conv_at)}")
+    assert all(conv_at)
+    for sp, g, pt in models:
+        idx = solved(g)
+        pp, lev = pt.get_pol_pops(), pt.get_levpop()
+        w = np.sqrt(2 * sp.j_lev + 1.0)
+        np.testing.assert_allclose(pp[idx][:, 0::2] @ w, 1.0, atol=1e-12)
+        assert np.all(np.isfinite(lev[:, idx])) and np.all(lev[:, idx] > 0)
+        # alignment within what any sublevel populations allow (|rho20/rho00| <= sqrt 2 for
+        # J = 1), on levels holding more than 1e-10 of their node
+        assert np.all(np.isfinite(pp[idx]))
+        r0, r2 = pp[idx][:, 2::2], pp[idx][:, 3::2]
+        held = r0 * w[None, 1:] > 1e-10
+        assert np.all(np.abs(r2[held]) < 2.0 * r0[held])
+    smallest = min(range(n), key=lambda m: models[m][1].node_num)
+    big = [m for m in range(n) if models[m][0].e_num == 20]
+    for m in [smallest] + big[:1]:
+        sp, g, pt = models[m]
+        lo = int(solved(g)[len(solved(g)) // 2]) + 1
+        ref = oracle.se_pass(g, sp, xth, 32, cloud.DX_INWARD, node_lo=lo, node_hi=lo + 7, nthreads=NTH,
+                             lev_p=pt.get_levpop())[0]
+        idx = np.array([i for i in range(lo - 1, lo + 7) if g.ab_all[i] > 0])
+        got = pt.get_pol_pops()
+        big_lv = ref[idx][:, 0::2] > 1e-6 * ref[idx][:, 0::2].max(axis=1, keepdims=True)
+        e = np.abs(got[idx][:, 0::2] / ref[idx][:, 0::2] - 1)[big_lv].max()
+        print(f"model {m} ({g.node_num} nodes, {sp.e_num} levels): fixed point vs oracle, max rel rho00 {e:.1e}")
+        assert e < 1e-5
+    for pt in pts:
+        pt.close()
