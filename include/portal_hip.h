@@ -115,22 +115,24 @@ typedef struct {
 } portal_img_params;
 
 typedef struct {
-    double t_walk_ms;        /* inward path construction (done once per geometry) */
+    double t_walk_ms;        /* inward path construction (done once per geometry); in the image
+                                pass the start search and march */
     double t_prep_ms;        /* per-iteration line coefficients */
     double t_rt_ms;          /* inward ray integration */
     double t_se_ms;          /* SE assembly + solve */
     double t_img_ms;         /* image pass */
     double t_total_ms;       /* device time of the call */
     double rt_kernel_avg_ms; /* average duration of one inward-integration launch */
-    long long n_cross;       /* ray-cell crossings integrated per pass */
-    long long n_paths;       /* (node, ray) paths */
+    long long n_cross;       /* ray-cell crossings integrated per pass (image pass: cells the
+                                pixels' rays visit) */
+    long long n_paths;       /* (node, ray) paths (image pass: pixels) */
     int iterations;
     int fallbacks;           /* nodes where the surrogate iterate was rejected */
     int lu_failures;
     double max_rel_change;   /* last iteration's population change */
     int rt_variant;          /* inward-integration kernel chosen by the autotuner */
     int rt_waves;            /* its waves per workgroup */
-    int max_path;            /* longest inward path (cells) */
+    int max_path;            /* longest inward path (cells; image pass: longest pixel ray) */
     int ng_steps;            /* Ng extrapolations applied by the lambda iteration */
     int converged;           /* 1: the lambda iteration met both tolerances */
     double max_align_change; /* last iteration's largest change of rho^2_0/rho^0_0 */

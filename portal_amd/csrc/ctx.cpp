@@ -1756,6 +1756,19 @@ int portal_image_pass(portal_ctx *x, const portal_img_params *p, const double *p
         st->t_rt_ms = ms_r;
         st->rt_kernel_avg_ms = ms_r;
         st->t_total_ms = ms_g + ms_r;
+        // cells visited by the pixels' rays (their march records), and the longest ray
+        std::vector<int> rn(n_rt);
+        HIP_TRY(&c, hipMemcpy(rn.data(), a.rec_n, sizeof(int) * (size_t)n_rt, hipMemcpyDeviceToHost));
+        long long tot = 0;
+        int mx = 0;
+        for (int v : rn)
+            if (v > 0) {
+                tot += v;
+                mx = std::max(mx, v);
+            }
+        st->n_cross = tot;
+        st->n_paths = n_rt;
+        st->max_path = mx;
     }
     return PORTAL_OK;
 }
