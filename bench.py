@@ -102,8 +102,8 @@ def cpu_baseline(g, sp, xth, args, log):
 def image_cpu_port(g, sp, pt, lines, vel_out, args, log):
     """The image pass's CPU baseline: the oracle's restatement of the reference's image
     loops (raytrace.f90, para_finish.f90:382-449; the reference's own image loops need
-    em_coefs.f90, which this image cannot build) on 128 pixels of the cube's central row,
-    FP64, on the host's threads."""
+    em_coefs.f90, which this image cannot build) on 1,024 consecutive pixels about the
+    cube's centre, FP64, on the host's threads."""
     if args.cpu_sample <= 0:
         return None
     try:
@@ -111,9 +111,9 @@ def image_cpu_port(g, sp, pt, lines, vel_out, args, log):
         from portal_amd import cloud, driver
         nthr = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
         rp = driver.rescale_pol_pops(pt.get_pol_pops(), pt.get_levpop(), sp.lev_g)
-        n_side = 2 * args.im_pix + 1
-        npx = min(128, n_side)
-        lo = args.im_pix * n_side + (n_side - npx) // 2 + 1
+        n_pix = (2 * args.im_pix + 1) ** 2
+        npx = min(1024, n_pix)
+        lo = (n_pix - npx) // 2 + 1
         t = time.perf_counter()
         Oracle().image_pass(g, sp, rp, lines, np.pi / 4, 0.0, 64, vel_out, args.im_pix, 0.7, cloud.DX_IMAGE,
                             n_depth_rt=2048, pix_lo=lo, pix_hi=lo + npx - 1, nthreads=nthr)
@@ -121,7 +121,8 @@ def image_cpu_port(g, sp, pt, lines, vel_out, args, log):
         rate = npx * 129 * len(lines) / t
         log(f"CPU image (C restatement): {rate:.3g} pixel-channels/s on {nthr} threads")
         return {"pixel_channels_per_s": rate, "cores": nthr, "kind": "port",
-                "sample": f"{npx} pixels of the central row, {len(lines)} line x 129 channels, FP64, {t:.1f}s"}
+                "sample": f"{npx} consecutive pixels about the centre, {len(lines)} line x 129 channels, FP64, "
+                          f"{t:.1f}s"}
     except Exception as e:  # checker not built
         log(f"cpu image baseline skipped: {e}")
         return None
