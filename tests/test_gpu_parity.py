@@ -353,6 +353,58 @@ def test_error_paths(cfg1):
     pt.close()
 
 
+def _disconnected_species():
+    """A 3-level rotor whose J = 2 level has no radiative or collisional transition: its
+    rows of the SE matrix are empty, so the matrix is singular."""
+    ct = np.array([1.0, 10.0, 100.0, 1e3, 1e4])
+    return cloud.Species(e_num=3, j_lev=np.array([0, 1, 2], dtype=np.int32), e_levs_cm=np.array([0.0, 3.845, 11.535]),
+                         mass_mol=28.0, r_trans=np.array([[1], [2]], dtype=np.int32), t_Aji=np.array([7.2e-8]),
+                         nu_ghz=np.array([115.27]), c_trans=np.array([[1], [2]], dtype=np.int32), col_temps=ct,
+                         t_Cji=np.asfortranarray(np.full((5, 1, 1), 1e-11)), kmax=2)
+
+
+def test_singular_se_system_is_reported(oracle):
+    """A singular SE matrix stops the reference (em_coefs.f90:170-178); the oracle and the
+    library report it (PORTAL_ERR_LU) in both precisions, single model and batch, instead
+    of returning a solution."""
+    sp = _disconnected_species()
+    g = cloud.make_cloud(sp, 300, seed=6)
+    xth = cloud.legendre_all(4, 8)
+    with pytest.raises(RuntimeError):
+        oracle.se_pass(g, sp, xth, 8, cloud.DX_INWARD, nthreads=NTH)
+    from portal_amd.api import batch_lambda_iterate
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    for prec in (PORTAL_FP64, PORTAL_MIXED):
+        pt.set_levpop(g.lev_p)
+        with pytest.raises(PortalError) as e:
+            pt.se_pass(xth, 8, cloud.DX_INWARD, precision=prec)
+        assert e.value.code == -5  # PORTAL_ERR_LU
+        pt.set_levpop(g.lev_p)
+        with pytest.raises(PortalError) as e:
+            batch_lambda_iterate([pt], [xth], [8], cloud.DX_INWARD, 2, precision=prec)
+        assert e.value.code == -5
+    pt.close()
+
+
+def test_species_beyond_the_solver_size_is_refused():
+    """A 40-level rotor's SE system (79 unknowns with 780 transitions) does not fit one
+    workgroup's LDS: the upload is refused with PORTAL_ERR_ARG, and the context stays usable."""
+    pt = Portal(0)
+    with pytest.raises(PortalError) as e:
+        pt.upload_species(cloud.co_like_rotor(40, kmax=2))
+    assert e.value.code == -1
+    sp = cloud.co_like_rotor(4)
+    g = cloud.make_cloud(sp, 300, seed=7)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    pp, _ = pt.se_pass(cloud.legendre_all(4, 8), 8, cloud.DX_INWARD)
+    assert np.all(np.isfinite(pp))
+    pt.close()
+
+
 def test_image_pass_fp64_parity(portal1, cfg1, oracle, oracle_cfg1):
     from portal_amd import driver
     sp, g, xth = cfg1

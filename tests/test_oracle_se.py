@@ -123,3 +123,17 @@ def test_rate_columns_conserve(oracle):
         w[sp.count_array[i]] = math.sqrt(2 * sp.j_lev[i] + 1)
     col = w @ A
     assert np.abs(col).max() < 1e-10 * np.abs(A).max()
+
+
+def test_singular_system_is_reported(oracle):
+    """A level with no transition leaves its rows of the SE matrix empty: the solve reports
+    the singular matrix (em_coefs.f90:170-178 stops the reference there) rather than
+    returning numbers."""
+    ct = np.array([1.0, 10.0, 100.0, 1e3, 1e4])
+    sp = cloud.Species(e_num=3, j_lev=np.array([0, 1, 2], dtype=np.int32), e_levs_cm=np.array([0.0, 3.845, 11.535]),
+                       mass_mol=28.0, r_trans=np.array([[1], [2]], dtype=np.int32), t_Aji=np.array([7.2e-8]),
+                       nu_ghz=np.array([115.27]), c_trans=np.array([[1], [2]], dtype=np.int32), col_temps=ct,
+                       t_Cji=np.asfortranarray(np.full((5, 1, 1), 1e-11)), kmax=2)
+    AC = oracle.make_ac(sp, 1e12, 20.0, np.array([1e-17]), np.array([1e-19]))
+    with pytest.raises(RuntimeError):
+        oracle.get_rhovec(sp, AC, 20.0)
