@@ -145,6 +145,7 @@ struct Ctx {
     double surr_perturb_factor = 1.0;
     std::vector<int> fallback_map;  // per work item of the last surrogate run: 0 kept, 1 invalid, 2 residual
     int se_waves = 1;               // waves per system of the single-model SE solve
+    bool se_force_slab = false;     // test hook: the single-model solve in the slab form
     // test hook for a batch led by this context: SE form (-1 autotuned, 0 one persistent
     // launch, 1 a launch per class) and the waves per system of the classes (0 autotuned)
     int se_forced_form = -1, se_forced_waves = 0;
@@ -338,6 +339,7 @@ hipError_t launch_se_slab(const SeArgs *models, const int2 *items, int n_items, 
 int se_slab_blocks_per_cu(int waves, int cd, int nt);
 size_t se_lds_bytes(int c_dim, int n_trans, bool m0_in_lds = true);
 size_t se_slab_doubles(int c_dim, int n_trans);
+constexpr int SE_SLAB_WAVES = 4;  // waves per system of a single model's slab-form solve
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves);
 // the cooperative form of the persistent integration, one launch per (slot count, dust)
 hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,

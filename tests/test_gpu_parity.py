@@ -389,11 +389,11 @@ def test_singular_se_system_is_reported(oracle):
 
 
 def test_species_beyond_the_solver_size_is_refused():
-    """A 40-level rotor's SE system (79 unknowns with 780 transitions) does not fit one
-    workgroup's LDS: the upload is refused with PORTAL_ERR_ARG, and the context stays usable."""
+    """A 41-level rotor at kmax = 6 has 158 unknowns, beyond the 128 the solver holds: the
+    upload is refused with PORTAL_ERR_ARG, and the context stays usable."""
     pt = Portal(0)
     with pytest.raises(PortalError) as e:
-        pt.upload_species(cloud.co_like_rotor(40, kmax=2))
+        pt.upload_species(cloud.co_like_rotor(41, kmax=6))
     assert e.value.code == -1
     sp = cloud.co_like_rotor(4)
     g = cloud.make_cloud(sp, 300, seed=7)
@@ -403,6 +403,38 @@ def test_species_beyond_the_solver_size_is_refused():
     pp, _ = pt.se_pass(cloud.legendre_all(4, 8), 8, cloud.DX_INWARD)
     assert np.all(np.isfinite(pp))
     pt.close()
+
+
+def test_large_se_systems_take_the_slab_form(oracle):
+    """A 41-level rotor (LAMDA's CO has 41 levels) at kmax = 4: 120 unknowns and 820
+    transitions, whose matrix, copy and scalars do not fit one workgroup's LDS, so the
+    solve keeps the copy and the scalars in a global slab.  FP64 against the oracle to
+    1e-10; and on 74-dimensional systems the slab form gives the LDS form's bits."""
+    sp = cloud.co_like_rotor(41, kmax=4)
+    g = cloud.make_cloud(sp, 400, seed=63)
+    xth = cloud.legendre_all(3, 6)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    assert pt.species_info()[1] == 120
+    pp, _ = pt.se_pass(xth, 8, cloud.DX_INWARD)
+    ref = oracle.se_pass(g, sp, xth, 8, cloud.DX_INWARD, nthreads=NTH)[0]
+    assert_pops_close(pp, ref, sp, solved(g), 1e-10)
+    pt.close()
+    sp = cloud.co_like_rotor(20)
+    g = cloud.make_cloud(sp, 400, seed=64)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    out = []
+    for form, waves in ((-1, 1), (2, 4), (2, 1)):
+        pt.force_se(form, waves)
+        pt.set_levpop(g.lev_p)
+        out.append(pt.se_pass(xth, 8, cloud.DX_INWARD)[0])
+    pt.close()
+    np.testing.assert_array_equal(out[1], out[0])
+    np.testing.assert_array_equal(out[2], out[0])
 
 
 def test_image_pass_fp64_parity(portal1, cfg1, oracle, oracle_cfg1):
