@@ -145,7 +145,7 @@ struct Ctx {
     double surr_perturb_factor = 1.0;
     std::vector<int> fallback_map;  // per work item of the last surrogate run: 0 kept, 1 invalid, 2 residual
     int se_waves = 1;               // waves per system of the single-model SE solve
-    bool se_force_slab = false;     // test hook: the single-model solve in the slab form
+    int se_force_slab = 0;          // test hook: the single-model solve in the slab form (1), with the factors in the slab too (2)
     // test hook for a batch led by this context: SE form (-1 autotuned, 0 one persistent
     // launch, 1 a launch per class) and the waves per system of the classes (0 autotuned)
     int se_forced_form = -1, se_forced_waves = 0;
@@ -335,11 +335,12 @@ hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_i
 // persistent workgroups of `waves` (1, 2 or 4) waves over the items of one system size
 // (cd, nt), each with a slab of cd*cd doubles at slabs + blockIdx.x*cd*cd
 hipError_t launch_se_slab(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd, int nt,
-                          int waves, int n_blocks, double *slabs, hipStream_t s);
-int se_slab_blocks_per_cu(int waves, int cd, int nt);
-size_t se_lds_bytes(int c_dim, int n_trans, bool m0_in_lds = true);
-size_t se_slab_doubles(int c_dim, int n_trans);
+                          int waves, int n_blocks, double *slabs, hipStream_t s, bool all_global = false);
+int se_slab_blocks_per_cu(int waves, int cd, int nt, bool all_global = false);
+size_t se_lds_bytes(int c_dim, int n_trans, bool m0_in_lds = true, bool all_global = false);
+size_t se_slab_doubles(int c_dim, int n_trans, bool all_global = false);
 constexpr int SE_SLAB_WAVES = 4;  // waves per system of a single model's slab-form solve
+constexpr int SE_MAX_UNKNOWNS = 512;  // largest SE system (c_dim) the solver takes
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves);
 // the cooperative form of the persistent integration, one launch per (slot count, dust)
 hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,

@@ -617,7 +617,8 @@ int run_se(Ctx &c, int extra_info, portal_stats *st) {
     if (extra_info) HIP_TRY(&c, hipMemsetAsync(c.d_rates, 0, sizeof(double) * c.g.N * 6 * s.e_num, c.stream));
     // a system whose matrix, copy and scalars do not fit one workgroup's LDS takes the
     // batch's slab form (copy and scalars in a global slab per workgroup) as a batch of one
-    const bool slab_form = se_lds_bytes(a.c_dim, a.n_trans, true) > 160 * 1024 || c.se_force_slab;
+    const bool slab_form = se_lds_bytes(a.c_dim, a.n_trans, true) > 160 * 1024 || c.se_force_slab > 0;
+    const bool all_global = c.se_force_slab == 2;
     SeArgs *d_a = nullptr;
     int2 *d_items = nullptr;
     int *d_cnt = nullptr;
@@ -634,7 +635,7 @@ int run_se(Ctx &c, int extra_info, portal_stats *st) {
         int n_cu = 256;
         hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c.device);
         a.waves = c.se_forced_waves > 0 ? c.se_forced_waves : SE_SLAB_WAVES;
-        const int bpc = se_slab_blocks_per_cu(a.waves, a.c_dim, a.n_trans);
+        const int bpc = se_slab_blocks_per_cu(a.waves, a.c_dim, a.n_trans, all_global);
         if (bpc < 1) return fail(c, PORTAL_ERR_ARG, "SE system too large for one workgroup's LDS");
         blocks = n_cu * bpc;
         std::vector<SeArgs> av(1, a);
@@ -643,13 +644,14 @@ int run_se(Ctx &c, int extra_info, portal_stats *st) {
         HIP_TRY(&c, upload(d_a, av));
         HIP_TRY(&c, upload(d_items, items));
         HIP_TRY(&c, hipMalloc((void **)&d_cnt, sizeof(int)));
-        HIP_TRY(&c, hipMalloc((void **)&d_slabs, sizeof(double) * (size_t)blocks * se_slab_doubles(a.c_dim, a.n_trans)));
+        HIP_TRY(&c, hipMalloc((void **)&d_slabs,
+                              sizeof(double) * (size_t)blocks * se_slab_doubles(a.c_dim, a.n_trans, all_global)));
         HIP_TRY(&c, launch_fill_zero(d_cnt, sizeof(int), c.stream));
     }
     Timer t(c.stream);
     if (slab_form)
         HIP_TRY(&c, launch_se_slab(d_a, d_items, (int)c.g.work.size(), d_cnt, a.c_dim, a.n_trans, a.waves, blocks,
-                                   d_slabs, c.stream));
+                                   d_slabs, c.stream, all_global));
     else
         HIP_TRY(&c, launch_se(a, c.stream));
     const double ms = t.stop();
@@ -897,12 +899,18 @@ int portal_upload_species(portal_ctx *x, int e_num, const int *j_lev, const doub
         if (s.r_trans[i] < 1 || s.r_trans[i] > e_num) return fail(c, PORTAL_ERR_ARG, "r_trans out of range");
     for (int i = 0; i < 2 * c_num; i++)
         if (s.c_trans[i] < 1 || s.c_trans[i] > e_num) return fail(c, PORTAL_ERR_ARG, "c_trans out of range");
+    {
+        // unknowns of the SE system (setup_count_array): K = 0, 2, .., min(2J, kmax) per level
+        long long unknowns = 0;
+        for (int i = 0; i < e_num; i++) unknowns += std::min(2 * std::max(s.j_lev[i], 0), kmax) / 2 + 1;
+        if (unknowns > SE_MAX_UNKNOWNS)
+            return fail(c, PORTAL_ERR_ARG, "SE system too large (more than 512 unknowns)");
+    }
     build_species_tables(s);
     const int cd = s.c_dim;
-    // the slab form holds only the factors and two vectors in LDS; the substitutions hold
-    // two rows per lane of one wave
-    if (cd > 128 || se_lds_bytes(cd, s.n_trans, false) > 150 * 1024)
-        return fail(c, PORTAL_ERR_ARG, "SE system too large (more than 128 unknowns)");
+    // larger systems take the slab form, whose LDS is a few vectors
+    if (cd > SE_MAX_UNKNOWNS || se_lds_bytes(cd, s.n_trans, false) > 150 * 1024)
+        return fail(c, PORTAL_ERR_ARG, "SE system too large (more than 512 unknowns)");
     // per radiative line
     std::vector<int> l1(r_num), l2(r_num);
     std::vector<double> c1B(r_num), c2A(r_num), g1(r_num), g2(r_num), cnu(r_num), Iin(r_num);
@@ -1270,13 +1278,14 @@ int portal_debug_force_rt(portal_ctx *x, int precision, int variant, int waves) 
 // Test hook: waves per system of this context's SE solve (1, 2 or 4), and for a batch
 // that this context leads, the SE form (-1 autotuned, 0 one persistent launch, 1 a
 // launch per class of system sizes) and the classes' waves (0 autotuned).  Form 2 puts
-// this context's own solve in the slab form that larger systems take.
+// this context's own solve in the slab form that larger systems take, form 3 in the form
+// of systems above 128 unknowns (factors in the slab, substitutions on all waves).
 int portal_debug_force_se(portal_ctx *x, int form, int waves) {
-    if (!x || form < -1 || form > 2 || !(waves == 0 || waves == 1 || waves == 2 || waves == 4)) return PORTAL_ERR_ARG;
+    if (!x || form < -1 || form > 3 || !(waves == 0 || waves == 1 || waves == 2 || waves == 4)) return PORTAL_ERR_ARG;
     x->c.se_waves = waves > 0 ? waves : 1;
-    x->c.se_forced_form = form == 2 ? -1 : form;
+    x->c.se_forced_form = form >= 2 ? -1 : form;
     x->c.se_forced_waves = waves;
-    x->c.se_force_slab = form == 2;
+    x->c.se_force_slab = form >= 2 ? form - 1 : 0;
     return PORTAL_OK;
 }
 

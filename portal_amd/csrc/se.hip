@@ -63,23 +63,73 @@ __device__ __forceinline__ void add_depol(const SeArgs &a, double *M, int cd, in
 // elementwise loops stride over all of them, the trailing update of each LU step splits
 // its columns between the waves (column j to wave j mod W), and every wave runs the
 // pivot search on its own, so the factors and the solution do not depend on W.
-// G: the copy of the assembled matrix that the refinement residual reads, and the
-// transition scalars, live in global memory at M0g (a slab of the workgroup's own,
-// cd^2 + 10 nt doubles) rather than in LDS.
-template <int W, bool G>
-__device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *sm, double *M0g) {
+// The substitutions and the refinement of se_node for more than 128 unknowns, on all
+// NT threads with the vectors in LDS (one barrier per step): thread t owns rows t, t+NT,
+// .., and every row sees the same operations in the same order as in the one-wave form:
+// y_i = b_perm(i); forward y_i -= l_ik y_k for k < i in order; backward y_i /= u_ii at
+// k = i and y_i -= u_ik y_k for k > i in decreasing k; the residual r = e1 - M0 v summed
+// over j in order with TwoProduct compensation.
+template <int NT>
+__device__ __forceinline__ void se_solve_coop(const double *M, const double *M0, double *v, double *y, double *b,
+                                              const int *perm, const int cd, const int tid) {
+    for (int it = 0; it < 2; it++) {
+        for (int i = tid; i < cd; i += NT) {
+            if (it == 0) {
+                b[i] = i == 0 ? 1.0 : 0.0;
+            } else {
+                double s0 = i == 0 ? 1.0 : 0.0, c0 = 0.0;
+                for (int j = 0; j < cd; j++) {
+                    double p, e;
+                    two_prod(-M0[i + (size_t)cd * j], v[j], p, e);
+                    const double t = s0 + p;
+                    const double z = t - s0;
+                    c0 += (s0 - (t - z)) + (p - z) + e;
+                    s0 = t;
+                }
+                b[i] = s0 + c0;
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < cd; i += NT) y[i] = b[perm[i]];
+        __syncthreads();
+        for (int k = 0; k < cd; k++) {
+            const double yk = y[k];
+            for (int i = k + 1 + tid; i < cd; i += NT) y[i] = y[i] - M[i + (size_t)cd * k] * yk;
+            __syncthreads();
+        }
+        for (int k = cd - 1; k >= 0; k--) {
+            if (tid == 0) y[k] = y[k] / M[k + (size_t)cd * k];
+            __syncthreads();
+            const double yk = y[k];
+            for (int i = tid; i < k; i += NT) y[i] = y[i] - M[i + (size_t)cd * k] * yk;
+            __syncthreads();
+        }
+        for (int i = tid; i < cd; i += NT) v[i] = it == 0 ? y[i] : v[i] + y[i];
+        __syncthreads();
+    }
+}
+
+// MODE 0: everything in LDS.  MODE 1 (slab form): the copy of the assembled matrix
+// that the refinement residual reads, and the transition scalars, live in global memory
+// at slab (the workgroup's own, cd^2 + 10 nt doubles) rather than in LDS.  MODE 2 (more
+// than 128 unknowns): the factors too are in the slab (2 cd^2 + 10 nt doubles), LDS holds
+// three vectors, and the substitutions run on all waves (se_solve_coop) instead of in one
+// wave's registers; the arithmetic of every row is the same.
+template <int W, int MODE>
+__device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *sm, double *slab) {
     constexpr int NT = 64 * W;
+    constexpr bool G = MODE >= 1;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int node = a.work[wi];
     const int cd = a.c_dim, nt = a.n_trans, L = a.L, R = a.R;
-    // the slab form (G) keeps the scalars next to the matrix copy in global memory, so
-    // that LDS holds only the factors and two vectors
-    double *x = G ? M0g + (size_t)cd * cd : sm;  // 10*nt scalars
-    double *M = G ? sm : x + 10 * nt;        // LU in place, column-major cd x cd
-    double *M0 = G ? M0g : M + cd * cd;     // assembled matrix (for the residual)
-    double *v = G ? M + cd * cd : M0 + cd * cd;  // solution
+    const size_t cd2 = (size_t)cd * cd;
+    double *x = MODE == 0 ? sm : MODE == 1 ? slab + cd2 : slab + 2 * cd2;  // 10*nt scalars
+    double *M = MODE == 0 ? x + 10 * nt : MODE == 1 ? sm : slab;          // LU in place, column-major
+    double *M0 = MODE == 0 ? M + cd2 : MODE == 1 ? slab : slab + cd2;     // assembled matrix (residual)
+    double *v = MODE == 0 ? M0 + cd2 : MODE == 1 ? M + cd2 : sm;          // solution
     double *y = v + cd;                      // rhs / work
-    int *perm = (int *)(y + cd);             // row permutation of the factorisation
+    double *bv = y + cd;                     // MODE 2: right-hand side
+    int *perm = (int *)(MODE == 2 ? bv + cd : y + cd);  // row permutation of the factorisation
     __shared__ int s_info;
 
     const double T = a.tkin[node];
@@ -238,7 +288,9 @@ __device__ __forceinline__ void se_node(const SeArgs &a, const int wi, double *s
     // registers with no barrier: lane l holds rows l and l + 64 (cd <= 128), the
     // permutation is a gather by perm, and each substitution step broadcasts one
     // element by readlane.
-    if (wv == 0) {
+    if constexpr (MODE == 2) {
+        se_solve_coop<NT>(M, M0, v, y, bv, perm, cd, tid);
+    } else if (wv == 0) {
         const int i0 = lane, i1 = lane + 64;
         const bool h0 = i0 < cd, h1 = i1 < cd;
         const int p0 = h0 ? perm[i0] : 0, p1 = h1 ? perm[i1] : 0;
@@ -381,7 +433,7 @@ template <int W>
 __global__ void __launch_bounds__(64 * W) se_kernel(SeArgs a) {
     extern __shared__ double sm[];
     if ((int)blockIdx.x >= a.n_work) return;
-    se_node<W, false>(a, blockIdx.x, sm, nullptr);
+    se_node<W, 0>(a, blockIdx.x, sm, nullptr);
 }
 
 // Persistent form over the (model, node) items of a batch of models, one wave per
@@ -397,20 +449,21 @@ __global__ void __launch_bounds__(64) se_persistent_kernel(const SeArgs *models,
         __syncthreads();
         if (it >= n_items) break;
         const int2 mi = items[it];
-        se_node<1, false>(models[mi.x], mi.y, sm, nullptr);
+        se_node<1, 0>(models[mi.x], mi.y, sm, nullptr);
         __syncthreads();
     }
 }
 
 // Persistent form for a class of models with one system size: W waves per system, the
 // residual's matrix copy and the transition scalars in the workgroup's slab of `slab`
-// doubles, so that LDS holds only the factors and two vectors.
-template <int W>
+// doubles, so that LDS holds only the factors and two vectors (MODE 1); above 128
+// unknowns the factors too (MODE 2, see se_node).
+template <int W, int MODE>
 __global__ void __launch_bounds__(64 * W) se_slab_kernel(const SeArgs *models, const int2 *items, int n_items,
                                                         int *counter, double *slabs, int slab) {
     extern __shared__ double sm[];
     __shared__ int s_item;
-    double *M0 = slabs + (size_t)blockIdx.x * slab;
+    double *mine = slabs + (size_t)blockIdx.x * slab;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
@@ -418,21 +471,26 @@ __global__ void __launch_bounds__(64 * W) se_slab_kernel(const SeArgs *models, c
         __syncthreads();
         if (it >= n_items) break;
         const int2 mi = items[it];
-        se_node<W, true>(models[mi.x], mi.y, sm, M0);
+        se_node<W, MODE>(models[mi.x], mi.y, sm, mine);
         __syncthreads();
     }
 }
 
 // LDS of one system: with m0_in_lds the scalars, the factors, the matrix copy and two
-// vectors; without (the slab form) the factors and two vectors
-size_t se_lds_bytes(int c_dim, int n_trans, bool m0_in_lds) {
-    return sizeof(double) * ((m0_in_lds ? 10 * (size_t)n_trans + 2 * (size_t)c_dim * c_dim : (size_t)c_dim * c_dim) +
-                             2 * c_dim) +
-           sizeof(int) * c_dim + 16;
+// vectors; without (the slab form) the factors and two vectors, or above 128 unknowns
+// three vectors
+size_t se_lds_bytes(int c_dim, int n_trans, bool m0_in_lds, bool all_global) {
+    const size_t cd = (size_t)c_dim;
+    const bool g = all_global || c_dim > 128;
+    const size_t dbl = m0_in_lds ? 10 * (size_t)n_trans + 2 * cd * cd + 2 * cd : g ? 3 * cd : cd * cd + 2 * cd;
+    return sizeof(double) * dbl + sizeof(int) * cd + 16;
 }
 
-// doubles of one workgroup's slab: the matrix copy and the transition scalars
-size_t se_slab_doubles(int c_dim, int n_trans) { return (size_t)c_dim * c_dim + 10 * (size_t)n_trans; }
+// doubles of one workgroup's slab: the matrix copy and the transition scalars, and above
+// 128 unknowns the factors
+size_t se_slab_doubles(int c_dim, int n_trans, bool all_global) {
+    return (size_t)(all_global || c_dim > 128 ? 2 : 1) * c_dim * c_dim + 10 * (size_t)n_trans;
+}
 
 hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd_max,
                                 int nt_max, int n_blocks, hipStream_t s) {
@@ -442,41 +500,55 @@ hipError_t launch_se_persistent(const SeArgs *models, const int2 *items, int n_i
     return hipGetLastError();
 }
 
-int se_slab_blocks_per_cu(int waves, int cd, int nt) {
-    const size_t lds = se_lds_bytes(cd, nt, false);
-    if (lds > 160 * 1024 || cd > 128) return 0;
+template <int MODE>
+static int slab_blocks(int waves, size_t lds) {
     int nb = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (waves) {
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, se_slab_kernel<1>, 64, lds); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, se_slab_kernel<2>, 128, lds); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, se_slab_kernel<4>, 256, lds); break;
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, se_slab_kernel<1, MODE>, 64, lds); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, se_slab_kernel<2, MODE>, 128, lds); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, se_slab_kernel<4, MODE>, 256, lds); break;
         default: break;
     }
     return e == hipSuccess ? nb : 0;
 }
 
-hipError_t launch_se_slab(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd, int nt,
-                          int waves, int n_blocks, double *slabs, hipStream_t s) {
-    const size_t lds = se_lds_bytes(cd, nt, false);
-    if (lds > 160 * 1024 || cd > 128 || n_items < 0 || n_blocks < 1) return hipErrorInvalidValue;
-    const int slab = (int)se_slab_doubles(cd, nt);
+int se_slab_blocks_per_cu(int waves, int cd, int nt, bool all_global) {
+    const size_t lds = se_lds_bytes(cd, nt, false, all_global);
+    if (lds > 160 * 1024 || cd > SE_MAX_UNKNOWNS) return 0;
+    // above 128 unknowns the factorisation works in the slab: a few systems per CU
+    // already keep it busy, and the slabs stay within a few GB
+    return all_global || cd > 128 ? std::min(4, slab_blocks<2>(waves, lds)) : slab_blocks<1>(waves, lds);
+}
+
+template <int MODE>
+static hipError_t launch_slab(const SeArgs *models, const int2 *items, int n_items, int *counter, int waves,
+                              int n_blocks, double *slabs, int slab, size_t lds, hipStream_t s) {
     switch (waves) {
         case 1:
-            hipLaunchKernelGGL(se_slab_kernel<1>, dim3(n_blocks), dim3(64), lds, s, models, items, n_items, counter,
-                               slabs, slab);
+            hipLaunchKernelGGL((se_slab_kernel<1, MODE>), dim3(n_blocks), dim3(64), lds, s, models, items, n_items,
+                               counter, slabs, slab);
             break;
         case 2:
-            hipLaunchKernelGGL(se_slab_kernel<2>, dim3(n_blocks), dim3(128), lds, s, models, items, n_items, counter,
-                               slabs, slab);
+            hipLaunchKernelGGL((se_slab_kernel<2, MODE>), dim3(n_blocks), dim3(128), lds, s, models, items, n_items,
+                               counter, slabs, slab);
             break;
         case 4:
-            hipLaunchKernelGGL(se_slab_kernel<4>, dim3(n_blocks), dim3(256), lds, s, models, items, n_items, counter,
-                               slabs, slab);
+            hipLaunchKernelGGL((se_slab_kernel<4, MODE>), dim3(n_blocks), dim3(256), lds, s, models, items, n_items,
+                               counter, slabs, slab);
             break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_se_slab(const SeArgs *models, const int2 *items, int n_items, int *counter, int cd, int nt,
+                          int waves, int n_blocks, double *slabs, hipStream_t s, bool all_global) {
+    const size_t lds = se_lds_bytes(cd, nt, false, all_global);
+    if (lds > 160 * 1024 || cd > SE_MAX_UNKNOWNS || n_items < 0 || n_blocks < 1) return hipErrorInvalidValue;
+    const int slab = (int)se_slab_doubles(cd, nt, all_global);
+    return all_global || cd > 128 ? launch_slab<2>(models, items, n_items, counter, waves, n_blocks, slabs, slab, lds, s)
+                    : launch_slab<1>(models, items, n_items, counter, waves, n_blocks, slabs, slab, lds, s);
 }
 
 hipError_t launch_se(const SeArgs &a, hipStream_t s) {

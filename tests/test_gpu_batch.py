@@ -382,3 +382,35 @@ def test_batch_surrogate_equals_single_models():
             np.testing.assert_array_equal(pt.fallback_map(), fmap)
         pt.close()
     assert tot["fallbacks"] == sum(s[2] for s in singles)
+
+
+def test_batch_with_large_systems_equals_single_models():
+    """A batch holding a model with 158-unknown SE systems (41 levels, kmax = 6: the class
+    solved with its factors in the slab) beside a 4-level model iterates as they do alone
+    (FP64, 1e-10)."""
+    specs = [(sp, cloud.make_cloud(sp, n, seed=seed)) for sp, n, seed in
+             ((cloud.co_like_rotor(41, kmax=6), 800, 51), (cloud.co_like_rotor(4), 600, 52))]
+    xth = cloud.legendre_all(3, 6)
+
+    def ctx(sp, g):
+        pt = Portal(0)
+        pt.upload_grid(g)
+        pt.upload_species(sp)
+        pt.set_levpop(g.lev_p)
+        return pt
+
+    singles = []
+    for sp, g in specs:
+        pt = ctx(sp, g)
+        singles.append(pt.lambda_iterate(xth, 8, cloud.DX_INWARD, 2, tol_pop=0.0, tol_align=0.0,
+                                         precision=PORTAL_FP64)[:2])
+        pt.close()
+    pts = [ctx(sp, g) for sp, g in specs]
+    per, tot = batch_lambda_iterate(pts, [xth] * 2, [8] * 2, cloud.DX_INWARD, 2, tol_pop=0.0, tol_align=0.0,
+                                    precision=PORTAL_FP64)
+    assert tot["se_form"] == 1
+    for (sp, g), pt, (pp, lev) in zip(specs, pts, singles):
+        idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+        np.testing.assert_allclose(pt.get_levpop()[:, idx], lev[:, idx], rtol=1e-10)
+        np.testing.assert_allclose(pt.get_pol_pops()[idx][:, 0::2], pp[idx][:, 0::2], rtol=1e-10)
+        pt.close()

@@ -389,11 +389,11 @@ def test_singular_se_system_is_reported(oracle):
 
 
 def test_species_beyond_the_solver_size_is_refused():
-    """A 41-level rotor at kmax = 6 has 158 unknowns, beyond the 128 the solver holds: the
+    """A 110-level rotor at kmax = 8 has 540 unknowns, beyond the 512 the solver takes: the
     upload is refused with PORTAL_ERR_ARG, and the context stays usable."""
     pt = Portal(0)
     with pytest.raises(PortalError) as e:
-        pt.upload_species(cloud.co_like_rotor(41, kmax=6))
+        pt.upload_species(cloud.co_like_rotor(110, kmax=8))
     assert e.value.code == -1
     sp = cloud.co_like_rotor(4)
     g = cloud.make_cloud(sp, 300, seed=7)
@@ -405,36 +405,44 @@ def test_species_beyond_the_solver_size_is_refused():
     pt.close()
 
 
-def test_large_se_systems_take_the_slab_form(oracle):
-    """A 41-level rotor (LAMDA's CO has 41 levels) at kmax = 4: 120 unknowns and 820
-    transitions, whose matrix, copy and scalars do not fit one workgroup's LDS, so the
-    solve keeps the copy and the scalars in a global slab.  FP64 against the oracle to
-    1e-10; and on 74-dimensional systems the slab form gives the LDS form's bits."""
-    sp = cloud.co_like_rotor(41, kmax=4)
-    g = cloud.make_cloud(sp, 400, seed=63)
+@pytest.mark.parametrize("kmax,unknowns", [(4, 120), (6, 158), (8, 195)])
+def test_large_se_systems_take_the_slab_form(oracle, kmax, unknowns):
+    """A 41-level rotor (LAMDA's CO has 41 levels, 820 transitions) at kmax = 4, 6 and 8.
+    At 120 unknowns the matrix, its copy and the scalars do not fit one workgroup's LDS,
+    so the copy and the scalars go to a global slab; above 128 the factors too, with the
+    substitutions on all waves.  FP64 against the oracle to 1e-10."""
+    sp = cloud.co_like_rotor(41, kmax=kmax)
+    g = cloud.make_cloud(sp, 300, seed=63)
     xth = cloud.legendre_all(3, 6)
     pt = Portal(0)
     pt.upload_grid(g)
     pt.upload_species(sp)
     pt.set_levpop(g.lev_p)
-    assert pt.species_info()[1] == 120
+    assert pt.species_info()[1] == unknowns
     pp, _ = pt.se_pass(xth, 8, cloud.DX_INWARD)
+    pt.close()
     ref = oracle.se_pass(g, sp, xth, 8, cloud.DX_INWARD, nthreads=NTH)[0]
     assert_pops_close(pp, ref, sp, solved(g), 1e-10)
-    pt.close()
+
+
+def test_se_forms_give_the_same_bits():
+    """On 74-unknown systems the LDS form, the slab form and the all-global form (factors
+    in the slab, substitutions on all waves), at 1 and 4 waves per system, give the same
+    bits: every row sees the same operations in the same order."""
     sp = cloud.co_like_rotor(20)
     g = cloud.make_cloud(sp, 400, seed=64)
+    xth = cloud.legendre_all(3, 6)
     pt = Portal(0)
     pt.upload_grid(g)
     pt.upload_species(sp)
     out = []
-    for form, waves in ((-1, 1), (2, 4), (2, 1)):
+    for form, waves in ((-1, 1), (2, 4), (2, 1), (3, 4), (3, 1)):
         pt.force_se(form, waves)
         pt.set_levpop(g.lev_p)
         out.append(pt.se_pass(xth, 8, cloud.DX_INWARD)[0])
     pt.close()
-    np.testing.assert_array_equal(out[1], out[0])
-    np.testing.assert_array_equal(out[2], out[0])
+    for o in out[1:]:
+        np.testing.assert_array_equal(o, out[0])
 
 
 def test_image_pass_fp64_parity(portal1, cfg1, oracle, oracle_cfg1):
