@@ -155,18 +155,45 @@ class Oracle:
             return pp, J0, J2, rates, jj
         return pp, J0, J2
 
-    def inverse_nodes(self, g, sp, xth, nvel, dx, nodes, n_depth=4096, nthreads=0, lev_p=None, star=None):
-        """inverse_rt for the given 1-based nodes -> J0, J2 (R x len(nodes))."""
+    def inverse_nodes(self, g, sp, xth, nvel, dx, nodes, n_depth=4096, nthreads=0, lev_p=None, star=None,
+                      thin_exact=False):
+        """inverse_rt for the given 1-based nodes -> J0, J2 (R x len(nodes)).  With
+        ``thin_exact`` the emission factor 1 - e^-tau is evaluated as -expm1(-tau) (the
+        mixed-precision path's definition) instead of as the reference writes it."""
         G = self.grid(g, lev_p)
         S = self.species(sp)
         P = self.params(xth, nvel, dx, g.gvec, g.bvec_all, n_depth, 0, star)
         nd = np.ascontiguousarray(nodes, dtype=np.int32)
         J0 = np.zeros((sp.r_num, len(nd)), order="F")
         J2 = np.zeros((sp.r_num, len(nd)), order="F")
-        rc = self.lib.orc_inverse_nodes(C.byref(G), C.byref(S), C.byref(P), _i(nd), len(nd), _d(J0), _d(J2), nthreads)
+        self.lib.orc_set_thin_exact(1 if thin_exact else 0)
+        try:
+            rc = self.lib.orc_inverse_nodes(C.byref(G), C.byref(S), C.byref(P), _i(nd), len(nd), _d(J0), _d(J2),
+                                            nthreads)
+        finally:
+            self.lib.orc_set_thin_exact(0)
         if rc:
             raise RuntimeError(f"oracle inverse_nodes failed ({rc})")
         return J0, J2
+
+    def ray_in(self, temp_node, vel_node, p1_node, p2_node, dens_1, dens_2, temp_list, vel_list, length_list,
+               nvel, dx, nu0, Aji, Bij, g1, g2, I_in, thin_exact=False):
+        """ray_in (inverse_rt.f90:217-344) without dust over one gathered path: index 0
+        of the per-cell arrays is the node's own cell -> the profile-weighted I_out."""
+        nc = len(dens_1)
+        z = np.zeros(nc)
+        L = self.lib
+        L.orc_ray_in.restype = C.c_double
+        self.lib.orc_set_thin_exact(1 if thin_exact else 0)
+        try:
+            return L.orc_ray_in(C.c_double(temp_node), C.c_double(vel_node), C.c_double(p1_node),
+                                C.c_double(p2_node), C.c_double(0.0), C.c_double(0.0), C.c_int(nc),
+                                _d(_f64(dens_1)), _d(_f64(dens_2)), _d(z), C.c_double(0.0), _d(z),
+                                _d(_f64(temp_list)), _d(_f64(vel_list)), _i(_i32(length_list)), C.c_int(nvel),
+                                C.c_double(dx), C.c_double(nu0), C.c_double(Aji), C.c_double(Bij), C.c_int(g1),
+                                C.c_int(g2), C.c_double(I_in))
+        finally:
+            self.lib.orc_set_thin_exact(0)
 
     def solve_from_J(self, g, sp, J0, J2, node_lo=1, node_hi=None):
         if node_hi is None:

@@ -16,6 +16,9 @@
  *   legendre nodes / find_dir  compiled reference (_ref) + closed forms
  *   check_neigh_fast / trace_out / ray_in / inverse_rt / black_body
  *                              compiled reference (_ref), bit-level and <=1e-12
+ *   ray_in with thin_exact     the closed form of a uniform ray, S + (I_in - S) e^-tau,
+ *                              at tau down to 1e-22 with S / I_in up to 1e25; equal to
+ *                              the reference form (<=1e-12) where no cell is thin
  *   make_ac                    compiled reference (_ref)
  *   t_xxx_const, r_xxx_const   exact rational re-evaluation (sympy) + selection rules
  *   setup_stateq / get_rhovec  an independent rate-equation model in the magnetic-
@@ -393,6 +396,16 @@ double orc_black_body(double T, double nu0) {
     return c1 * pow(nu0, 3.0) * pow(exp(c2 * nu0) - 1.0, -1.0);
 }
 
+/* The emission factor 1 - e^-tau of ray_in.  The reference forms it as written
+ * (inverse_rt.f90:314, :337), which in FP64 is exactly zero below tau ~ 1.1e-16 and
+ * coarse above: a thin cell's emission S tau is lost.  It matters only where S tau
+ * is comparable to what the ray already carries, i.e. S / I_in > 1e16 (lines far in
+ * the Wien tail of the 2.73 K background).  orc_set_thin_exact(1) evaluates the
+ * factor as -expm1(-tau), the definition the mixed-precision path follows. */
+static int thin_exact = 0;
+void orc_set_thin_exact(int on) { thin_exact = on != 0; }
+static double emit_factor(double tau, double e_tau) { return thin_exact ? -expm1(-tau) : 1.0 - e_tau; }
+
 /* ray_in, inverse_rt.f90:217-344.  Arrays are the per-cell gathers of the ray
  * path, index 0 = the central node's own cell. */
 double orc_ray_in(double temp_node, double vel_node, double p1_node, double p2_node, double dust_node,
@@ -425,7 +438,7 @@ double orc_ray_in(double temp_node, double vel_node, double p1_node, double p2_n
             if (tau_v > 1e-30) {
                 double Sv = (ev * phi_v + j_v) / (kv * phi_v + alpha_v);
                 double e_tau = exp(-tau_v);
-                Iv = Iv * e_tau + Sv * (1.0 - e_tau);
+                Iv = Iv * e_tau + Sv * emit_factor(tau_v, e_tau);
             }
         }
         double kv = c1 * Bij * (p1_node - (1.0 * g1) * p2_node / (1.0 * g2));
@@ -437,7 +450,7 @@ double orc_ray_in(double temp_node, double vel_node, double p1_node, double p2_n
         double Sv = (ev * phi_v + j_v) / (kv * phi_v + alpha_v);
         double tau_v = (kv * phi_v + alpha_v) * dx * (double)length_list[0];
         double e_tau = exp(-tau_v);
-        double I_out = Iv * e_tau + Sv * (1.0 - e_tau);
+        double I_out = Iv * e_tau + Sv * emit_factor(tau_v, e_tau);
         I_tot = I_tot + phi_v0 * I_out * d_v;
     }
     return I_tot;

@@ -92,6 +92,9 @@ double orc_ray_in(double temp_node, double vel_node, double p1_node, double p2_n
                   double Aji, double Bij, int g1, int g2, double I_in);
 int  orc_inverse_rt(int node, const orc_grid *g, const orc_species *s, const orc_se_params *p,
                     const double *bvec, double *J0, double *J2);
+/* 1: ray_in's emission factor 1 - e^-tau as -expm1(-tau) (no cancellation for thin
+ * cells); 0 (default): as the reference writes it.  Process-wide; set it between calls. */
+void orc_set_thin_exact(int on);
 
 /* statistical equilibrium */
 void orc_t_abs_const(int j, int k, int jl, int kl, double *t0, double *t2);

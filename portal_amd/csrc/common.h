@@ -228,8 +228,23 @@ struct PrepArgs {
     const double *dust, *tdust, *kd, *nu;
     double *KSD;
     float *KSD32;
+    const double *I_in;      // per line background, which sets the unit of S in KS32/KSD32
 };
 hipError_t launch_line_prep(const PrepArgs &a, hipStream_t s);
+
+// The FP32 inward pass carries a line's intensities (source functions, backgrounds,
+// and so J before the FP64 channel sum) in units of 2^e, e = line_iexp(I_in): the
+// power of two at or below the line's background B_nu(2.73 K), but not below 2^-100
+// and not above 1 (W m^-2 Hz^-1 sr^-1).  The backgrounds of lines above ~3 THz lie
+// below FP32's range in those SI units; in units of 2^e they are 1 (or, under the
+// 2^-100 floor, still normal FP32 numbers for any background above 1e-68), while a
+// source function of 1e-9, far above any thermal gas's, stays below 2^72.  A power of
+// two makes the scaling and its undoing exact.
+__host__ __device__ inline int line_iexp(double iin) {
+    if (!(iin > 0.0) || !(iin < 1e300)) return -100;
+    const int e = ilogb(iin);
+    return e < -100 ? -100 : (e > 0 ? 0 : e);
+}
 
 struct RtArgs {
     int n_work, D, R, nvel, C;

@@ -447,6 +447,7 @@ int inward_pass(Ctx &c, const portal_se_params *p, const double *xth, const doub
     const bool mixed = p->precision == PORTAL_MIXED;
     pa.KS = mixed ? nullptr : c.d_KS;
     pa.KS32 = mixed ? c.d_KS32 : nullptr;
+    pa.I_in = s.d_Iin;
     // dust enters the inward pass when some node has dust and some line a dust opacity
     bool dust = false;
     if (c.g.has_dust)
@@ -1881,6 +1882,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         Ctx &c = xs[m]->c;
         PrepArgs pa{c.g.N, c.sp.r_num, c.sp.e_num, c.d_levp, c.g.d_velb, c.sp.d_l1, c.sp.d_l2, c.sp.d_c1B, c.sp.d_c2A,
                     nullptr, c.sp.d_g1, c.sp.d_g2, c.sp.d_cnu_spi, mixed ? nullptr : c.d_KS, mixed ? c.d_KS32 : nullptr};
+        pa.I_in = c.sp.d_Iin;
         if (dusty[m]) {
             pa.dust = c.g.d_dust;
             pa.tdust = c.g.d_tdust;

@@ -58,6 +58,8 @@ struct Arith<double> {
     __device__ static double att(double tau) { return exp(-tau); }
     __device__ static double om(double tau, double et) { return 1.0 - et; }
     __device__ static double thr() { return 1e-30; }
+    // exponent of the intensity unit: FP64 carries intensities in W m^-2 Hz^-1 sr^-1
+    __device__ static int uexp(double) { return 0; }
     __device__ static R2 ks(const RtArgs &a, size_t i) { return *reinterpret_cast<const double2 *>(a.KS + 2 * i); }
     __device__ static R2 ksd(const RtArgs &a, size_t i) { return *reinterpret_cast<const double2 *>(a.KSD + 2 * i); }
     // dust of a segment: (alpha dl, B_nu(T_dust)) in the units of tau_of
@@ -104,6 +106,8 @@ struct Arith<float> {
         return __int_as_float((__float_as_int(series) & m) | (__float_as_int(direct) & ~m));
     }
     __device__ static float thr() { return 1.44269504e-30f; }  // 1e-30 in units of tau2
+    // FP32 carries a line's intensities in units of 2^line_iexp(I_in) (S is staged so)
+    __device__ static int uexp(double iin) { return line_iexp(iin); }
     __device__ static R2 ks(const RtArgs &a, size_t i) { return *reinterpret_cast<const float2 *>(a.KS32 + 2 * i); }
     __device__ static R2 ksd(const RtArgs &a, size_t i) { return *reinterpret_cast<const float2 *>(a.KSD32 + 2 * i); }
     __device__ static R2 stage_dust(R2 d, float dl, bool cell) {
@@ -221,7 +225,8 @@ __device__ __forceinline__ void rt_staged_node(const RtArgs &a, const int bid, c
 #pragma unroll
                 for (int q = 0; q < GROUP; q++) {
                     if (!on[q]) continue;
-                    const Real Iout = Em[q] + Tr[q] * (Real)a.I_in[lg[q]];
+                    const double iin = a.I_in[lg[q]];
+                    const Real Iout = Em[q] + Tr[q] * (Real)ldexp(iin, -A::uexp(iin));
                     const int s = g0 + q;
                     if (A::kahan) {
                         Real y = w * Iout - C0[s];
@@ -274,8 +279,9 @@ __device__ __forceinline__ void rt_staged_node(const RtArgs &a, const int bid, c
     }
     (void)jacc;
     if (threadIdx.x < R) {
-        a.J0[(size_t)node * R + threadIdx.x] = jl0;
-        a.J2[(size_t)node * R + threadIdx.x] = jl2 / 1.4142135623730951;
+        const int ue = A::uexp(a.I_in[threadIdx.x]);  // back from the intensity unit (exact)
+        a.J0[(size_t)node * R + threadIdx.x] = ldexp(jl0, ue);
+        a.J2[(size_t)node * R + threadIdx.x] = ldexp(jl2 / 1.4142135623730951, ue);
     }
 }
 

@@ -186,8 +186,10 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     const R2 k0 = A::ks(a, (size_t)node * R + lo);  // the node's own cell, no skip
     R2 d0;
     if (DUST) d0 = A::ksd(a, (size_t)node * R + lo);
-    const Real iin_bg = (Real)a.I_in[lo];
-    const Real iin_star = a.xthn ? (Real)a.I_star[lo] : iin_bg;
+    // backgrounds in the line's intensity unit, the unit of the staged S
+    const int ue = A::uexp(a.I_in[lo]);
+    const Real iin_bg = (Real)ldexp(a.I_in[lo], -ue);
+    const Real iin_star = a.xthn ? (Real)ldexp(a.I_star[lo], -ue) : iin_bg;
     const int n_star = a.xthn ? a.nstar[bid] : 0;
 #pragma unroll
     for (int q = 0; q < SLOTS; q++) S0[q] = C0[q] = S2[q] = C2[q] = (Real)0;
@@ -399,8 +401,10 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             j2 = j2 + phi0 * d_v * red2[l * C + c];
             if (LAM) lam = lam + phi0 * d_v * red3[l * C + c];
         }
-        a.J0[(size_t)node * R + l] = j0;
-        a.J2[(size_t)node * R + l] = j2 / 1.4142135623730951;
+        // back from the intensity unit (exact); the own-cell operator is per unit of S
+        const int ul = A::uexp(a.I_in[l]);
+        a.J0[(size_t)node * R + l] = ldexp(j0, ul);
+        a.J2[(size_t)node * R + l] = ldexp(j2 / 1.4142135623730951, ul);
         if (LAM) a.LAM[(size_t)node * R + l] = lam;
     }
 }

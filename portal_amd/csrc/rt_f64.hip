@@ -43,9 +43,11 @@ __global__ void __launch_bounds__(256) line_prep_kernel(PrepArgs a) {
         a.KS[2 * id] = K;
         a.KS[2 * id + 1] = S;
     }
+    // FP32: S in the line's intensity unit 2^line_iexp(I_in)
+    const int ue = a.KS32 || a.KSD32 ? line_iexp(a.I_in[l]) : 0;
     if (a.KS32) {
         a.KS32[2 * id] = (float)K;
-        a.KS32[2 * id + 1] = (float)S;
+        a.KS32[2 * id + 1] = (float)ldexp(S, -ue);
     }
     if (a.KSD || a.KSD32) {
         // dust: alpha = kd dust_p, j = alpha B_nu(T_dust) (inverse_rt.f90:292-297,
@@ -59,7 +61,7 @@ __global__ void __launch_bounds__(256) line_prep_kernel(PrepArgs a) {
         }
         if (a.KSD32) {
             a.KSD32[2 * id] = (float)alpha;
-            a.KSD32[2 * id + 1] = (float)Bb;
+            a.KSD32[2 * id + 1] = (float)ldexp(Bb, -ue);
         }
     }
 }

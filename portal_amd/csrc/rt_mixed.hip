@@ -6,7 +6,8 @@
 // below tau = 1e-2 so thin cells keep full relative accuracy, and the per-pair sums
 // over rays are Kahan-compensated FP32.  The line coefficients are formed in FP64
 // (line_prep_kernel) and rounded once, because the reference's constants (h nu/c,
-// h nu/4pi) underflow FP32.  Geometry (the stored paths) is the FP64 walk's.
+// h nu/4pi) underflow FP32; S and the background are carried in the line's
+// intensity unit 2^line_iexp(I_in).  Geometry (the stored paths) is the FP64 walk's.
 // The channel sum and the sum over waves are done in FP64.
 #include "common.h"
 #include "rt_body.h"
@@ -78,7 +79,8 @@ __global__ void __launch_bounds__(256) rt_mixed_kernel(RtArgs a) {
                             Tr = Tr * et;
                         }
                     }
-                    const float Iout = Em + Tr * (float)a.I_in[l];
+                    const double iin = a.I_in[l];
+                    const float Iout = Em + Tr * (float)ldexp(iin, -line_iexp(iin));
                     // Kahan-compensated sums over rays
                     float y = w * Iout - C0s[s];
                     float t = S0s[s] + y;
@@ -124,8 +126,10 @@ __global__ void __launch_bounds__(256) rt_mixed_kernel(RtArgs a) {
         __syncthreads();
     }
     if (threadIdx.x < R) {
-        a.J0[(size_t)node * R + threadIdx.x] = jacc[threadIdx.x];
-        a.J2[(size_t)node * R + threadIdx.x] = jacc[64 + threadIdx.x] / 1.4142135623730951;
+        // back from the line's intensity unit (exact)
+        const int ue = line_iexp(a.I_in[threadIdx.x]);
+        a.J0[(size_t)node * R + threadIdx.x] = ldexp(jacc[threadIdx.x], ue);
+        a.J2[(size_t)node * R + threadIdx.x] = ldexp(jacc[64 + threadIdx.x] / 1.4142135623730951, ue);
     }
 }
 

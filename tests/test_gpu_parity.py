@@ -678,3 +678,39 @@ def test_image_depth_overflow(portal1, cfg1, oracle_cfg1):
         portal1.image_pass(pp, driver.image_lines(sp, [1], [2]), 0.3, 0.0, 2, driver.channel_velocities(2, 100.0),
                            4, 0.7, cloud.DX_IMAGE, n_depth_rt=3)
     assert e.value.code == -4  # PORTAL_ERR_DEPTH_RT
+
+
+@pytest.mark.parametrize("precision,tol", [(PORTAL_FP64, 1e-10), (PORTAL_MIXED, 2e-5)])
+def test_forty_line_species_at_129_channels(oracle, precision, tol):
+    """LAMDA's CO size: 41 levels, 40 lines, 129 channels.  The cooperative kernel needs 8
+    waves (12 slots) for 5,160 (line, channel) pairs; the autotuner chooses among the
+    variants that take them.  J of 16 seeded nodes against the oracle, and the SE pass
+    (81 unknowns at kmax = 2).  The lines above ~3 THz have backgrounds below FP32's
+    range (the mixed path's per-line intensity unit keeps them), and lines in the Wien
+    tail of the background take thin cells' emission, which the reference's FP64
+    1 - e^-tau drops; the mixed path is checked against the oracle's exact form there."""
+    sp = cloud.co_like_rotor(41, kmax=2)
+    g = cloud.make_cloud(sp, 1500, seed=71)
+    xth = cloud.legendre_all(4, 8)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    pt.set_levpop(g.lev_p)
+    J0, J2, st = pt.inverse_rt(xth, 64, cloud.DX_INWARD, precision=precision)
+    rng = np.random.default_rng(72)
+    sub = np.sort(rng.choice(solved(g), 16, replace=False)) + 1
+    J0o, J2o = oracle.inverse_nodes(g, sp, xth, 64, cloud.DX_INWARD, sub, nthreads=NTH,
+                                    thin_exact=precision == PORTAL_MIXED)
+    print(f"40 lines x 129 channels: variant {st['rt_variant']}/{st['rt_waves']}, "
+          f"max rel J0 err {np.abs(J0[:, sub - 1] / J0o - 1).max():.2e}")
+    np.testing.assert_allclose(J0[:, sub - 1], J0o, rtol=tol, atol=0)
+    for k in range(16):
+        assert np.abs(J2[:, sub[k] - 1] - J2o[:, k]).max() <= tol * np.abs(J0o[:, k]).max()
+    if precision == PORTAL_FP64:
+        pt.set_levpop(g.lev_p)
+        pp, _ = pt.se_pass(xth, 64, cloud.DX_INWARD)
+        lo = int(sub[0])
+        ref = oracle.se_pass(g, sp, xth, 64, cloud.DX_INWARD, node_lo=lo, node_hi=lo + 5, nthreads=NTH)[0]
+        idx = np.array([i for i in range(lo - 1, lo + 5) if g.ab_all[i] > 0])
+        assert_pops_close(pp, ref, sp, idx, 1e-10)
+    pt.close()
