@@ -176,6 +176,16 @@ class Oracle:
             raise RuntimeError(f"oracle inverse_nodes failed ({rc})")
         return J0, J2
 
+    def polarize_prop(self, eI, eQ, pI, pQ, ds, I, Q, U, thin_exact=False):
+        """One Stokes segment (raytrace.f90:1-56) -> (I, Q, U)."""
+        a = [C.c_double(v) for v in (I, Q, U)]
+        self.lib.orc_set_thin_exact(1 if thin_exact else 0)
+        try:
+            self.lib.orc_polarize_prop(*[C.c_double(v) for v in (eI, eQ, pI, pQ, ds)], *[C.byref(v) for v in a])
+        finally:
+            self.lib.orc_set_thin_exact(0)
+        return tuple(v.value for v in a)
+
     def ray_in(self, temp_node, vel_node, p1_node, p2_node, dens_1, dens_2, temp_list, vel_list, length_list,
                nvel, dx, nu0, Aji, Bij, g1, g2, I_in, thin_exact=False):
         """ray_in (inverse_rt.f90:217-344) without dust over one gathered path: index 0
@@ -273,8 +283,10 @@ class Oracle:
 
     def image_pass(self, g, sp, pol_pops, lines, th_inc, ph_az, nvel, vel_out, im_pix, im_size, dx,
                    alpha=0.15, n_depth_rt=4096, pix_lo=1, pix_hi=None, nthreads=0, bvec_all=None,
-                   with_dust=False, star=None, node_pixels=False):
+                   with_dust=False, star=None, node_pixels=False, thin_exact=False):
         """One inclination of the image pass.  lines = list of (l1, l2, nu0, Aji, kd).
+        thin_exact: polarize_prop without the cancellation for thin cells (the
+        mixed-precision path's definition) instead of as the reference writes it.
         star = dict(R_star=, T_star=, X_star=) (unit-sphere units, X_star before rotation)
         puts the star behind the pixels it covers (raytrace.f90:103-114).  node_pixels: one
         pixel per bulk node at its rotated (x, y), the reference's pix = 0 mode."""
@@ -295,14 +307,18 @@ class Oracle:
         pp = _f64(pol_pops)
         vo = _f64(vel_out)
         bv = _f64(g.bvec_all if bvec_all is None else bvec_all)
-        rc = self.lib.orc_image_pass(C.byref(G), C.byref(S), nl, _i(l1), _i(l2), _d(nu), _d(A), _d(kd),
-                                     C.c_double(th_inc), C.c_double(ph_az), _d(bv), _d(pp), nvel, _d(vo), im_pix,
-                                     C.c_double(im_size), C.c_double(dx), C.c_double(alpha), n_depth_rt, pix_lo,
-                                     pix_hi, 1 if node_pixels else 0, 1 if star else 0,
-                                     C.c_double(star["R_star"] if star else 0.0),
-                                     C.c_double(star["T_star"] if star else 0.0),
-                                     _d(_f64(star["X_star"])) if star else None,
-                                     *[_d(o) for o in outs], _d(dust) if with_dust else None, nthreads)
+        self.lib.orc_set_thin_exact(1 if thin_exact else 0)
+        try:
+            rc = self.lib.orc_image_pass(C.byref(G), C.byref(S), nl, _i(l1), _i(l2), _d(nu), _d(A), _d(kd),
+                                         C.c_double(th_inc), C.c_double(ph_az), _d(bv), _d(pp), nvel, _d(vo), im_pix,
+                                         C.c_double(im_size), C.c_double(dx), C.c_double(alpha), n_depth_rt, pix_lo,
+                                         pix_hi, 1 if node_pixels else 0, 1 if star else 0,
+                                         C.c_double(star["R_star"] if star else 0.0),
+                                         C.c_double(star["T_star"] if star else 0.0),
+                                         _d(_f64(star["X_star"])) if star else None,
+                                         *[_d(o) for o in outs], _d(dust) if with_dust else None, nthreads)
+        finally:
+            self.lib.orc_set_thin_exact(0)
         if rc:
             raise RuntimeError(f"oracle image_pass failed ({rc})")
         if with_dust:

@@ -19,6 +19,8 @@
  *   ray_in with thin_exact     the closed form of a uniform ray, S + (I_in - S) e^-tau,
  *                              at tau down to 1e-22 with S / I_in up to 1e25; equal to
  *                              the reference form (<=1e-12) where no cell is thin
+ *   polarize_prop, thin_exact  the reference's own formula evaluated in 60-digit
+ *                              arithmetic (mpmath) at t from 1e-22 to 3, q/t to 0.9
  *   make_ac                    compiled reference (_ref)
  *   t_xxx_const, r_xxx_const   exact rational re-evaluation (sympy) + selection rules
  *   setup_stateq / get_rhovec  an independent rate-equation model in the magnetic-
@@ -1286,10 +1288,26 @@ void orc_comp_emcoefs(double r1_0, double r2_0, double r1_2, double r2_2, int j1
     *esQ = es_Q;
 }
 
-/* polarize_prop, raytrace.f90:1-56 */
+/* (1 - e^-x)/x without cancellation, 1 at x = 0 */
+static double phi_exact(double x) { return x == 0.0 ? 1.0 : -expm1(-x) / x; }
+
+/* polarize_prop, raytrace.f90:1-56.  With orc_set_thin_exact(1) the same propagator
+ * is evaluated in the basis I +- Q, where it is diagonal: o11 +- o12 =
+ * (1 - e^-(t +- q))/(eta_I +- eta_Q) and the attenuation e^-t (ch q I -+ sh q Q)
+ * splits into e^-(t +- q)(I +- Q); 1 - e^-x as -expm1(-x).  The reference's form
+ * loses a thin cell's emission to the cancellation in 1 - (ch q + ...) e^-t. */
 void orc_polarize_prop(double eta_I, double eta_Q, double eps_I, double eps_Q, double ds, double *I, double *Q,
                        double *U) {
     double I0 = *I, Q0 = *Q;
+    if (thin_exact) {
+        const double m = (eta_I + eta_Q) * ds, p = (eta_I - eta_Q) * ds;
+        const double X = (I0 + Q0) * exp(-m) + (eps_I + eps_Q) * ds * phi_exact(m);
+        const double Y = (I0 - Q0) * exp(-p) + (eps_I - eps_Q) * ds * phi_exact(p);
+        *I = 0.5 * (X + Y);
+        *Q = 0.5 * (X - Y);
+        *U = *U * exp(-(eta_I * ds));
+        return;
+    }
     double shQ = sinh(eta_Q * ds);
     double chQ = cosh(eta_Q * ds);
     double e_tau = exp(-(eta_I * ds));

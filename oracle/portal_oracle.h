@@ -92,8 +92,9 @@ double orc_ray_in(double temp_node, double vel_node, double p1_node, double p2_n
                   double Aji, double Bij, int g1, int g2, double I_in);
 int  orc_inverse_rt(int node, const orc_grid *g, const orc_species *s, const orc_se_params *p,
                     const double *bvec, double *J0, double *J2);
-/* 1: ray_in's emission factor 1 - e^-tau as -expm1(-tau) (no cancellation for thin
- * cells); 0 (default): as the reference writes it.  Process-wide; set it between calls. */
+/* 1: ray_in's emission factor 1 - e^-tau as -expm1(-tau), and polarize_prop in the
+ * basis I +- Q with -expm1 (no cancellation for thin cells); 0 (default): both as the
+ * reference writes them.  Process-wide; set it between calls. */
 void orc_set_thin_exact(int on);
 
 /* statistical equilibrium */

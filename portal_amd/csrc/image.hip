@@ -369,8 +369,11 @@ __device__ __forceinline__ float g_f(float t, float et) {
 // where s = (sinh q / q - 1)/q^2, c = (cosh q - 1)/q^2 and g(t) = (1 - e^-t (1 + t))/t^2,
 // each by its series where small arguments would cancel.  Near total polarisation
 // (|r| >= 0.999) the transported I + Q and I - Q are used instead.  Intensities are
-// carried in units of the line's background intensity I_in, so that emissivities of weak
-// lines stay in the FP32 normal range; the optional dust branch stays FP64.
+// carried in units of u = max(I_in, 2^-100), the pixel's background intensity, so that
+// emissivities of weak lines stay in the FP32 normal range; the floor keeps the unit
+// finite for lines whose background lies below FP32's range (above ~3 THz), and the
+// coefficients are scaled in FP64 before they are rounded.  The optional dust branch
+// stays FP64.
 __global__ void __launch_bounds__(256) stokes_f32_kernel(ImgArgs a) {
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)a.n_rt * a.nlines * a.C;
@@ -385,13 +388,15 @@ __global__ void __launch_bounds__(256) stokes_f32_kernel(ImgArgs a) {
     const double nu0 = a.nu0[j];
     const double I_cmb = a.Iin[j];
     const double I_in = a.start[pix].w != 0.0 ? a.Istar[j] : I_cmb;
-    // eta scale c/nu (float literal as in raytrace.f90), eps additionally / I_in
-    const float cnu = (float)((double)2.99792E8f / nu0);
-    const float cnu_e = (float)((double)2.99792E8f / nu0 / I_in);
+    // eta scale c/nu (float literal as in raytrace.f90), eps additionally / u
+    const double u = fmax(I_in, 0x1p-100);
+    const double cnu = (double)2.99792E8f / nu0;
+    const double cnu_e = cnu / u;
     const float vel_obs = (float)a.vel_out[k];
     const float rspi = 0.56418958f;  // 1/sqrt(pi)
     const double kd = a.kd[j], alpha = a.alpha;
-    float I = 1.0f, Q_in = 0.0f, U_in = 0.0f, Q_out = 0.0f, U_out = 0.0f, tau = 0.0f;
+    const float I0 = (float)(I_in / u);  // 1 unless the floor applies
+    float I = I0, Q_in = 0.0f, U_in = 0.0f, Q_out = 0.0f, U_out = 0.0f, tau = 0.0f;
     double I_dust = I_in, sig0 = 0.0, sig2 = 0.0, Qd = 0.0, Ud = 0.0, tau_dust_tot = 0.0;
     const double dxl = a.dl_scale;
     for (int i = 0; i < nc - 1; i++) {
@@ -404,10 +409,10 @@ __global__ void __launch_bounds__(256) stokes_f32_kernel(ImgArgs a) {
         Q_out = fmaf(Q_in, c2a, U_in * s2a);
         U_out = fmaf(U_in, c2a, -(Q_in * s2a));
         const double4 pc = a.polco[(size_t)c * a.nlines + j];
-        const float eta_I = cnu * v_corr * (float)pc.x;
-        const float eta_Q = cnu * v_corr * (float)pc.y;
-        const float eps_I = cnu_e * v_corr * (float)pc.z;
-        const float eps_Q = cnu_e * v_corr * (float)pc.w;
+        const float eta_I = v_corr * (float)(cnu * pc.x);
+        const float eta_Q = v_corr * (float)(cnu * pc.y);
+        const float eps_I = v_corr * (float)(cnu_e * pc.z);
+        const float eps_Q = v_corr * (float)(cnu_e * pc.w);
         const double delta_x = dxl * (double)(steps[i + 1] - steps[i]);
         const float ds = (float)delta_x;
         if (ds * eta_I > 1e-30f) {
@@ -462,10 +467,10 @@ __global__ void __launch_bounds__(256) stokes_f32_kernel(ImgArgs a) {
         U_in = fmaf(U_out, c2a, Q_out * s2a);
     }
     const size_t o = (size_t)pix + (size_t)a.n_rt * ((size_t)j + (size_t)a.nlines * k);
-    // I_out - B_nu(2.73 K) with I_out = I I_in; the second term is 0 off the star
-    a.I[o] = (double)(I - 1.0f) * I_in + (I_in - I_cmb);
-    a.Q[o] = (double)Q_in * I_in;
-    a.U[o] = (double)U_in * I_in;
+    // I_out - B_nu(2.73 K) with I_out = I u; the second term is 0 off the star
+    a.I[o] = (double)(I - I0) * u + (I_in - I_cmb);
+    a.Q[o] = (double)Q_in * u;
+    a.U[o] = (double)U_in * u;
     a.TAU[o] = (double)tau;
     if (a.dust_out) {
         I_dust = I_dust - I_cmb;

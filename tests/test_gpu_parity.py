@@ -714,3 +714,46 @@ def test_forty_line_species_at_129_channels(oracle, precision, tol):
         idx = np.array([i for i in range(lo - 1, lo + 5) if g.ab_all[i] > 0])
         assert_pops_close(pp, ref, sp, idx, 1e-10)
     pt.close()
+
+
+def test_forty_line_species_image(oracle):
+    """Images of lines from 115 GHz to 3.5 THz of the 40-line species.  FP64 follows the
+    reference's propagator; on lines 25 and 30 (2.9 and 3.5 THz at 20 K) that form drops
+    the thin cells' emission, which exceeds the 2.73 K background there, and the mixed
+    path's cancellation-free form keeps it: the mixed image is checked against the oracle's
+    exact form on every line, FP64 against the reference's form where the two agree."""
+    from portal_amd import driver
+    sp = cloud.co_like_rotor(41, kmax=2)
+    g = cloud.make_cloud(sp, 1500, seed=71)
+    L = sp.e_num
+    pp = np.zeros((g.node_num, 2 * L), order="F")
+    for i in range(L):
+        pp[:, 2 * i] = g.lev_p[i] / np.sqrt(sp.lev_g[i])
+        if i:
+            pp[:, 2 * i + 1] = 0.01 * pp[:, 2 * i]
+    lines = driver.image_lines(sp, [1, 17, 21, 23, 25, 30], [2, 18, 22, 24, 26, 31], nu0_overwrite=False)
+    vel = driver.channel_velocities(8, 150.0)
+    args = (pp, lines, 0.6, 0.2, 8, vel, 10, 0.7, cloud.DX_IMAGE)
+    pt = Portal(0)
+    pt.upload_grid(g)
+    pt.upload_species(sp)
+    (I, Q, U, T), _ = pt.image_pass(*args, n_depth_rt=2048)
+    (Im, Qm, Um, Tm), _ = pt.image_pass(*args, n_depth_rt=2048, precision=PORTAL_MIXED)
+    pt.close()
+    Io, Qo, Uo, To = oracle.image_pass(g, sp, *args, n_depth_rt=2048, nthreads=NTH)
+    It, Qt, Ut, _ = oracle.image_pass(g, sp, *args, n_depth_rt=2048, nthreads=NTH, thin_exact=True)
+    assert np.isfinite(Im).all() and np.isfinite(Qm).all() and np.isfinite(Um).all()
+    for j in range(len(lines)):
+        sI, sQ, sU = (np.abs(x[:, j]).max() for x in (It, Qt, Ut))
+        e = [np.abs(Im[:, j] - It[:, j]).max() / sI, np.abs(Qm[:, j] - Qt[:, j]).max() / sQ,
+             np.abs(Um[:, j] - Ut[:, j]).max() / sU]
+        print(f"line {j} ({lines[j][2] / 1e9:.0f} GHz): max |I| {sI:.2e}; mixed against the exact form "
+              f"{e[0]:.1e} {e[1]:.1e} {e[2]:.1e} (each on its own scale); "
+              f"reference form off by {np.abs(Io[:, j] - It[:, j]).max() / sI:.1e}")
+        assert sI > 0 and sQ > 0
+        assert max(e) < 2e-5
+        if j < 3:
+            np.testing.assert_allclose(I[:, j], Io[:, j], rtol=1e-10, atol=1e-10 * sI)
+            np.testing.assert_allclose(Q[:, j], Qo[:, j], rtol=0, atol=1e-10 * np.abs(Qo[:, j]).max())
+    np.testing.assert_allclose(Tm, T, rtol=1e-5, atol=1e-6 * np.abs(T).max())
+    np.testing.assert_allclose(T, To, rtol=1e-10, atol=1e-12)

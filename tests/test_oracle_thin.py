@@ -78,3 +78,38 @@ def test_forms_part_only_in_the_wien_tail_of_the_background(oracle):
     rel = np.abs(thin / ref - 1).max(axis=1)
     assert rel[:11].max() < 1e-12
     assert rel[16:].max() > 1e-2
+
+
+def _prop_mp(eI, eQ, pI, pQ, ds, I, Q, U):
+    """polarize_prop's own formula (raytrace.f90:1-56) in 60-digit arithmetic."""
+    import mpmath as mp
+    mp.mp.dps = 60
+    eI, eQ, pI, pQ, ds, I, Q, U = (mp.mpf(x) for x in (eI, eQ, pI, pQ, ds, I, Q, U))
+    shQ, chQ, et = mp.sinh(eQ * ds), mp.cosh(eQ * ds), mp.exp(-(eI * ds))
+    fac = 1 / (eI * eI - eQ * eQ)
+    o11 = fac * eI * (1 - (chQ + (eQ / eI) * shQ) * et)
+    o12 = -(fac * eQ * (1 - (chQ + (eI / eQ) * shQ) * et))
+    return (float(o11 * pI + o12 * pQ + et * (chQ * I - shQ * Q)),
+            float(o12 * pI + o11 * pQ + et * (-(shQ * I) + chQ * Q)), float(U * et))
+
+
+@pytest.mark.parametrize("t", [1e-22, 1e-16, 1e-12, 1e-6, 1e-2, 0.5, 3.0])
+@pytest.mark.parametrize("r", [1e-3, 0.3, -0.5, 0.9])
+def test_thin_exact_propagator_is_the_formula_evaluated_exactly(oracle, t, r):
+    ds = 2.0e14
+    eI = t / ds
+    eQ = r * eI
+    S = 1e25  # source function in units of the incoming I: thin cells' emission matters
+    pI, pQ = S * eI, 0.2 * S * eQ
+    I, Q, U = 1.0, 0.01, 0.02
+    want = _prop_mp(eI, eQ, pI, pQ, ds, I, Q, U)
+    got = oracle.polarize_prop(eI, eQ, pI, pQ, ds, I, Q, U, thin_exact=True)
+    assert got[0] == pytest.approx(want[0], rel=1e-12)
+    assert abs(got[1] - want[1]) <= 1e-12 * abs(want[0])
+    assert got[2] == pytest.approx(want[2], rel=1e-15)
+    if t >= 0.5:  # thick: the reference's form is exact to rounding too
+        ref = oracle.polarize_prop(eI, eQ, pI, pQ, ds, I, Q, U)
+        assert ref[0] == pytest.approx(want[0], rel=1e-12)
+    if t == 1e-22:  # the reference's form drops the emission (S t = 1e3 here)
+        ref = oracle.polarize_prop(eI, eQ, pI, pQ, ds, I, Q, U)
+        assert abs(ref[0] - want[0]) > 0.5 * abs(want[0])
