@@ -130,7 +130,11 @@ typedef struct {
     int fallbacks;           /* nodes where the surrogate iterate was rejected */
     int lu_failures;
     double max_rel_change;   /* last iteration's population change */
-    int rt_variant;          /* inward-integration kernel chosen by the autotuner */
+    int rt_variant;          /* inward-integration kernel chosen by the autotuner: 0 global,
+                                1 staged, 2 cooperative, 3 cooperative with each ray's cells
+                                back to front; in a batch, 1 the staged persistent form, 2 the
+                                cooperative persistent form, 3 the cooperative form with one
+                                workgroup per item */
     int rt_waves;            /* its waves per workgroup */
     int max_path;            /* longest inward path (cells; image pass: longest pixel ray) */
     int ng_steps;            /* Ng extrapolations applied by the lambda iteration */
@@ -142,6 +146,8 @@ typedef struct {
                                 OR of the classes' wave counts (1, 2, 4) */
     double t_mlp_ms;         /* surrogate network inference in the surrogate step (input
                                 normalisation and the BF16 GEMMs), 0 without it */
+    int rt_order;            /* 1: the inward integration took each ray's cells back to front
+                                (as ray_in does), 0: front to back, leaving exhausted rays */
 } portal_stats;
 
 /* Context -------------------------------------------------------------------- */

@@ -208,9 +208,15 @@ class _Core:
         self._check(self.lib.portal_debug_walk_chunks(self.ctx, C.c_double(scale)))
 
     def force_rt_kernel(self, precision, variant, waves=4):
-        """Test hook: fix the inward-integration kernel (0 global, 1 staged, 2 cooperative)
-        and its waves per workgroup for one precision; variant -1 returns to autotuning."""
+        """Test hook: fix the inward-integration kernel (0 global, 1 staged, 2 cooperative,
+        3 cooperative with each ray's cells back to front) and its waves per workgroup for
+        one precision; variant -1 returns to autotuning."""
         self._check(self.lib.portal_debug_force_rt(self.ctx, int(precision), int(variant), int(waves)))
+
+    def force_batch_rt(self, form=-1):
+        """Test hook: the cooperative form of a batch this context leads (-1 timed; bit 0 one
+        workgroup per item, else persistent workgroups; bit 1 rays back to front)."""
+        self._check(self.lib.portal_debug_force_batch_rt(self.ctx, int(form)))
 
     def surrogate_perturb(self, stride, factor=1e3):
         """Test hook: every stride-th work item's surrogate prediction gets its first level's

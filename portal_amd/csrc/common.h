@@ -90,6 +90,7 @@ struct Grid {
     std::vector<double> XYZ, vel, temp, b_all, h2, ab, dust;
     std::vector<int> nbr_ptr, nbr_idx;  // CSR, 0-based ids
     bool has_dust = false;
+    double b_ratio = 0;  // largest over smallest positive finite Doppler width
     // device
     double4 *d_pos = nullptr;    // x, y, z, 0
     double4 *d_velb = nullptr;   // vx, vy, vz, b
@@ -105,6 +106,9 @@ struct Grid {
     int *d_work = nullptr;
     bool valid = false;
 };
+
+// autotuner candidates of the inward integration whose timings a context keeps
+constexpr int RT_TUNE_SLOTS = 12;
 
 struct Ctx {
     int device = 0;
@@ -135,7 +139,7 @@ struct Ctx {
     // autotuned inward-integration launch (per precision): variant, waves per block
     int tuned_key[2] = {0, 0};
     int tuned_variant[2] = {0, 0}, tuned_waves[2] = {4, 4};
-    double tune_ms[2][8] = {{0}};
+    double tune_ms[2][RT_TUNE_SLOTS] = {{0}};
     int forced_variant[2] = {-1, -1}, forced_waves[2] = {4, 4};  // test hook: bypass the autotuner
     double walk_chunk_scale = 1.1;  // chunk store of the single-pass walk / sampled estimate
     int ng_first = 3, ng_gap = 3;   // Ng schedule: first iteration, least gap between extrapolations
@@ -149,6 +153,9 @@ struct Ctx {
     // test hook for a batch led by this context: SE form (-1 autotuned, 0 one persistent
     // launch, 1 a launch per class) and the waves per system of the classes (0 autotuned)
     int se_forced_form = -1, se_forced_waves = 0;
+    // test hook: the cooperative form of a batch this context leads (-1 timed; bit 0 one
+    // workgroup per item, bit 1 rays back to front)
+    int batch_rt_form = -1;
     // d_pol holds the SE solution from which the current d_levp was formed (set by the
     // lambda iteration; cleared by anything that changes the state otherwise), so that
     // a lambda iteration continued by a further call measures its first change exactly
@@ -276,6 +283,7 @@ struct RtArgs {
     // of w phi0 d_v (1 - e^-tau_own) (R x N), or null
     double *LAM;
     int unit;               // cooperative kernel: cells per staged unit (0: COOP_UNIT)
+    double b_ratio;         // largest over smallest positive Doppler width of the grid (0: unknown)
 };
 hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s);
 hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s);
@@ -357,10 +365,13 @@ size_t se_slab_doubles(int c_dim, int n_trans, bool all_global = false);
 constexpr int SE_SLAB_WAVES = 4;  // waves per system of a single model's slab-form solve
 constexpr int SE_MAX_UNKNOWNS = 512;  // largest SE system (c_dim) the solver takes
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves);
-// the cooperative form of the persistent integration, one launch per (slot count, dust)
+// the cooperative form of the persistent integration, one launch per (slot count, dust);
+// btf: each ray's cells back to front; rec: the FP32 profile recurrence, which needs
+// coop_profile_rec_host for every model of the launch
 hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,
-                                     int cap, int slots, int waves, size_t lds, int n_blocks, bool dust,
-                                     hipStream_t s);
+                                     int cap, int slots, int waves, size_t lds, int n_blocks, bool dust, bool btf,
+                                     bool rec, hipStream_t s);
+bool coop_profile_rec_host(double b_ratio, int nvel, int slots);
 size_t rt_coop_persistent_lds(bool mixed, int cap, int R_max, int C, bool dust);
 int coop_slots_host(int R, int C, int W);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);

@@ -367,10 +367,12 @@ int check_rt_args(Ctx &c, const RtArgs &a, bool mixed) {
 // Autotuner of the inward-integration launch: on the first pass of a geometry the
 // candidate (kernel variant, waves per workgroup) pairs are timed on a slice of the
 // work list and the fastest is kept for every later pass with the same shape.
-// inward-integration launch shapes the autotuner times: (variant, waves per workgroup)
-constexpr int N_RT_CANDS = 7;
+// inward-integration launch shapes the autotuner times: (variant, waves per workgroup);
+// variant 3 is the cooperative kernel taking the cells of a ray back to front
+constexpr int N_RT_CANDS = 10;
 constexpr struct RtCand { int variant, waves; } RT_CANDS[N_RT_CANDS] = {{0, 4}, {1, 4}, {1, 2}, {2, 2}, {2, 4},
-                                                                      {2, 5}, {2, 8}};
+                                                                      {2, 5}, {2, 8}, {3, 4}, {3, 5}, {3, 8}};
+static_assert(N_RT_CANDS <= RT_TUNE_SLOTS, "tune_ms holds every candidate");
 
 // the autotuner's key: the paths and the launch's shape
 int rt_tune_key(const Ctx &c, const RtArgs &ra) {
@@ -400,6 +402,12 @@ int autotune_rt(Ctx &c, RtArgs ra, bool mixed) {
     double best = 1e30;
     int bv = 0, bw = 4;
     for (int k = 0; k < NC; k++) {
+        // FP64 keeps each ray front to back: its candidates then give the same bits, so the
+        // timed choice cannot change an FP64 result (back to front stays a forced option)
+        if (!mixed && cands[k].variant == 3) {
+            c.tune_ms[pi][k] = -1.0;
+            continue;
+        }
         ra.variant = cands[k].variant;
         ra.waves = cands[k].waves;
         Timer t(c.stream);
@@ -481,6 +489,7 @@ int inward_pass(Ctx &c, const portal_se_params *p, const double *xth, const doub
     ra.work = c.g.d_work;
     ra.velb = c.g.d_velb;
     ra.inv_b = c.g.d_inv_b;
+    ra.b_ratio = c.g.b_ratio;
     ra.KS = c.d_KS;
     ra.KS32 = c.d_KS32;
     ra.xth = xth;
@@ -512,14 +521,14 @@ int inward_pass(Ctx &c, const portal_se_params *p, const double *xth, const doub
     if (c.ali) {
         // the own-cell operator comes from the cooperative kernel: its fastest timed shape
         ra.LAM = c.d_LAM;
-        if (ra.variant != 2) {
+        if (ra.variant != 2 && ra.variant != 3) {
             const int pi = mixed ? 1 : 0;
             int best = -1;
             for (int k = 0; k < N_RT_CANDS; k++)
-                if (RT_CANDS[k].variant == 2 && c.tune_ms[pi][k] > 0 &&
+                if ((RT_CANDS[k].variant == 2 || RT_CANDS[k].variant == 3) && c.tune_ms[pi][k] > 0 &&
                     (best < 0 || c.tune_ms[pi][k] < c.tune_ms[pi][best]))
                     best = k;
-            ra.variant = 2;
+            ra.variant = best < 0 ? 2 : RT_CANDS[best].variant;
             ra.waves = best < 0 ? 4 : RT_CANDS[best].waves;
         }
     }
@@ -539,6 +548,7 @@ int inward_pass(Ctx &c, const portal_se_params *p, const double *xth, const doub
         st->n_paths = (long long)c.paths.n_work * c.paths.D;
         st->rt_variant = ra.variant;
         st->rt_waves = ra.waves;
+        st->rt_order = ra.variant == 3 ? 1 : 0;
         st->max_path = c.paths.max_nc;
     }
     return PORTAL_OK;
@@ -827,13 +837,20 @@ int portal_upload_grid(portal_ctx *x, int N, int N_b, const double *XYZ, const d
     }
     std::vector<double4> pos(N), velb(N);
     std::vector<double> invb(N), tdust(N), tkin(N);
+    double b_lo = 0.0, b_hi = 0.0;
     for (int i = 0; i < N; i++) {
         pos[i] = make_double4(XYZ[3 * (size_t)i], XYZ[3 * (size_t)i + 1], XYZ[3 * (size_t)i + 2], 0.0);
         velb[i] = make_double4(vel[3 * (size_t)i], vel[3 * (size_t)i + 1], vel[3 * (size_t)i + 2], b_all[i]);
         invb[i] = 1.0 / b_all[i];
         tkin[i] = temp[2 * (size_t)i];
         tdust[i] = temp[2 * (size_t)i + 1];
+        const double b = b_all[i];
+        if (b > 0.0 && std::isfinite(b)) {
+            b_lo = b_lo > 0.0 ? std::min(b_lo, b) : b;
+            b_hi = std::max(b_hi, b);
+        }
     }
+    g.b_ratio = b_lo > 0.0 ? b_hi / b_lo : 0.0;
     // work list: nodes 1..bulk_node_num with ab_all > 0, ordered along a Morton curve
     std::vector<std::pair<uint64_t, int>> key;
     for (int i = 0; i < N_b; i++)
@@ -1203,12 +1220,13 @@ static int surrogate_step(Ctx &c, int *n_fallback, int *d_item_flags, double *t_
 }
 
 // the autotuner's timings of the inward-integration candidates (variant/waves: global 4,
-// staged 4, staged 2, cooperative 2, 4, 5, 8; one unused slot) for one precision; -1 for
-// a candidate that could not run
-int portal_debug_tune_ms(const portal_ctx *x, int precision, double *out8) {
-    if (!x || !out8 || precision < 0 || precision > 1) return PORTAL_ERR_ARG;
-    for (int k = 0; k < 8; k++) out8[k] = x->c.tune_ms[precision][k];
-    return PORTAL_OK;
+// staged 4, staged 2, cooperative 2, 4, 5, 8, cooperative back to front 4, 5, 8) for one
+// precision, -1 for a candidate that could not run; out holds n doubles, returns the
+// number of candidates
+int portal_debug_tune_ms(const portal_ctx *x, int precision, double *out, int n) {
+    if (!x || !out || n < 0 || precision < 0 || precision > 1) return PORTAL_ERR_ARG;
+    for (int k = 0; k < n && k < N_RT_CANDS; k++) out[k] = x->c.tune_ms[precision][k];
+    return N_RT_CANDS;
 }
 
 // Test hooks of the surrogate's residual fallback: every stride-th work item's
@@ -1266,13 +1284,23 @@ int portal_debug_coop_unit(portal_ctx *x, int cells) {
 }
 
 // Test hook: run the inward integration of one precision with a fixed kernel variant
-// (0 global, 1 staged, 2 cooperative) and wave count; variant -1 returns to autotuning.
+// (0 global, 1 staged, 2 cooperative, 3 cooperative back to front) and wave count;
+// variant -1 returns to autotuning.
 int portal_debug_force_rt(portal_ctx *x, int precision, int variant, int waves) {
-    if (!x || precision < 0 || precision > 1 || variant < -1 || variant > 2 || waves < 1 || waves > 8)
+    if (!x || precision < 0 || precision > 1 || variant < -1 || variant > 3 || waves < 1 || waves > 8)
         return PORTAL_ERR_ARG;
     x->c.forced_variant[precision] = variant;
     x->c.forced_waves[precision] = waves;
     x->c.tuned_key[precision] = 0;
+    return PORTAL_OK;
+}
+
+// Test hook: the cooperative form of the inward integration in a batch this context
+// leads: -1 timed, else bit 0 one workgroup per item (else persistent workgroups), bit 1
+// each ray's cells back to front.
+int portal_debug_force_batch_rt(portal_ctx *x, int form) {
+    if (!x || form < -1 || form > 3) return PORTAL_ERR_ARG;
+    x->c.batch_rt_form = form;
     return PORTAL_OK;
 }
 
@@ -1916,6 +1944,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         ra.work = c.g.d_work;
         ra.velb = c.g.d_velb;
         ra.inv_b = c.g.d_inv_b;
+        ra.b_ratio = c.g.b_ratio;
         ra.KS = c.d_KS;
         ra.KS32 = c.d_KS32;
         ra.offsets = c.paths.offsets;
@@ -2025,6 +2054,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     struct CoopClass {
         int slots = 0, cap = 8, R_max = 1, C_max = 1;
         bool dust = false;
+        bool rec = true;  // every model of the class allows the profile recurrence
         std::vector<int2> items;
         size_t lds = 0;
         int2 *d_items = nullptr;
@@ -2048,6 +2078,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
                 classes.back().dust = dusty[m] != 0;
             }
             CoopClass &k = classes[cls_of[m]];
+            k.rec = k.rec && coop_profile_rec_host(rts[m].b_ratio, rts[m].nvel, sl);
             k.cap = std::max(k.cap, std::min(rts[m].max_nc, COOP_UNIT));
             k.R_max = std::max(k.R_max, rts[m].R);
             k.C_max = std::max(k.C_max, rts[m].C);
@@ -2141,9 +2172,9 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     HIP_TRY(&c0, hipMemset(c0.d_flags, 0, sizeof(int) * 4));
     int n_cu = 256;
     hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c0.device);
-    // the cooperative form: persistent workgroups or one workgroup per item, timed on the
-    // largest class's first items
-    bool coop_flat = false;
+    // the cooperative form: persistent workgroups or one workgroup per item, each with the
+    // rays front to back or back to front, timed on the largest class's first items
+    bool coop_flat = false, coop_btf = false;
     if (coop) {
         size_t kb = 0;
         for (size_t k = 1; k < classes.size(); k++)
@@ -2154,20 +2185,26 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
             Ctx &c = xs[m]->c;
             HIP_TRY(&c0, launch_line_prep(prep_of(m), s));
         }
-        double ms_form[2] = {0, 0};
-        for (int f = 0; f < 2; f++) {
+        // f: bit 0 the flat form, bit 1 back to front (mixed precision only, as for a single
+        // model: the FP64 forms give the same bits)
+        double ms_form[4] = {0, 0, 0, 0};
+        int fb = c0.batch_rt_form >= 0 ? c0.batch_rt_form : 0;
+        for (int f = 0; f < (mixed ? 4 : 2) && c0.batch_rt_form < 0; f++) {
             HIP_TRY(&c0, launch_fill_zero(d_cnt, sizeof(int) * n_cnt, s));
             const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / std::max<size_t>(cl.lds, 1))));
             Timer t(s);
             hipError_t e = launch_rt_coop_persistent(mixed, d_rt, cl.d_items, ns, d_cnt + 2, cl.cap, cl.slots,
-                                                     COOP_W, cl.lds, f == 0 ? n_cu * bpc : 0, cl.dust, s);
+                                                     COOP_W, cl.lds, (f & 1) ? 0 : n_cu * bpc, cl.dust,
+                                                     (f & 2) != 0, cl.rec, s);
             ms_form[f] = t.stop();
             if (e != hipSuccess) {
                 (void)hipGetLastError();
                 ms_form[f] = 1e30;
             }
+            if (ms_form[f] < ms_form[fb]) fb = f;
         }
-        coop_flat = ms_form[1] < ms_form[0];
+        coop_flat = (fb & 1) != 0;
+        coop_btf = (fb & 2) != 0;
     }
     // autotune the staged persistent integration's waves per workgroup on a slice of the
     // items (when the cooperative form serves the batch, it runs with COOP_W waves)
@@ -2328,7 +2365,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
             const CoopClass &cl = classes[k];
             const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / std::max<size_t>(cl.lds, 1))));
             launch_rt_coop_persistent(mixed, d_rt, cl.d_items, (int)cl.items.size(), d_cnt + 2 + k, cl.cap, cl.slots,
-                                      COOP_W, cl.lds, coop_flat ? 0 : n_cu * bpc, cl.dust, s);
+                                      COOP_W, cl.lds, coop_flat ? 0 : n_cu * bpc, cl.dust, coop_btf, cl.rec, s);
         }
     } else {
         launch_rt_persistent(mixed, d_rt, d_ri, n_items, d_cnt, cap, R_max, best_w, rt_blocks, s);
@@ -2536,6 +2573,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         total->n_cross = ncross;
         total->rt_waves = best_w;
         total->rt_variant = coop ? (coop_flat ? 3 : 2) : 1;  // 3: the cooperative form, one workgroup per item
+        total->rt_order = coop_btf ? 1 : 0;
         total->ng_steps = n_ng;
         total->se_form = se_form;
         total->se_waves = se_waves_or;

@@ -62,6 +62,61 @@ __device__ __forceinline__ SegF fold_seg(float2 k, float dl, bool cell) {
     return g;
 }
 
+// I <- I e + x in I's own register.  Written as fmaf(I, e, x), the compiler's FMAC form
+// ties the result to x's register, and the loop over cells then copies every slot's
+// intensity back into place (six v_mov per five-slot cell).  e comes from v_exp_f32 at
+// least two dependent instructions earlier (x is formed from it), so no trans-forwarding
+// wait state falls inside the asm.
+__device__ __forceinline__ void bt_update(float &I, float e, float x) {
+    asm("v_fma_f32 %0, %0, %1, %2" : "+v"(I) : "v"(e), "v"(x));
+}
+
+// Per cell record of the cooperative kernel: v.dir/b and 1/b in the profile's units, and
+// for the FP32 folded record the terms of the profile recurrence over a lane's
+// consecutive channels.  With delta = (2 b_node / nvel) / b_cell the step of the profile
+// argument u from one channel to the next, tau2(u + delta) = tau2(u) rho(u) with
+// rho(u) = 2^(-delta (2 u + delta)), and rho(u + delta) = rho(u) cc, cc = 2^(-2 delta^2);
+// nd = -delta.
+template <typename Real>
+struct __attribute__((aligned(16))) CoopCell {
+    Real vb, inv_b, nd, cc;
+};
+// per staged unit, the ray's first path entry (the node's own cell): its v.dir and
+// segment length
+template <typename Real>
+struct __attribute__((aligned(16))) CoopHdr {
+    Real vel_node, dl0;
+};
+
+// The profile terms tau2_q = 2^(L - u_q^2) of a lane's SLOTS consecutive channels,
+// u_q = u_0 + q delta.  REC: one exponential and the recurrence of CoopCell (two products
+// per further slot; the exponentials, not the products, bound the update), else one
+// exponential per slot.  The recurrence is taken only where (SLOTS - 1) delta <= 1 for
+// every cell of the grid (coop_profile_rec).  Then a slot whose term the recurrence
+// loses because tau2_0 fell below FP32's range holds less than 2^-100 for line-centre
+// optical depths below 1e12 per cell, and the bound on rho's exponent acts only where
+// |u_0| > 32 / delta >= 32, so that every slot's term is zero in FP32 anyway.
+template <int SLOTS, bool REC>
+__device__ __forceinline__ void fold_profile(const float (&v0s)[SLOTS], const CoopCell<float> &c, float L,
+                                             float (&t2)[SLOTS]) {
+    if constexpr (REC && SLOTS > 1) {
+        const float u0 = fmaf(v0s[0], c.inv_b, -c.vb);
+        t2[0] = __builtin_amdgcn_exp2f(fmaf(-u0, u0, L));
+        float rho = __builtin_amdgcn_exp2f(fminf(c.nd * fmaf(2.0f, u0, -c.nd), 64.0f));
+#pragma unroll
+        for (int q = 1; q < SLOTS; q++) {
+            t2[q] = t2[q - 1] * rho;
+            if (q + 1 < SLOTS) rho = rho * c.cc;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < SLOTS; q++) {
+            const float uu = fmaf(v0s[q], c.inv_b, -c.vb);
+            t2[q] = __builtin_amdgcn_exp2f(fmaf(-uu, uu, L));
+        }
+    }
+}
+
 // the staged record of a (cell, line): SegF for FP32 without dust, else (L, S) of Arith
 template <typename Real, bool DUST>
 struct CoopSeg {
@@ -79,11 +134,12 @@ __host__ __device__ constexpr size_t coop_seg_bytes(bool dust) {
     return dust || !std::is_same<Real, float>::value ? sizeof(typename Arith<Real>::R2) : sizeof(SegF);
 }
 
-// one staging buffer: cap cell records, cap x R per-line records, and with dust cap x R
-// (alpha dl, B_nu(T_dust))
+// one staging buffer: the unit's header, cap cell records, cap x R per-line records, and
+// with dust cap x R (alpha dl, B_nu(T_dust))
 template <typename Real>
 __host__ __device__ constexpr size_t coop_buf_bytes(int cap, int R, bool dust) {
-    return ((size_t)cap * (sizeof(CellRec<Real>) + (size_t)R * coop_seg_bytes<Real>(dust) +
+    return (sizeof(CoopHdr<Real>) +
+            (size_t)cap * (sizeof(CoopCell<Real>) + (size_t)R * coop_seg_bytes<Real>(dust) +
                            (dust ? (size_t)R * sizeof(typename Arith<Real>::R2) : 0)) +
             15) &
            ~(size_t)15;
@@ -95,8 +151,8 @@ __host__ __device__ constexpr size_t coop_buf_bytes(int cap, int R, bool dust) {
 // FP32 record (24 bytes per line) reaches that bound at the configurations' line counts.
 template <typename Real>
 __host__ __device__ constexpr int coop_unit_cells(int R, bool dust) {
-    const size_t per_wg = (160 * 1024) / 7;
-    const size_t per_cell = 2 * (sizeof(CellRec<Real>) + (size_t)R * coop_seg_bytes<Real>(dust) +
+    const size_t per_wg = (160 * 1024) / 7 - 2 * sizeof(CoopHdr<Real>);
+    const size_t per_cell = 2 * (sizeof(CoopCell<Real>) + (size_t)R * coop_seg_bytes<Real>(dust) +
                                  (dust ? (size_t)R * sizeof(typename Arith<Real>::R2) : 0));
     const int fit = (int)(per_wg / per_cell) & ~7;
     return fit >= COOP_UNIT ? COOP_UNIT : (fit < 8 ? 8 : fit);
@@ -112,21 +168,29 @@ size_t coop_lds_bytes(int cap, int R, int npairs, bool dust, bool lam = true) {
 }
 
 // All threads of the workgroup stage cells c0 .. c0+cnt-1 of ray r of work item bid
-// into buf (cnt <= cap).
+// into buf (cnt <= cap); dv is the channel spacing 2 b_node / nvel.
 template <typename Real, bool DUST>
 __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const int r, const int c0, const int cnt,
-                                           const int cap, char *buf) {
+                                           const int cap, const Real dv, char *buf) {
     typedef Arith<Real> A;
     typedef typename A::R2 R2;
     typedef typename CoopSeg<Real, DUST>::T Seg;
     const int R = a.R;
     const long long off = a.offsets[(long long)bid * a.D + r];
     const PathEntry *P = a.paths + off + c0;
-    CellRec<Real> *cr = reinterpret_cast<CellRec<Real> *>(buf);
+    CoopHdr<Real> *hdr = reinterpret_cast<CoopHdr<Real> *>(buf);
+    CoopCell<Real> *cr = reinterpret_cast<CoopCell<Real> *>(hdr + 1);
     Seg *ks = reinterpret_cast<Seg *>(cr + cap);
     R2 *kds = reinterpret_cast<R2 *>(ks + (size_t)cap * R);
     const Real dls = (Real)a.dl_scale;
     const int n = cnt * R;
+    if (threadIdx.x == 0) {
+        const PathEntry e0 = a.paths[off];
+        CoopHdr<Real> h;
+        h.vel_node = (Real)e0.vray;
+        h.dl0 = (Real)e0.len * dls;
+        *hdr = h;
+    }
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const int i = j / R, l = j - i * R;
         const PathEntry e = P[i];
@@ -140,11 +204,16 @@ __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const
             ks[j] = A::stage(A::ks(a, (size_t)e.cell * R + l), dl, ok);
         if (DUST) kds[j] = A::stage_dust(A::ksd(a, (size_t)e.cell * R + l), dl, ok);
         if (l == 0) {
-            CellRec<Real> c;
+            CoopCell<Real> c;
             c.vb = ok ? vb : (Real)0;
             c.inv_b = ok ? sib : (Real)0;
-            c.dl = dl;
-            c.pad = (Real)e.vray;
+            c.nd = (Real)0;
+            c.cc = (Real)1;
+            if constexpr (CoopSeg<Real, DUST>::fold) {
+                const float nd = -(dv * c.inv_b);
+                c.nd = nd;
+                c.cc = __builtin_amdgcn_exp2f(-2.0f * (nd * nd));
+            }
             cr[i] = c;
         }
     }
@@ -157,7 +226,13 @@ __device__ __forceinline__ unsigned nonzero_bits(double x) {
     return (unsigned)(b >> 32) | (unsigned)b;
 }
 
-template <typename Real, int SLOTS, bool DUST, bool LAM>
+// BTF: the cells of a ray are taken back to front, from the far end to the node, as the
+// reference's ray_in does (I <- I e^-tau + S (1 - e^-tau)); the units of a ray are then
+// staged last to first.  The update needs one product fewer than front to back (no
+// transmission to carry) but cannot leave an exhausted ray early.
+// REC: the FP32 folded update takes the profile terms of a lane's slots by recurrence
+// (fold_profile).
+template <typename Real, int SLOTS, bool DUST, bool LAM, bool BTF = false, bool REC = false>
 __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, const int cap, char *smem) {
     typedef Arith<Real> A;
     typedef typename A::R2 R2;
@@ -171,6 +246,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     const Real b = (Real)bd;
     const Real inv_bn = (Real)(1.0 / bd);
     const int C = a.C, nvel = a.nvel, R = a.R, D = a.D;
+    const Real dv = ((Real)2 * b) / (Real)nvel;  // channel spacing
     const int npairs = R * C;
     const size_t bb = coop_buf_bytes<Real>(cap, R, DUST);
 
@@ -206,38 +282,67 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     auto stage_unit = [&](int r, int u, char *buf) {
         const int nc = ray_cells(r);
         const int c0 = u * cap;
-        coop_stage<Real, DUST>(a, bid, r, c0, min(cap, nc - c0), cap, buf);
+        coop_stage<Real, DUST>(a, bid, r, c0, min(cap, nc - c0), cap, dv, buf);
     };
-    // per-ray state, carried across the units of a ray
+    // per-ray state, carried across the units of a ray.  Front to back: Em the emission
+    // reaching the node so far and Tr the transmission to the current cell.  Back to
+    // front: Em the intensity entering the current cell (Tr unused).
     Real Tr[SLOTS], Em[SLOTS], v0s[SLOTS];
     Real w = (Real)0, wc = (Real)0, iin = (Real)0;
     bool ray_done = false;
-    stage_unit(0, 0, smem);
+    auto units = [&](int r) { return max(1, (ray_cells(r) + cap - 1) / cap); };
+    stage_unit(0, BTF ? units(0) - 1 : 0, smem);
     __syncthreads();
-    int r = 0, u = 0, half = 0;
+    int r = 0, u = BTF ? units(0) - 1 : 0, half = 0;
     for (;;) {
         const char *cur = smem + half * bb;
         const int nc = ray_cells(r);
         const int nu = max(1, (nc + cap - 1) / cap);
-        int rn = r, un = u + 1;
-        if (un >= nu) {
-            rn = r + 1;
-            un = 0;
+        int rn = r, un;
+        if constexpr (BTF) {
+            un = u - 1;
+            if (un < 0) {
+                rn = r + 1;
+                un = rn < D ? units(rn) - 1 : 0;
+            }
+        } else {
+            un = u + 1;
+            if (un >= nu) {
+                rn = r + 1;
+                un = 0;
+            }
         }
         if (rn < D) stage_unit(rn, un, smem + (half ^ 1) * bb);
-        const CellRec<Real> *cr = reinterpret_cast<const CellRec<Real> *>(cur);
+        const CoopHdr<Real> *hdr = reinterpret_cast<const CoopHdr<Real> *>(cur);
+        const CoopCell<Real> *cr = reinterpret_cast<const CoopCell<Real> *>(hdr + 1);
         const Seg *ks = reinterpret_cast<const Seg *>(cr + cap);
         const R2 *kds = reinterpret_cast<const R2 *>(ks + (size_t)cap * R);
         const int c0 = u * cap, cnt = min(cap, nc - c0);
-        if (u == 0) {
+        if (BTF && u == nu - 1) {
+            // the ray's first unit back to front: its direction, background and channels
+            const double *xr = a.xthn ? a.xthn + 3 * ((size_t)bid * D + r) : a.xth + 3 * r;
+            const Real mu = (Real)xr[0];
+            w = (Real)xr[2];
+            iin = r < n_star ? iin_star : iin_bg;
+            wc = w * (((Real)3 * (mu * mu) - (Real)1) / (Real)2);
+            const Real vel_node = hdr->vel_node;
+#pragma unroll
+            for (int q = 0; q < SLOTS; q++) {
+                const int nv = on(q) ? cb + q - nvel : 0;
+                const Real v0 = ((Real)2 * b) * (Real)nv / (Real)nvel;
+                v0s[q] = v0 + vel_node;
+                Em[q] = iin;
+            }
+        }
+        if (!BTF && u == 0) {
             // star mode: the node's own directions; rays towards the star start with B_nu(T_star)
             const double *xr = a.xthn ? a.xthn + 3 * ((size_t)bid * D + r) : a.xth + 3 * r;
             const Real mu = (Real)xr[0];
             w = (Real)xr[2];
             iin = r < n_star ? iin_star : iin_bg;
             wc = w * (((Real)3 * (mu * mu) - (Real)1) / (Real)2);
-            const Real vel_node = cr[0].pad;
-            const Real dl0 = cr[0].dl;
+            const Real vel_node = hdr->vel_node;
+            const Real dl0 = hdr->dl0;
             ray_done = false;
 #pragma unroll
             for (int q = 0; q < SLOTS; q++) {
@@ -266,7 +371,62 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                 if (!on(q)) Tr[q] = (Real)0;
             }
         }
-        if (!ray_done) {
+        if constexpr (BTF) {
+            // cells c0 + cnt - 1 down to max(c0, 1) (local index j = i - c0); the next
+            // cell's staged data are read one cell ahead, within the unit
+            const int jb = c0 == 0 ? 1 : 0;
+            const char *kl = reinterpret_cast<const char *>(ks + lo);
+            const int kstride = R * (int)sizeof(Seg);
+            const int jt = cnt - 1 >= jb ? cnt - 1 : 0;
+            CoopCell<Real> cn = cr[jt];
+            Seg kn = *reinterpret_cast<const Seg *>(kl + jt * kstride);
+            R2 dn;
+            if (DUST) dn = kds[(size_t)jt * R + lo];
+            for (int j = cnt - 1; j >= jb; j--) {
+                const CoopCell<Real> c = cn;
+                const Seg k = kn;
+                R2 d;
+                if (DUST) d = dn;
+                const int jn = j - 1 >= jb ? j - 1 : j;
+                cn = cr[jn];
+                kn = *reinterpret_cast<const Seg *>(kl + jn * kstride);
+                if (DUST) dn = kds[(size_t)jn * R + lo];
+                if constexpr (FOLD) {
+                    // every slot's profile term, then its attenuation, then the updates:
+                    // independent exponentials in a row rather than a dependent pair per slot
+                    float t2v[SLOTS], etv[SLOTS];
+                    fold_profile<SLOTS, REC>(v0s, c, k.L, t2v);
+#pragma unroll
+                    for (int q = 0; q < SLOTS; q++) etv[q] = __builtin_amdgcn_exp2f(-t2v[q]);
+#pragma unroll
+                    for (int q = 0; q < SLOTS; q++) {
+                        const float tau2 = t2v[q], et = etv[q];
+                        const float cub = tau2 * fmaf(-tau2, fmaf(-tau2, k.s2, k.s1), k.s0);
+                        const float dir = fmaf(-k.S, et, k.Sc);
+                        bt_update(Em[q], et, __builtin_fminf(cub, dir));
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < SLOTS; q++) {
+                        const Real uu = fma(v0s[q], c.inv_b, -c.vb);
+                        Real tau = A::tau_of(k.x, uu), S = k.y;
+                        if (DUST) {
+                            const Real tl = tau;
+                            tau = tl + d.x;
+                            S = (k.y * tl + d.y * d.x) * A::rcp(tau);
+                            if (!A::skip_test && !(tau > (Real)0)) S = (Real)0;
+                        }
+                        const Real et = A::att(tau);
+                        // the reference's Iv * e_tau + Sv * (1 - e_tau) (inverse_rt.f90:314)
+                        const Real iv = Em[q] * et + S * A::om(tau, et);
+                        if constexpr (A::skip_test)
+                            Em[q] = tau > A::thr() ? iv : Em[q];
+                        else
+                            Em[q] = iv;
+                    }
+                }
+            }
+        } else if (!ray_done) {
             // cells max(c0, 1) .. c0 + cnt - 1 of the ray (local index j = i - c0); the
             // next cell's staged data are read one cell ahead, within the unit
             const int jb = c0 == 0 ? 1 : 0;
@@ -274,7 +434,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             // offset on one address register
             const char *kl = reinterpret_cast<const char *>(ks + lo);
             const int kstride = R * (int)sizeof(Seg);
-            CellRec<Real> cn = cr[jb < cnt ? jb : 0];
+            CoopCell<Real> cn = cr[jb < cnt ? jb : 0];
             Seg kn = *reinterpret_cast<const Seg *>(kl + (jb < cnt ? jb : 0) * kstride);
             R2 dn;
             if (DUST) dn = kds[(size_t)(jb < cnt ? jb : 0) * R + lo];
@@ -302,7 +462,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                         break;
                     }
                 }
-                const CellRec<Real> c = cn;
+                const CoopCell<Real> c = cn;
                 const Seg k = kn;
                 R2 d;
                 if (DUST) d = dn;
@@ -311,10 +471,11 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                 kn = *reinterpret_cast<const Seg *>(kl + jn * kstride);
                 if (DUST) dn = kds[(size_t)jn * R + lo];
                 if constexpr (FOLD) {
+                    float t2v[SLOTS];
+                    fold_profile<SLOTS, REC>(v0s, c, k.L, t2v);
 #pragma unroll
                     for (int q = 0; q < SLOTS; q++) {
-                        const float uu = fmaf(v0s[q], c.inv_b, -c.vb);
-                        const float tau2 = __builtin_amdgcn_exp2f(fmaf(-uu, uu, k.L));
+                        const float tau2 = t2v[q];
                         const float et = __builtin_amdgcn_exp2f(-tau2);
                         const float cub = tau2 * fmaf(-tau2, fmaf(-tau2, k.s2, k.s1), k.s0);
                         const float dir = fmaf(-k.S, et, k.Sc);
@@ -349,7 +510,32 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                 }
             }
         }
-        if (u == nu - 1) {
+        if (BTF ? u == 0 : u == nu - 1) {
+            if constexpr (BTF) {
+                // the node's own cell last, with no skip (inverse_rt.f90:318-337)
+                const Real dl0 = hdr->dl0;
+#pragma unroll
+                for (int q = 0; q < SLOTS; q++) {
+                    const int nv = on(q) ? cb + q - nvel : 0;
+                    const Real v0 = ((Real)2 * b) * (Real)nv / (Real)nvel;
+                    const Real aa = v0 * inv_bn;
+                    const Real e0 = A::ex(-(aa * aa));
+                    Real tau0, Sc;
+                    if (DUST) {
+                        const Real kp = k0.x * e0;
+                        tau0 = (kp + d0.x) * dl0;
+                        Sc = (k0.y * kp + d0.y * d0.x) / (kp + d0.x);
+                    } else {
+                        tau0 = k0.x * e0 * dl0;
+                        Sc = k0.y;
+                    }
+                    const Real t0 = A::ex(-tau0);
+                    const Real om0 = A::one_minus(tau0, t0);
+                    Em[q] = Em[q] * t0 + Sc * om0;
+                    if (LAM) Lw[q] = Lw[q] + w * om0;
+                    Tr[q] = (Real)0;
+                }
+            }
 #pragma unroll
             for (int q = 0; q < SLOTS; q++) {
                 const Real Iout = Em[q] + Tr[q] * iin;
@@ -409,10 +595,10 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     }
 }
 
-template <typename Real, int SLOTS, bool DUST, bool LAM>
+template <typename Real, int SLOTS, bool DUST, bool LAM, bool BTF, bool REC>
 __global__ void __launch_bounds__(512) rt_coop_kernel(RtArgs a, int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    rt_coop_node<Real, SLOTS, DUST, LAM>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+    rt_coop_node<Real, SLOTS, DUST, LAM, BTF, REC>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
 }
 
 // The FP32 forms with up to 6 slots are compiled for 7 waves per SIMD (at most 72
@@ -420,19 +606,44 @@ __global__ void __launch_bounds__(512) rt_coop_kernel(RtArgs a, int cap) {
 // which more resident waves hide: 723 ms per production pass against 781 ms at the 6
 // waves the kernel's natural 80 VGPRs give (one box, same inputs; 12 dwords spill).
 // With more slots, or in the persistent form, the budget would spill far more.
-template <int SLOTS, bool DUST, bool LAM>
+template <int SLOTS, bool DUST, bool LAM, bool BTF, bool REC>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8)))
 rt_coop_kernel_f32(RtArgs a, int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    rt_coop_node<float, SLOTS, DUST, LAM>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+    rt_coop_node<float, SLOTS, DUST, LAM, BTF, REC>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
 }
 
+// The profile recurrence (fold_profile) is taken where (SLOTS - 1) delta <= 1 for every
+// cell: delta = (2 b_node / nvel) sqrt(log2 e) / b_cell <= 2 b_ratio sqrt(log2 e) / nvel,
+// b_ratio the grid's largest over its smallest positive Doppler width (0: unknown).
+inline bool coop_profile_rec(double b_ratio, int nvel, int slots) {
+    return slots > 1 && b_ratio > 0.0 && nvel > 0 &&
+           (slots - 1) * 2.0 * b_ratio * 1.2011224087864498 / nvel <= 1.0;
+}
+
+// variant 3 of RtArgs takes the cells of a ray back to front
 template <typename Real, int SLOTS, bool DUST, bool LAM>
 void launch_coop_k(dim3 g, dim3 b, size_t lds, hipStream_t s, const RtArgs &a, int cap) {
-    if constexpr (std::is_same<Real, float>::value && SLOTS <= 6)
-        hipLaunchKernelGGL((rt_coop_kernel_f32<SLOTS, DUST, LAM>), g, b, lds, s, a, cap);
-    else
-        hipLaunchKernelGGL((rt_coop_kernel<Real, SLOTS, DUST, LAM>), g, b, lds, s, a, cap);
+    auto go = [&](auto btf_c, auto rec_c) {
+        constexpr bool BTF = decltype(btf_c)::value;
+        constexpr bool REC = decltype(rec_c)::value && CoopSeg<Real, DUST>::fold && SLOTS > 1;
+        if constexpr (std::is_same<Real, float>::value && SLOTS <= 6)
+            hipLaunchKernelGGL((rt_coop_kernel_f32<SLOTS, DUST, LAM, BTF, REC>), g, b, lds, s, a, cap);
+        else
+            hipLaunchKernelGGL((rt_coop_kernel<Real, SLOTS, DUST, LAM, BTF, REC>), g, b, lds, s, a, cap);
+    };
+    const bool rec = coop_profile_rec(a.b_ratio, a.nvel, SLOTS);
+    if (a.variant == 3) {
+        if (rec)
+            go(std::true_type{}, std::true_type{});
+        else
+            go(std::true_type{}, std::false_type{});
+    } else {
+        if (rec)
+            go(std::false_type{}, std::true_type{});
+        else
+            go(std::false_type{}, std::false_type{});
+    }
 }
 
 template <typename Real, int SLOTS>
@@ -453,7 +664,7 @@ hipError_t launch_coop_s(const RtArgs &a, int cap, size_t lds, hipStream_t s) {
 // and channels need the same slot count: workgroups take items from a counter until the
 // list is exhausted (every workgroup reaches the exit test after each item).  DUST: the
 // class's models carry dust (every model of a class agrees).
-template <typename Real, int SLOTS, bool DUST>
+template <typename Real, int SLOTS, bool DUST, bool BTF, bool REC>
 __device__ __forceinline__ void coop_persistent_body(const RtArgs *__restrict__ models,
                                                      const int2 *__restrict__ items, int n_items, int *counter,
                                                      int cap) {
@@ -469,16 +680,16 @@ __device__ __forceinline__ void coop_persistent_body(const RtArgs *__restrict__ 
         const int2 mi = items[it];
         const int m = __builtin_amdgcn_readfirstlane(mi.x), w = __builtin_amdgcn_readfirstlane(mi.y);
         const RtArgs am = models[m];
-        rt_coop_node<Real, SLOTS, DUST, false>(am, w, cap, smem);
+        rt_coop_node<Real, SLOTS, DUST, false, BTF, REC>(am, w, cap, smem);
         __syncthreads();
     }
 }
 
-template <typename Real, int SLOTS, bool DUST>
+template <typename Real, int SLOTS, bool DUST, bool BTF, bool REC>
 __global__ void __launch_bounds__(512) rt_coop_persistent_kernel(const RtArgs *__restrict__ models,
                                                                  const int2 *__restrict__ items, int n_items,
                                                                  int *counter, int cap) {
-    coop_persistent_body<Real, SLOTS, DUST>(models, items, n_items, counter, cap);
+    coop_persistent_body<Real, SLOTS, DUST, BTF, REC>(models, items, n_items, counter, cap);
 }
 
 // Slots per lane: the fewest consecutive channels per lane with which every line's
@@ -498,72 +709,76 @@ inline int coop_slots(int R, int C, int W) {
 // Flat form over the same items: one workgroup per item, in the items' (cost) order,
 // the hardware's dispatch doing the balancing.  It needs no loop state, so it keeps the
 // register budget of rt_coop_kernel.
-template <typename Real, int SLOTS, bool DUST>
+template <typename Real, int SLOTS, bool DUST, bool BTF, bool REC>
 __device__ __forceinline__ void coop_flat_body(const RtArgs *__restrict__ models, const int2 *__restrict__ items,
                                                int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int2 mi = items[blockIdx.x];
     const int m = __builtin_amdgcn_readfirstlane(mi.x), w = __builtin_amdgcn_readfirstlane(mi.y);
-    rt_coop_node<Real, SLOTS, DUST, false>(models[m], w, cap, smem);
+    rt_coop_node<Real, SLOTS, DUST, false, BTF, REC>(models[m], w, cap, smem);
 }
 
-template <typename Real, int SLOTS, bool DUST>
+template <typename Real, int SLOTS, bool DUST, bool BTF, bool REC>
 __global__ void __launch_bounds__(512) rt_coop_flat_kernel(const RtArgs *__restrict__ models,
                                                            const int2 *__restrict__ items, int cap) {
-    coop_flat_body<Real, SLOTS, DUST>(models, items, cap);
+    coop_flat_body<Real, SLOTS, DUST, BTF, REC>(models, items, cap);
 }
 
-template <int SLOTS, bool DUST>
+template <int SLOTS, bool DUST, bool BTF, bool REC>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8)))
 rt_coop_flat_kernel_f32(const RtArgs *__restrict__ models, const int2 *__restrict__ items, int cap) {
-    coop_flat_body<float, SLOTS, DUST>(models, items, cap);
+    coop_flat_body<float, SLOTS, DUST, BTF, REC>(models, items, cap);
 }
 
-// n_blocks > 0: the persistent form on n_blocks workgroups; n_blocks == 0: the flat form
+// n_blocks > 0: the persistent form on n_blocks workgroups; n_blocks == 0: the flat form.
+// btf: rays back to front; rec: the profile recurrence (coop_profile_rec for every model
+// of the launch).
 template <typename Real, bool DUST>
 hipError_t launch_coop_persistent_d(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
-                                    int slots, int waves, size_t lds, int n_blocks, hipStream_t s) {
+                                    int slots, int waves, size_t lds, int n_blocks, bool btf, bool rec,
+                                    hipStream_t s) {
     if (lds > 160 * 1024 || waves < 1 || waves > 8) return hipErrorInvalidValue;
     if (n_items <= 0) return hipSuccess;
-    if (n_blocks == 0) {
-        const dim3 g(n_items), b(64 * waves);
-        auto flat = [&](auto slots_c) {
-            constexpr int S = decltype(slots_c)::value;
-            if constexpr (std::is_same<Real, float>::value && S <= 6)
-                hipLaunchKernelGGL((rt_coop_flat_kernel_f32<S, DUST>), g, b, lds, s, models, items, cap);
-            else
-                hipLaunchKernelGGL((rt_coop_flat_kernel<Real, S, DUST>), g, b, lds, s, models, items, cap);
-        };
-        switch (slots) {
-            case 1: flat(std::integral_constant<int, 1>{}); break;
-            case 2: flat(std::integral_constant<int, 2>{}); break;
-            case 3: flat(std::integral_constant<int, 3>{}); break;
-            case 4: flat(std::integral_constant<int, 4>{}); break;
-            case 5: flat(std::integral_constant<int, 5>{}); break;
-            case 6: flat(std::integral_constant<int, 6>{}); break;
-            case 8: flat(std::integral_constant<int, 8>{}); break;
-            case 10: flat(std::integral_constant<int, 10>{}); break;
-            case 12: flat(std::integral_constant<int, 12>{}); break;
-            default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
-    }
-    const dim3 g(n_blocks), b(64 * waves);
-    auto go = [&](auto slots_c) {
+    auto launch = [&](auto slots_c, auto btf_c, auto rec_c) {
         constexpr int S = decltype(slots_c)::value;
-        hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, S, DUST>), g, b, lds, s, models, items, n_items, counter,
-                           cap);
+        constexpr bool BTF = decltype(btf_c)::value;
+        constexpr bool REC = decltype(rec_c)::value && CoopSeg<Real, DUST>::fold && S > 1;
+        if (n_blocks == 0) {
+            const dim3 g(n_items), b(64 * waves);
+            if constexpr (std::is_same<Real, float>::value && S <= 6)
+                hipLaunchKernelGGL((rt_coop_flat_kernel_f32<S, DUST, BTF, REC>), g, b, lds, s, models, items, cap);
+            else
+                hipLaunchKernelGGL((rt_coop_flat_kernel<Real, S, DUST, BTF, REC>), g, b, lds, s, models, items,
+                                   cap);
+        } else {
+            const dim3 g(n_blocks), b(64 * waves);
+            hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, S, DUST, BTF, REC>), g, b, lds, s, models, items,
+                               n_items, counter, cap);
+        }
+    };
+    auto by_order = [&](auto slots_c) {
+        if (btf) {
+            if (rec)
+                launch(slots_c, std::true_type{}, std::true_type{});
+            else
+                launch(slots_c, std::true_type{}, std::false_type{});
+        } else {
+            if (rec)
+                launch(slots_c, std::false_type{}, std::true_type{});
+            else
+                launch(slots_c, std::false_type{}, std::false_type{});
+        }
     };
     switch (slots) {
-        case 1: go(std::integral_constant<int, 1>{}); break;
-        case 2: go(std::integral_constant<int, 2>{}); break;
-        case 3: go(std::integral_constant<int, 3>{}); break;
-        case 4: go(std::integral_constant<int, 4>{}); break;
-        case 5: go(std::integral_constant<int, 5>{}); break;
-        case 6: go(std::integral_constant<int, 6>{}); break;
-        case 8: go(std::integral_constant<int, 8>{}); break;
-        case 10: go(std::integral_constant<int, 10>{}); break;
-        case 12: go(std::integral_constant<int, 12>{}); break;
+        case 1: by_order(std::integral_constant<int, 1>{}); break;
+        case 2: by_order(std::integral_constant<int, 2>{}); break;
+        case 3: by_order(std::integral_constant<int, 3>{}); break;
+        case 4: by_order(std::integral_constant<int, 4>{}); break;
+        case 5: by_order(std::integral_constant<int, 5>{}); break;
+        case 6: by_order(std::integral_constant<int, 6>{}); break;
+        case 8: by_order(std::integral_constant<int, 8>{}); break;
+        case 10: by_order(std::integral_constant<int, 10>{}); break;
+        case 12: by_order(std::integral_constant<int, 12>{}); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -572,11 +787,12 @@ hipError_t launch_coop_persistent_d(const RtArgs *models, const int2 *items, int
 // n_blocks > 0: the persistent form on n_blocks workgroups; n_blocks == 0: the flat form
 template <typename Real>
 hipError_t launch_coop_persistent(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
-                                  int slots, int waves, size_t lds, int n_blocks, bool dust, hipStream_t s) {
+                                  int slots, int waves, size_t lds, int n_blocks, bool dust, bool btf, bool rec,
+                                  hipStream_t s) {
     return dust ? launch_coop_persistent_d<Real, true>(models, items, n_items, counter, cap, slots, waves, lds,
-                                                       n_blocks, s)
+                                                       n_blocks, btf, rec, s)
                 : launch_coop_persistent_d<Real, false>(models, items, n_items, counter, cap, slots, waves, lds,
-                                                        n_blocks, s);
+                                                        n_blocks, btf, rec, s);
 }
 
 template <typename Real>

@@ -172,9 +172,9 @@ hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
     if (a.R > 64) return hipErrorInvalidValue;
     // dust, star and the own-cell operator: cooperative kernel only
-    if ((a.dust || a.xthn || a.LAM) && a.variant != 2) return hipErrorInvalidValue;
+    if ((a.dust || a.xthn || a.LAM) && a.variant != 2 && a.variant != 3) return hipErrorInvalidValue;
     if (a.variant == 1) return launch_staged<double>(a, a.max_nc, s);
-    if (a.variant == 2) return launch_coop<double>(a, a.max_nc, s);
+    if (a.variant == 2 || a.variant == 3) return launch_coop<double>(a, a.max_nc, s);
     const int W = a.waves;
     const int npairs = a.R * a.C;
     if (npairs <= 64 * 4)
@@ -194,8 +194,10 @@ hipError_t launch_rt_persistent_f64(const RtArgs *models, const int2 *items, int
 size_t rt_persistent_lds_f64(int cap, int R_max, int waves) { return staged_lds_bytes<double, 20>(cap, R_max, waves); }
 
 hipError_t launch_coop_persistent_f64(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
-                                      int slots, int waves, size_t lds, int n_blocks, bool dust, hipStream_t s) {
-    return launch_coop_persistent<double>(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust, s);
+                                      int slots, int waves, size_t lds, int n_blocks, bool dust, bool btf,
+                                      hipStream_t s) {
+    return launch_coop_persistent<double>(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust,
+                                          btf, false, s);
 }
 
 }  // namespace portal

@@ -137,9 +137,9 @@ hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s) {
     if (a.n_work == 0) return hipSuccess;
     if (a.R > 64) return hipErrorInvalidValue;
     // dust, star and the own-cell operator: cooperative kernel only
-    if ((a.dust || a.xthn || a.LAM) && a.variant != 2) return hipErrorInvalidValue;
+    if ((a.dust || a.xthn || a.LAM) && a.variant != 2 && a.variant != 3) return hipErrorInvalidValue;
     if (a.variant == 1) return launch_staged<float>(a, a.max_nc, s);
-    if (a.variant == 2) return launch_coop<float>(a, a.max_nc, s);
+    if (a.variant == 2 || a.variant == 3) return launch_coop<float>(a, a.max_nc, s);
     const int W = a.waves;
     const int npairs = a.R * a.C;
     if (npairs <= 64 * 4)
@@ -162,15 +162,20 @@ hipError_t launch_rt_persistent(bool mixed, const RtArgs *models, const int2 *it
 }
 
 hipError_t launch_coop_persistent_f64(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
-                                      int slots, int waves, size_t lds, int n_blocks, bool dust, hipStream_t s);
+                                      int slots, int waves, size_t lds, int n_blocks, bool dust, bool btf,
+                                      hipStream_t s);
 
 hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,
-                                     int cap, int slots, int waves, size_t lds, int n_blocks, bool dust,
-                                     hipStream_t s) {
+                                     int cap, int slots, int waves, size_t lds, int n_blocks, bool dust, bool btf,
+                                     bool rec, hipStream_t s) {
     if (!mixed)
-        return launch_coop_persistent_f64(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust, s);
-    return launch_coop_persistent<float>(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust, s);
+        return launch_coop_persistent_f64(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust,
+                                          btf, s);
+    return launch_coop_persistent<float>(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust,
+                                         btf, rec, s);
 }
+
+bool coop_profile_rec_host(double b_ratio, int nvel, int slots) { return coop_profile_rec(b_ratio, nvel, slots); }
 
 int coop_slots_host(int R, int C, int W) { return coop_slots(R, C, W); }
 

@@ -62,6 +62,7 @@ module portal_hip
     real(c_double) :: max_align_change
     integer(c_int) :: se_form, se_waves
     real(c_double) :: t_mlp_ms
+    integer(c_int) :: rt_order
   end type
 
   interface

@@ -62,7 +62,7 @@ class Stats(C.Structure):
                 ("max_rel_change", C.c_double), ("rt_variant", C.c_int), ("rt_waves", C.c_int),
                 ("max_path", C.c_int), ("ng_steps", C.c_int), ("converged", C.c_int),
                 ("max_align_change", C.c_double), ("se_form", C.c_int), ("se_waves", C.c_int),
-                ("t_mlp_ms", C.c_double)]
+                ("t_mlp_ms", C.c_double), ("rt_order", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -96,3 +96,27 @@ def test_ali_fixed_point_is_exact(small):
     al0 = pp0[idx][:, 3::2] / pp0[idx][:, 2::2]
     al1 = pp1[idx][:, 3::2] / pp1[idx][:, 2::2]
     assert np.abs(al1 - al0).max() < 1e-12
+
+
+@pytest.mark.parametrize("precision,tol", [(PORTAL_FP64, 1e-12), (PORTAL_MIXED, 2e-5)])
+def test_ali_operator_is_the_same_in_both_orders(small, precision, tol):
+    # the own-cell operator of the accelerated iteration comes from the cooperative
+    # kernel in either order of a ray's cells (variant 2 front to back, 3 back to front):
+    # five accelerated iterations give the same level densities
+    sp, g, xth, pt = small
+    out = []
+    try:
+        for variant in (2, 3):
+            pt.force_rt_kernel(precision, variant, 4)
+            pt.set_levpop(g.lev_p)
+            _, lev, st = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, 5, tol_pop=0.0, tol_align=0.0,
+                                           precision=precision, ali=1)
+            assert st["iterations"] == 5 and st["rt_variant"] == variant
+            out.append(lev)
+    finally:
+        pt.force_rt_kernel(precision, -1)
+    idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+    big = out[0][:, idx] > 1e-8 * out[0][:, idx].max(axis=0)
+    rel = np.abs(out[1][:, idx] - out[0][:, idx]) / np.abs(out[0][:, idx])
+    print(f"precision {precision}: max rel difference between the orders {rel[big].max():.2e}")
+    assert rel[big].max() < tol

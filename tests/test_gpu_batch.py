@@ -226,6 +226,7 @@ def test_batch_se_forms_give_the_same_bits(precision):
     for form, waves in [(0, 0), (1, 1), (1, 2), (1, 4), (-1, 0)]:
         pts = _open(models)
         pts[0].force_se(form, waves)
+        pts[0].force_batch_rt(3)  # one inward form for every run: the SE forms are compared
         per, tot = batch_lambda_iterate(pts, [xth] * 3, [8] * 3, cloud.DX_INWARD, 3, tol_pop=0.0, tol_align=0.0,
                                         precision=precision)
         print(form, waves, tot["se_form"], tot["se_waves"])
@@ -413,4 +414,47 @@ def test_batch_with_large_systems_equals_single_models():
         idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
         np.testing.assert_allclose(pt.get_levpop()[:, idx], lev[:, idx], rtol=1e-10)
         np.testing.assert_allclose(pt.get_pol_pops()[idx][:, 0::2], pp[idx][:, 0::2], rtol=1e-10)
+        pt.close()
+
+
+@pytest.mark.parametrize("form", [0, 1, 2, 3], ids=["persistent", "flat", "persistent-btf", "flat-btf"])
+@pytest.mark.parametrize("precision,tol", [(PORTAL_FP64, 1e-10), (PORTAL_MIXED, 1e-5)])
+def test_batch_forms_equal_single_models(precision, tol, form):
+    """Each cooperative form of the batch's inward integration (persistent workgroups or
+    one per item; rays front to back or back to front) at 65 channels, where the 6- and
+    10-level models take 2 and 3 channels per lane and the mixed path's profile
+    recurrence: three iterations equal the single-model ones."""
+    specs = [(700, 4, 41), (1200, 10, 42), (900, 7, 43)]
+    models = []
+    for n, L, seed in specs:
+        sp = cloud.co_like_rotor(L)
+        models.append((sp, cloud.make_cloud(sp, n, seed=seed)))
+    xth = cloud.legendre_all(4, 8)
+    iters = 3
+    singles = []
+    for sp, g in models:
+        pt = Portal(0)
+        pt.upload_grid(g)
+        pt.upload_species(sp)
+        pt.set_levpop(g.lev_p)
+        pp, lev, _ = pt.lambda_iterate(xth, 32, cloud.DX_INWARD, iters, tol_pop=0.0, tol_align=0.0,
+                                       precision=precision)
+        singles.append((pp, lev))
+        pt.close()
+    pts = []
+    for sp, g in models:
+        pt = Portal(0)
+        pt.upload_grid(g)
+        pt.upload_species(sp)
+        pt.set_levpop(g.lev_p)
+        pts.append(pt)
+    pts[0].force_batch_rt(form)
+    per, tot = batch_lambda_iterate(pts, [xth] * 3, [32] * 3, cloud.DX_INWARD, iters, tol_pop=0.0, tol_align=0.0,
+                                    precision=precision)
+    assert tot["iterations"] == iters
+    assert tot["rt_variant"] == (3 if form & 1 else 2) and tot["rt_order"] == (form >> 1)
+    for (sp, g), pt, (pp, lev) in zip(models, pts, singles):
+        idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+        np.testing.assert_allclose(pt.get_levpop()[:, idx], lev[:, idx], rtol=tol)
+        np.testing.assert_allclose(pt.get_pol_pops()[idx][:, 0::2], pp[idx][:, 0::2], rtol=tol)
         pt.close()

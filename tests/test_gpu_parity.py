@@ -97,10 +97,12 @@ def test_walk_forms_agree():
         assert nc == out[0][2] and mp == out[0][3]
 
 
+@pytest.mark.parametrize("variant", [2, 3])
 @pytest.mark.parametrize("precision", [PORTAL_FP64, PORTAL_MIXED])
-def test_cooperative_units_agree(precision):
+def test_cooperative_units_agree(precision, variant):
     # the cooperative kernel stages rays in units of at most 64 cells; the unit size
-    # changes where the staging breaks a ray, not the arithmetic: J is identical
+    # changes where the staging breaks a ray, not the arithmetic: J is identical (back
+    # to front, variant 3, the units of a ray are staged last to first)
     sp = cloud.co_like_rotor(6)
     g = cloud.make_cloud(sp, 20000, seed=9)
     xth = cloud.legendre_all(4, 8)
@@ -108,7 +110,7 @@ def test_cooperative_units_agree(precision):
     pt.upload_grid(g)
     pt.upload_species(sp)
     pt.set_levpop(g.lev_p)
-    pt.force_rt_kernel(precision, 2, 4)
+    pt.force_rt_kernel(precision, variant, 4)
     out = []
     for unit in (0, 8, 4096):
         pt.coop_unit(unit)
@@ -263,7 +265,8 @@ def test_full_size_subset_parity(oracle):
     pt.close()
 
 
-KERNELS = [(0, 4), (1, 4), (1, 2), (2, 2), (2, 4), (2, 5), (2, 8)]
+# variant 3: the cooperative kernel taking the cells of a ray back to front
+KERNELS = [(0, 4), (1, 4), (1, 2), (2, 2), (2, 4), (2, 5), (2, 8), (3, 4), (3, 5), (3, 8)]
 
 
 @pytest.mark.parametrize("variant,waves", KERNELS)
@@ -489,7 +492,7 @@ def test_image_pass_mixed_error(portal1, cfg1, oracle_cfg1):
     assert np.abs(Qm).max() > 0
 
 
-@pytest.mark.parametrize("variant,waves", [(2, 2), (2, 4)])
+@pytest.mark.parametrize("variant,waves", [(2, 2), (2, 4), (3, 4)])
 def test_inward_pass_with_dust(oracle, variant, waves):
     # dust absorption and emission in the inward integration (inverse_rt.f90:273-297)
     sp = cloud.co_like_rotor(4)
@@ -524,9 +527,11 @@ def test_inward_pass_with_dust(oracle, variant, waves):
     pt.close()
 
 
-def test_star_mode(oracle):
+@pytest.mark.parametrize("variant", [-1, 3])
+def test_star_mode(oracle, variant):
     # star mode (main_sub.f90:56-79): per-node star_all directions, star rays ending on the
-    # star with B_nu(T_star) behind them; nodes inside the star are not solved
+    # star with B_nu(T_star) behind them; nodes inside the star are not solved (variant
+    # -1: the autotuner's choice, which must be a cooperative form)
     sp = cloud.co_like_rotor(4)
     g = cloud.make_cloud(sp, 2500, seed=42)
     star = dict(nth=6, nph=11, R_star=0.15, T_star=6000.0, X_star=np.zeros(3))
@@ -539,13 +544,16 @@ def test_star_mode(oracle):
     pt.upload_species(sp)
     pt.set_levpop(g.lev_p)
     pt.star = star
+    if variant >= 0:
+        pt.force_rt_kernel(PORTAL_FP64, variant, 4)
+        pt.force_rt_kernel(PORTAL_MIXED, variant, 4)
     idx = solved(g)
     rng = np.random.default_rng(8)
     sub = np.sort(np.concatenate([idx[np.argsort(r[idx])[:8]], rng.choice(idx, 24, replace=False)])) + 1
     sub = np.unique(sub)
     J0o, J2o = oracle.inverse_nodes(g, sp, xth, 32, cloud.DX_INWARD, sub, star=star, nthreads=NTH)
     J0, J2, st = pt.inverse_rt(xth, 32, cloud.DX_INWARD)
-    assert st["rt_variant"] == 2
+    assert st["rt_variant"] in ((2, 3) if variant < 0 else (variant,))
     np.testing.assert_allclose(J0[:, sub - 1], J0o, rtol=1e-10)
     np.testing.assert_allclose(J2[:, sub - 1], J2o, rtol=0, atol=1e-10 * np.abs(J0o).max(axis=1).min())
     J0m, _, _ = pt.inverse_rt(xth, 32, cloud.DX_INWARD, precision=PORTAL_MIXED)
