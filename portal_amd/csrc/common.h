@@ -366,13 +366,14 @@ constexpr int SE_SLAB_WAVES = 4;  // waves per system of a single model's slab-f
 constexpr int SE_MAX_UNKNOWNS = 512;  // largest SE system (c_dim) the solver takes
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves);
 // the cooperative form of the persistent integration, one launch per (slot count, dust);
-// btf: each ray's cells back to front; rec: the FP32 profile recurrence, which needs
-// coop_profile_rec_host for every model of the launch
+// btf: each ray's cells back to front; prof: the FP32 profile form, at most what
+// coop_profile_form_host gives for every model of the launch (2 shared, 1 recurrence,
+// 0 one exponential per slot)
 hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,
                                      int cap, int slots, int waves, size_t lds, int n_blocks, bool dust, bool btf,
-                                     bool rec, hipStream_t s);
-bool coop_profile_rec_host(double b_ratio, int nvel, int slots);
-size_t rt_coop_persistent_lds(bool mixed, int cap, int R_max, int C, bool dust);
+                                     int prof, hipStream_t s);
+int coop_profile_form_host(int R, int C, int W, int slots, double b_ratio, int nvel);
+size_t rt_coop_persistent_lds(bool mixed, int cap, int R_max, int C, bool dust, int waves);
 int coop_slots_host(int R, int C, int W);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 hipError_t launch_ng_dots(int n_work, int L, const int *work, const double *x0, const double *x1, const double *x2,

@@ -2054,7 +2054,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     struct CoopClass {
         int slots = 0, cap = 8, R_max = 1, C_max = 1;
         bool dust = false;
-        bool rec = true;  // every model of the class allows the profile recurrence
+        int prof = 2;  // the FP32 profile form every model of the class allows
         std::vector<int2> items;
         size_t lds = 0;
         int2 *d_items = nullptr;
@@ -2078,7 +2078,8 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
                 classes.back().dust = dusty[m] != 0;
             }
             CoopClass &k = classes[cls_of[m]];
-            k.rec = k.rec && coop_profile_rec_host(rts[m].b_ratio, rts[m].nvel, sl);
+            k.prof = std::min(k.prof, coop_profile_form_host(rts[m].R, rts[m].C, COOP_W, sl, rts[m].b_ratio,
+                                                             rts[m].nvel));
             k.cap = std::max(k.cap, std::min(rts[m].max_nc, COOP_UNIT));
             k.R_max = std::max(k.R_max, rts[m].R);
             k.C_max = std::max(k.C_max, rts[m].C);
@@ -2086,7 +2087,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         if (coop) {
             for (const int2 &itm : ri) classes[cls_of[itm.x]].items.push_back(itm);
             for (auto &k : classes) {
-                k.lds = rt_coop_persistent_lds(mixed, k.cap, k.R_max, k.C_max, k.dust);
+                k.lds = rt_coop_persistent_lds(mixed, k.cap, k.R_max, k.C_max, k.dust, COOP_W);
                 coop = coop && k.lds <= 160 * 1024;
             }
         }
@@ -2195,7 +2196,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
             Timer t(s);
             hipError_t e = launch_rt_coop_persistent(mixed, d_rt, cl.d_items, ns, d_cnt + 2, cl.cap, cl.slots,
                                                      COOP_W, cl.lds, (f & 1) ? 0 : n_cu * bpc, cl.dust,
-                                                     (f & 2) != 0, cl.rec, s);
+                                                     (f & 2) != 0, cl.prof, s);
             ms_form[f] = t.stop();
             if (e != hipSuccess) {
                 (void)hipGetLastError();
@@ -2365,7 +2366,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
             const CoopClass &cl = classes[k];
             const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / std::max<size_t>(cl.lds, 1))));
             launch_rt_coop_persistent(mixed, d_rt, cl.d_items, (int)cl.items.size(), d_cnt + 2 + k, cl.cap, cl.slots,
-                                      COOP_W, cl.lds, coop_flat ? 0 : n_cu * bpc, cl.dust, coop_btf, cl.rec, s);
+                                      COOP_W, cl.lds, coop_flat ? 0 : n_cu * bpc, cl.dust, coop_btf, cl.prof, s);
         }
     } else {
         launch_rt_persistent(mixed, d_rt, d_ri, n_items, d_cnt, cap, R_max, best_w, rt_blocks, s);

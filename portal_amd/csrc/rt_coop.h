@@ -43,17 +43,22 @@ namespace portal {
 // min(tau^4/24, 2.4e-7) of the value, relative error below 7e-6 at any tau, where the
 // select at tau = 1e-2 of the cubic or 1 - e^-tau left up to 9e-6.  A skipped segment
 // (K dl not > 0, or a cell whose profile argument is not finite) is staged with
-// L = -inf and S = 0, which makes its update an exact no-op.
+// S = 0 and L = -inf (tau2 = 2^(L - u^2)) or, for the shared profile (fold_profile), the
+// factor K = 0 (tau2 = K 2^(-u^2)), which makes its update an exact no-op.
 struct __attribute__((aligned(8))) SegF {
-    float L, S, Sc, s0, s1, s2;
+    float L, S, Sc, s0, s1, s2;  // L: log2 of K dl log2 e, or K dl log2 e itself (KF)
 };
 
+template <bool KF>
 __device__ __forceinline__ SegF fold_seg(float2 k, float dl, bool cell) {
     const float kdl = k.x * dl;
     const bool pos = cell && kdl > 0.0f;
     const float S = pos ? k.y : 0.0f;
     SegF g;
-    g.L = pos ? __log2f(kdl * 1.44269504f) : -INFINITY;
+    if (KF)
+        g.L = pos ? kdl * 1.44269504f : 0.0f;
+    else
+        g.L = pos ? __log2f(kdl * 1.44269504f) : -INFINITY;
     g.S = S;
     g.Sc = S * 1.00000024f;  // 1 + 2^-22
     g.s0 = S * 0.693147182f;
@@ -88,18 +93,36 @@ struct __attribute__((aligned(16))) CoopHdr {
     Real vel_node, dl0;
 };
 
-// The profile terms tau2_q = 2^(L - u_q^2) of a lane's SLOTS consecutive channels,
-// u_q = u_0 + q delta.  REC: one exponential and the recurrence of CoopCell (two products
-// per further slot; the exponentials, not the products, bound the update), else one
-// exponential per slot.  The recurrence is taken only where (SLOTS - 1) delta <= 1 for
-// every cell of the grid (coop_profile_rec).  Then a slot whose term the recurrence
-// loses because tau2_0 fell below FP32's range holds less than 2^-100 for line-centre
-// optical depths below 1e12 per cell, and the bound on rho's exponent acts only where
-// |u_0| > 32 / delta >= 32, so that every slot's term is zero in FP32 anyway.
-template <int SLOTS, bool REC>
+// How the FP32 folded update forms the profile terms tau2_q of a lane's SLOTS
+// consecutive channels (u_q = u_0 + q delta), fold_profile:
+//   PROF_DIRECT: tau2_q = 2^(L - u_q^2), one exponential per slot;
+//   PROF_REC: one exponential and the recurrence of CoopCell (two products per further
+//     slot), taken only where (SLOTS - 1) delta <= 1 for every cell of the grid
+//     (coop_profile_rec).  Then a slot whose term the recurrence loses because tau2_0
+//     fell below FP32's range holds less than 2^-100 for line-centre optical depths below
+//     1e12 per cell, and the bound on rho's exponent acts only where
+//     |u_0| > 32 / delta >= 32, so that every slot's term is zero in FP32 anyway;
+//   PROF_SHARE: the profile 2^(-u^2) depends on the channel, not the line, and the lanes
+//     of a wave that hold one channel group differ only in their line.  Each lane
+//     evaluates 2^(-u^2) for one channel of its wave's window (the window's channel
+//     cw0 + lane) into the wave's 64-entry LDS row, then reads its slots' values and
+//     multiplies them by its line's K dl log2 e: one exponential per lane and cell
+//     instead of one per slot, and the lanes of the same channel see the same value.  It
+//     needs every wave's window to fit 64 channels (coop_profile_share).  A term lost
+//     because 2^(-u^2) fell below FP32's range is below 2^-86 for K dl log2 e < 2^40.
+constexpr int PROF_DIRECT = 0, PROF_REC = 1, PROF_SHARE = 2;
+
+template <int SLOTS, int PROF>
 __device__ __forceinline__ void fold_profile(const float (&v0s)[SLOTS], const CoopCell<float> &c, float L,
-                                             float (&t2)[SLOTS]) {
-    if constexpr (REC && SLOTS > 1) {
+                                             float (&t2)[SLOTS], float vg, float *gw, int lane, int go) {
+    if constexpr (PROF == PROF_SHARE) {
+        const float ug = fmaf(vg, c.inv_b, -c.vb);
+        gw[lane] = __builtin_amdgcn_exp2f(-(ug * ug));
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < SLOTS; q++) t2[q] = L * gw[go + q];
+        __builtin_amdgcn_wave_barrier();
+    } else if constexpr (PROF == PROF_REC && SLOTS > 1) {
         const float u0 = fmaf(v0s[0], c.inv_b, -c.vb);
         t2[0] = __builtin_amdgcn_exp2f(fmaf(-u0, u0, L));
         float rho = __builtin_amdgcn_exp2f(fminf(c.nd * fmaf(2.0f, u0, -c.nd), 64.0f));
@@ -145,31 +168,39 @@ __host__ __device__ constexpr size_t coop_buf_bytes(int cap, int R, bool dust) {
            ~(size_t)15;
 }
 
-// Cells per staged unit: COOP_UNIT, or fewer where the two staging buffers of a
-// workgroup would not leave room for 7 workgroups of 4 waves per CU (the FP32 kernel's
-// occupancy; LDS is 160 KiB per CU), rounded down to a multiple of 8.  Only the folded
-// FP32 record (24 bytes per line) reaches that bound at the configurations' line counts.
+// the shared profile's LDS rows (fold_profile): 64 floats per wave, for the FP32 folded
+// record only
 template <typename Real>
-__host__ __device__ constexpr int coop_unit_cells(int R, bool dust) {
-    const size_t per_wg = (160 * 1024) / 7 - 2 * sizeof(CoopHdr<Real>);
+__host__ __device__ constexpr size_t coop_share_bytes(bool dust, int waves) {
+    return CoopSeg<Real, false>::fold && !dust ? (size_t)waves * 64 * sizeof(float) : 0;
+}
+
+// Cells per staged unit: COOP_UNIT, or fewer where the two staging buffers of a
+// workgroup (and the shared profile's rows) would not leave room for 7 workgroups per CU
+// (the FP32 kernel's occupancy; LDS is 160 KiB per CU), rounded down to a multiple of 8.
+// Only the folded FP32 record (24 bytes per line) reaches that bound at the
+// configurations' line counts.
+template <typename Real>
+__host__ __device__ constexpr int coop_unit_cells(int R, bool dust, int waves) {
+    const size_t per_wg = (160 * 1024) / 7 - 2 * sizeof(CoopHdr<Real>) - coop_share_bytes<Real>(dust, waves);
     const size_t per_cell = 2 * (sizeof(CoopCell<Real>) + (size_t)R * coop_seg_bytes<Real>(dust) +
                                  (dust ? (size_t)R * sizeof(typename Arith<Real>::R2) : 0));
     const int fit = (int)(per_wg / per_cell) & ~7;
     return fit >= COOP_UNIT ? COOP_UNIT : (fit < 8 ? 8 : fit);
 }
 
-// the two staging buffers, or the reduction of J0, J2 (and the own-cell operator) per
-// (line, channel), whichever is larger
+// the two staging buffers and the shared profile's rows, or the reduction of J0, J2
+// (and the own-cell operator) per (line, channel), whichever is larger
 template <typename Real>
-size_t coop_lds_bytes(int cap, int R, int npairs, bool dust, bool lam = true) {
-    const size_t stage = 2 * coop_buf_bytes<Real>(cap, R, dust);
+size_t coop_lds_bytes(int cap, int R, int npairs, bool dust, bool lam, int waves) {
+    const size_t stage = 2 * coop_buf_bytes<Real>(cap, R, dust) + coop_share_bytes<Real>(dust, waves);
     const size_t red = sizeof(double) * (lam ? 3 : 2) * (size_t)npairs;
     return stage > red ? stage : red;
 }
 
 // All threads of the workgroup stage cells c0 .. c0+cnt-1 of ray r of work item bid
 // into buf (cnt <= cap); dv is the channel spacing 2 b_node / nvel.
-template <typename Real, bool DUST>
+template <typename Real, bool DUST, bool KF>
 __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const int r, const int c0, const int cnt,
                                            const int cap, const Real dv, char *buf) {
     typedef Arith<Real> A;
@@ -199,7 +230,7 @@ __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const
         const Real vb = (Real)(e.vray * ib) * (Real)A::vscale, sib = (Real)ib * (Real)A::vscale;
         const bool ok = A::cell_ok(vb, sib);
         if constexpr (CoopSeg<Real, DUST>::fold)
-            ks[j] = fold_seg(A::ks(a, (size_t)e.cell * R + l), dl, ok);
+            ks[j] = fold_seg<KF>(A::ks(a, (size_t)e.cell * R + l), dl, ok);
         else
             ks[j] = A::stage(A::ks(a, (size_t)e.cell * R + l), dl, ok);
         if (DUST) kds[j] = A::stage_dust(A::ksd(a, (size_t)e.cell * R + l), dl, ok);
@@ -230,9 +261,8 @@ __device__ __forceinline__ unsigned nonzero_bits(double x) {
 // reference's ray_in does (I <- I e^-tau + S (1 - e^-tau)); the units of a ray are then
 // staged last to first.  The update needs one product fewer than front to back (no
 // transmission to carry) but cannot leave an exhausted ray early.
-// REC: the FP32 folded update takes the profile terms of a lane's slots by recurrence
-// (fold_profile).
-template <typename Real, int SLOTS, bool DUST, bool LAM, bool BTF = false, bool REC = false>
+// PROF: how the FP32 folded update forms its profile terms (fold_profile).
+template <typename Real, int SLOTS, bool DUST, bool LAM, bool BTF = false, int PROF = PROF_DIRECT>
 __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, const int cap, char *smem) {
     typedef Arith<Real> A;
     typedef typename A::R2 R2;
@@ -255,6 +285,13 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     const bool lane_on = t < G * R;
     const int lo = lane_on ? t % R : 0;      // this lane's line
     const int cb = (t / R) * SLOTS;          // its first channel
+    // the shared profile (PROF_SHARE): the wave's window starts at channel cw0, lane l
+    // evaluates channel cw0 + l into the wave's row gw, and this lane's slots are the
+    // row's entries go .. go + SLOTS - 1
+    const int cw0 = ((64 * wave) / R) * SLOTS;
+    const int go = lane_on ? cb - cw0 : 0;
+    float *gw = reinterpret_cast<float *>(smem + 2 * coop_buf_bytes<Real>(cap, R, DUST)) + 64 * wave;
+    Real vg = (Real)0;  // the velocity of channel cw0 + lane, relative as v0s
     // slot q of this lane holds channel cb + q when on(q); the channel's velocity offset
     // and profile weight at the node are recomputed per ray, which keeps registers free
     auto on = [&](int q) { return lane_on && cb + q < C; };
@@ -282,7 +319,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     auto stage_unit = [&](int r, int u, char *buf) {
         const int nc = ray_cells(r);
         const int c0 = u * cap;
-        coop_stage<Real, DUST>(a, bid, r, c0, min(cap, nc - c0), cap, dv, buf);
+        coop_stage<Real, DUST, PROF == PROF_SHARE>(a, bid, r, c0, min(cap, nc - c0), cap, dv, buf);
     };
     // per-ray state, carried across the units of a ray.  Front to back: Em the emission
     // reaching the node so far and Tr the transmission to the current cell.  Back to
@@ -333,6 +370,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                 v0s[q] = v0 + vel_node;
                 Em[q] = iin;
             }
+            if (PROF == PROF_SHARE) vg = ((Real)2 * b) * (Real)(cw0 + lane - nvel) / (Real)nvel + vel_node;
         }
         if (!BTF && u == 0) {
             // star mode: the node's own directions; rays towards the star start with B_nu(T_star)
@@ -344,6 +382,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             const Real vel_node = hdr->vel_node;
             const Real dl0 = hdr->dl0;
             ray_done = false;
+            if (PROF == PROF_SHARE) vg = ((Real)2 * b) * (Real)(cw0 + lane - nvel) / (Real)nvel + vel_node;
 #pragma unroll
             for (int q = 0; q < SLOTS; q++) {
                 const int nv = on(q) ? cb + q - nvel : 0;
@@ -372,30 +411,23 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             }
         }
         if constexpr (BTF) {
-            // cells c0 + cnt - 1 down to max(c0, 1) (local index j = i - c0); the next
-            // cell's staged data are read one cell ahead, within the unit
+            // cells c0 + cnt - 1 down to max(c0, 1) (local index j = i - c0); the staged
+            // data are read at the top of each cell (reading a cell ahead made the compiler
+            // copy the read-ahead registers every cell; the other resident waves hide the
+            // LDS latency)
             const int jb = c0 == 0 ? 1 : 0;
             const char *kl = reinterpret_cast<const char *>(ks + lo);
             const int kstride = R * (int)sizeof(Seg);
-            const int jt = cnt - 1 >= jb ? cnt - 1 : 0;
-            CoopCell<Real> cn = cr[jt];
-            Seg kn = *reinterpret_cast<const Seg *>(kl + jt * kstride);
-            R2 dn;
-            if (DUST) dn = kds[(size_t)jt * R + lo];
             for (int j = cnt - 1; j >= jb; j--) {
-                const CoopCell<Real> c = cn;
-                const Seg k = kn;
+                const CoopCell<Real> c = cr[j];
+                const Seg k = *reinterpret_cast<const Seg *>(kl + j * kstride);
                 R2 d;
-                if (DUST) d = dn;
-                const int jn = j - 1 >= jb ? j - 1 : j;
-                cn = cr[jn];
-                kn = *reinterpret_cast<const Seg *>(kl + jn * kstride);
-                if (DUST) dn = kds[(size_t)jn * R + lo];
+                if (DUST) d = kds[(size_t)j * R + lo];
                 if constexpr (FOLD) {
                     // every slot's profile term, then its attenuation, then the updates:
                     // independent exponentials in a row rather than a dependent pair per slot
                     float t2v[SLOTS], etv[SLOTS];
-                    fold_profile<SLOTS, REC>(v0s, c, k.L, t2v);
+                    fold_profile<SLOTS, PROF>(v0s, c, k.L, t2v, vg, gw, lane, go);
 #pragma unroll
                     for (int q = 0; q < SLOTS; q++) etv[q] = __builtin_amdgcn_exp2f(-t2v[q]);
 #pragma unroll
@@ -427,17 +459,13 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                 }
             }
         } else if (!ray_done) {
-            // cells max(c0, 1) .. c0 + cnt - 1 of the ray (local index j = i - c0); the
-            // next cell's staged data are read one cell ahead, within the unit
+            // cells max(c0, 1) .. c0 + cnt - 1 of the ray (local index j = i - c0), their
+            // staged data read at the top of each cell
             const int jb = c0 == 0 ? 1 : 0;
             // this lane's record of cell j is at byte j R sizeof(Seg) from kl: a uniform
             // offset on one address register
             const char *kl = reinterpret_cast<const char *>(ks + lo);
             const int kstride = R * (int)sizeof(Seg);
-            CoopCell<Real> cn = cr[jb < cnt ? jb : 0];
-            Seg kn = *reinterpret_cast<const Seg *>(kl + (jb < cnt ? jb : 0) * kstride);
-            R2 dn;
-            if (DUST) dn = kds[(size_t)(jb < cnt ? jb : 0) * R + lo];
             for (int j = jb; j < cnt; j++) {
                 if (((c0 + j) & 3) == 0) {
                     // mixed: the wave leaves the ray once every channel's transmission is at
@@ -462,17 +490,13 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                         break;
                     }
                 }
-                const CoopCell<Real> c = cn;
-                const Seg k = kn;
+                const CoopCell<Real> c = cr[j];
+                const Seg k = *reinterpret_cast<const Seg *>(kl + j * kstride);
                 R2 d;
-                if (DUST) d = dn;
-                const int jn = j + 1 < cnt ? j + 1 : j;
-                cn = cr[jn];
-                kn = *reinterpret_cast<const Seg *>(kl + jn * kstride);
-                if (DUST) dn = kds[(size_t)jn * R + lo];
+                if (DUST) d = kds[(size_t)j * R + lo];
                 if constexpr (FOLD) {
                     float t2v[SLOTS];
-                    fold_profile<SLOTS, REC>(v0s, c, k.L, t2v);
+                    fold_profile<SLOTS, PROF>(v0s, c, k.L, t2v, vg, gw, lane, go);
 #pragma unroll
                     for (int q = 0; q < SLOTS; q++) {
                         const float tau2 = t2v[q];
@@ -595,10 +619,10 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     }
 }
 
-template <typename Real, int SLOTS, bool DUST, bool LAM, bool BTF, bool REC>
+template <typename Real, int SLOTS, bool DUST, bool LAM, bool BTF, int PROF>
 __global__ void __launch_bounds__(512) rt_coop_kernel(RtArgs a, int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    rt_coop_node<Real, SLOTS, DUST, LAM, BTF, REC>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+    rt_coop_node<Real, SLOTS, DUST, LAM, BTF, PROF>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
 }
 
 // The FP32 forms with up to 6 slots are compiled for 7 waves per SIMD (at most 72
@@ -606,11 +630,11 @@ __global__ void __launch_bounds__(512) rt_coop_kernel(RtArgs a, int cap) {
 // which more resident waves hide: 723 ms per production pass against 781 ms at the 6
 // waves the kernel's natural 80 VGPRs give (one box, same inputs; 12 dwords spill).
 // With more slots, or in the persistent form, the budget would spill far more.
-template <int SLOTS, bool DUST, bool LAM, bool BTF, bool REC>
+template <int SLOTS, bool DUST, bool LAM, bool BTF, int PROF>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8)))
 rt_coop_kernel_f32(RtArgs a, int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    rt_coop_node<float, SLOTS, DUST, LAM, BTF, REC>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+    rt_coop_node<float, SLOTS, DUST, LAM, BTF, PROF>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
 }
 
 // The profile recurrence (fold_profile) is taken where (SLOTS - 1) delta <= 1 for every
@@ -621,29 +645,49 @@ inline bool coop_profile_rec(double b_ratio, int nvel, int slots) {
            (slots - 1) * 2.0 * b_ratio * 1.2011224087864498 / nvel <= 1.0;
 }
 
+// The shared profile needs every wave's window, the channels of the groups its lanes
+// hold (lane t = 64 w + l holds group t / R), to fit the wave's 64 lanes.
+inline bool coop_profile_share(int R, int C, int W, int slots) {
+    if (slots < 2 || R < 2) return false;
+    const int G = (C + slots - 1) / slots, n_on = G * R;
+    for (int w = 0; w < W; w++) {
+        const int t0 = 64 * w, t1 = std::min(64 * w + 63, n_on - 1);
+        if (t0 > t1) break;
+        if ((t1 / R - t0 / R + 1) * slots > 64) return false;
+    }
+    return true;
+}
+
+// the profile form of a launch: shared where it fits, else the recurrence where it is
+// exact enough, else one exponential per slot
+inline int coop_profile_form(int R, int C, int W, int slots, double b_ratio, int nvel) {
+    if (coop_profile_share(R, C, W, slots)) return PROF_SHARE;
+    if (coop_profile_rec(b_ratio, nvel, slots)) return PROF_REC;
+    return PROF_DIRECT;
+}
+
 // variant 3 of RtArgs takes the cells of a ray back to front
 template <typename Real, int SLOTS, bool DUST, bool LAM>
 void launch_coop_k(dim3 g, dim3 b, size_t lds, hipStream_t s, const RtArgs &a, int cap) {
-    auto go = [&](auto btf_c, auto rec_c) {
+    auto go = [&](auto btf_c, auto prof_c) {
         constexpr bool BTF = decltype(btf_c)::value;
-        constexpr bool REC = decltype(rec_c)::value && CoopSeg<Real, DUST>::fold && SLOTS > 1;
+        constexpr int PROF = CoopSeg<Real, DUST>::fold && SLOTS > 1 ? decltype(prof_c)::value : PROF_DIRECT;
         if constexpr (std::is_same<Real, float>::value && SLOTS <= 6)
-            hipLaunchKernelGGL((rt_coop_kernel_f32<SLOTS, DUST, LAM, BTF, REC>), g, b, lds, s, a, cap);
+            hipLaunchKernelGGL((rt_coop_kernel_f32<SLOTS, DUST, LAM, BTF, PROF>), g, b, lds, s, a, cap);
         else
-            hipLaunchKernelGGL((rt_coop_kernel<Real, SLOTS, DUST, LAM, BTF, REC>), g, b, lds, s, a, cap);
+            hipLaunchKernelGGL((rt_coop_kernel<Real, SLOTS, DUST, LAM, BTF, PROF>), g, b, lds, s, a, cap);
     };
-    const bool rec = coop_profile_rec(a.b_ratio, a.nvel, SLOTS);
-    if (a.variant == 3) {
-        if (rec)
-            go(std::true_type{}, std::true_type{});
-        else
-            go(std::true_type{}, std::false_type{});
-    } else {
-        if (rec)
-            go(std::false_type{}, std::true_type{});
-        else
-            go(std::false_type{}, std::false_type{});
-    }
+    auto by_prof = [&](auto btf_c) {
+        switch (coop_profile_form(a.R, a.C, a.waves, SLOTS, a.b_ratio, a.nvel)) {
+            case PROF_SHARE: go(btf_c, std::integral_constant<int, PROF_SHARE>{}); break;
+            case PROF_REC: go(btf_c, std::integral_constant<int, PROF_REC>{}); break;
+            default: go(btf_c, std::integral_constant<int, PROF_DIRECT>{}); break;
+        }
+    };
+    if (a.variant == 3)
+        by_prof(std::true_type{});
+    else
+        by_prof(std::false_type{});
 }
 
 template <typename Real, int SLOTS>
@@ -664,7 +708,7 @@ hipError_t launch_coop_s(const RtArgs &a, int cap, size_t lds, hipStream_t s) {
 // and channels need the same slot count: workgroups take items from a counter until the
 // list is exhausted (every workgroup reaches the exit test after each item).  DUST: the
 // class's models carry dust (every model of a class agrees).
-template <typename Real, int SLOTS, bool DUST, bool BTF, bool REC>
+template <typename Real, int SLOTS, bool DUST, bool BTF, int PROF>
 __device__ __forceinline__ void coop_persistent_body(const RtArgs *__restrict__ models,
                                                      const int2 *__restrict__ items, int n_items, int *counter,
                                                      int cap) {
@@ -680,16 +724,16 @@ __device__ __forceinline__ void coop_persistent_body(const RtArgs *__restrict__ 
         const int2 mi = items[it];
         const int m = __builtin_amdgcn_readfirstlane(mi.x), w = __builtin_amdgcn_readfirstlane(mi.y);
         const RtArgs am = models[m];
-        rt_coop_node<Real, SLOTS, DUST, false, BTF, REC>(am, w, cap, smem);
+        rt_coop_node<Real, SLOTS, DUST, false, BTF, PROF>(am, w, cap, smem);
         __syncthreads();
     }
 }
 
-template <typename Real, int SLOTS, bool DUST, bool BTF, bool REC>
+template <typename Real, int SLOTS, bool DUST, bool BTF, int PROF>
 __global__ void __launch_bounds__(512) rt_coop_persistent_kernel(const RtArgs *__restrict__ models,
                                                                  const int2 *__restrict__ items, int n_items,
                                                                  int *counter, int cap) {
-    coop_persistent_body<Real, SLOTS, DUST, BTF, REC>(models, items, n_items, counter, cap);
+    coop_persistent_body<Real, SLOTS, DUST, BTF, PROF>(models, items, n_items, counter, cap);
 }
 
 // Slots per lane: the fewest consecutive channels per lane with which every line's
@@ -709,65 +753,65 @@ inline int coop_slots(int R, int C, int W) {
 // Flat form over the same items: one workgroup per item, in the items' (cost) order,
 // the hardware's dispatch doing the balancing.  It needs no loop state, so it keeps the
 // register budget of rt_coop_kernel.
-template <typename Real, int SLOTS, bool DUST, bool BTF, bool REC>
+template <typename Real, int SLOTS, bool DUST, bool BTF, int PROF>
 __device__ __forceinline__ void coop_flat_body(const RtArgs *__restrict__ models, const int2 *__restrict__ items,
                                                int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int2 mi = items[blockIdx.x];
     const int m = __builtin_amdgcn_readfirstlane(mi.x), w = __builtin_amdgcn_readfirstlane(mi.y);
-    rt_coop_node<Real, SLOTS, DUST, false, BTF, REC>(models[m], w, cap, smem);
+    rt_coop_node<Real, SLOTS, DUST, false, BTF, PROF>(models[m], w, cap, smem);
 }
 
-template <typename Real, int SLOTS, bool DUST, bool BTF, bool REC>
+template <typename Real, int SLOTS, bool DUST, bool BTF, int PROF>
 __global__ void __launch_bounds__(512) rt_coop_flat_kernel(const RtArgs *__restrict__ models,
                                                            const int2 *__restrict__ items, int cap) {
-    coop_flat_body<Real, SLOTS, DUST, BTF, REC>(models, items, cap);
+    coop_flat_body<Real, SLOTS, DUST, BTF, PROF>(models, items, cap);
 }
 
-template <int SLOTS, bool DUST, bool BTF, bool REC>
+template <int SLOTS, bool DUST, bool BTF, int PROF>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8)))
 rt_coop_flat_kernel_f32(const RtArgs *__restrict__ models, const int2 *__restrict__ items, int cap) {
-    coop_flat_body<float, SLOTS, DUST, BTF, REC>(models, items, cap);
+    coop_flat_body<float, SLOTS, DUST, BTF, PROF>(models, items, cap);
 }
 
 // n_blocks > 0: the persistent form on n_blocks workgroups; n_blocks == 0: the flat form.
-// btf: rays back to front; rec: the profile recurrence (coop_profile_rec for every model
-// of the launch).
+// btf: rays back to front; prof: the profile form (fold_profile), one that every model of
+// the launch allows.
 template <typename Real, bool DUST>
 hipError_t launch_coop_persistent_d(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
-                                    int slots, int waves, size_t lds, int n_blocks, bool btf, bool rec,
+                                    int slots, int waves, size_t lds, int n_blocks, bool btf, int prof,
                                     hipStream_t s) {
     if (lds > 160 * 1024 || waves < 1 || waves > 8) return hipErrorInvalidValue;
     if (n_items <= 0) return hipSuccess;
-    auto launch = [&](auto slots_c, auto btf_c, auto rec_c) {
+    auto launch = [&](auto slots_c, auto btf_c, auto prof_c) {
         constexpr int S = decltype(slots_c)::value;
         constexpr bool BTF = decltype(btf_c)::value;
-        constexpr bool REC = decltype(rec_c)::value && CoopSeg<Real, DUST>::fold && S > 1;
+        constexpr int PROF = CoopSeg<Real, DUST>::fold && S > 1 ? decltype(prof_c)::value : PROF_DIRECT;
         if (n_blocks == 0) {
             const dim3 g(n_items), b(64 * waves);
             if constexpr (std::is_same<Real, float>::value && S <= 6)
-                hipLaunchKernelGGL((rt_coop_flat_kernel_f32<S, DUST, BTF, REC>), g, b, lds, s, models, items, cap);
+                hipLaunchKernelGGL((rt_coop_flat_kernel_f32<S, DUST, BTF, PROF>), g, b, lds, s, models, items, cap);
             else
-                hipLaunchKernelGGL((rt_coop_flat_kernel<Real, S, DUST, BTF, REC>), g, b, lds, s, models, items,
+                hipLaunchKernelGGL((rt_coop_flat_kernel<Real, S, DUST, BTF, PROF>), g, b, lds, s, models, items,
                                    cap);
         } else {
             const dim3 g(n_blocks), b(64 * waves);
-            hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, S, DUST, BTF, REC>), g, b, lds, s, models, items,
+            hipLaunchKernelGGL((rt_coop_persistent_kernel<Real, S, DUST, BTF, PROF>), g, b, lds, s, models, items,
                                n_items, counter, cap);
         }
     };
-    auto by_order = [&](auto slots_c) {
-        if (btf) {
-            if (rec)
-                launch(slots_c, std::true_type{}, std::true_type{});
-            else
-                launch(slots_c, std::true_type{}, std::false_type{});
-        } else {
-            if (rec)
-                launch(slots_c, std::false_type{}, std::true_type{});
-            else
-                launch(slots_c, std::false_type{}, std::false_type{});
+    auto by_prof = [&](auto slots_c, auto btf_c) {
+        switch (prof) {
+            case PROF_SHARE: launch(slots_c, btf_c, std::integral_constant<int, PROF_SHARE>{}); break;
+            case PROF_REC: launch(slots_c, btf_c, std::integral_constant<int, PROF_REC>{}); break;
+            default: launch(slots_c, btf_c, std::integral_constant<int, PROF_DIRECT>{}); break;
         }
+    };
+    auto by_order = [&](auto slots_c) {
+        if (btf)
+            by_prof(slots_c, std::true_type{});
+        else
+            by_prof(slots_c, std::false_type{});
     };
     switch (slots) {
         case 1: by_order(std::integral_constant<int, 1>{}); break;
@@ -787,22 +831,22 @@ hipError_t launch_coop_persistent_d(const RtArgs *models, const int2 *items, int
 // n_blocks > 0: the persistent form on n_blocks workgroups; n_blocks == 0: the flat form
 template <typename Real>
 hipError_t launch_coop_persistent(const RtArgs *models, const int2 *items, int n_items, int *counter, int cap,
-                                  int slots, int waves, size_t lds, int n_blocks, bool dust, bool btf, bool rec,
+                                  int slots, int waves, size_t lds, int n_blocks, bool dust, bool btf, int prof,
                                   hipStream_t s) {
     return dust ? launch_coop_persistent_d<Real, true>(models, items, n_items, counter, cap, slots, waves, lds,
-                                                       n_blocks, btf, rec, s)
+                                                       n_blocks, btf, prof, s)
                 : launch_coop_persistent_d<Real, false>(models, items, n_items, counter, cap, slots, waves, lds,
-                                                        n_blocks, btf, rec, s);
+                                                        n_blocks, btf, prof, s);
 }
 
 template <typename Real>
 hipError_t launch_coop(const RtArgs &a, int cap, hipStream_t s) {
     const int W = a.waves;
     if (W < 1 || W > 8 || a.R < 1) return hipErrorInvalidValue;
-    const int unit = a.unit > 0 ? a.unit : coop_unit_cells<Real>(a.R, a.dust != 0);
+    const int unit = a.unit > 0 ? a.unit : coop_unit_cells<Real>(a.R, a.dust != 0, W);
     cap = cap < unit ? (cap < 1 ? 1 : cap) : unit;  // cells per staged unit
     const int npairs = a.R * a.C;
-    const size_t lds = coop_lds_bytes<Real>(cap, a.R, npairs, a.dust != 0, a.LAM != nullptr);
+    const size_t lds = coop_lds_bytes<Real>(cap, a.R, npairs, a.dust != 0, a.LAM != nullptr, W);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const int per_line = (64 * W) / a.R;
     if (per_line < 1) return hipErrorInvalidValue;

@@ -197,7 +197,7 @@ hipError_t launch_coop_persistent_f64(const RtArgs *models, const int2 *items, i
                                       int slots, int waves, size_t lds, int n_blocks, bool dust, bool btf,
                                       hipStream_t s) {
     return launch_coop_persistent<double>(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust,
-                                          btf, false, s);
+                                          btf, PROF_DIRECT, s);
 }
 
 }  // namespace portal

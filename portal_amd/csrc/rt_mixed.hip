@@ -167,21 +167,23 @@ hipError_t launch_coop_persistent_f64(const RtArgs *models, const int2 *items, i
 
 hipError_t launch_rt_coop_persistent(bool mixed, const RtArgs *models, const int2 *items, int n_items, int *counter,
                                      int cap, int slots, int waves, size_t lds, int n_blocks, bool dust, bool btf,
-                                     bool rec, hipStream_t s) {
+                                     int prof, hipStream_t s) {
     if (!mixed)
         return launch_coop_persistent_f64(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust,
                                           btf, s);
     return launch_coop_persistent<float>(models, items, n_items, counter, cap, slots, waves, lds, n_blocks, dust,
-                                         btf, rec, s);
+                                         btf, prof, s);
 }
 
-bool coop_profile_rec_host(double b_ratio, int nvel, int slots) { return coop_profile_rec(b_ratio, nvel, slots); }
+int coop_profile_form_host(int R, int C, int W, int slots, double b_ratio, int nvel) {
+    return coop_profile_form(R, C, W, slots, b_ratio, nvel);
+}
 
 int coop_slots_host(int R, int C, int W) { return coop_slots(R, C, W); }
 
-size_t rt_coop_persistent_lds(bool mixed, int cap, int R_max, int C, bool dust) {
-    return mixed ? coop_lds_bytes<float>(cap, R_max, R_max * C, dust, false)
-                 : coop_lds_bytes<double>(cap, R_max, R_max * C, dust, false);
+size_t rt_coop_persistent_lds(bool mixed, int cap, int R_max, int C, bool dust, int waves) {
+    return mixed ? coop_lds_bytes<float>(cap, R_max, R_max * C, dust, false, waves)
+                 : coop_lds_bytes<double>(cap, R_max, R_max * C, dust, false, waves);
 }
 
 size_t rt_persistent_lds(bool mixed, int cap, int R_max, int waves) {
