@@ -177,15 +177,17 @@ __host__ __device__ constexpr size_t coop_share_bytes(bool dust, int waves) {
 
 // Cells per staged unit: COOP_UNIT, or fewer where the two staging buffers of a
 // workgroup (and the shared profile's rows) would not leave room for 7 workgroups per CU
-// (the FP32 kernel's occupancy; LDS is 160 KiB per CU), rounded down to a multiple of 8.
-// Only the folded FP32 record (24 bytes per line) reaches that bound at the
-// configurations' line counts.
+// (the FP32 kernel's occupancy).  LDS is 160 KiB per CU, allocated to a workgroup in
+// blocks of 512 bytes, so 7 workgroups have 45 blocks each: at 46 the CU holds 6.  Only
+// the folded FP32 record (24 bytes per line) reaches that bound at the configurations'
+// line counts (47 cells at 9 lines and 4 waves).
 template <typename Real>
 __host__ __device__ constexpr int coop_unit_cells(int R, bool dust, int waves) {
-    const size_t per_wg = (160 * 1024) / 7 - 2 * sizeof(CoopHdr<Real>) - coop_share_bytes<Real>(dust, waves);
+    const size_t per_wg =
+        (160 * 1024) / 7 / 512 * 512 - 2 * sizeof(CoopHdr<Real>) - coop_share_bytes<Real>(dust, waves);
     const size_t per_cell = 2 * (sizeof(CoopCell<Real>) + (size_t)R * coop_seg_bytes<Real>(dust) +
                                  (dust ? (size_t)R * sizeof(typename Arith<Real>::R2) : 0));
-    const int fit = (int)(per_wg / per_cell) & ~7;
+    const int fit = (int)(per_wg / per_cell);
     return fit >= COOP_UNIT ? COOP_UNIT : (fit < 8 ? 8 : fit);
 }
 
