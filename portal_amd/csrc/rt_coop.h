@@ -204,7 +204,7 @@ size_t coop_lds_bytes(int cap, int R, int npairs, bool dust, bool lam, int waves
 // into buf (cnt <= cap); dv is the channel spacing 2 b_node / nvel.
 template <typename Real, bool DUST, bool KF>
 __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const int r, const int c0, const int cnt,
-                                           const int cap, const Real dv, char *buf) {
+                                           const int cap, const Real dv, const int4 pos, char *buf) {
     typedef Arith<Real> A;
     typedef typename A::R2 R2;
     typedef typename CoopSeg<Real, DUST>::T Seg;
@@ -224,8 +224,10 @@ __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const
         h.dl0 = (Real)e0.len * dls;
         *hdr = h;
     }
+    // item j = i R + l: the thread's first item is (pos.x, pos.y), and a step of the block
+    // size moves it by (pos.z, pos.w) with a carry, so no item divides by R
+    int i = pos.x, l = pos.y;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const int i = j / R, l = j - i * R;
         const PathEntry e = P[i];
         const double ib = a.inv_b[e.cell];
         const Real dl = (Real)e.len * dls;
@@ -248,6 +250,12 @@ __device__ __forceinline__ void coop_stage(const RtArgs &a, const int bid, const
                 c.cc = __builtin_amdgcn_exp2f(-2.0f * (nd * nd));
             }
             cr[i] = c;
+        }
+        i += pos.z;
+        l += pos.w;
+        if (l >= R) {
+            l -= R;
+            i++;
         }
     }
 }
@@ -279,6 +287,25 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     const Real inv_bn = (Real)(1.0 / bd);
     const int C = a.C, nvel = a.nvel, R = a.R, D = a.D;
     const Real dv = ((Real)2 * b) / (Real)nvel;  // channel spacing
+    const Real da = (Real)2 / (Real)nvel;        // its profile argument at the node
+    // channel nv's velocity offset 2 b nv / nvel and profile argument v / b at the node:
+    // FP64 in the operation order of the other inward kernels (which give the same bits),
+    // FP32 as products with the spacing (no division per channel and ray)
+    auto chan_v = [&](int nv) -> Real {
+        if constexpr (std::is_same<Real, float>::value)
+            return (Real)nv * dv;
+        else
+            return ((Real)2 * b) * (Real)nv / (Real)nvel;
+    };
+    auto chan_a = [&](int nv) -> Real {
+        if constexpr (std::is_same<Real, float>::value)
+            return (Real)nv * da;
+        else
+            return (((Real)2 * b) * (Real)nv / (Real)nvel) * inv_bn;
+    };
+    // this thread's first staging item (cell, line) and the step of one block
+    const int4 spos = make_int4((int)threadIdx.x / a.R, (int)threadIdx.x % a.R, (int)blockDim.x / a.R,
+                                (int)blockDim.x % a.R);
     const int npairs = R * C;
     const size_t bb = coop_buf_bytes<Real>(cap, R, DUST);
 
@@ -321,7 +348,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
     auto stage_unit = [&](int r, int u, char *buf) {
         const int nc = ray_cells(r);
         const int c0 = u * cap;
-        coop_stage<Real, DUST, PROF == PROF_SHARE>(a, bid, r, c0, min(cap, nc - c0), cap, dv, buf);
+        coop_stage<Real, DUST, PROF == PROF_SHARE>(a, bid, r, c0, min(cap, nc - c0), cap, dv, spos, buf);
     };
     // per-ray state, carried across the units of a ray.  Front to back: Em the emission
     // reaching the node so far and Tr the transmission to the current cell.  Back to
@@ -368,11 +395,10 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
 #pragma unroll
             for (int q = 0; q < SLOTS; q++) {
                 const int nv = on(q) ? cb + q - nvel : 0;
-                const Real v0 = ((Real)2 * b) * (Real)nv / (Real)nvel;
-                v0s[q] = v0 + vel_node;
+                v0s[q] = chan_v(nv) + vel_node;
                 Em[q] = iin;
             }
-            if (PROF == PROF_SHARE) vg = ((Real)2 * b) * (Real)(cw0 + lane - nvel) / (Real)nvel + vel_node;
+            if (PROF == PROF_SHARE) vg = chan_v(cw0 + lane - nvel) + vel_node;
         }
         if (!BTF && u == 0) {
             // star mode: the node's own directions; rays towards the star start with B_nu(T_star)
@@ -384,12 +410,12 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             const Real vel_node = hdr->vel_node;
             const Real dl0 = hdr->dl0;
             ray_done = false;
-            if (PROF == PROF_SHARE) vg = ((Real)2 * b) * (Real)(cw0 + lane - nvel) / (Real)nvel + vel_node;
+            if (PROF == PROF_SHARE) vg = chan_v(cw0 + lane - nvel) + vel_node;
 #pragma unroll
             for (int q = 0; q < SLOTS; q++) {
                 const int nv = on(q) ? cb + q - nvel : 0;
-                const Real v0 = ((Real)2 * b) * (Real)nv / (Real)nvel;
-                const Real aa = v0 * inv_bn;
+                const Real v0 = chan_v(nv);
+                const Real aa = chan_a(nv);
                 const Real e0 = A::ex(-(aa * aa));
                 if (DUST) {
                     // S = (e phi + j)/(k phi + alpha), tau = (k phi + alpha) dl (inverse_rt.f90:289-297)
@@ -543,8 +569,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
 #pragma unroll
                 for (int q = 0; q < SLOTS; q++) {
                     const int nv = on(q) ? cb + q - nvel : 0;
-                    const Real v0 = ((Real)2 * b) * (Real)nv / (Real)nvel;
-                    const Real aa = v0 * inv_bn;
+                    const Real aa = chan_a(nv);
                     const Real e0 = A::ex(-(aa * aa));
                     Real tau0, Sc;
                     if (DUST) {
