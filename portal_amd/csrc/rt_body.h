@@ -77,13 +77,13 @@ struct Arith<float> {
     // below 1e-9 of the remaining source function)
     static constexpr float tr_floor = 1e-9f;
     __device__ static float ex(float x) { return __expf(x); }
-    // 1 - exp(-tau) without cancellation for small tau.  Both forms are evaluated and
-    // one is picked by a bit select, so that no branch splits the update.
+    // 1 - exp(-tau) without cancellation for small tau: the smaller of two upper bounds,
+    // the cubic Taylor sum (above 1 - e^-tau for every tau > 0, exact to tau^4/24) and
+    // 1 + 2^-22 - e^-tau (the offset exceeds the error of et = __expf(-tau), at most
+    // 1.5e-7 absolute); relative error below 5e-6 at any tau, with no branch or select
     __device__ static float one_minus(float tau, float et) {
-        const float series = tau * (1.0f - tau * (0.5f - tau * (0.16666667f - tau * 0.041666668f)));
-        const float direct = 1.0f - et;
-        const int m = -(int)(tau < 1e-2f);
-        return __int_as_float((__float_as_int(series) & m) | (__float_as_int(direct) & ~m));
+        const float cub = tau * fmaf(-tau, fmaf(-tau, 0.16666667f, 0.5f), 1.0f);
+        return __builtin_fminf(cub, 1.00000024f - et);
     }
     __device__ static R2 stage(R2 k, float dl, bool cell) {
         const float kdl = k.x * dl;

@@ -612,31 +612,35 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
         half ^= 1;
     }
 
+    // every lane weights its own channels' sums with phi0 d_v (inverse_rt.f90:258,340);
+    // one thread per line then adds them in channel order
     double *red0 = reinterpret_cast<double *>(smem);
     double *red2 = red0 + npairs;
     double *red3 = red2 + npairs;
+    {
+        const double spi = 1.7724538509055160273;
+        const double d_v = 2.0 * bd / nvel;
 #pragma unroll
-    for (int q = 0; q < SLOTS; q++) {
-        const int p = lo * C + cb + q;
-        if (on(q)) {
-            red0[p] = (double)S0[q] - (double)C0[q];
-            red2[p] = (double)S2[q] - (double)C2[q];
-            if (LAM) red3[p] = (double)Lw[q];
+        for (int q = 0; q < SLOTS; q++) {
+            const int p = lo * C + cb + q;
+            if (on(q)) {
+                const double vv = (2.0 * bd) * (cb + q - nvel) / nvel;
+                const double aa = vv / bd;
+                const double wgt = (1.0 / (spi * bd)) * exp(-(aa * aa)) * d_v;
+                red0[p] = wgt * ((double)S0[q] - (double)C0[q]);
+                red2[p] = wgt * ((double)S2[q] - (double)C2[q]);
+                if (LAM) red3[p] = wgt * (double)Lw[q];
+            }
         }
     }
     __syncthreads();
     if (threadIdx.x < R) {
         const int l = threadIdx.x;
-        const double spi = 1.7724538509055160273;
-        const double d_v = 2.0 * bd / nvel;
         double j0 = 0.0, j2 = 0.0, lam = 0.0;
         for (int c = 0; c < C; c++) {
-            const double vv = (2.0 * bd) * (c - nvel) / nvel;
-            const double aa = vv / bd;
-            const double phi0 = (1.0 / (spi * bd)) * exp(-(aa * aa));
-            j0 = j0 + phi0 * d_v * red0[l * C + c];
-            j2 = j2 + phi0 * d_v * red2[l * C + c];
-            if (LAM) lam = lam + phi0 * d_v * red3[l * C + c];
+            j0 = j0 + red0[l * C + c];
+            j2 = j2 + red2[l * C + c];
+            if (LAM) lam = lam + red3[l * C + c];
         }
         // back from the intensity unit (exact); the own-cell operator is per unit of S
         const int ul = A::uexp(a.I_in[l]);

@@ -148,11 +148,15 @@ def test_batch_continued_calls_are_one_iteration():
         pt.close()
 
 
-def test_batch_ng_reaches_the_plain_fixed_point():
-    # Ng acceleration per model in the batch: fewer iterations, same fixed point
+@pytest.mark.parametrize("precision,tol,bound", [(PORTAL_FP64, 1e-7, 2e-5), (PORTAL_MIXED, 1e-6, 5e-5)])
+def test_batch_ng_reaches_the_plain_fixed_point(precision, tol, bound):
+    # Ng acceleration per model in the batch: fewer iterations, same fixed point.  Mixed
+    # precision at the production tolerances: an alignment change of 1e-9 (tol_pop 1e-7)
+    # is about two FP32 roundings of rho20/rho00 ~ 1e-2, below what the FP32 transfer
+    # resolves from one iteration to the next
     models = _models()
     xth = cloud.legendre_all(4, 8)
-    kw = dict(tol_pop=1e-7, tol_align=1e-9)
+    kw = dict(tol_pop=tol, tol_align=tol * 1e-2, precision=precision)
     out = []
     for ng in (0, 1):
         pts = []
@@ -173,7 +177,7 @@ def test_batch_ng_reaches_the_plain_fixed_point():
     for (sp, g), a, b in zip(models, lev0, lev1):
         idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
         big = a[:, idx] > 1e-8 * a[:, idx].max(axis=0)
-        assert (np.abs(b[:, idx] - a[:, idx])[big] / a[:, idx][big]).max() < 2e-5
+        assert (np.abs(b[:, idx] - a[:, idx])[big] / a[:, idx][big]).max() < bound
 
 
 def _open(models):
