@@ -442,11 +442,12 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             // cells c0 + cnt - 1 down to max(c0, 1) (local index j = i - c0); the staged
             // data are read at the top of each cell (reading a cell ahead made the compiler
             // copy the read-ahead registers every cell; the other resident waves hide the
-            // LDS latency)
-            const int jb = c0 == 0 ? 1 : 0;
+            // LDS latency); the bounds are uniform, which keeps the loop test scalar
+            const int jb = __builtin_amdgcn_readfirstlane(c0 == 0 ? 1 : 0);
+            const int jt = __builtin_amdgcn_readfirstlane(cnt - 1);
             const char *kl = reinterpret_cast<const char *>(ks + lo);
             const int kstride = R * (int)sizeof(Seg);
-            for (int j = cnt - 1; j >= jb; j--) {
+            for (int j = jt; j >= jb; j--) {
                 const CoopCell<Real> c = cr[j];
                 const Seg k = *reinterpret_cast<const Seg *>(kl + j * kstride);
                 R2 d;
@@ -488,13 +489,14 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             }
         } else if (!ray_done) {
             // cells max(c0, 1) .. c0 + cnt - 1 of the ray (local index j = i - c0), their
-            // staged data read at the top of each cell
-            const int jb = c0 == 0 ? 1 : 0;
+            // staged data read at the top of each cell; uniform bounds keep the test scalar
+            const int jb = __builtin_amdgcn_readfirstlane(c0 == 0 ? 1 : 0);
+            const int je = __builtin_amdgcn_readfirstlane(cnt);
             // this lane's record of cell j is at byte j R sizeof(Seg) from kl: a uniform
             // offset on one address register
             const char *kl = reinterpret_cast<const char *>(ks + lo);
             const int kstride = R * (int)sizeof(Seg);
-            for (int j = jb; j < cnt; j++) {
+            for (int j = jb; j < je; j++) {
                 if (((c0 + j) & 3) == 0) {
                     // mixed: the wave leaves the ray once every channel's transmission is at
                     // or below the floor (the dropped tail is below the floor times the
