@@ -284,7 +284,16 @@ struct RtArgs {
     double *LAM;
     int unit;               // cooperative kernel: cells per staged unit (0: COOP_UNIT)
     double b_ratio;         // largest over smallest positive Doppler width of the grid (0: unknown)
+    // the autotuner's sample of the work list: with item_run > 0, block b integrates work
+    // item (b / item_run) * item_stride + b % item_run, runs of consecutive items spread
+    // over the list (0: block b integrates item b)
+    int item_run, item_stride;
 };
+
+// the work item of block b (RtArgs.item_run, item_stride)
+__device__ __forceinline__ int rt_item(const RtArgs &a, int b) {
+    return a.item_run > 0 ? (b / a.item_run) * a.item_stride + b % a.item_run : b;
+}
 hipError_t launch_rt_f64(const RtArgs &a, hipStream_t s);
 hipError_t launch_rt_mixed(const RtArgs &a, hipStream_t s);
 

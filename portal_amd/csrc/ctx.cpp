@@ -368,10 +368,11 @@ int check_rt_args(Ctx &c, const RtArgs &a, bool mixed) {
 // candidate (kernel variant, waves per workgroup) pairs are timed on a slice of the
 // work list and the fastest is kept for every later pass with the same shape.
 // inward-integration launch shapes the autotuner times: (variant, waves per workgroup);
-// variant 3 is the cooperative kernel taking the cells of a ray back to front
-constexpr int N_RT_CANDS = 10;
-constexpr struct RtCand { int variant, waves; } RT_CANDS[N_RT_CANDS] = {{0, 4}, {1, 4}, {1, 2}, {2, 2}, {2, 4},
-                                                                      {2, 5}, {2, 8}, {3, 4}, {3, 5}, {3, 8}};
+// variant 3 is the cooperative kernel taking the cells of a ray back to front.  The
+// staged and global kernels are timed only when no cooperative shape fits the launch.
+constexpr int N_RT_CANDS = 11;
+constexpr struct RtCand { int variant, waves; } RT_CANDS[N_RT_CANDS] = {
+    {2, 2}, {2, 4}, {2, 5}, {2, 8}, {3, 2}, {3, 4}, {3, 5}, {3, 8}, {1, 4}, {1, 2}, {0, 4}};
 static_assert(N_RT_CANDS <= RT_TUNE_SLOTS, "tune_ms holds every candidate");
 
 // the autotuner's key: the paths and the launch's shape
@@ -397,22 +398,43 @@ int autotune_rt(Ctx &c, RtArgs ra, bool mixed) {
     if (c.tuned_key[pi] == key) return PORTAL_OK;
     constexpr int NC = N_RT_CANDS;
     const RtCand *cands = RT_CANDS;
-    const int sub = std::min(ra.n_work, 2048);
-    ra.n_work = sub;
+    // the sample: one launch over 16 runs of 1,024 consecutive items spread evenly over the
+    // Morton-ordered work list (the whole list when it is shorter).  The runs keep the
+    // locality of a full pass (neighbouring nodes' rays share cells in L2), the spread lets
+    // the cloud's centre and its edge both weigh in, and 16,384 items are several times the
+    // workgroups the device keeps resident, so the timing is one of throughput, not of one
+    // workgroup's latency.  The first 2,048 items alone, or a sample of every 122nd item,
+    // chose kernels that ran the 10^6-node cloud 10 to 25% slower than this one's choice.
+    const int n_all = ra.n_work, run = 1024, n_runs = 16;
+    const bool whole = n_all <= run * n_runs;
+    auto launch_slice = [&](int n, bool sample) {
+        RtArgs rs = ra;
+        rs.n_work = n;
+        if (sample) {
+            rs.item_run = run;
+            rs.item_stride = n_all / n_runs;
+        }
+        return mixed ? launch_rt_mixed(rs, c.stream) : launch_rt_f64(rs, c.stream);
+    };
     double best = 1e30;
     int bv = 0, bw = 4;
     for (int k = 0; k < NC; k++) {
         // FP64 keeps each ray front to back: its candidates then give the same bits, so the
         // timed choice cannot change an FP64 result (back to front stays a forced option)
-        if (!mixed && cands[k].variant == 3) {
+        if ((!mixed && cands[k].variant == 3) || (cands[k].variant < 2 && best < 1e30)) {
             c.tune_ms[pi][k] = -1.0;
             continue;
         }
         ra.variant = cands[k].variant;
         ra.waves = cands[k].waves;
-        Timer t(c.stream);
-        hipError_t e = mixed ? launch_rt_mixed(ra, c.stream) : launch_rt_f64(ra, c.stream);
-        const double ms = t.stop();
+        // an untimed launch first: a kernel's first launch also loads its code
+        hipError_t e = launch_slice(std::min(n_all, 64), false);
+        double ms = 0.0;
+        if (e == hipSuccess) {
+            Timer t(c.stream);
+            e = whole ? launch_slice(n_all, false) : launch_slice(run * n_runs, true);
+            ms = t.stop();
+        }
         c.tune_ms[pi][k] = e == hipSuccess ? ms : -1.0;
         if (e != hipSuccess) {
             (void)hipGetLastError();  // an LDS budget the variant cannot meet: skip it
@@ -1219,10 +1241,10 @@ static int surrogate_step(Ctx &c, int *n_fallback, int *d_item_flags, double *t_
     return PORTAL_OK;
 }
 
-// the autotuner's timings of the inward-integration candidates (variant/waves: global 4,
-// staged 4, staged 2, cooperative 2, 4, 5, 8, cooperative back to front 4, 5, 8) for one
-// precision, -1 for a candidate that could not run; out holds n doubles, returns the
-// number of candidates
+// the autotuner's timings of the inward-integration candidates (variant/waves: cooperative
+// 2, 4, 5, 8, cooperative back to front 2, 4, 5, 8, staged 4, staged 2, global 4) for one
+// precision, -1 for a candidate that could not run or was not timed; out holds n doubles,
+// returns the number of candidates
 int portal_debug_tune_ms(const portal_ctx *x, int precision, double *out, int n) {
     if (!x || !out || n < 0 || precision < 0 || precision > 1) return PORTAL_ERR_ARG;
     for (int k = 0; k < n && k < N_RT_CANDS; k++) out[k] = x->c.tune_ms[precision][k];

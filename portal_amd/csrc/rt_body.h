@@ -288,7 +288,7 @@ __device__ __forceinline__ void rt_staged_node(const RtArgs &a, const int bid, c
 template <typename Real, int MAXS, int GROUP>
 __global__ void __launch_bounds__(256) rt_staged_kernel(RtArgs a, int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    rt_staged_node<Real, MAXS, GROUP>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+    rt_staged_node<Real, MAXS, GROUP>(a, rt_item(a, xcd_block(blockIdx.x, gridDim.x)), cap, smem);
 }
 
 // Persistent form for batches of independent models: every workgroup takes the

@@ -496,6 +496,19 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             // offset on one address register
             const char *kl = reinterpret_cast<const char *>(ks + lo);
             const int kstride = R * (int)sizeof(Seg);
+            // a lane with one channel reads a cell ahead: it has too little work per cell to
+            // hide the LDS latency otherwise (with more slots the read-ahead registers cost
+            // copies every cell)
+            constexpr bool AHEAD = SLOTS == 1;
+            CoopCell<Real> cn{};
+            Seg kn{};
+            R2 dn{};
+            if constexpr (AHEAD) {
+                const int j0 = jb < je ? jb : 0;
+                cn = cr[j0];
+                kn = *reinterpret_cast<const Seg *>(kl + j0 * kstride);
+                if (DUST) dn = kds[(size_t)j0 * R + lo];
+            }
             for (int j = jb; j < je; j++) {
                 if (((c0 + j) & 3) == 0) {
                     // mixed: the wave leaves the ray once every channel's transmission is at
@@ -520,10 +533,22 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                         break;
                     }
                 }
-                const CoopCell<Real> c = cr[j];
-                const Seg k = *reinterpret_cast<const Seg *>(kl + j * kstride);
-                R2 d;
-                if (DUST) d = kds[(size_t)j * R + lo];
+                CoopCell<Real> c;
+                Seg k;
+                R2 d{};
+                if constexpr (AHEAD) {
+                    c = cn;
+                    k = kn;
+                    d = dn;
+                    const int jn = j + 1 < je ? j + 1 : j;
+                    cn = cr[jn];
+                    kn = *reinterpret_cast<const Seg *>(kl + jn * kstride);
+                    if (DUST) dn = kds[(size_t)jn * R + lo];
+                } else {
+                    c = cr[j];
+                    k = *reinterpret_cast<const Seg *>(kl + j * kstride);
+                    if (DUST) d = kds[(size_t)j * R + lo];
+                }
                 if constexpr (FOLD) {
                     float t2v[SLOTS];
                     fold_profile<SLOTS, PROF>(v0s, c, k.L, t2v, vg, gw, lane, go);
@@ -655,7 +680,7 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
 template <typename Real, int SLOTS, bool DUST, bool LAM, bool BTF, int PROF>
 __global__ void __launch_bounds__(512) rt_coop_kernel(RtArgs a, int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    rt_coop_node<Real, SLOTS, DUST, LAM, BTF, PROF>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+    rt_coop_node<Real, SLOTS, DUST, LAM, BTF, PROF>(a, rt_item(a, xcd_block(blockIdx.x, gridDim.x)), cap, smem);
 }
 
 // The FP32 forms with up to 6 slots are compiled for 7 waves per SIMD (at most 72
@@ -667,7 +692,7 @@ template <int SLOTS, bool DUST, bool LAM, bool BTF, int PROF>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8)))
 rt_coop_kernel_f32(RtArgs a, int cap) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    rt_coop_node<float, SLOTS, DUST, LAM, BTF, PROF>(a, xcd_block(blockIdx.x, gridDim.x), cap, smem);
+    rt_coop_node<float, SLOTS, DUST, LAM, BTF, PROF>(a, rt_item(a, xcd_block(blockIdx.x, gridDim.x)), cap, smem);
 }
 
 // The profile recurrence (fold_profile) is taken where (SLOTS - 1) delta <= 1 for every
@@ -693,7 +718,10 @@ inline bool coop_profile_share(int R, int C, int W, int slots) {
 
 // the profile form of a launch: shared where it fits, else the recurrence where it is
 // exact enough, else one exponential per slot
+// (with fewer than three slots neither saves an exponential's worth of work: the shared
+// row's LDS round trip cost a production-sized pass of the 4-level cloud 11% at two slots)
 inline int coop_profile_form(int R, int C, int W, int slots, double b_ratio, int nvel) {
+    if (slots < 3) return PROF_DIRECT;
     if (coop_profile_share(R, C, W, slots)) return PROF_SHARE;
     if (coop_profile_rec(b_ratio, nvel, slots)) return PROF_REC;
     return PROF_DIRECT;

@@ -75,7 +75,7 @@ hipError_t launch_line_prep(const PrepArgs &a, hipStream_t s) {
 template <int MAXS>
 __global__ void __launch_bounds__(256) rt_f64_kernel(RtArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int bid = rt_item(a, xcd_remap(blockIdx.x, gridDim.x));
     const int node = a.work[bid];
     const double b = a.velb[node].w;
     const double spi = 1.7724538509055160273;  // sqrt(pi)

@@ -30,7 +30,7 @@ __device__ __forceinline__ float one_minus_exp_neg(float tau, float et) {
 template <int MAXS>
 __global__ void __launch_bounds__(256) rt_mixed_kernel(RtArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
-    const int bid = xcd_remap_m(blockIdx.x, gridDim.x);
+    const int bid = rt_item(a, xcd_remap_m(blockIdx.x, gridDim.x));
     const int node = a.work[bid];
     const double bd = a.velb[node].w;
     const float b = (float)bd;
