@@ -156,6 +156,12 @@ struct Ctx {
     // test hook: the cooperative form of a batch this context leads (-1 timed; bit 0 one
     // workgroup per item, bit 1 rays back to front)
     int batch_rt_form = -1;
+    // a batch this context leads: the timed choices of its last call, kept for the next call
+    // over the same models with the same paths and shapes (batch_key)
+    unsigned long long batch_key = 0;
+    bool batch_flat = false, batch_btf = false;
+    int batch_se_form = 1;
+    std::vector<std::pair<int, int>> batch_se_waves;  // (c_dim, waves per system)
     // d_pol holds the SE solution from which the current d_levp was formed (set by the
     // lambda iteration; cleared by anything that changes the state otherwise), so that
     // a lambda iteration continued by a further call measures its first change exactly
