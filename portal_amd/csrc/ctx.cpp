@@ -2421,52 +2421,94 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
             for (const auto &k : se_cls) c0.batch_se_waves.push_back({k.cd, k.waves});
         }
     }
-    // capture one iteration
+    // one iteration captured as a graph over the models still iterating: a model leaves
+    // once it has converged, as its single-model iteration stops there (the work lists
+    // are compacted in place, in their cost order, and the graph captured again)
+    std::vector<char> live(n, 1);
+    std::vector<int> n_live_cls(classes.size()), n_live_se(se_cls.size());
+    int n_live_ri = n_items, n_live_si = (int)si.size();
+    auto compact = [&](const std::vector<int2> &all, int2 *d, int &cnt) -> hipError_t {
+        std::vector<int2> v;
+        v.reserve(all.size());
+        for (const int2 &x : all)
+            if (live[x.x]) v.push_back(x);
+        cnt = (int)v.size();
+        return v.empty() ? hipSuccess : hipMemcpy(d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice);
+    };
+    auto compact_all = [&]() -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (coop)
+            for (size_t k = 0; k < classes.size() && e == hipSuccess; k++)
+                e = compact(classes[k].items, classes[k].d_items, n_live_cls[k]);
+        else
+            e = compact(ri, d_ri, n_live_ri);
+        if (e != hipSuccess) return e;
+        if (se_form == 0) return compact(si, d_si, n_live_si);
+        for (size_t k = 0; k < se_cls.size() && e == hipSuccess; k++)
+            e = compact(se_cls[k].items, se_cls[k].d_items, n_live_se[k]);
+        return e;
+    };
+    for (size_t k = 0; k < classes.size(); k++) n_live_cls[k] = (int)classes[k].items.size();
+    for (size_t k = 0; k < se_cls.size(); k++) n_live_se[k] = (int)se_cls[k].items.size();
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     Defer graph_at_exit{[&] {
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
     }};
-    HIP_TRY(&c0, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    launch_fill_zero(d_cnt, sizeof(int) * n_cnt, s);
-    launch_fill_zero(d_se_cnt, sizeof(int) * std::max<size_t>(1, se_cls.size()), s);
-    launch_fill_zero(d_max, sizeof(unsigned long long) * 2 * n, s);
-    for (int m = 0; m < n; m++) {
-        Ctx &c = xs[m]->c;
-        launch_line_prep(prep_of(m), s);
-    }
-    if (coop) {
-        for (size_t k = 0; k < classes.size(); k++) {
-            const CoopClass &cl = classes[k];
-            const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / std::max<size_t>(cl.lds, 1))));
-            launch_rt_coop_persistent(mixed, d_rt, cl.d_items, (int)cl.items.size(), d_cnt + 2 + k, cl.cap, cl.slots,
-                                      COOP_W, cl.lds, coop_flat ? 0 : n_cu * bpc, cl.dust, coop_btf, cl.prof, s);
+    auto capture = [&]() -> hipError_t {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        launch_fill_zero(d_cnt, sizeof(int) * n_cnt, s);
+        launch_fill_zero(d_se_cnt, sizeof(int) * std::max<size_t>(1, se_cls.size()), s);
+        launch_fill_zero(d_max, sizeof(unsigned long long) * 2 * n, s);
+        for (int m = 0; m < n; m++)
+            if (live[m]) launch_line_prep(prep_of(m), s);
+        if (coop) {
+            for (size_t k = 0; k < classes.size(); k++) {
+                const CoopClass &cl = classes[k];
+                if (n_live_cls[k] == 0) continue;
+                const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / std::max<size_t>(cl.lds, 1))));
+                launch_rt_coop_persistent(mixed, d_rt, cl.d_items, n_live_cls[k], d_cnt + 2 + k, cl.cap, cl.slots,
+                                          COOP_W, cl.lds, coop_flat ? 0 : n_cu * bpc, cl.dust, coop_btf, cl.prof, s);
+            }
+        } else if (n_live_ri > 0) {
+            launch_rt_persistent(mixed, d_rt, d_ri, n_live_ri, d_cnt, cap, R_max, best_w, rt_blocks, s);
         }
-    } else {
-        launch_rt_persistent(mixed, d_rt, d_ri, n_items, d_cnt, cap, R_max, best_w, rt_blocks, s);
-    }
-    if (se_form == 0)
-        launch_se_persistent(d_se, d_si, (int)si.size(), d_cnt + 1, cd_max, nt_max, se_blocks, s);
-    else
-        for (size_t k = 0; k < se_cls.size(); k++) {
-            const SeClass &cl = se_cls[k];
-            launch_se_slab(d_se, cl.d_items, (int)cl.items.size(), d_se_cnt + k, cl.cd, cl.nt, cl.waves, cl.blocks,
-                           d_slabs, s);
+        if (se_form == 0) {
+            if (n_live_si > 0)
+                launch_se_persistent(d_se, d_si, n_live_si, d_cnt + 1, cd_max, nt_max, se_blocks, s);
+        } else {
+            for (size_t k = 0; k < se_cls.size(); k++) {
+                const SeClass &cl = se_cls[k];
+                if (n_live_se[k] == 0) continue;
+                launch_se_slab(d_se, cl.d_items, n_live_se[k], d_se_cnt + k, cl.cd, cl.nt, cl.waves, cl.blocks,
+                               d_slabs, s);
+            }
         }
-    for (int m = 0; m < n; m++) {
-        Ctx &c = xs[m]->c;
-        if (use_surr[m])
-            launch_fill_zero(sb[m].res_cur, sizeof(unsigned long long) * std::max<size_t>(c.g.work.size(), 1), s);
-        launch_lambda_update((int)c.g.work.size(), c.g.N, c.sp.e_num, c.g.d_work, c.g.d_ab, c.g.d_h2, c.sp.d_jlev,
-                             c.d_pol, prev[m], c.d_levp, d_max + 2 * m, s, prev[m], sb[m].res_cur);
+        for (int m = 0; m < n; m++) {
+            if (!live[m]) continue;
+            Ctx &c = xs[m]->c;
+            if (use_surr[m])
+                launch_fill_zero(sb[m].res_cur, sizeof(unsigned long long) * std::max<size_t>(c.g.work.size(), 1), s);
+            launch_lambda_update((int)c.g.work.size(), c.g.N, c.sp.e_num, c.g.d_work, c.g.d_ab, c.g.d_h2,
+                                 c.sp.d_jlev, c.d_pol, prev[m], c.d_levp, d_max + 2 * m, s, prev[m], sb[m].res_cur);
+        }
+        e = hipStreamEndCapture(s, &graph);
+        if (e != hipSuccess) return e;
+        return hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    };
+    {
+        hipError_t ce = capture();
+        if (ce != hipSuccess) {
+            c0.err = std::string("graph capture failed: ") + hipGetErrorString(ce);
+            return PORTAL_ERR_DEVICE;
+        }
     }
-    hipError_t ce = hipStreamEndCapture(s, &graph);
-    if (ce != hipSuccess) {
-        c0.err = std::string("graph capture failed: ") + hipGetErrorString(ce);
-        return PORTAL_ERR_DEVICE;
-    }
-    HIP_TRY(&c0, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     std::vector<unsigned long long> mx(2 * n);
     std::vector<int> conv_at(n, 0);
     std::vector<double> last(n, 0.0), last_al(n, 0.0);
@@ -2491,7 +2533,8 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         HIP_TRY(&c0, hipMalloc((void **)&d_ngp, sizeof(double) * ng_partial_doubles()));
         HIP_TRY(&c0, hipMalloc((void **)&d_ngs, sizeof(double) * 8 * n));
     }
-    int nhist = 0, n_ng = 0, last_ng = -1;
+    int nhist = 0, n_ng = 0;
+    std::vector<int> last_ng(n, -1);  // per model, the iteration of its last extrapolation
     std::vector<double> sums(8 * (size_t)n);
     std::vector<int> nfb(n, 0);
     double t_mlp = 0.0;
@@ -2504,7 +2547,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
         if (it->surrogate && k <= 1) {
             HIP_TRY(&c0, hipStreamSynchronize(s));
             for (int m = 0; m < n; m++) {
-                if (!use_surr[m]) continue;
+                if (!use_surr[m] || !live[m]) continue;
                 Ctx &c = xs[m]->c;
                 const int nw = (int)c.g.work.size();
                 const size_t NL = (size_t)c.g.N * c.sp.e_num;
@@ -2546,23 +2589,38 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
                 HIP_TRY(&c0, hipMemcpyAsync(prev[m], c.d_pol, sizeof(double) * 2 * NL, hipMemcpyDeviceToDevice, s));
             }
         }
-        bool all = true;
+        bool all = true, left = false;
         for (int m = 0; m < n; m++) {
+            if (!live[m]) continue;
             double dp, da;
             std::memcpy(&dp, &mx[2 * m], 8);
             std::memcpy(&da, &mx[2 * m + 1], 8);
             last[m] = dp;
             last_al[m] = da;
             // the first pass has no previous alignment to compare with, unless it continues
-            if ((k > 0 || chained[m]) && !conv_at[m] && dp <= it->tol_pop && da <= it->tol_align) conv_at[m] = k + 1;
+            if ((k > 0 || chained[m]) && dp <= it->tol_pop && da <= it->tol_align) {
+                conv_at[m] = k + 1;
+                live[m] = 0;
+                left = true;
+            }
             all = all && conv_at[m];
         }
         if (all) break;
+        if (left && k + 1 < it->max_iter) {
+            HIP_TRY(&c0, hipStreamSynchronize(s));
+            HIP_TRY(&c0, compact_all());
+            hipError_t ce = capture();
+            if (ce != hipSuccess) {
+                c0.err = std::string("graph capture failed: ") + hipGetErrorString(ce);
+                return PORTAL_ERR_DEVICE;
+            }
+        }
         if (ng) {
             // history x1 = hist[3m] (newest) .. x3 = hist[3m+2]; x0 the level densities just formed
             bool applied = false;
             if (nhist == 3 && k >= 3) {
                 for (int m = 0; m < n; m++) {
+                    if (!live[m]) continue;
                     Ctx &c = xs[m]->c;
                     HIP_TRY(&c0, launch_ng_dots((int)c.g.work.size(), c.sp.e_num, c.g.d_work, c.d_levp, hist[3 * m],
                                                 hist[3 * m + 1], hist[3 * m + 2], d_ngp, d_ngs + 8 * m, s));
@@ -2570,6 +2628,7 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
                 HIP_TRY(&c0, hipMemcpyAsync(sums.data(), d_ngs, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, s));
                 HIP_TRY(&c0, hipStreamSynchronize(s));
                 for (int m = 0; m < n; m++) {
+                    if (!live[m]) continue;
                     Ctx &c = xs[m]->c;
                     const double *q = &sums[8 * (size_t)m];
                     const double a11 = q[0], a12 = q[1], a22 = q[2], b1 = q[3], b2 = q[4];
@@ -2580,14 +2639,13 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
                     HIP_TRY(&c0, launch_ng_apply((int)c.g.work.size(), c.sp.e_num, c.g.d_work, c.d_levp, hist[3 * m],
                                                  hist[3 * m + 1], ca, cb, s));
                     applied = true;
+                    last_ng[m] = k;
                 }
-                if (applied) {
-                    n_ng++;
-                    last_ng = k;
-                }
+                if (applied) n_ng++;
                 nhist = 0;  // the extrapolated iterate starts a new history
             }
             for (int m = 0; m < n; m++) {
+                if (!live[m]) continue;
                 Ctx &c = xs[m]->c;
                 std::swap(hist[3 * m + 2], hist[3 * m + 1]);
                 std::swap(hist[3 * m + 1], hist[3 * m]);
@@ -2622,8 +2680,8 @@ int portal_batch_lambda_iterate(portal_ctx **xs, int n, const portal_se_params *
     for (int m = 0; m < n; m++) {
         Ctx &c = xs[m]->c;
         ncross += c.paths.total;
-        // d_pol formed d_levp in the last pass, unless an extrapolation followed it
-        c.pol_chain = last_ng < iters - 1;
+        // d_pol formed d_levp in the model's last pass, unless an extrapolation followed it
+        c.pol_chain = last_ng[m] < (conv_at[m] ? conv_at[m] : iters) - 1;
         all_conv = all_conv && conv_at[m];
         if (use_surr[m]) {
             const int nw = (int)c.g.work.size();

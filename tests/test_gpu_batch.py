@@ -105,6 +105,47 @@ def test_batch_converges_each_model():
         pt.close()
 
 
+@pytest.mark.parametrize("ng", [0, 1])
+def test_batch_models_stop_where_their_single_runs_stop(ng):
+    # a model leaves the batch's iteration once it meets the tolerance: its iteration
+    # count, level densities and SE solution equal those of its single-model run to the
+    # same tolerance, with Ng extrapolation on the same schedule or without
+    models = _models()
+    xth = cloud.legendre_all(4, 8)
+    kw = dict(tol_pop=1e-6, tol_align=1e-8, precision=PORTAL_FP64, ng_accel=ng)
+    singles = []
+    for sp, g in models:
+        pt = Portal(0)
+        pt.upload_grid(g)
+        pt.upload_species(sp)
+        pt.set_levpop(g.lev_p)
+        pp, lev, st = pt.lambda_iterate(xth, 8, cloud.DX_INWARD, 200, **kw)
+        assert st["converged"]
+        singles.append((pp, lev, st["iterations"]))
+        pt.close()
+    pts = []
+    for sp, g in models:
+        pt = Portal(0)
+        pt.upload_grid(g)
+        pt.upload_species(sp)
+        pt.set_levpop(g.lev_p)
+        pts.append(pt)
+    per, tot = batch_lambda_iterate(pts, [xth] * 3, [8] * 3, cloud.DX_INWARD, 200, **kw)
+    its = [s[2] for s in singles]
+    print(f"ng={ng}: single-model iterations {its}, batch {[p['iterations'] for p in per]}")
+    assert tot["converged"] and tot["iterations"] == max(its)
+    assert len(set(its)) > 1  # the models leave at different iterations
+    for (sp, g), pt, p, (pp, lev, n_it) in zip(models, pts, per, singles):
+        assert p["converged"] and p["iterations"] == n_it
+        idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+        np.testing.assert_allclose(pt.get_levpop()[:, idx], lev[:, idx], rtol=1e-10)
+        pb = pt.get_pol_pops()
+        np.testing.assert_allclose(pb[idx][:, 0::2], pp[idx][:, 0::2], rtol=1e-10)
+        scale = np.abs(pp[idx][:, 0::2]).max(axis=1, keepdims=True)
+        assert np.all(np.abs(pb[idx][:, 1::2] - pp[idx][:, 1::2]) <= 1e-10 * scale + 1e-300)
+        pt.close()
+
+
 def test_batch_continued_calls_are_one_iteration():
     # 4 + 3 batch iterations in two calls equal 7 in one (the second call continues each
     # model's iteration), and the iteration at which each model meets the tolerance does
