@@ -338,6 +338,8 @@ struct SurrArgs {
     const int *work;
     const double *h2, *tkin, *ab, *J0, *J2, *Iin;
     const int *j_lev;
+    const int *l1, *l2;      // per line: lower and upper level (0-based)
+    const double *g1, *g2;   // per line: their statistical weights
     float *feats;            // n_work x n_in
     const float *out;        // n_work x n_out (network outputs, physical units)
     double *levp, *pol;      // updated for accepted nodes

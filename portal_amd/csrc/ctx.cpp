@@ -1220,6 +1220,10 @@ static int surrogate_step(Ctx &c, int *n_fallback, int *d_item_flags, double *t_
     a.J2 = c.d_J2;
     a.Iin = s.d_Iin;
     a.j_lev = s.d_jlev;
+    a.l1 = s.d_l1;
+    a.l2 = s.d_l2;
+    a.g1 = s.d_g1;
+    a.g2 = s.d_g2;
     a.feats = d_f;
     a.out = d_o;
     a.levp = c.d_levp;

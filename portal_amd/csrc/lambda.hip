@@ -334,18 +334,27 @@ __global__ void __launch_bounds__(256) surr_features_kernel(SurrArgs a) {
 }
 
 // Decode the network output into populations and alignment; reject non-finite or
-// badly normalised predictions (those nodes keep the plain iterate).
+// badly normalised predictions, and ones that invert a line that rho_1 does not invert
+// (those nodes keep the plain iterate).  A level whose sqrt(2J+1) rho_1^0_0 is at or
+// below the features' floor (1e-12) keeps rho_1: its features and training targets were
+// floored, so the network's output says nothing about it.
+__device__ __forceinline__ double surr_decode(double x1, float o) {
+    return x1 > 1e-12 ? exp10(surr_logpop(x1) + (double)o) : x1;
+}
+
 __global__ void __launch_bounds__(256) surr_apply_kernel(SurrArgs a) {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= a.n_work) return;
     const int node = a.work[w];
     const float *o = a.out + (size_t)w * a.n_out;
     const size_t N = a.N;
+    // sqrt(2J+1) rho^0_0 of level i (proportional to its population): first solution, prediction
+    auto first = [&](int i) { return sqrt(2.0 * a.j_lev[i] + 1.0) * a.pol[node + N * (2 * i)]; };
+    auto pred = [&](int i) { return surr_decode(first(i), o[i]); };
     double s = 0.0;
     bool ok = true;
     for (int i = 0; i < a.L; i++) {
-        const double sg = sqrt(2.0 * a.j_lev[i] + 1.0);
-        const double p = exp10(surr_logpop(sg * a.pol[node + N * (2 * i)]) + (double)o[i]);
+        const double p = pred(i);
         ok = ok && isfinite(p);
         s += p;
     }
@@ -357,6 +366,13 @@ __global__ void __launch_bounds__(256) surr_apply_kernel(SurrArgs a) {
         const double rat = (r0 != 0.0 ? r2 / r0 : 0.0) + 1e-3 * (double)o[q++];
         ok = ok && isfinite(rat) && fabs(rat) < 0.5;
     }
+    // line k between levels l1 (lower) and l2 (upper) is inverted where n1 g2 < g1 n2
+    for (int k = 0; k < a.R && ok; k++) {
+        const int lo = a.l1[k], up = a.l2[k];
+        const bool inv_pred = pred(lo) * a.g2[k] < a.g1[k] * pred(up);
+        const bool inv_first = first(lo) * a.g2[k] < a.g1[k] * first(up);
+        ok = !inv_pred || inv_first;
+    }
     if (a.flags) a.flags[w] = ok ? 0 : 1;
     if (!ok) {
         atomicAdd(a.n_fallback, 1);
@@ -367,7 +383,7 @@ __global__ void __launch_bounds__(256) surr_apply_kernel(SurrArgs a) {
     const bool pert = a.perturb_stride > 0 && w % a.perturb_stride == 0;
     double p0 = 0.0;
     if (pert) {
-        p0 = exp10(surr_logpop(sqrt(2.0 * a.j_lev[0] + 1.0) * a.pol[node]) + (double)o[0]);
+        p0 = pred(0);
         s += (a.perturb_factor - 1.0) * p0;
     }
     const double nab = a.ab[node] * a.h2[node];
@@ -375,7 +391,7 @@ __global__ void __launch_bounds__(256) surr_apply_kernel(SurrArgs a) {
     for (int i = 0; i < a.L; i++) {
         const double sg = sqrt(2.0 * a.j_lev[i] + 1.0);
         const double r0_1 = a.pol[node + N * (2 * i)], r2_1 = a.pol[node + N * (2 * i + 1)];
-        const double p = exp10(surr_logpop(sg * r0_1) + (double)o[i]) * (pert && i == 0 ? a.perturb_factor : 1.0) / s;
+        const double p = surr_decode(sg * r0_1, o[i]) * (pert && i == 0 ? a.perturb_factor : 1.0) / s;
         const double r0 = p / sg;
         const double rat = a.j_lev[i] != 0 ? (r0_1 != 0.0 ? r2_1 / r0_1 : 0.0) + 1e-3 * (double)o[q++] : 0.0;
         a.levp[(size_t)node * a.L + i] = nab * p;

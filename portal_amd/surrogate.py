@@ -13,7 +13,12 @@ path, and the first-pass features are paired with the converged solution.
 
     python -m portal_amd.surrogate --levels 10 --clouds 10 --nodes 200000
 
-writes portal_amd/surrogate_weights/<species>.npz, which ``load_into`` finds again.
+writes portal_amd/surrogate_weights/<species>.npz, which ``load_into`` finds again.  The
+networks of the batch's other rotors (5 to 9 and 11 to 20 levels) are trained on clouds of
+the batch recipe (density slopes in [-2, -1], 48 directions, 65 channels):
+
+    python -m portal_amd.surrogate --levels 5 6 7 8 9 11 12 13 14 15 16 17 18 19 20 \
+        --clouds 6 --nodes 40000 --nth 6 --nph 11 --nvel 32 --n-exp -2 -1
 """
 from __future__ import annotations
 
@@ -105,7 +110,7 @@ def make_dataset(pt, sp, g, xth, nvel, dx, max_iter=80, tol=1e-6, tol_align=1e-8
 
 
 def train(levels=10, clouds=10, nodes=200000, nth=9, nph=32, nvel=64, epochs=100, batch=4096, lr=1e-3,
-          seed0=101, out=None, log=print):
+          seed0=101, out=None, log=print, n_exp=(-1.7, -1.3)):
     from . import cloud
     from .api import Portal
     sp = cloud.co_like_rotor(levels)
@@ -117,7 +122,7 @@ def train(levels=10, clouds=10, nodes=200000, nth=9, nph=32, nvel=64, epochs=100
         t = time.time()
         rng = np.random.default_rng(seed0 + k)
         # the production recipe with the density slope and field direction varied per cloud
-        g = cloud.make_cloud(sp, nodes, seed=seed0 + k, n_exp=float(rng.uniform(-1.7, -1.3)))
+        g = cloud.make_cloud(sp, nodes, seed=seed0 + k, n_exp=float(rng.uniform(*n_exp)))
         f, y, it = make_dataset(pt, sp, g, xth, nvel, cloud.DX_INWARD)
         X.append(f)
         Y.append(y)
@@ -160,15 +165,19 @@ def train(levels=10, clouds=10, nodes=200000, nth=9, nph=32, nvel=64, epochs=100
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
-    ap.add_argument("--levels", type=int, default=10)
+    ap.add_argument("--levels", type=int, nargs="+", default=[10])
     ap.add_argument("--clouds", type=int, default=10)
     ap.add_argument("--nodes", type=int, default=200000)
     ap.add_argument("--nth", type=int, default=9)
     ap.add_argument("--nph", type=int, default=32)
     ap.add_argument("--nvel", type=int, default=64)
     ap.add_argument("--epochs", type=int, default=100)
+    ap.add_argument("--n-exp", type=float, nargs=2, default=[-1.7, -1.3], help="range of the density slope")
     ap.add_argument("--out", default=None, help="weights file (default: the species' file in surrogate_weights/)")
     a = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
-    print(train(a.levels, a.clouds, a.nodes, a.nth, a.nph, a.nvel, a.epochs, out=a.out,
-                log=lambda *x: print(*x, flush=True)))
+    for lv in a.levels:
+        t0 = time.time()
+        print(train(lv, a.clouds, a.nodes, a.nth, a.nph, a.nvel, a.epochs, out=a.out if len(a.levels) == 1 else None,
+                    log=lambda *x: print(*x, flush=True), n_exp=tuple(a.n_exp)), f"{time.time() - t0:.0f}s",
+              flush=True)

@@ -2,6 +2,7 @@
 
     python -m portal_amd.workloads small        # 4,000 nodes, 4 levels: FP64, then mixed with its errors
     python -m portal_amd.workloads batch        # 64 independent clouds in one batched iteration (Ng)
+    python -m portal_amd.workloads batch --surrogate   # the same with each rotor's network
     python -m portal_amd.workloads million      # the 4,000-node recipe at 1e6 nodes with the surrogate
 
 `small` runs the 4,000-node cloud (48 directions, 65 channels, 30 iterations, a 129 x 129
@@ -10,7 +11,8 @@ median relative errors of the level populations, of rho^2_0/rho^0_0 and of the S
 images against the FP64 run.  `batch` builds 64 seeded clouds (node counts log-uniform
 in [5e3, 1e5], rotors of 4..20 levels, density slopes in [-2, -1], random fields) and
 iterates all of them to convergence in mixed precision with one persistent launch per
-kernel per iteration.  `million` iterates the 1e6-node cloud to convergence with and
+kernel per iteration; with --surrogate every model whose rotor has a trained network
+starts from its prediction.  `million` iterates the 1e6-node cloud to convergence with and
 without the surrogate and reports iterations and the Stokes error against FP64.
 """
 from __future__ import annotations
@@ -125,10 +127,11 @@ def build_batch(n_models=64, seed=4000):
     return out
 
 
-def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True):
+def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True, surr=False):
     xth = cloud.legendre_all(nth, nph)
     models = build_batch(n_models, seed)
     pts = [pt for _, _, pt in models]
+    have = [surr and surrogate.load_into(pt, sp) for sp, _, pt in models]
     sizes = [(g.node_num, sp.e_num, int((g.ab_all[: g.bulk_node_num] > 0).sum())) for sp, g, _ in models]
     t = time.perf_counter()
     # calls of at most 10 iterations, each continuing the last (the same iteration as one
@@ -137,7 +140,10 @@ def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True):
     while True:
         per, tot = batch_lambda_iterate(pts, [xth] * n_models, [nvel] * n_models, cloud.DX_INWARD, 10,
                                         tol_pop=TOL_POP, tol_align=TOL_ALIGN, precision=PORTAL_MIXED,
-                                        rebuild_paths=1 if done == 0 else 0, ng_accel=1 if ng else 0)
+                                        rebuild_paths=1 if done == 0 else 0, ng_accel=1 if ng else 0,
+                                        surrogate=1 if surr and done == 0 else 0)
+        if done == 0:
+            fb = [p["fallbacks"] for p in per]
         for m, p in enumerate(per):
             if not iters_at[m] and p["converged"]:
                 iters_at[m] = done + p["iterations"]
@@ -151,8 +157,10 @@ def batch(n_models=64, seed=4000, nth=6, nph=11, nvel=32, ng=True):
     t = time.perf_counter() - t
     n_solved = sum(s[2] for s in sizes)
     out = {"workload": f"{n_models} clouds, nodes log-uniform in [5e3, 1e5], 4..20 levels, "
-                       f"{xth.shape[1]} directions, {2 * nvel + 1} channels, mixed{' + Ng' if ng else ''}, "
-                       f"to {TOL_POP:g}/{TOL_ALIGN:g}",
+                       f"{xth.shape[1]} directions, {2 * nvel + 1} channels, mixed{' + surrogate' if surr else ''}"
+                       f"{' + Ng' if ng else ''}, to {TOL_POP:g}/{TOL_ALIGN:g}",
+           "models_with_surrogate": int(sum(have)),
+           "surrogate_fallbacks": int(sum(fb)) if surr else None,
            "nodes": n_solved, "seconds": t, "converged_nodes_per_s": n_solved / t,
            "iterations_max": done, "all_converged": bool(all(iters_at)),
            "iterations_per_model": [v if v else done for v in iters_at],
@@ -220,8 +228,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("which", choices=["small", "batch", "million"])
     ap.add_argument("--no-ng", action="store_true", help="batch: plain iteration")
+    ap.add_argument("--surrogate", action="store_true", help="batch: start each model with a network from its "
+                                                             "rotor's prediction")
     a = ap.parse_args()
-    res = batch(ng=not a.no_ng) if a.which == "batch" else {"small": small, "million": million}[a.which]()
+    res = batch(ng=not a.no_ng, surr=a.surrogate) if a.which == "batch" else {"small": small, "million": million}[a.which]()
     print(json.dumps(res), flush=True)
 
 

@@ -390,13 +390,14 @@ def test_batch_staged_form_when_a_model_does_not_fit_the_cooperative_kernel():
 def test_batch_surrogate_equals_single_models():
     """The surrogate iterate in a batch: models whose context holds a network (the 4- and
     10-level rotors) get the prediction after the first replay and the residual test
-    after the second, as they do alone; a model without a network iterates plainly.
-    FP64, four iterations, against single-model runs to 1e-10, with the same fallbacks."""
+    after the second, as they do alone; a model without a network (a 6-level rotor with
+    another dipole moment, for which none is trained) iterates plainly.  FP64, four
+    iterations, against single-model runs to 1e-10, with the same fallbacks."""
     from portal_amd import surrogate
-    specs = [(3000, 4, 41), (2000, 10, 42), (1500, 6, 43)]
+    specs = [(3000, 4, 41, 0.11), (2000, 10, 42, 0.11), (1500, 6, 43, 0.12)]
     models = []
-    for n, L, seed in specs:
-        sp = cloud.co_like_rotor(L)
+    for n, L, seed, mu in specs:
+        sp = cloud.co_like_rotor(L, dipole_debye=mu)
         models.append((sp, cloud.make_cloud(sp, n, seed=seed)))
     xth = cloud.legendre_all(4, 8)
 
