@@ -65,6 +65,43 @@ def test_invalid_network_output_falls_back_to_the_plain_iterate():
     pt.close()
 
 
+def test_prediction_that_inverts_a_line_falls_back():
+    # network outputs fixed through the output normalisation (zero spread): all-zero
+    # corrections predict rho_1 itself and every node accepts it; a correction of -1 on
+    # the J=0 level's log10 population inverts the 1-0 line wherever n1/n0 > 0.3, and
+    # every node falls back, including those whose renormalisation alone would pass
+    sp, g, pt = _setup(4, 1500, seed=780)
+    n_in, n_out = surrogate.dims(sp)
+    xth = cloud.legendre_all(4, 8)
+    pt.set_levpop(g.lev_p)
+    pp1, _ = pt.se_pass(xth, 16, cloud.DX_INWARD)  # rho_1: the state the network corrects
+    idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
+    f0 = pp1[idx, 0] * 1.0 / (pp1[idx][:, 0::2] * np.sqrt(2.0 * sp.j_lev + 1.0)).sum(axis=1)
+    n1_n0 = pp1[idx, 2] * np.sqrt(3.0) / pp1[idx, 0]
+    assert n1_n0.min() > 0.3
+    sum_passes = (1.0 - 0.9 * f0 > 0.5).sum()
+    assert sum_passes > len(idx) // 2
+
+    def run(y_mean):
+        pt.mlp_create(n_in, 64, 2, n_out, seed=1)
+        p, nm = pt.mlp_get_params()
+        nm[2 * n_in: 2 * n_in + n_out] = y_mean
+        nm[2 * n_in + n_out:] = 0.0
+        pt.mlp_set_params(p, nm)
+        pt.set_levpop(g.lev_p)
+        return pt.lambda_iterate(xth, 16, cloud.DX_INWARD, 1, tol_pop=0, tol_align=0, surrogate=1)
+
+    y = np.zeros(n_out, dtype=np.float32)
+    pp_id, _, st_id = run(y)
+    assert st_id["fallbacks"] == 0
+    np.testing.assert_allclose(pp_id[idx][:, 0::2], pp1[idx][:, 0::2], rtol=1e-12)
+    y[0] = -1.0
+    _, _, st_inv = run(y)
+    print(f"{len(idx)} nodes, {sum_passes} pass the normalisation test; fallbacks {st_inv['fallbacks']}")
+    assert st_inv["fallbacks"] == len(idx)
+    pt.close()
+
+
 def test_features_are_the_first_pass_state():
     # the device features against the same quantities formed on the host
     sp, g, pt = _setup(4, 1200, seed=779)
