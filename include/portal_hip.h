@@ -238,7 +238,9 @@ int portal_lambda_iterate(portal_ctx *ctx, const portal_se_params *p, const port
  * every model whose context holds a network (portal_mlp_create + weights) the
  * network's prediction after the first replay and the residual test (fallback_tol)
  * after the second, as portal_lambda_iterate does; models without one iterate
- * plainly.  Iterates until every model meets the tolerances (or max_iter); per_model
+ * plainly.  Iterates until every model meets the tolerances (or max_iter); a model
+ * that meets them leaves the iteration there, as portal_lambda_iterate stops, so its
+ * results are those of its single-model call.  per_model
  * (n, may be NULL) gets each model's iteration count at convergence, whether it
  * converged and its surrogate fallbacks, total the device time of the whole batch.  A
  * further call continues each model's iteration (not with the surrogate, which starts
