@@ -590,6 +590,9 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
             }
         }
         if (BTF ? u == 0 : u == nu - 1) {
+            // back to front: the intensity leaving the node's own cell, kept apart from Em
+            // (which the next ray sets afresh), so that no register copy carries it back
+            Real Io[SLOTS];
             if constexpr (BTF) {
                 // the node's own cell last, with no skip (inverse_rt.f90:318-337)
                 const Real dl0 = hdr->dl0;
@@ -609,14 +612,14 @@ __device__ __forceinline__ void rt_coop_node(const RtArgs &a, const int bid, con
                     }
                     const Real t0 = A::ex(-tau0);
                     const Real om0 = A::one_minus(tau0, t0);
-                    Em[q] = Em[q] * t0 + Sc * om0;
+                    Io[q] = Em[q] * t0 + Sc * om0;
                     if (LAM) Lw[q] = Lw[q] + w * om0;
-                    Tr[q] = (Real)0;
                 }
             }
 #pragma unroll
             for (int q = 0; q < SLOTS; q++) {
-                const Real Iout = Em[q] + Tr[q] * iin;
+                // front to back the background arrives through the ray's transmission
+                const Real Iout = BTF ? Io[q] : Em[q] + Tr[q] * iin;
                 if (A::kahan) {
                     Real y = w * Iout - C0[q];
                     Real t2 = S0[q] + y;
