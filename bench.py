@@ -32,7 +32,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SURVEY_BYTES_PER_CROSS_MIXED = None  # computed from the grid: 28*nn + 4*(4+L)
 
 
 def log(*a):
@@ -293,8 +292,9 @@ def main():
     img_m, stim = image(PORTAL_MIXED)
     err = stokes_error(img64, img_m) if img64 else None
 
-    # roofline of the dominant kernel (the mixed inward integration), per-crossing bytes of
-    # the SURVEY's measurement section: 28 nn + 4 (4 + L)
+    # roofline of the dominant kernel (the mixed inward integration): algorithmic bytes per
+    # ray-cell crossing, 28 nn (the walk's neighbour ids and positions) + 4 (4 + L) (the
+    # FP32 cell record and its L level populations)
     nn = float(np.diff(g.nbr_ptr).mean())
     bytes_per_cross = 28.0 * nn + 4.0 * (4 + sp.e_num)
     launches = max(1, stm["iterations"])
