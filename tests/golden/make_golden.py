@@ -82,7 +82,7 @@ def main():
             wofs.append(wofs[-1] + dl.shape[1])
             wnode.append(n)
             wdir.append(d)
-    np.savez_compressed(os.path.join(OUT, "inward_cfg1.npz"), digest=cloud_digest(g), nodes=nodes, J0=J0, J2=J2,
+    np.savez_compressed(os.path.join(OUT, "inward_4000_nodes.npz"), digest=cloud_digest(g), nodes=nodes, J0=J0, J2=J2,
                         walk=np.concatenate(walks, axis=1), walk_ofs=np.array(wofs), walk_node=np.array(wnode),
                         walk_dir=np.array(wdir))
 

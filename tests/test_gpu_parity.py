@@ -18,7 +18,7 @@ NTH = 16  # oracle threads on the GPU host
 
 
 @pytest.fixture(scope="module")
-def cfg1():
+def cloud4k():
     sp = cloud.co_like_rotor(4)
     g = cloud.make_cloud(sp, 4000, seed=1)
     xth = cloud.legendre_all(6, 11)
@@ -26,8 +26,8 @@ def cfg1():
 
 
 @pytest.fixture(scope="module")
-def portal1(cfg1):
-    sp, g, xth = cfg1
+def portal1(cloud4k):
+    sp, g, xth = cloud4k
     pt = Portal(0)
     pt.upload_grid(g)
     pt.upload_species(sp)
@@ -37,8 +37,8 @@ def portal1(cfg1):
 
 
 @pytest.fixture(scope="module")
-def oracle_cfg1(oracle, cfg1):
-    sp, g, xth = cfg1
+def oracle_cloud4k(oracle, cloud4k):
+    sp, g, xth = cloud4k
     pp, J0, J2, rates, jj = oracle.se_pass(g, sp, xth, 32, cloud.DX_INWARD, extra_info=1, nthreads=NTH)
     return pp, J0, J2, rates, jj
 
@@ -60,9 +60,9 @@ def assert_pops_close(pp, ref, sp, idx, rtol):
         assert np.all(d2 <= rtol * scale), f"level {i}: max |d rho20| / rho00 = {np.max(d2 / scale):.2e}"
 
 
-def test_inverse_rt_fp64_parity(portal1, cfg1, oracle_cfg1):
-    sp, g, xth = cfg1
-    _, J0o, J2o, _, _ = oracle_cfg1
+def test_inverse_rt_fp64_parity(portal1, cloud4k, oracle_cloud4k):
+    sp, g, xth = cloud4k
+    _, J0o, J2o, _, _ = oracle_cloud4k
     J0, J2, st = portal1.inverse_rt(xth, 32, cloud.DX_INWARD)
     idx = solved(g)
     np.testing.assert_allclose(J0[:, idx], J0o[:, idx], rtol=1e-10, atol=0)
@@ -123,9 +123,9 @@ def test_cooperative_units_agree(precision, variant):
         np.testing.assert_array_equal(J2, out[0][1])
 
 
-def test_se_pass_fp64_parity(portal1, cfg1, oracle_cfg1):
-    sp, g, xth = cfg1
-    ppo, _, _, rates_o, jj_o = oracle_cfg1
+def test_se_pass_fp64_parity(portal1, cloud4k, oracle_cloud4k):
+    sp, g, xth = cloud4k
+    ppo, _, _, rates_o, jj_o = oracle_cloud4k
     pp, rates, jj, st = portal1.se_pass(xth, 32, cloud.DX_INWARD, extra_info=1)
     idx = solved(g)
     assert_pops_close(pp, ppo, sp, idx, 1e-10)
@@ -168,9 +168,9 @@ def test_se_pass_with_collisional_depolarisation(oracle):
     assert al(pp).mean() < 0.8 * al(pp0).mean()
 
 
-def test_se_solve_from_oracle_J(portal1, cfg1, oracle, oracle_cfg1):
-    sp, g, xth = cfg1
-    _, J0o, J2o, _, _ = oracle_cfg1
+def test_se_solve_from_oracle_J(portal1, cloud4k, oracle, oracle_cloud4k):
+    sp, g, xth = cloud4k
+    _, J0o, J2o, _, _ = oracle_cloud4k
     pp = portal1.se_solve(J0o, J2o)
     ref = oracle.solve_from_J(g, sp, J0o, J2o)
     assert_pops_close(pp, ref, sp, solved(g), 1e-11)
@@ -203,9 +203,9 @@ def test_se_solve_74_dimensional_systems(oracle, waves):
         np.testing.assert_allclose(pp[idx, 2 * i + 1], ref[idx, 2 * i + 1], rtol=0, atol=1e-10 * np.abs(o0).max())
 
 
-def test_mixed_precision_error_bounds(portal1, cfg1, oracle_cfg1):
-    sp, g, xth = cfg1
-    ppo, J0o, J2o, _, _ = oracle_cfg1
+def test_mixed_precision_error_bounds(portal1, cloud4k, oracle_cloud4k):
+    sp, g, xth = cloud4k
+    ppo, J0o, J2o, _, _ = oracle_cloud4k
     idx = solved(g)
     J0, J2, _ = portal1.inverse_rt(xth, 32, cloud.DX_INWARD, precision=PORTAL_MIXED)
     e0 = np.abs(J0[:, idx] / J0o[:, idx] - 1).max()
@@ -218,7 +218,7 @@ def test_mixed_precision_error_bounds(portal1, cfg1, oracle_cfg1):
     assert r0 < 1e-5
 
 
-def test_lambda_iteration_matches_oracle(cfg1, oracle):
+def test_lambda_iteration_matches_oracle(cloud4k, oracle):
     # two closure steps lev_p <- ab n sqrt(2J+1) rho00 on a small cloud
     sp = cloud.co_like_rotor(4)
     g = cloud.make_cloud(sp, 800, seed=9)
@@ -270,10 +270,10 @@ KERNELS = [(0, 4), (1, 4), (1, 2), (2, 2), (2, 4), (2, 5), (2, 8), (3, 4), (3, 5
 
 
 @pytest.mark.parametrize("variant,waves", KERNELS)
-def test_each_inward_kernel_cfg1(portal1, cfg1, oracle_cfg1, variant, waves):
+def test_each_inward_kernel_4000_nodes(portal1, cloud4k, oracle_cloud4k, variant, waves):
     # every kernel the autotuner may pick, FP64 against the oracle and mixed within bounds
-    sp, g, xth = cfg1
-    _, J0o, J2o, _, _ = oracle_cfg1
+    sp, g, xth = cloud4k
+    _, J0o, J2o, _, _ = oracle_cloud4k
     idx = solved(g)
     try:
         portal1.force_rt_kernel(PORTAL_FP64, variant, waves)
@@ -328,8 +328,8 @@ def test_each_inward_kernel_10_levels(cloud10, variant, waves):
         pt.force_rt_kernel(PORTAL_MIXED, -1)
 
 
-def test_error_paths(cfg1):
-    sp, g, xth = cfg1
+def test_error_paths(cloud4k):
+    sp, g, xth = cloud4k
     pt = Portal(0)
     pt.upload_grid(g)
     pt.upload_species(sp)
@@ -448,10 +448,10 @@ def test_se_forms_give_the_same_bits():
         np.testing.assert_array_equal(o, out[0])
 
 
-def test_image_pass_fp64_parity(portal1, cfg1, oracle, oracle_cfg1):
+def test_image_pass_fp64_parity(portal1, cloud4k, oracle, oracle_cloud4k):
     from portal_amd import driver
-    sp, g, xth = cfg1
-    ppo = oracle_cfg1[0]
+    sp, g, xth = cloud4k
+    ppo = oracle_cloud4k[0]
     pp = driver.rescale_pol_pops(ppo, g.lev_p, sp.lev_g)
     lines = driver.image_lines(sp, [1], [2])
     vel_out = driver.channel_velocities(32, 93.75)
@@ -469,11 +469,11 @@ def test_image_pass_fp64_parity(portal1, cfg1, oracle, oracle_cfg1):
     assert np.abs(Qo).max() > 1e-6 * Imax and np.abs(Uo).max() > 0  # polarisation is present
 
 
-def test_image_pass_mixed_error(portal1, cfg1, oracle_cfg1):
+def test_image_pass_mixed_error(portal1, cloud4k, oracle_cloud4k):
     # FP32 Stokes transfer on the same populations: error on the scale of max |I|
     from portal_amd import driver
-    sp, g, xth = cfg1
-    pp = driver.rescale_pol_pops(oracle_cfg1[0], g.lev_p, sp.lev_g)
+    sp, g, xth = cloud4k
+    pp = driver.rescale_pol_pops(oracle_cloud4k[0], g.lev_p, sp.lev_g)
     lines = driver.image_lines(sp, [1, 2], [2, 3])
     vel_out = driver.channel_velocities(32, 93.75)
     args = (pp, lines, np.pi / 4, 0.0, 32, vel_out, 24, 0.7, cloud.DX_IMAGE)
@@ -647,10 +647,10 @@ def test_image_pass_node_pixels(oracle):
     pt.close()
 
 
-def test_image_pass_dust_branch(cfg1, oracle):
+def test_image_pass_dust_branch(cloud4k, oracle):
     # dust present in the image pass: both branches agree
     from portal_amd import driver
-    sp, g0, xth = cfg1
+    sp, g0, xth = cloud4k
     g = cloud.make_cloud(sp, 1500, seed=12)
     g.dust_p[: g.bulk_node_num - 1] = 1e-17 * g.h2_dens[: g.bulk_node_num - 1] / 1e12
     sp.kd_v = np.full(sp.r_num, 0.3)
@@ -678,10 +678,10 @@ def test_image_pass_dust_branch(cfg1, oracle):
     pt.close()
 
 
-def test_image_depth_overflow(portal1, cfg1, oracle_cfg1):
+def test_image_depth_overflow(portal1, cloud4k, oracle_cloud4k):
     from portal_amd import driver
-    sp, g, xth = cfg1
-    pp = driver.rescale_pol_pops(oracle_cfg1[0], g.lev_p, sp.lev_g)
+    sp, g, xth = cloud4k
+    pp = driver.rescale_pol_pops(oracle_cloud4k[0], g.lev_p, sp.lev_g)
     with pytest.raises(PortalError) as e:
         portal1.image_pass(pp, driver.image_lines(sp, [1], [2]), 0.3, 0.0, 2, driver.channel_velocities(2, 100.0),
                            4, 0.7, cloud.DX_IMAGE, n_depth_rt=3)

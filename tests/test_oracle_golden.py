@@ -24,7 +24,7 @@ def _digest(g):
 
 
 @pytest.fixture(scope="module")
-def cfg1():
+def cloud4k():
     sp = cloud.co_like_rotor(4)
     g = cloud.make_cloud(sp, 4000, seed=1)
     return sp, g
@@ -71,9 +71,9 @@ def test_find_dir_and_black_body(oracle, golden_dir):
                 assert v == pytest.approx(2 * 6.62607015e-34 * nu**3 / 299792458.0**2 / math.expm1(6.62607015e-34 * nu / (1.380649e-23 * T)), rel=2e-6)
 
 
-def test_walk_matches_reference(oracle, cfg1, golden_dir):
-    sp, g = cfg1
-    z = np.load(os.path.join(golden_dir, "inward_cfg1.npz"))
+def test_walk_matches_reference(oracle, cloud4k, golden_dir):
+    sp, g = cloud4k
+    z = np.load(os.path.join(golden_dir, "inward_4000_nodes.npz"))
     if str(z["digest"]) != _digest(g):
         pytest.skip("cloud generator output differs from the fixture's")
     G = oracle.grid(g)
@@ -89,9 +89,9 @@ def test_walk_matches_reference(oracle, cfg1, golden_dir):
         np.testing.assert_array_equal(lens[:nc], ref[1])
 
 
-def test_inverse_rt_matches_reference(oracle, cfg1, golden_dir):
-    sp, g = cfg1
-    z = np.load(os.path.join(golden_dir, "inward_cfg1.npz"))
+def test_inverse_rt_matches_reference(oracle, cloud4k, golden_dir):
+    sp, g = cloud4k
+    z = np.load(os.path.join(golden_dir, "inward_4000_nodes.npz"))
     if str(z["digest"]) != _digest(g):
         pytest.skip("cloud generator output differs from the fixture's")
     xth = cloud.legendre_all(6, 11)
@@ -153,8 +153,8 @@ def test_coupling_selection_rules(oracle):
                         assert t0 == 0
 
 
-def test_image_pieces_match_reference(oracle, cfg1, golden_dir):
-    sp, g = cfg1
+def test_image_pieces_match_reference(oracle, cloud4k, golden_dir):
+    sp, g = cloud4k
     z = np.load(os.path.join(golden_dir, "image_pieces.npz"))
     N = g.node_num
     vn, bn, xn = (np.zeros((3, N), order="F") for _ in range(3))
