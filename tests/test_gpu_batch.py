@@ -347,26 +347,29 @@ def test_batch_with_dust_and_star_equals_single_models(precision, tol):
     pt.close()
 
 
-def test_batch_staged_form_when_a_model_does_not_fit_the_cooperative_kernel():
+@pytest.mark.parametrize("to_tol", [False, True])
+def test_batch_staged_form_when_a_model_does_not_fit_the_cooperative_kernel(to_tol):
     """A 13-level rotor with 255 channels has 12 x 255 pairs, more than the cooperative
     kernel's 256 lanes x 12 slots hold, so the whole batch takes the staged persistent
     form (rt_variant 1), whose counters the batch clears in four ints; in FP64 each model
-    equals its single-model iterations to 1e-10."""
+    equals its single-model iterations to 1e-10, over three iterations or to a tolerance
+    (where the models leave the staged form's work list at different iterations)."""
     specs = [(500, 13, 41, 127), (700, 4, 42, 127)]
     models = []
     for n, L, seed, nvel in specs:
         sp = cloud.co_like_rotor(L)
         models.append((sp, cloud.make_cloud(sp, n, seed=seed), nvel))
     xth = cloud.legendre_all(3, 6)
-    iters = 3
-    singles = []
+    iters, kw = (200, dict(tol_pop=1e-5, tol_align=1e-7)) if to_tol else (3, dict(tol_pop=0.0, tol_align=0.0))
+    singles, its = [], []
     for sp, g, nvel in models:
         pt = Portal(0)
         pt.upload_grid(g)
         pt.upload_species(sp)
         pt.set_levpop(g.lev_p)
-        singles.append(pt.lambda_iterate(xth, nvel, cloud.DX_INWARD, iters, tol_pop=0.0, tol_align=0.0,
-                                         precision=PORTAL_FP64)[:2])
+        pp, lev, st = pt.lambda_iterate(xth, nvel, cloud.DX_INWARD, iters, precision=PORTAL_FP64, **kw)
+        singles.append((pp, lev))
+        its.append(st["iterations"])
         pt.close()
     pts = []
     for sp, g, nvel in models:
@@ -375,10 +378,13 @@ def test_batch_staged_form_when_a_model_does_not_fit_the_cooperative_kernel():
         pt.upload_species(sp)
         pt.set_levpop(g.lev_p)
         pts.append(pt)
-    per, tot = batch_lambda_iterate(pts, [xth] * 2, [m[2] for m in models], cloud.DX_INWARD, iters, tol_pop=0.0,
-                                    tol_align=0.0, precision=PORTAL_FP64)
-    print(tot)
-    assert tot["rt_variant"] == 1 and tot["iterations"] == iters
+    per, tot = batch_lambda_iterate(pts, [xth] * 2, [m[2] for m in models], cloud.DX_INWARD, iters,
+                                    precision=PORTAL_FP64, **kw)
+    print(tot, "single-model iterations", its, "batch", [p["iterations"] for p in per])
+    assert tot["rt_variant"] == 1 and tot["iterations"] == max(its)
+    assert [p["iterations"] for p in per] == its
+    if to_tol:
+        assert tot["converged"] and len(set(its)) > 1
     for (sp, g, nvel), pt, (pp, lev) in zip(models, pts, singles):
         idx = np.where(g.ab_all[: g.bulk_node_num] > 0)[0]
         np.testing.assert_allclose(pt.get_levpop()[:, idx], lev[:, idx], rtol=1e-10)
