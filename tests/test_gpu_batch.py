@@ -105,11 +105,12 @@ def test_batch_converges_each_model():
         pt.close()
 
 
-@pytest.mark.parametrize("ng", [0, 1])
-def test_batch_models_stop_where_their_single_runs_stop(ng):
+@pytest.mark.parametrize("ng,se_form", [(0, -1), (1, -1), (1, 0)])
+def test_batch_models_stop_where_their_single_runs_stop(ng, se_form):
     # a model leaves the batch's iteration once it meets the tolerance: its iteration
     # count, level densities and SE solution equal those of its single-model run to the
-    # same tolerance, with Ng extrapolation on the same schedule or without
+    # same tolerance, with Ng extrapolation on the same schedule or without, and with the
+    # SE solves in the timed form or in the one persistent launch over all sizes
     models = _models()
     xth = cloud.legendre_all(4, 8)
     kw = dict(tol_pop=1e-6, tol_align=1e-8, precision=PORTAL_FP64, ng_accel=ng)
@@ -130,7 +131,11 @@ def test_batch_models_stop_where_their_single_runs_stop(ng):
         pt.upload_species(sp)
         pt.set_levpop(g.lev_p)
         pts.append(pt)
+    if se_form >= 0:
+        pts[0].force_se(se_form, 0)
     per, tot = batch_lambda_iterate(pts, [xth] * 3, [8] * 3, cloud.DX_INWARD, 200, **kw)
+    if se_form >= 0:
+        assert tot["se_form"] == se_form
     its = [s[2] for s in singles]
     print(f"ng={ng}: single-model iterations {its}, batch {[p['iterations'] for p in per]}")
     assert tot["converged"] and tot["iterations"] == max(its)
